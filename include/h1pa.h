@@ -1,0 +1,215 @@
+/*
+ * h1pa.h -- C ABI of the MI355X-native H1 partial-assembly (PA) operator path.
+ *
+ * Drop-in boundary for MFEM's matrix-free H1 path on hexahedra:
+ *   BilinearForm::SetAssemblyLevel(AssemblyLevel::PARTIAL)  (fem/bilinearform.cpp:109-136)
+ *   BilinearForm::Assemble / Operator::Mult / AssembleDiagonal
+ *                                                 (fem/bilinearform_ext.cpp:332-454, :487-677)
+ *   ElementRestriction                            (fem/restriction.hpp:40-108, restriction.cpp:26-290)
+ *   QuadratureInterpolator (geometric factors)    (fem/quadinterpolator.cpp:226-341)
+ *   ConstrainedOperator + OperatorJacobiSmoother + CGSolver
+ *                                                 (linalg/operator.cpp:129-145, :526-662;
+ *                                                  linalg/solvers.cpp:324-453, :869-1050)
+ *
+ * Conventions
+ *   - Every function returns an int status (H1PA_OK = 0); h1pa_last_error()
+ *     returns a thread-local message for the last failure.  No exception
+ *     crosses this boundary (MFEM: MFEM_VERIFY/MFEM_ABORT, general/error.hpp:104-166).
+ *   - Vector arguments named d_* are DEVICE pointers (FP64 unless stated);
+ *     arguments named h_* are HOST pointers.  Layouts follow MFEM: L-vectors
+ *     are indexed by global DOF; E-vectors are [dof][ne] with dof =
+ *     (p+1)^3 in lexicographic order (ElementDofOrdering::LEXICOGRAPHIC).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the default stream).
+ *     Calls on one handle are not re-entrant; distinct handles may run on
+ *     distinct streams.
+ */
+#ifndef H1PA_H
+#define H1PA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------- */
+enum {
+    H1PA_OK = 0,
+    H1PA_EINVAL = 1,    /* bad argument                          */
+    H1PA_ENOMEM = 2,    /* host or device allocation failed      */
+    H1PA_EHIP = 3,      /* HIP runtime error                     */
+    H1PA_ENOTCONV = 4,  /* solver did not converge               */
+    H1PA_EUNSUP = 5     /* unsupported order / shape             */
+};
+
+typedef struct h1pa_mesh h1pa_mesh;
+typedef struct h1pa_space h1pa_space;
+typedef struct h1pa_restriction h1pa_restriction;
+typedef struct h1pa_form h1pa_form;
+
+/* Last error message of the calling thread ("" if none). */
+const char *h1pa_last_error(void);
+/* Library version string. */
+const char *h1pa_version(void);
+/* Select the HIP device for subsequent handle creation. */
+int h1pa_set_device(int device);
+/* Block until all work the library queued on `stream` is done. */
+int h1pa_stream_synchronize(void *stream);
+
+/* ---- mesh: Mesh::MakeCartesian3D(nx, ny, nz, HEXAHEDRON, 1, 1, 1, sfc)
+ *      (mesh/mesh.cpp:3841-4121, :4786-4794) -------------------------------- */
+int h1pa_mesh_create_cartesian(int nx, int ny, int nz, int sfc_ordering, h1pa_mesh **mesh);
+int h1pa_mesh_destroy(h1pa_mesh *mesh);
+/* Perturb interior vertices by U(-amp*h, amp*h) per coordinate (counter-based
+ * splitmix64 with `seed`); boundary vertices stay on the cube faces. */
+int h1pa_mesh_perturb(h1pa_mesh *mesh, double amp, uint64_t seed);
+/* Give the mesh order-`order` nodes (Mesh::SetCurvature, mesh/mesh.cpp:7211)
+ * mapped by x' = x + amp sin(2 pi y) sin(2 pi z) and its cyclic permutations
+ * (Mesh::Transform, mesh/mesh.cpp:14056).  amp = 0 keeps the trilinear map. */
+int h1pa_mesh_set_curvature(h1pa_mesh *mesh, int order, double amp);
+/* counts[0..3] = NV, NE, NEdges, NFaces. */
+int h1pa_mesh_counts(const h1pa_mesh *mesh, int64_t *counts);
+/* Host copies of the topology (NULL to skip): element vertices [ne][8],
+ * edges [ne][12], edge orientations (+-1) [ne][12], faces [ne][6], face
+ * orientations [ne][6]  (Mesh::GetElementVertices/Edges/Faces,
+ * mesh/mesh.cpp:8044-8073, :8318-8340). */
+int h1pa_mesh_topology(const h1pa_mesh *mesh, int *h_ev, int *h_ee, int *h_eeo,
+                       int *h_ef, int *h_efo);
+/* Host copy of the vertex coordinates [nv][3]. */
+int h1pa_mesh_vertices(const h1pa_mesh *mesh, double *h_xyz);
+
+/* ---- H1 space: FiniteElementSpace(mesh, H1_FECollection(p, 3, GaussLobatto))
+ *      (fem/fespace.cpp:2769-2900, :3428-3535; fem/fe_coll.cpp:1741-2107) ---- */
+int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space);
+int h1pa_space_destroy(h1pa_space *space);
+/* info[0..3] = order, NE, dof per element, NDofs (= true VSize). */
+int h1pa_space_info(const h1pa_space *space, int64_t *info);
+/* Element-to-DOF table in MFEM's native element order
+ * (FiniteElementSpace::GetElementToDofTable), device [ne][dof]. */
+int h1pa_space_element_dofs(const h1pa_space *space, const int **d_elem_dof);
+/* Essential true DOFs for all boundary attributes, ascending
+ * (FiniteElementSpace::GetBoundaryTrueDofs, fem/fespace.cpp:668-680). */
+int h1pa_space_boundary_dofs(const h1pa_space *space, const int **d_ess, int64_t *n_ess);
+/* Borrowed ElementRestriction of the space (LEXICOGRAPHIC;
+ * FiniteElementSpace::GetElementRestriction, fem/fespace.cpp:1476-1505). */
+int h1pa_space_restriction(h1pa_space *space, h1pa_restriction **restriction);
+
+/* ---- ElementRestriction (fem/restriction.hpp:40-108) ---------------------- */
+/* Build from MFEM's own element-to-DOF table and lexicographic dof_map, as the
+ * ElementRestriction constructor does (fem/restriction.cpp:26-114).  Signed
+ * entries follow MFEM's -1-i convention.  Host inputs. */
+int h1pa_restriction_create(int ne, int dof, int ndofs, const int *h_elem_dof,
+                            const int *h_dof_map, h1pa_restriction **restriction);
+int h1pa_restriction_destroy(h1pa_restriction *restriction);
+/* Device pointers of gather_map [ne*dof], offsets [ndofs+1], indices [ne*dof]
+ * (GatherMap/Offsets/Indices, fem/restriction.hpp:105-107). */
+int h1pa_restriction_maps(const h1pa_restriction *restriction, const int **d_gather_map,
+                          const int **d_offsets, const int **d_indices);
+/* L->E: y[i%dof, i/dof] = +-x[gather_map[i]]   (restriction.cpp:116-136) */
+int h1pa_restriction_mult(const h1pa_restriction *r, const double *d_x, double *d_y, void *stream);
+/* L->E without signs (restriction.cpp:138-157) */
+int h1pa_restriction_abs_mult(const h1pa_restriction *r, const double *d_x, double *d_y, void *stream);
+/* E->L set, deterministic transpose-table sum (restriction.cpp:159-193) */
+int h1pa_restriction_mult_transpose(const h1pa_restriction *r, const double *d_x, double *d_y,
+                                    void *stream);
+/* E->L add; a must be 1.0 as in MFEM (restriction.cpp:195-201) */
+int h1pa_restriction_add_mult_transpose(const h1pa_restriction *r, const double *d_x,
+                                        double *d_y, double a, void *stream);
+/* E->L set without signs (restriction.cpp:203-228) */
+int h1pa_restriction_abs_mult_transpose(const h1pa_restriction *r, const double *d_x,
+                                        double *d_y, void *stream);
+/* E->L taking the last E-entry of each L-DOF (restriction.cpp:230-253) */
+int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *d_x,
+                                       double *d_y, void *stream);
+
+/* ---- quadrature interpolation of the mesh nodes (geometric factors)
+ *      GeometricFactors::Compute (mesh/mesh.cpp:15445-15497) via
+ *      QuadratureInterpolator::Mult (fem/quadinterpolator.cpp:226-341) ------- */
+enum { H1PA_QI_VALUES = 1, H1PA_QI_DERIVATIVES = 2, H1PA_QI_DETERMINANTS = 4 };
+/* Evaluate the mesh nodes at the q1d^3 Gauss-Legendre points: X[nq][3][ne],
+ * J[nq][3][3][ne] (J(q,r,c,e) = dx_r/dxi_c), detJ[nq][ne]; NULL skips. */
+int h1pa_mesh_geometric_factors(const h1pa_mesh *mesh, int q1d, unsigned flags,
+                                double *d_X, double *d_J, double *d_detJ, void *stream);
+
+/* ---- PA bilinear form ---------------------------------------------------- */
+enum { H1PA_INTEG_MASS = 1, H1PA_INTEG_DIFFUSION = 2 };
+enum {
+    H1PA_COEFF_CONSTANT = 0,    /* ConstantCoefficient(value)                         */
+    H1PA_COEFF_SINCOS = 1,      /* 1 + 0.5 sin(2 pi x) cos(2 pi y) sin(2 pi z)        */
+    H1PA_COEFF_PER_ELEMENT = 2, /* PWConstCoefficient-like: d_data[ne] device values  */
+    H1PA_COEFF_QUADRATURE = 3,  /* QuadratureFunction: d_data[q1d^3][ne] device       */
+    H1PA_COEFF_FOURIER = 4      /* exp(sum of 8 seeded Fourier modes, amplitude 0.5)  */
+};
+typedef struct {
+    unsigned integrators;   /* H1PA_INTEG_* bitmask                         */
+    int coeff_kind;         /* H1PA_COEFF_*                                  */
+    double coeff_value;     /* constant value                                */
+    const double *d_coeff;  /* device data for PER_ELEMENT / QUADRATURE      */
+    uint64_t coeff_seed;    /* seed for FOURIER                              */
+    int q1d;                /* 0 -> MFEM default p+2                          */
+} h1pa_form_desc;
+
+/* BilinearForm a(fes); a.SetAssemblyLevel(PARTIAL); a.AddDomainIntegrator(...) */
+int h1pa_form_create(h1pa_space *space, const h1pa_form_desc *desc, h1pa_form **form);
+int h1pa_form_destroy(h1pa_form *form);
+/* a.Assemble(): geometric factors, coefficient and PA data on the device
+ * (DiffusionIntegrator::AssemblePA, fem/integ/bilininteg_diffusion_pa.cpp:108-171;
+ *  MassIntegrator::AssemblePA, fem/integ/bilininteg_mass_pa.cpp:25-90). */
+int h1pa_form_assemble(h1pa_form *form, void *stream);
+/* Operator::Mult: y = A x on L-vectors, y overwritten
+ * (PABilinearFormExtension::Mult, fem/bilinearform_ext.cpp:487-560). */
+int h1pa_form_mult(h1pa_form *form, const double *d_x, double *d_y, void *stream);
+/* AddMultPA on E-vectors: yE += A_E xE (fem/bilininteg.hpp:74-81). */
+int h1pa_form_add_mult_e(h1pa_form *form, const double *d_xE, double *d_yE, void *stream);
+/* Operator::AssembleDiagonal: L-vector diagonal, set
+ * (PABilinearFormExtension::AssembleDiagonal, fem/bilinearform_ext.cpp:370-454). */
+int h1pa_form_assemble_diagonal(h1pa_form *form, double *d_diag, void *stream);
+/* Device pointers of the PA data: diffusion D[q1d^3][6][ne] (comps 11,21,31,
+ * 22,32,33) and mass v[q1d^3][ne]; NULL where the integrator is absent. */
+int h1pa_form_pa_data(const h1pa_form *form, const double **d_diffusion, const double **d_mass);
+/* info[0..2] = q1d, integrators, ndofs */
+int h1pa_form_info(const h1pa_form *form, int64_t *info);
+/* Benchmark helper (no MFEM counterpart): `steps` back-to-back h1pa_form_mult
+ * calls with HIP events recorded on `stream` around each launch.  Returns the
+ * wall time of the whole region and the average duration of the element
+ * kernel(s) and of the E->L scatter, in milliseconds. */
+int h1pa_form_mult_timed(h1pa_form *form, const double *d_x, double *d_y, int steps,
+                         double *ms_total, double *ms_element, double *ms_scatter, void *stream);
+
+/* ---- essential BCs and Jacobi-PCG --------------------------------------- */
+/* ConstrainedOperator (DIAG_ONE) Mult with the ess list (linalg/operator.cpp:601-662). */
+int h1pa_form_constrained_mult(h1pa_form *form, const int *d_ess, int64_t n_ess,
+                               const double *d_x, double *d_y, void *stream);
+/* Operator::FormLinearSystem with copy_interior = 0 (linalg/operator.cpp:129-145,
+ * :574-599): X = x on ess, 0 elsewhere; B = b - A X_ess; B[ess] = X[ess]. */
+int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess,
+                            const double *d_x, const double *d_b, double *d_X, double *d_B,
+                            void *stream);
+/* DomainLFIntegrator(ConstantCoefficient(f)) assembled into an L-vector with
+ * q1d = p+1 (fem/lininteg.cpp:51-79, LinearForm::Assemble). */
+int h1pa_space_domain_lf(h1pa_space *space, double f, double *d_b, void *stream);
+
+typedef struct {
+    double rel_tol;     /* CGSolver::SetRelTol (preconditioned-norm criterion) */
+    double abs_tol;     /* CGSolver::SetAbsTol                                 */
+    int max_iter;       /* CGSolver::SetMaxIter                                */
+    int check_every;    /* iterations per host convergence check (0 -> 8)      */
+} h1pa_pcg_opts;
+typedef struct {
+    int converged;
+    int iterations;       /* CGSolver::GetNumIterations                    */
+    double initial_norm;  /* sqrt((B r0, r0))                              */
+    double final_norm;    /* sqrt((B r, r)), CGSolver::GetFinalNorm        */
+    double seconds;       /* wall time of the iteration loop               */
+} h1pa_pcg_stats;
+/* OperatorJacobiSmoother(a, ess) + CGSolver on the DIAG_ONE ConstrainedOperator,
+ * iterative_mode = false (x is overwritten; initial guess 0). */
+int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *d_b,
+                   double *d_x, const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats,
+                   void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H1PA_H */

@@ -1,0 +1,722 @@
+"""mfem_amd -- MI355X-native H1 partial-assembly operator path.
+
+A thin Python mirror of MFEM's interface for this path (same class and method
+names, argument meaning and error behaviour) over the C ABI of libh1pa.so
+(include/h1pa.h).  All arithmetic runs in the HIP library; vectors are torch
+float64 tensors on the GPU, used only as device memory.
+
+    mesh = Mesh.MakeCartesian3D(16, 16, 16)
+    fes = FiniteElementSpace(mesh, H1_FECollection(3, 3))
+    a = BilinearForm(fes)
+    a.SetAssemblyLevel(AssemblyLevel.PARTIAL)
+    a.AddDomainIntegrator(DiffusionIntegrator(ConstantCoefficient(1.0)))
+    a.Assemble()
+    a.Mult(x, y)
+
+Reference: fem/bilinearform.hpp:31-53 (AssemblyLevel), fem/restriction.hpp:40-108
+(ElementRestriction), fem/bilinearform_ext.cpp (PABilinearFormExtension),
+linalg/solvers.cpp (CGSolver, OperatorJacobiSmoother).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libh1pa.so")
+
+_lib = None
+
+_i, _i64, _u, _d, _vp = C.c_int, C.c_int64, C.c_uint, C.c_double, C.c_void_p
+_pi64 = C.POINTER(C.c_int64)
+
+H1PA_OK, H1PA_EINVAL, H1PA_ENOMEM, H1PA_EHIP, H1PA_ENOTCONV, H1PA_EUNSUP = range(6)
+INTEG_MASS, INTEG_DIFFUSION = 1, 2
+COEFF_CONSTANT, COEFF_SINCOS, COEFF_PER_ELEMENT, COEFF_QUADRATURE, COEFF_FOURIER = range(5)
+QI_VALUES, QI_DERIVATIVES, QI_DETERMINANTS = 1, 2, 4
+
+
+class H1paError(RuntimeError):
+    """A non-zero status from libh1pa (MFEM would call mfem_error here)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"h1pa error {code}: {msg}")
+        self.code = code
+
+
+class FormDesc(C.Structure):
+    _fields_ = [("integrators", _u), ("coeff_kind", _i), ("coeff_value", _d),
+                ("d_coeff", _vp), ("coeff_seed", C.c_uint64), ("q1d", _i)]
+
+
+class PcgOpts(C.Structure):
+    _fields_ = [("rel_tol", _d), ("abs_tol", _d), ("max_iter", _i), ("check_every", _i)]
+
+
+class PcgStats(C.Structure):
+    _fields_ = [("converged", _i), ("iterations", _i), ("initial_norm", _d),
+                ("final_norm", _d), ("seconds", _d)]
+
+
+# name -> (restype, argtypes); every entry point of include/h1pa.h
+SIGNATURES = {
+    "h1pa_last_error": (C.c_char_p, []),
+    "h1pa_version": (C.c_char_p, []),
+    "h1pa_set_device": (_i, [_i]),
+    "h1pa_stream_synchronize": (_i, [_vp]),
+    "h1pa_mesh_create_cartesian": (_i, [_i, _i, _i, _i, C.POINTER(_vp)]),
+    "h1pa_mesh_destroy": (_i, [_vp]),
+    "h1pa_mesh_perturb": (_i, [_vp, _d, C.c_uint64]),
+    "h1pa_mesh_set_curvature": (_i, [_vp, _i, _d]),
+    "h1pa_mesh_counts": (_i, [_vp, _pi64]),
+    "h1pa_mesh_topology": (_i, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "h1pa_mesh_vertices": (_i, [_vp, _vp]),
+    "h1pa_mesh_geometric_factors": (_i, [_vp, _i, _u, _vp, _vp, _vp, _vp]),
+    "h1pa_space_create": (_i, [_vp, _i, C.POINTER(_vp)]),
+    "h1pa_space_destroy": (_i, [_vp]),
+    "h1pa_space_info": (_i, [_vp, _pi64]),
+    "h1pa_space_element_dofs": (_i, [_vp, C.POINTER(_vp)]),
+    "h1pa_space_boundary_dofs": (_i, [_vp, C.POINTER(_vp), _pi64]),
+    "h1pa_space_restriction": (_i, [_vp, C.POINTER(_vp)]),
+    "h1pa_restriction_create": (_i, [_i, _i, _i, _vp, _vp, C.POINTER(_vp)]),
+    "h1pa_restriction_destroy": (_i, [_vp]),
+    "h1pa_restriction_maps": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "h1pa_restriction_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_restriction_abs_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_restriction_mult_transpose": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_restriction_add_mult_transpose": (_i, [_vp, _vp, _vp, _d, _vp]),
+    "h1pa_restriction_abs_mult_transpose": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_restriction_mult_left_inverse": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_form_create": (_i, [_vp, C.POINTER(FormDesc), C.POINTER(_vp)]),
+    "h1pa_form_destroy": (_i, [_vp]),
+    "h1pa_form_assemble": (_i, [_vp, _vp]),
+    "h1pa_form_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_form_add_mult_e": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_form_assemble_diagonal": (_i, [_vp, _vp, _vp]),
+    "h1pa_form_pa_data": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "h1pa_form_info": (_i, [_vp, _pi64]),
+    "h1pa_form_mult_timed": (_i, [_vp, _vp, _vp, _i, C.POINTER(_d), C.POINTER(_d),
+                                  C.POINTER(_d), _vp]),
+    "h1pa_form_constrained_mult": (_i, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "h1pa_form_linear_system": (_i, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "h1pa_space_domain_lf": (_i, [_vp, _d, _vp, _vp]),
+    "h1pa_pcg_solve": (_i, [_vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
+                            _vp]),
+}
+
+
+def lib():
+    """Load libh1pa.so (built by mfem_amd.build); fail loudly if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run `python -m mfem_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(status):
+    if status != H1PA_OK:
+        raise H1paError(status, lib().h1pa_last_error().decode())
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _stream():
+    torch = _torch()
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError("expected a CUDA (HIP) tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _check_vec(t, n, dtype_name="float64"):
+    torch = _torch()
+    want = getattr(torch, dtype_name)
+    if t.dtype != want or t.numel() != n:
+        raise ValueError(f"vector must be {dtype_name}[{n}], got {t.dtype}[{t.numel()}]")
+
+
+def _wrap_device_int(ptr, n):
+    """Copy n int32 values from a device pointer into a new torch tensor."""
+    torch = _torch()
+    out = torch.empty(int(n), dtype=torch.int32, device="cuda")
+    if n:
+        from ctypes import c_size_t
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [_vp, _vp, c_size_t, _i]
+        hip.hipMemcpy.restype = _i
+        rc = hip.hipMemcpy(C.c_void_p(out.data_ptr()), C.c_void_p(ptr), 4 * int(n), 3)
+        if rc != 0:
+            raise H1paError(H1PA_EHIP, f"hipMemcpy failed ({rc})")
+    return out
+
+
+def _wrap_device_double(ptr, n):
+    torch = _torch()
+    out = torch.empty(int(n), dtype=torch.float64, device="cuda")
+    if n:
+        from ctypes import c_size_t
+        hip = C.CDLL("libamdhip64.so")
+        hip.hipMemcpy.argtypes = [_vp, _vp, c_size_t, _i]
+        hip.hipMemcpy.restype = _i
+        rc = hip.hipMemcpy(C.c_void_p(out.data_ptr()), C.c_void_p(ptr), 8 * int(n), 3)
+        if rc != 0:
+            raise H1paError(H1PA_EHIP, f"hipMemcpy failed ({rc})")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# enums
+# ---------------------------------------------------------------------------
+class AssemblyLevel:
+    """fem/bilinearform.hpp:31-53"""
+    LEGACY, FULL, ELEMENT, PARTIAL, NONE = 0, 1, 2, 3, 4
+
+
+class ElementDofOrdering:
+    NATIVE, LEXICOGRAPHIC = 0, 1
+
+
+class Element:
+    HEXAHEDRON = 5
+
+
+# ---------------------------------------------------------------------------
+# mesh
+# ---------------------------------------------------------------------------
+class Mesh:
+    """Cartesian hexahedral mesh (Mesh::MakeCartesian3D, mesh/mesh.cpp:4786-4794)."""
+
+    def __init__(self, handle, nx, ny, nz):
+        self._h = handle
+        self.nx, self.ny, self.nz = nx, ny, nz
+
+    @staticmethod
+    def MakeCartesian3D(nx, ny, nz, type=Element.HEXAHEDRON, sx=1.0, sy=1.0, sz=1.0,
+                        sfc_ordering=True):
+        if type != Element.HEXAHEDRON:
+            raise H1paError(H1PA_EUNSUP, "only HEXAHEDRON meshes are supported")
+        if (sx, sy, sz) != (1.0, 1.0, 1.0):
+            raise H1paError(H1PA_EUNSUP, "only the unit cube is supported")
+        h = C.c_void_p()
+        _check(lib().h1pa_mesh_create_cartesian(nx, ny, nz, 1 if sfc_ordering else 0,
+                                                C.byref(h)))
+        return Mesh(h, nx, ny, nz)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().h1pa_mesh_destroy(self._h)
+            self._h = None
+
+    def _counts(self):
+        c = (C.c_int64 * 4)()
+        _check(lib().h1pa_mesh_counts(self._h, c))
+        return list(c)
+
+    def GetNV(self):
+        return self._counts()[0]
+
+    def GetNE(self):
+        return self._counts()[1]
+
+    def GetNEdges(self):
+        return self._counts()[2]
+
+    def GetNFaces(self):
+        return self._counts()[3]
+
+    def Perturb(self, amp, seed):
+        """Move interior vertices by U(-amp h, amp h) per coordinate (seeded)."""
+        _check(lib().h1pa_mesh_perturb(self._h, float(amp), int(seed)))
+
+    def SetCurvature(self, order, amp=0.0):
+        """SetCurvature(order) followed by the curving map of amplitude amp."""
+        _check(lib().h1pa_mesh_set_curvature(self._h, int(order), float(amp)))
+
+    def GetVertices(self):
+        xyz = np.zeros(3 * self.GetNV())
+        _check(lib().h1pa_mesh_vertices(self._h, xyz.ctypes.data_as(_vp)))
+        return xyz.reshape(-1, 3)
+
+    def Topology(self):
+        ne = self.GetNE()
+        ev = np.zeros(8 * ne, np.int32); ee = np.zeros(12 * ne, np.int32)
+        eeo = np.zeros(12 * ne, np.int32); ef = np.zeros(6 * ne, np.int32)
+        efo = np.zeros(6 * ne, np.int32)
+        _check(lib().h1pa_mesh_topology(self._h, *(a.ctypes.data_as(_vp)
+                                                   for a in (ev, ee, eeo, ef, efo))))
+        return dict(ev=ev.reshape(ne, 8), ee=ee.reshape(ne, 12), eeo=eeo.reshape(ne, 12),
+                    ef=ef.reshape(ne, 6), efo=efo.reshape(ne, 6))
+
+    def GetGeometricFactors(self, q1d, flags=QI_VALUES | QI_DERIVATIVES | QI_DETERMINANTS):
+        """X[nq][3][ne], J[nq][3][3][ne], detJ[nq][ne] at the q1d^3 GL points."""
+        torch = _torch()
+        ne, nq = self.GetNE(), q1d ** 3
+        X = torch.empty(nq * 3 * ne, dtype=torch.float64, device="cuda") if flags & QI_VALUES else None
+        J = torch.empty(nq * 9 * ne, dtype=torch.float64, device="cuda") if flags & QI_DERIVATIVES else None
+        dJ = torch.empty(nq * ne, dtype=torch.float64, device="cuda") if flags & QI_DETERMINANTS else None
+        _check(lib().h1pa_mesh_geometric_factors(self._h, q1d, flags, _ptr(X), _ptr(J), _ptr(dJ),
+                                                 _stream()))
+        return X, J, dJ
+
+
+# ---------------------------------------------------------------------------
+# space and restriction
+# ---------------------------------------------------------------------------
+class H1_FECollection:
+    """H1 Gauss-Lobatto collection (fem/fe_coll.cpp:1741)."""
+
+    def __init__(self, p, dim=3):
+        if dim != 3:
+            raise H1paError(H1PA_EUNSUP, "only dim = 3 is supported")
+        if not 1 <= p <= 8:
+            raise H1paError(H1PA_EUNSUP, "order must be in [1, 8]")
+        self.p, self.dim = p, dim
+
+    def GetOrder(self):
+        return self.p
+
+
+class ElementRestriction:
+    """L <-> E maps (fem/restriction.hpp:40-108)."""
+
+    def __init__(self, handle, ne, dof, ndofs, owner=None, owned=True):
+        self._h, self.ne, self.dof, self.ndofs = handle, ne, dof, ndofs
+        self._owner, self._owned = owner, owned
+
+    @staticmethod
+    def FromTable(elem_dof, dof_map, ndofs):
+        """ElementRestriction ctor from MFEM's element-to-DOF table + dof_map
+        (fem/restriction.cpp:26-114).  Host numpy int32 inputs."""
+        ed = np.ascontiguousarray(elem_dof, dtype=np.int32)
+        ne, dof = ed.shape
+        dm = None if dof_map is None else np.ascontiguousarray(dof_map, dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().h1pa_restriction_create(ne, dof, int(ndofs), ed.ctypes.data_as(_vp),
+                                             None if dm is None else dm.ctypes.data_as(_vp),
+                                             C.byref(h)))
+        return ElementRestriction(h, ne, dof, int(ndofs))
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "_h", None):
+            lib().h1pa_restriction_destroy(self._h)
+            self._h = None
+
+    def Height(self):
+        return self.ne * self.dof
+
+    def Width(self):
+        return self.ndofs
+
+    def _maps(self):
+        g, o, i = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().h1pa_restriction_maps(self._h, C.byref(g), C.byref(o), C.byref(i)))
+        return g.value, o.value, i.value
+
+    def GatherMap(self):
+        return _wrap_device_int(self._maps()[0], self.ne * self.dof)
+
+    def Offsets(self):
+        return _wrap_device_int(self._maps()[1], self.ndofs + 1)
+
+    def Indices(self):
+        return _wrap_device_int(self._maps()[2], self.ne * self.dof)
+
+    def Mult(self, x, y):
+        _check_vec(x, self.ndofs); _check_vec(y, self.ne * self.dof)
+        _check(lib().h1pa_restriction_mult(self._h, _ptr(x), _ptr(y), _stream()))
+
+    def AbsMult(self, x, y):
+        _check_vec(x, self.ndofs); _check_vec(y, self.ne * self.dof)
+        _check(lib().h1pa_restriction_abs_mult(self._h, _ptr(x), _ptr(y), _stream()))
+
+    def MultTranspose(self, x, y):
+        _check_vec(x, self.ne * self.dof); _check_vec(y, self.ndofs)
+        _check(lib().h1pa_restriction_mult_transpose(self._h, _ptr(x), _ptr(y), _stream()))
+
+    def AddMultTranspose(self, x, y, a=1.0):
+        _check_vec(x, self.ne * self.dof); _check_vec(y, self.ndofs)
+        _check(lib().h1pa_restriction_add_mult_transpose(self._h, _ptr(x), _ptr(y), float(a),
+                                                         _stream()))
+
+    def AbsMultTranspose(self, x, y):
+        _check_vec(x, self.ne * self.dof); _check_vec(y, self.ndofs)
+        _check(lib().h1pa_restriction_abs_mult_transpose(self._h, _ptr(x), _ptr(y), _stream()))
+
+    def MultLeftInverse(self, x, y):
+        _check_vec(x, self.ne * self.dof); _check_vec(y, self.ndofs)
+        _check(lib().h1pa_restriction_mult_left_inverse(self._h, _ptr(x), _ptr(y), _stream()))
+
+
+class FiniteElementSpace:
+    """FiniteElementSpace(mesh, H1_FECollection) on a conforming hex mesh."""
+
+    def __init__(self, mesh, fec):
+        self.mesh, self.fec = mesh, fec
+        h = C.c_void_p()
+        _check(lib().h1pa_space_create(mesh._h, fec.p, C.byref(h)))
+        self._h = h
+        info = (C.c_int64 * 4)()
+        _check(lib().h1pa_space_info(h, info))
+        self.p, self.ne, self.dof, self.ndofs = (int(v) for v in info)
+        r = C.c_void_p()
+        _check(lib().h1pa_space_restriction(h, C.byref(r)))
+        self._restr = ElementRestriction(r, self.ne, self.dof, self.ndofs, owner=self, owned=False)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._restr = None
+            lib().h1pa_space_destroy(self._h)
+            self._h = None
+
+    def GetNE(self):
+        return self.ne
+
+    def GetNDofs(self):
+        return self.ndofs
+
+    def GetVSize(self):
+        return self.ndofs
+
+    def GetTrueVSize(self):
+        return self.ndofs
+
+    def GetElementRestriction(self, ordering=ElementDofOrdering.LEXICOGRAPHIC):
+        if ordering != ElementDofOrdering.LEXICOGRAPHIC:
+            raise H1paError(H1PA_EUNSUP, "only LEXICOGRAPHIC element ordering is provided")
+        return self._restr
+
+    def GetElementToDofTable(self):
+        """Native-order element-to-DOF table as an int32 [ne, dof] device tensor."""
+        d = C.c_void_p()
+        _check(lib().h1pa_space_element_dofs(self._h, C.byref(d)))
+        return _wrap_device_int(d.value, self.ne * self.dof).view(self.ne, self.dof)
+
+    def GetBoundaryTrueDofs(self):
+        d, n = C.c_void_p(), C.c_int64()
+        _check(lib().h1pa_space_boundary_dofs(self._h, C.byref(d), C.byref(n)))
+        return _wrap_device_int(d.value, n.value)
+
+    def _boundary_raw(self):
+        d, n = C.c_void_p(), C.c_int64()
+        _check(lib().h1pa_space_boundary_dofs(self._h, C.byref(d), C.byref(n)))
+        return d, n.value
+
+
+# ---------------------------------------------------------------------------
+# coefficients and integrators
+# ---------------------------------------------------------------------------
+class Coefficient:
+    kind = COEFF_CONSTANT
+    value = 1.0
+    data = None
+    seed = 0
+
+
+class ConstantCoefficient(Coefficient):
+    def __init__(self, value=1.0):
+        self.kind, self.value = COEFF_CONSTANT, float(value)
+
+
+class SinCosCoefficient(Coefficient):
+    """k(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y) sin(2 pi z), evaluated at the
+    physical quadrature points (a FunctionCoefficient in MFEM terms)."""
+
+    def __init__(self):
+        self.kind = COEFF_SINCOS
+
+
+class PWConstCoefficient(Coefficient):
+    """One value per element (device float64 tensor of size NE)."""
+
+    def __init__(self, values):
+        self.kind, self.data = COEFF_PER_ELEMENT, values
+
+
+class QuadratureFunctionCoefficient(Coefficient):
+    """Values at every quadrature point, [q1d^3][ne] (device float64)."""
+
+    def __init__(self, values):
+        self.kind, self.data = COEFF_QUADRATURE, values
+
+
+class FourierCoefficient(Coefficient):
+    """k = exp(g), g = sum of 8 seeded Fourier modes (amplitude 0.5/sqrt 8 each)."""
+
+    def __init__(self, seed):
+        self.kind, self.seed = COEFF_FOURIER, int(seed)
+
+
+class BilinearFormIntegrator:
+    flag = 0
+
+    def __init__(self, coeff=None, q1d=0):
+        self.coeff = coeff if coeff is not None else ConstantCoefficient(1.0)
+        self.q1d = q1d
+
+
+class DiffusionIntegrator(BilinearFormIntegrator):
+    flag = INTEG_DIFFUSION
+
+
+class MassIntegrator(BilinearFormIntegrator):
+    flag = INTEG_MASS
+
+
+# ---------------------------------------------------------------------------
+# forms, operators, solvers
+# ---------------------------------------------------------------------------
+class _PAForm:
+    """One C form (a set of integrators sharing a coefficient and q1d)."""
+
+    def __init__(self, fes, flags, coeff, q1d):
+        desc = FormDesc()
+        desc.integrators = flags
+        desc.coeff_kind = coeff.kind
+        desc.coeff_value = coeff.value
+        desc.d_coeff = _ptr(coeff.data).value if coeff.data is not None else None
+        desc.coeff_seed = coeff.seed
+        desc.q1d = q1d
+        self._keep = coeff.data
+        h = C.c_void_p()
+        _check(lib().h1pa_form_create(fes._h, C.byref(desc), C.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().h1pa_form_destroy(self._h)
+            self._h = None
+
+
+class BilinearForm:
+    """BilinearForm with AssemblyLevel::PARTIAL (fem/bilinearform.cpp:109-136)."""
+
+    def __init__(self, fes):
+        self.fes = fes
+        self._level = None
+        self._integs = []
+        self._forms = []
+        self._tmp = None
+
+    def SetAssemblyLevel(self, level):
+        if self._level is not None:
+            raise H1paError(H1PA_EINVAL, "AssemblyLevel can only be set once")
+        if level != AssemblyLevel.PARTIAL:
+            raise H1paError(H1PA_EUNSUP, "only AssemblyLevel.PARTIAL is provided")
+        self._level = level
+
+    def AddDomainIntegrator(self, integ):
+        self._integs.append(integ)
+
+    def Assemble(self):
+        if self._level != AssemblyLevel.PARTIAL:
+            raise H1paError(H1PA_EINVAL, "call SetAssemblyLevel(PARTIAL) first")
+        if not self._integs:
+            raise H1paError(H1PA_EINVAL, "no integrators")
+        # integrators with the same coefficient object and q1d share one C form
+        groups = {}
+        for it in self._integs:
+            key = (id(it.coeff), it.q1d)
+            g = groups.setdefault(key, [0, it.coeff, it.q1d])
+            if g[0] & it.flag:
+                raise H1paError(H1PA_EUNSUP, "duplicate integrator with the same coefficient")
+            g[0] |= it.flag
+        self._forms = [_PAForm(self.fes, f, c, q) for f, c, q in groups.values()]
+        for f in self._forms:
+            _check(lib().h1pa_form_assemble(f._h, _stream()))
+
+    def Height(self):
+        return self.fes.ndofs
+
+    def Width(self):
+        return self.fes.ndofs
+
+    def _scratch(self):
+        torch = _torch()
+        if self._tmp is None:
+            self._tmp = torch.empty(self.fes.ndofs, dtype=torch.float64, device="cuda")
+        return self._tmp
+
+    def Mult(self, x, y):
+        n = self.fes.ndofs
+        _check_vec(x, n); _check_vec(y, n)
+        _check(lib().h1pa_form_mult(self._forms[0]._h, _ptr(x), _ptr(y), _stream()))
+        for f in self._forms[1:]:
+            t = self._scratch()
+            _check(lib().h1pa_form_mult(f._h, _ptr(x), _ptr(t), _stream()))
+            y.add_(t)
+
+    def MultTimed(self, x, y, steps):
+        """`steps` applies with HIP events around every launch; returns
+        (total_ms, element_kernel_ms_avg, scatter_ms_avg)."""
+        f = self._single()
+        tot, te, ts = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().h1pa_form_mult_timed(f._h, _ptr(x), _ptr(y), int(steps), C.byref(tot),
+                                          C.byref(te), C.byref(ts), _stream()))
+        return tot.value, te.value, ts.value
+
+    def AddMultPA_E(self, xE, yE):
+        """yE += A_E xE on E-vectors (BilinearFormIntegrator::AddMultPA)."""
+        for f in self._forms:
+            _check(lib().h1pa_form_add_mult_e(f._h, _ptr(xE), _ptr(yE), _stream()))
+
+    def AssembleDiagonal(self, diag):
+        _check_vec(diag, self.fes.ndofs)
+        _check(lib().h1pa_form_assemble_diagonal(self._forms[0]._h, _ptr(diag), _stream()))
+        for f in self._forms[1:]:
+            t = self._scratch()
+            _check(lib().h1pa_form_assemble_diagonal(f._h, _ptr(t), _stream()))
+            diag.add_(t)
+
+    def PAData(self):
+        """(diffusion D[q^3*6*ne], mass v[q^3*ne]) copies of the first form's PA data."""
+        dd, dm = C.c_void_p(), C.c_void_p()
+        f = self._forms[0]
+        _check(lib().h1pa_form_pa_data(f._h, C.byref(dd), C.byref(dm)))
+        info = (C.c_int64 * 3)()
+        _check(lib().h1pa_form_info(f._h, info))
+        q = int(info[0])
+        nq = q ** 3 * self.fes.ne
+        return (_wrap_device_double(dd.value, 6 * nq) if dd.value else None,
+                _wrap_device_double(dm.value, nq) if dm.value else None)
+
+    def _single(self):
+        if len(self._forms) != 1:
+            raise H1paError(H1PA_EUNSUP, "this call needs all integrators to share a coefficient")
+        return self._forms[0]
+
+    def FormLinearSystem(self, ess_tdof_list, x, b):
+        """Returns (A, X, B) as MFEM's FormLinearSystem (copy_interior = 0)."""
+        n = self.fes.ndofs
+        torch = _torch()
+        X = torch.empty(n, dtype=torch.float64, device="cuda")
+        B = torch.empty(n, dtype=torch.float64, device="cuda")
+        f = self._single()
+        _check(lib().h1pa_form_linear_system(f._h, _ptr(ess_tdof_list), ess_tdof_list.numel(),
+                                             _ptr(x), _ptr(b), _ptr(X), _ptr(B), _stream()))
+        return ConstrainedOperator(self, ess_tdof_list), X, B
+
+    def RecoverFEMSolution(self, X, b, x):
+        x.copy_(X)
+
+
+class ConstrainedOperator:
+    """DIAG_ONE constrained operator (linalg/operator.cpp:526-662)."""
+
+    def __init__(self, form, ess):
+        self.form, self.ess = form, ess
+
+    def Height(self):
+        return self.form.fes.ndofs
+
+    def Mult(self, x, y):
+        f = self.form._single()
+        _check(lib().h1pa_form_constrained_mult(f._h, _ptr(self.ess), self.ess.numel(), _ptr(x),
+                                                _ptr(y), _stream()))
+
+
+class OperatorJacobiSmoother:
+    """Jacobi from the PA diagonal, ess rows 1 (linalg/solvers.cpp:324-453)."""
+
+    def __init__(self, a, ess_tdof_list, damping=1.0):
+        if damping != 1.0:
+            raise H1paError(H1PA_EUNSUP, "only damping = 1 is provided")
+        self.a, self.ess = a, ess_tdof_list
+
+
+class CGSolver:
+    """CGSolver (linalg/solvers.hpp:626, solvers.cpp:869-1050) with the PA
+    operator and OperatorJacobiSmoother, run fully on the device."""
+
+    def __init__(self):
+        self.rel_tol, self.abs_tol, self.max_iter = 0.0, 0.0, 10
+        self.op = None
+        self.prec = None
+        self.stats = PcgStats()
+        self.check_every = 8
+
+    def SetRelTol(self, t):
+        self.rel_tol = float(t)
+
+    def SetAbsTol(self, t):
+        self.abs_tol = float(t)
+
+    def SetMaxIter(self, n):
+        self.max_iter = int(n)
+
+    def SetPrintLevel(self, _lvl):
+        pass
+
+    def SetOperator(self, op):
+        if not isinstance(op, ConstrainedOperator):
+            raise H1paError(H1PA_EUNSUP, "the operator must come from FormLinearSystem")
+        self.op = op
+
+    def SetPreconditioner(self, prec):
+        if not isinstance(prec, OperatorJacobiSmoother):
+            raise H1paError(H1PA_EUNSUP, "only OperatorJacobiSmoother is provided")
+        self.prec = prec
+
+    def Mult(self, b, x):
+        if self.op is None or self.prec is None:
+            raise H1paError(H1PA_EINVAL, "SetOperator and SetPreconditioner first")
+        f = self.op.form._single()
+        o = PcgOpts(self.rel_tol, self.abs_tol, self.max_iter, self.check_every)
+        st = self.stats
+        rc = lib().h1pa_pcg_solve(f._h, _ptr(self.op.ess), self.op.ess.numel(), _ptr(b), _ptr(x),
+                                  C.byref(o), C.byref(st), _stream())
+        if rc not in (H1PA_OK, H1PA_ENOTCONV):
+            _check(rc)
+
+    def GetNumIterations(self):
+        return self.stats.iterations
+
+    def GetFinalNorm(self):
+        return self.stats.final_norm
+
+    def GetConverged(self):
+        return bool(self.stats.converged)
+
+
+class LinearForm:
+    """LinearForm with one DomainLFIntegrator(ConstantCoefficient(f))."""
+
+    def __init__(self, fes):
+        self.fes, self.f = fes, None
+
+    def AddDomainIntegrator(self, integ):
+        if not isinstance(integ, DomainLFIntegrator):
+            raise H1paError(H1PA_EUNSUP, "only DomainLFIntegrator is provided")
+        if not isinstance(integ.coeff, ConstantCoefficient):
+            raise H1paError(H1PA_EUNSUP, "only a constant source is provided")
+        self.f = integ.coeff.value
+
+    def Assemble(self):
+        torch = _torch()
+        b = torch.empty(self.fes.ndofs, dtype=torch.float64, device="cuda")
+        _check(lib().h1pa_space_domain_lf(self.fes._h, float(self.f), _ptr(b), _stream()))
+        return b
+
+
+class DomainLFIntegrator:
+    def __init__(self, coeff):
+        self.coeff = coeff

@@ -1,0 +1,613 @@
+// capi.cpp -- extern "C" entry points of libh1pa (declared in include/h1pa.h).
+//
+// Every entry point catches h1pa::Error (and std::exception) and turns it into
+// a status code plus a thread-local message; nothing throws across the ABI.
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "internal.hpp"
+
+namespace h1pa {
+namespace {
+thread_local std::string g_last_error;
+}
+void set_error(const std::string &msg) { g_last_error = msg; }
+}  // namespace h1pa
+
+using namespace h1pa;
+
+#define API_BEGIN try {
+#define API_END                                                     \
+    }                                                               \
+    catch (const h1pa::Error &e) {                                  \
+        set_error(e.msg);                                           \
+        return e.code;                                              \
+    }                                                               \
+    catch (const std::bad_alloc &) {                                \
+        set_error("host allocation failed");                        \
+        return H1PA_ENOMEM;                                         \
+    }                                                               \
+    catch (const std::exception &e) {                               \
+        set_error(e.what());                                        \
+        return H1PA_EINVAL;                                         \
+    }                                                               \
+    set_error("");                                                  \
+    return H1PA_OK;
+
+#define REQUIRE(cond, msg) H1PA_CHECK(cond, H1PA_EINVAL, msg)
+
+extern "C" {
+
+const char *h1pa_last_error(void) { return g_last_error.c_str(); }
+const char *h1pa_version(void) { return "h1pa 0.1 (gfx950)"; }
+
+int h1pa_set_device(int device) {
+    API_BEGIN
+    H1PA_HIP(hipSetDevice(device));
+    API_END
+}
+
+int h1pa_stream_synchronize(void *stream) {
+    API_BEGIN
+    H1PA_HIP(hipStreamSynchronize(as_stream(stream)));
+    API_END
+}
+
+// ---- mesh -----------------------------------------------------------------
+int h1pa_mesh_create_cartesian(int nx, int ny, int nz, int sfc, h1pa_mesh **mesh) {
+    API_BEGIN
+    REQUIRE(mesh, "mesh: null output");
+    REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "mesh: nx, ny, nz must be >= 1");
+    REQUIRE((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (1LL << 31), "mesh: too many vertices");
+    auto *m = new h1pa_mesh();
+    m->m.nx = nx;
+    m->m.ny = ny;
+    m->m.nz = nz;
+    try {
+        m->m.build_topology(sfc != 0);
+    } catch (...) {
+        delete m;
+        throw;
+    }
+    *mesh = m;
+    API_END
+}
+
+int h1pa_mesh_destroy(h1pa_mesh *mesh) {
+    API_BEGIN
+    delete mesh;
+    API_END
+}
+
+int h1pa_mesh_perturb(h1pa_mesh *mesh, double amp, uint64_t seed) {
+    API_BEGIN
+    REQUIRE(mesh, "mesh: null");
+    REQUIRE(amp >= 0.0 && amp < 0.5, "perturbation amplitude must be in [0, 0.5)");
+    kern::perturb_vertices(mesh->m, amp, seed);
+    API_END
+}
+
+int h1pa_mesh_set_curvature(h1pa_mesh *mesh, int order, double amp) {
+    API_BEGIN
+    REQUIRE(mesh, "mesh: null");
+    REQUIRE(order >= 1 && order <= 8, "curvature order must be in [1, 8]");
+    mesh->m.node_order = order;
+    mesh->m.curve_amp = amp;
+    mesh->m.nodes_valid = false;
+    API_END
+}
+
+int h1pa_mesh_counts(const h1pa_mesh *mesh, int64_t *c) {
+    API_BEGIN
+    REQUIRE(mesh && c, "mesh_counts: null argument");
+    c[0] = mesh->m.nv;
+    c[1] = mesh->m.ne;
+    c[2] = mesh->m.nedges;
+    c[3] = mesh->m.nfaces;
+    API_END
+}
+
+int h1pa_mesh_topology(const h1pa_mesh *mesh, int *ev, int *ee, int *eeo, int *ef, int *efo) {
+    API_BEGIN
+    REQUIRE(mesh, "mesh: null");
+    const Mesh &m = mesh->m;
+    if (ev) std::memcpy(ev, m.ev.data(), sizeof(int) * m.ev.size());
+    if (ee) std::memcpy(ee, m.ee.data(), sizeof(int) * m.ee.size());
+    if (eeo) std::memcpy(eeo, m.eeo.data(), sizeof(int) * m.eeo.size());
+    if (ef) std::memcpy(ef, m.ef.data(), sizeof(int) * m.ef.size());
+    if (efo) std::memcpy(efo, m.efo.data(), sizeof(int) * m.efo.size());
+    API_END
+}
+
+int h1pa_mesh_vertices(const h1pa_mesh *mesh, double *xyz) {
+    API_BEGIN
+    REQUIRE(mesh && xyz, "mesh_vertices: null argument");
+    std::memcpy(xyz, mesh->m.xyz.data(), sizeof(double) * mesh->m.xyz.size());
+    API_END
+}
+
+int h1pa_mesh_geometric_factors(const h1pa_mesh *mesh, int q1d, unsigned flags, double *X,
+                                double *J, double *detJ, void *stream) {
+    API_BEGIN
+    REQUIRE(mesh, "mesh: null");
+    hipStream_t s = as_stream(stream);
+    Mesh &m = const_cast<h1pa_mesh *>(mesh)->m;
+    m.ensure_nodes(s);
+    kern::geometric_factors(m, q1d, (flags & H1PA_QI_VALUES) ? X : nullptr,
+                            (flags & H1PA_QI_DERIVATIVES) ? J : nullptr,
+                            (flags & H1PA_QI_DETERMINANTS) ? detJ : nullptr, s);
+    API_END
+}
+
+// ---- space ----------------------------------------------------------------
+int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space) {
+    API_BEGIN
+    REQUIRE(mesh && space, "space_create: null argument");
+    REQUIRE(order >= 1 && order <= 8, "order must be in [1, 8]");
+    Mesh &m = mesh->m;
+    auto *sp = new h1pa_space();
+    try {
+        Space &s = sp->s;
+        s.mesh = &m;
+        s.p = order;
+        s.ne = m.ne;
+        const int64_t D = order + 1, pm1 = order - 1;
+        s.dof = D * D * D;
+        s.nvdofs = m.nv;
+        s.nedofs = m.nedges * pm1;
+        s.nfdofs = m.nfaces * pm1 * pm1;
+        s.ndofs = s.nvdofs + s.nedofs + s.nfdofs + m.ne * pm1 * pm1 * pm1;
+        REQUIRE(s.ndofs < (1LL << 31) && s.ne * s.dof < (1LL << 31), "space too large for int32 maps");
+        hipStream_t st = nullptr;
+        m.ensure_device(st);
+        s.elem_dof.alloc(s.ne * s.dof);
+        Restriction &r = s.restr;
+        r.ne = s.ne;
+        r.dof = s.dof;
+        r.ndofs = s.ndofs;
+        r.gather_map.alloc(s.ne * s.dof);
+        r.offsets.alloc(s.ndofs + 1);
+        r.indices.alloc(s.ne * s.dof);
+        kern::build_gather_map(m, order, s.nvdofs, s.nedofs, s.nfdofs, s.elem_dof.p,
+                               r.gather_map.p, st);
+        kern::build_transpose(s.ne, (int)s.dof, s.ndofs, r.gather_map.p, r.offsets.p, r.indices.p, st);
+        s.ess_mask.alloc(s.ndofs);
+        kern::boundary_mask(m, order, r.gather_map.p, s.ess_mask.p, s.ndofs, st);
+        s.n_ess = kern::mask_to_list(s.ess_mask.p, s.ndofs, s.ess, st);
+        auto *rh = new h1pa_restriction();
+        rh->owned_by_space = true;
+        sp->r = rh;
+    } catch (...) {
+        delete sp;
+        throw;
+    }
+    *space = sp;
+    API_END
+}
+
+int h1pa_space_destroy(h1pa_space *space) {
+    API_BEGIN
+    if (space) delete space->r;
+    delete space;
+    API_END
+}
+
+int h1pa_space_info(const h1pa_space *space, int64_t *info) {
+    API_BEGIN
+    REQUIRE(space && info, "space_info: null argument");
+    info[0] = space->s.p;
+    info[1] = space->s.ne;
+    info[2] = space->s.dof;
+    info[3] = space->s.ndofs;
+    API_END
+}
+
+int h1pa_space_element_dofs(const h1pa_space *space, const int **d) {
+    API_BEGIN
+    REQUIRE(space && d, "space_element_dofs: null argument");
+    *d = space->s.elem_dof.p;
+    API_END
+}
+
+int h1pa_space_boundary_dofs(const h1pa_space *space, const int **d_ess, int64_t *n) {
+    API_BEGIN
+    REQUIRE(space && d_ess && n, "space_boundary_dofs: null argument");
+    *d_ess = space->s.ess.p;
+    *n = space->s.n_ess;
+    API_END
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int h1pa_space_restriction(h1pa_space *space, h1pa_restriction **r) {
+    API_BEGIN
+    REQUIRE(space && r, "space_restriction: null argument");
+    // The borrowed handle is a view of the space's device maps.
+    space->r->view = &space->s.restr;
+    *r = space->r;
+    API_END
+}
+
+// ---- restriction --------------------------------------------------------------
+int h1pa_restriction_create(int ne, int dof, int ndofs, const int *h_elem_dof, const int *h_dof_map,
+                            h1pa_restriction **restriction) {
+    API_BEGIN
+    REQUIRE(restriction && h_elem_dof, "restriction_create: null argument");
+    REQUIRE(ne >= 0 && dof >= 1 && ndofs >= 0, "restriction_create: bad sizes");
+    for (int64_t i = 0; i < (int64_t)ne * dof; ++i) {
+        const int v = h_elem_dof[i], g = v >= 0 ? v : -1 - v;
+        REQUIRE(g < ndofs, "restriction_create: element DOF out of range");
+    }
+    if (h_dof_map) {
+        std::vector<char> seen(dof, 0);
+        for (int d = 0; d < dof; ++d) {
+            const int v = h_dof_map[d], g = v >= 0 ? v : -1 - v;
+            REQUIRE(g < dof && !seen[g], "restriction_create: dof_map is not a permutation");
+            seen[g] = 1;
+        }
+    }
+    auto *r = new h1pa_restriction();
+    try {
+        Restriction &R = r->own;
+        R.ne = ne;
+        R.dof = dof;
+        R.ndofs = ndofs;
+        hipStream_t s = nullptr;
+        DevBuf<int> ed, dm;
+        ed.upload(h_elem_dof, (size_t)ne * dof, s);
+        if (h_dof_map) dm.upload(h_dof_map, dof, s);
+        R.gather_map.alloc((size_t)ne * dof);
+        R.offsets.alloc(ndofs + 1);
+        R.indices.alloc((size_t)ne * dof);
+        kern::gather_from_table(ne, dof, ed.p, h_dof_map ? dm.p : nullptr, R.gather_map.p, s);
+        kern::build_transpose(ne, dof, ndofs, R.gather_map.p, R.offsets.p, R.indices.p, s);
+    } catch (...) {
+        delete r;
+        throw;
+    }
+    *restriction = r;
+    API_END
+}
+
+int h1pa_restriction_destroy(h1pa_restriction *r) {
+    API_BEGIN
+    if (r && r->owned_by_space) {
+        throw Error{H1PA_EINVAL, "restriction is owned by its space"};
+    }
+    delete r;
+    API_END
+}
+
+int h1pa_restriction_maps(const h1pa_restriction *r, const int **g, const int **o, const int **i) {
+    API_BEGIN
+    REQUIRE(r, "restriction: null");
+    if (g) *g = r->R().gather_map.p;
+    if (o) *o = r->R().offsets.p;
+    if (i) *i = r->R().indices.p;
+    API_END
+}
+
+int h1pa_restriction_mult(const h1pa_restriction *r, const double *x, double *y, void *stream) {
+    API_BEGIN
+    REQUIRE(r && x && y, "restriction_mult: null argument");
+    kern::restrict_mult(r->R().ne * r->R().dof, r->R().gather_map.p, x, y, false, as_stream(stream));
+    API_END
+}
+
+int h1pa_restriction_abs_mult(const h1pa_restriction *r, const double *x, double *y, void *stream) {
+    API_BEGIN
+    REQUIRE(r && x && y, "restriction_abs_mult: null argument");
+    kern::restrict_mult(r->R().ne * r->R().dof, r->R().gather_map.p, x, y, true, as_stream(stream));
+    API_END
+}
+
+int h1pa_restriction_mult_transpose(const h1pa_restriction *r, const double *x, double *y,
+                                    void *stream) {
+    API_BEGIN
+    REQUIRE(r && x && y, "restriction_mult_transpose: null argument");
+    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 0,
+                                  as_stream(stream));
+    API_END
+}
+
+int h1pa_restriction_add_mult_transpose(const h1pa_restriction *r, const double *x, double *y,
+                                        double a, void *stream) {
+    API_BEGIN
+    REQUIRE(r && x && y, "restriction_add_mult_transpose: null argument");
+    REQUIRE(a == 1.0, "General coefficient case is not yet supported!");
+    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 1,
+                                  as_stream(stream));
+    API_END
+}
+
+int h1pa_restriction_abs_mult_transpose(const h1pa_restriction *r, const double *x, double *y,
+                                        void *stream) {
+    API_BEGIN
+    REQUIRE(r && x && y, "restriction_abs_mult_transpose: null argument");
+    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 2,
+                                  as_stream(stream));
+    API_END
+}
+
+int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *x, double *y,
+                                       void *stream) {
+    API_BEGIN
+    REQUIRE(r && x && y, "restriction_mult_left_inverse: null argument");
+    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 3,
+                                  as_stream(stream));
+    API_END
+}
+
+// ---- form -------------------------------------------------------------------
+int h1pa_form_create(h1pa_space *space, const h1pa_form_desc *desc, h1pa_form **form) {
+    API_BEGIN
+    REQUIRE(space && desc && form, "form_create: null argument");
+    REQUIRE(desc->integrators != 0 &&
+                (desc->integrators & ~(unsigned)(H1PA_INTEG_MASS | H1PA_INTEG_DIFFUSION)) == 0,
+            "form_create: integrators must be a non-empty subset of MASS|DIFFUSION");
+    REQUIRE(desc->coeff_kind >= H1PA_COEFF_CONSTANT && desc->coeff_kind <= H1PA_COEFF_FOURIER,
+            "form_create: unknown coefficient kind");
+    REQUIRE(!(desc->coeff_kind == H1PA_COEFF_PER_ELEMENT || desc->coeff_kind == H1PA_COEFF_QUADRATURE) ||
+                desc->d_coeff, "form_create: coefficient data required");
+    const int p = space->s.p;
+    const int q1d = desc->q1d > 0 ? desc->q1d : p + 2;
+    H1PA_CHECK(kern::apply_supported(p, q1d), H1PA_EUNSUP, "form_create: q1d must be p+1 or p+2");
+    auto *f = new h1pa_form();
+    f->f.space = &space->s;
+    f->f.desc = *desc;
+    f->f.q1d = q1d;
+    *form = f;
+    API_END
+}
+
+int h1pa_form_destroy(h1pa_form *form) {
+    API_BEGIN
+    delete form;
+    API_END
+}
+
+int h1pa_form_assemble(h1pa_form *form, void *stream) {
+    API_BEGIN
+    REQUIRE(form, "form: null");
+    Form &f = form->f;
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    sp.mesh->ensure_nodes(s);
+    const int Q = f.q1d, D = sp.p + 1;
+    std::vector<double> xq(Q), wq(Q), B(Q * D), G(Q * D);
+    gauss_legendre(Q, xq.data(), wq.data());
+    lagrange_basis(sp.p, Q, xq.data(), B.data(), G.data());
+    f.B.upload(B.data(), B.size(), s);
+    f.G.upload(G.data(), G.size(), s);
+    const int64_t nq = (int64_t)Q * Q * Q;
+    if (f.desc.integrators & H1PA_INTEG_DIFFUSION) f.pa_diff.alloc(nq * 6 * sp.ne);
+    else f.pa_diff.release();
+    if (f.desc.integrators & H1PA_INTEG_MASS) f.pa_mass.alloc(nq * sp.ne);
+    else f.pa_mass.release();
+    kern::pa_setup(*sp.mesh, f, s);
+    f.ework0.alloc(sp.ne * sp.dof);
+    H1PA_HIP(hipStreamSynchronize(s));
+    f.assembled = true;
+    API_END
+}
+
+}  // extern "C"
+
+namespace {
+void require_assembled(const Form &f) {
+    H1PA_CHECK(f.assembled, H1PA_EINVAL, "form: Assemble() has not been called");
+}
+
+// E-part of the operator: ework0 = sum of integrators applied to x.  `gmap`
+// null means x is already an E-vector.
+void element_apply(Form &f, const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                   bool accumulate, hipStream_t s) {
+    Space &sp = *f.space;
+    const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
+    const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
+    if (diff)
+        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.B.p, f.G.p, f.pa_diff.p, gmap, mask, x, ye,
+                              accumulate, s);
+    if (mass)
+        kern::mass_apply(sp.p, f.q1d, sp.ne, f.B.p, f.pa_mass.p, gmap, mask, x, ye,
+                         accumulate || diff, s);
+}
+
+const uint8_t *constraint_mask(Form &f, const int *d_ess, int64_t n_ess, hipStream_t s) {
+    Space &sp = *f.space;
+    if (d_ess == sp.ess.p && n_ess == sp.n_ess) return sp.ess_mask.p;
+    if (!f.cmask.p) f.cmask.alloc(sp.ndofs);
+    if (f.cmask_src != d_ess || f.cmask_n != n_ess) {
+        kern::list_to_mask(d_ess, n_ess, f.cmask.p, sp.ndofs, s);
+        f.cmask_src = d_ess;
+        f.cmask_n = n_ess;
+    }
+    return f.cmask.p;
+}
+}  // namespace
+
+extern "C" {
+
+int h1pa_form_mult(h1pa_form *form, const double *x, double *y, void *stream) {
+    API_BEGIN
+    REQUIRE(form && x && y, "form_mult: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, false, s);
+    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p, y,
+                                  0, s);
+    API_END
+}
+
+int h1pa_form_mult_timed(h1pa_form *form, const double *x, double *y, int steps, double *ms_total,
+                         double *ms_element, double *ms_scatter, void *stream) {
+    API_BEGIN
+    REQUIRE(form && x && y && steps >= 1, "form_mult_timed: bad argument");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    std::vector<hipEvent_t> ev(3 * (size_t)steps);
+    for (auto &e : ev) H1PA_HIP(hipEventCreate(&e));
+    try {
+        for (int k = 0; k < steps; ++k) {
+            H1PA_HIP(hipEventRecord(ev[3 * k], s));
+            element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, false, s);
+            H1PA_HIP(hipEventRecord(ev[3 * k + 1], s));
+            kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p,
+                                          f.ework0.p, y, 0, s);
+            H1PA_HIP(hipEventRecord(ev[3 * k + 2], s));
+        }
+        H1PA_HIP(hipEventSynchronize(ev.back()));
+        double te = 0.0, ts = 0.0;
+        for (int k = 0; k < steps; ++k) {
+            float a = 0.f, b = 0.f;
+            H1PA_HIP(hipEventElapsedTime(&a, ev[3 * k], ev[3 * k + 1]));
+            H1PA_HIP(hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]));
+            te += a;
+            ts += b;
+        }
+        float tot = 0.f;
+        H1PA_HIP(hipEventElapsedTime(&tot, ev[0], ev.back()));
+        if (ms_total) *ms_total = tot;
+        if (ms_element) *ms_element = te / steps;
+        if (ms_scatter) *ms_scatter = ts / steps;
+    } catch (...) {
+        for (auto &e : ev) (void)hipEventDestroy(e);
+        throw;
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    API_END
+}
+
+int h1pa_form_add_mult_e(h1pa_form *form, const double *xE, double *yE, void *stream) {
+    API_BEGIN
+    REQUIRE(form && xE && yE, "form_add_mult_e: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    element_apply(f, nullptr, nullptr, xE, yE, true, as_stream(stream));
+    API_END
+}
+
+int h1pa_form_assemble_diagonal(h1pa_form *form, double *diag, void *stream) {
+    API_BEGIN
+    REQUIRE(form && diag, "form_assemble_diagonal: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
+    const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
+    if (diff)
+        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.G.p, f.pa_diff.p, f.ework0.p, false, s);
+    if (mass)
+        kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.pa_mass.p, f.ework0.p, diff, s);
+    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
+                                  diag, 2, s);
+    API_END
+}
+
+int h1pa_form_pa_data(const h1pa_form *form, const double **dd, const double **dm) {
+    API_BEGIN
+    REQUIRE(form, "form: null");
+    if (dd) *dd = form->f.pa_diff.p;
+    if (dm) *dm = form->f.pa_mass.p;
+    API_END
+}
+
+int h1pa_form_info(const h1pa_form *form, int64_t *info) {
+    API_BEGIN
+    REQUIRE(form && info, "form_info: null argument");
+    info[0] = form->f.q1d;
+    info[1] = form->f.desc.integrators;
+    info[2] = form->f.space->ndofs;
+    API_END
+}
+
+int h1pa_form_constrained_mult(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *x,
+                               double *y, void *stream) {
+    API_BEGIN
+    REQUIRE(form && x && y && (n_ess == 0 || d_ess), "form_constrained_mult: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
+    element_apply(f, sp.restr.gather_map.p, mask, x, f.ework0.p, false, s);
+    kern::scatter_constrained(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p, mask, x,
+                              y, s);
+    API_END
+}
+
+int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *x,
+                            const double *b, double *X, double *B, void *stream) {
+    API_BEGIN
+    REQUIRE(form && x && b && X && B && (n_ess == 0 || d_ess), "form_linear_system: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    DevBuf<uint8_t> zero_mask;
+    const uint8_t *mask;
+    if (n_ess > 0) {
+        mask = constraint_mask(f, d_ess, n_ess, s);
+    } else {
+        zero_mask.alloc(sp.ndofs);
+        H1PA_HIP(hipMemsetAsync(zero_mask.p, 0, sp.ndofs, s));
+        mask = zero_mask.p;
+    }
+    DevBuf<double> z(sp.ndofs);
+    kern::form_x(sp.ndofs, mask, x, X, s);  // X = x on ess, 0 elsewhere (= w)
+    element_apply(f, sp.restr.gather_map.p, nullptr, X, f.ework0.p, false, s);
+    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
+                                  z.p, 0, s);
+    kern::form_b(sp.ndofs, mask, X, b, z.p, B, s);
+    H1PA_HIP(hipStreamSynchronize(s));
+    API_END
+}
+
+int h1pa_space_domain_lf(h1pa_space *space, double fval, double *b, void *stream) {
+    API_BEGIN
+    REQUIRE(space && b, "space_domain_lf: null argument");
+    Space &sp = space->s;
+    hipStream_t s = as_stream(stream);
+    sp.mesh->ensure_nodes(s);
+    DevBuf<double> bE(sp.ne * sp.dof);
+    kern::domain_lf(*sp.mesh, sp.p, fval, bE.p, s);
+    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, bE.p, b, 0, s);
+    H1PA_HIP(hipStreamSynchronize(s));
+    API_END
+}
+
+int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *b, double *x,
+                   const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats, void *stream) {
+    API_BEGIN
+    REQUIRE(form && b && x && opts && stats && (n_ess == 0 || d_ess), "pcg_solve: null argument");
+    REQUIRE(opts->max_iter >= 1, "pcg_solve: max_iter must be >= 1");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
+    DevBuf<double> diag(sp.ndofs);
+    const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
+    const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
+    if (diff)
+        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.G.p, f.pa_diff.p, f.ework0.p, false, s);
+    if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.pa_mass.p, f.ework0.p, diff, s);
+    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
+                                  diag.p, 2, s);
+    pcg_solve(f, mask, diag.p, b, x, *opts, *stats, s);
+    if (!stats->converged) {
+        set_error("PCG: No convergence!");
+        return H1PA_ENOTCONV;
+    }
+    API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,194 @@
+// internal.hpp -- shared declarations of the h1pa library (host C++ + HIP).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/h1pa.h"
+
+namespace h1pa {
+
+// ---- errors ---------------------------------------------------------------
+void set_error(const std::string &msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+#define H1PA_HIP(call)                                                                   \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::h1pa::Error{H1PA_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+#define H1PA_CHECK(cond, code, msg)                                                      \
+    do {                                                                                 \
+        if (!(cond)) throw ::h1pa::Error{(code), (msg)};                                 \
+    } while (0)
+
+inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// ---- device buffer ---------------------------------------------------------
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        if (count == 0) return;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw Error{H1PA_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t s) {
+        if (count > n) alloc(count);
+        if (count) H1PA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+// ---- 1D rules and basis (host, long double) --------------------------------
+void gauss_legendre(int n, double *x, double *w);
+void gauss_lobatto(int n, double *x, double *w);
+// B[q + n*d] = l_d(x_q), G likewise; Lagrange basis on p+1 GLL nodes.
+void lagrange_basis(int p, int n, const double *xq, double *B, double *G);
+void h1_dof_map(int p, int *dof_map);  // lex -> native (fem/fe/fe_base.cpp:2557-2674)
+
+// ---- mesh (host topology, device coordinates) -------------------------------
+struct Mesh {
+    int nx = 0, ny = 0, nz = 0;
+    int64_t nv = 0, ne = 0, nedges = 0, nfaces = 0;
+    std::vector<int> ev, ee, eeo, ef, efo;   // host topology
+    std::vector<int> ecoord;                 // [ne][3] grid position of each element
+    std::vector<double> xyz;                 // host vertex coordinates [nv][3]
+    // geometry: order-pg nodes (pg = 1: vertices), device E-vector [Dg^3][3][ne]
+    int node_order = 1;
+    double curve_amp = 0.0;
+    DevBuf<double> d_nodes;
+    bool nodes_valid = false;
+    DevBuf<int> d_topo;   // [ne][29] vertex, edge, face ids + grid coordinates
+    DevBuf<int8_t> d_ori; // [ne][12+6] edge (+-1) and face (0..7) orientations
+    void build_topology(bool sfc);
+    void ensure_device(hipStream_t s);
+    void ensure_nodes(hipStream_t s);
+};
+
+// ---- restriction -----------------------------------------------------------
+struct Restriction {
+    int64_t ne = 0, dof = 0, ndofs = 0;
+    DevBuf<int> gather_map, offsets, indices;
+    // transpose table with the trailing (last) entry per row, for the left inverse
+    void build_transpose(hipStream_t s);  // from gather_map
+};
+
+struct Space {
+    Mesh *mesh = nullptr;
+    int p = 1;
+    int64_t ne = 0, dof = 0, ndofs = 0;
+    int64_t nvdofs = 0, nedofs = 0, nfdofs = 0;
+    DevBuf<int> elem_dof;   // native order [ne][dof]
+    DevBuf<int> ess;        // boundary true DOFs, ascending
+    int64_t n_ess = 0;
+    Restriction restr;
+    DevBuf<uint8_t> ess_mask;  // [ndofs] 1 on boundary DOFs
+};
+
+struct Form {
+    Space *space = nullptr;
+    h1pa_form_desc desc{};
+    int q1d = 0;
+    DevBuf<double> B, G;       // [Q][D] MFEM layout B[q + Q*d]
+    DevBuf<double> Bt;         // packed B|G for kernels (see apply kernels)
+    DevBuf<double> pa_diff;    // [Q^3][6][ne]
+    DevBuf<double> pa_mass;    // [Q^3][ne]
+    DevBuf<double> ework0, ework1;  // E-vector scratch
+    DevBuf<double> lwork0, lwork1;  // L-vector scratch
+    DevBuf<uint8_t> cmask;          // constraint mask for the current ess list
+    const int *cmask_src = nullptr;
+    int64_t cmask_n = -1;
+    bool assembled = false;
+};
+
+// ---- kernel launchers (HIP translation units) ------------------------------
+namespace kern {
+void build_gather_map(const Mesh &m, int p, int64_t nvdofs, int64_t nedofs, int64_t nfdofs,
+                      int *elem_dof_native, int *gather_map, hipStream_t s);
+void gather_from_table(int64_t ne, int dof, const int *elem_dof, const int *dof_map,
+                       int *gather_map, hipStream_t s);
+void build_transpose(int64_t ne, int dof, int64_t ndofs, const int *gather_map, int *offsets,
+                     int *indices, hipStream_t s);
+void boundary_mask(const Mesh &m, int p, const int *gather_map, uint8_t *mask, int64_t ndofs,
+                   hipStream_t s);
+int64_t mask_to_list(const uint8_t *mask, int64_t n, DevBuf<int> &list, hipStream_t s);
+void list_to_mask(const int *list, int64_t nlist, uint8_t *mask, int64_t n, hipStream_t s);
+
+void restrict_mult(int64_t nE, const int *gmap, const double *x, double *y, bool abs_,
+                   hipStream_t s);
+// mode 0 set, 1 add, 2 abs set, 3 left inverse
+void restrict_mult_transpose(int64_t ndofs, const int *offsets, const int *indices,
+                             const double *xe, double *y, int mode, hipStream_t s);
+// E->L set with constraint handling: y[i] = mask[i] ? xin[i] : sum (DIAG_ONE)
+void scatter_constrained(int64_t ndofs, const int *offsets, const int *indices,
+                         const double *xe, const uint8_t *mask, const double *xin, double *y,
+                         hipStream_t s);
+
+void perturb_vertices(Mesh &m, double amp, uint64_t seed);  // host
+void nodes_trilinear(const Mesh &m, double *d_nodes, hipStream_t s);
+void nodes_curved(const Mesh &m, int pg, double amp, double *d_nodes, hipStream_t s);
+void geometric_factors(const Mesh &m, int q1d, double *X, double *J, double *detJ,
+                       hipStream_t s);
+void pa_setup(const Mesh &m, const Form &f, hipStream_t s);
+void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s);
+
+// E->E: ye = A_E xe (overwrite) or += ; x may be gathered from an L-vector
+// through gmap (gmap != nullptr) with constrained entries masked to zero.
+void diffusion_apply(int p, int q1d, int64_t ne, const double *B, const double *G,
+                     const double *pa, const int *gmap, const uint8_t *mask, const double *x,
+                     double *ye, bool accumulate, hipStream_t s);
+void mass_apply(int p, int q1d, int64_t ne, const double *B, const double *pa,
+                const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                bool accumulate, hipStream_t s);
+void diffusion_diagonal(int p, int q1d, int64_t ne, const double *B, const double *G,
+                        const double *pa, double *ye, bool accumulate, hipStream_t s);
+void mass_diagonal(int p, int q1d, int64_t ne, const double *B, const double *pa, double *ye,
+                   bool accumulate, hipStream_t s);
+bool apply_supported(int p, int q1d);
+void jacobi_dinv(int64_t n, const double *diag, const uint8_t *mask, double *dinv, hipStream_t s);
+void form_x(int64_t n, const uint8_t *mask, const double *x, double *X, hipStream_t s);
+void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, const double *z,
+            double *B, hipStream_t s);
+}  // namespace kern
+
+void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
+               const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s);
+
+}  // namespace h1pa
+
+struct h1pa_mesh { h1pa::Mesh m; };
+struct h1pa_space { h1pa::Space s; h1pa_restriction *r = nullptr; };
+struct h1pa_restriction {
+    h1pa::Restriction own;                  // maps built by h1pa_restriction_create
+    const h1pa::Restriction *view = nullptr; // or the space's maps (borrowed)
+    bool owned_by_space = false;
+    const h1pa::Restriction &R() const { return view ? *view : own; }
+};
+struct h1pa_form { h1pa::Form f; };

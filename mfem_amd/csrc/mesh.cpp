@@ -1,0 +1,276 @@
+// mesh.cpp -- Cartesian hexahedral mesh with MFEM's element, edge and face
+// numbering, built in O(NE) with direct-indexed structured arrays.
+//
+// MFEM numbers edges and faces by first encounter while it walks the elements
+// in order and each element's local edges/faces in Geometry::Constants<CUBE>
+// order (DSTable/STable3D pushes, mesh/mesh.cpp:8526-8549, :8996-9080).  On a
+// Cartesian grid every edge and face has a structured address, so the same
+// numbering falls out of one pass with an "already numbered?" array instead of
+// hash tables.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+
+#include "internal.hpp"
+
+namespace h1pa {
+
+namespace {
+
+// Generalized Hilbert curve of an a x b x c box (mesh/ncmesh.cpp:5435-5597):
+// the sub-box splits and their order fix MFEM's element numbering.
+struct Gilbert {
+    std::vector<int> &out;
+    static int sg(int v) { return (v > 0) - (v < 0); }
+    void emit(int x, int y, int z) {
+        out.push_back(x);
+        out.push_back(y);
+        out.push_back(z);
+    }
+    void run(int x, int y, int z, int ax, int ay, int az, int bx, int by, int bz, int cx, int cy,
+             int cz) {
+        const int w = std::abs(ax + ay + az), h = std::abs(bx + by + bz), d = std::abs(cx + cy + cz);
+        const int dax = sg(ax), day = sg(ay), daz = sg(az);
+        const int dbx = sg(bx), dby = sg(by), dbz = sg(bz);
+        const int dcx = sg(cx), dcy = sg(cy), dcz = sg(cz);
+        if (h == 1 && d == 1) {
+            for (int i = 0; i < w; ++i) emit(x + i * dax, y + i * day, z + i * daz);
+            return;
+        }
+        if (w == 1 && d == 1) {
+            for (int i = 0; i < h; ++i) emit(x + i * dbx, y + i * dby, z + i * dbz);
+            return;
+        }
+        if (w == 1 && h == 1) {
+            for (int i = 0; i < d; ++i) emit(x + i * dcx, y + i * dcy, z + i * dcz);
+            return;
+        }
+        int ax2 = ax / 2, ay2 = ay / 2, az2 = az / 2;
+        int bx2 = bx / 2, by2 = by / 2, bz2 = bz / 2;
+        int cx2 = cx / 2, cy2 = cy / 2, cz2 = cz / 2;
+        // keep the halves even where the side allows it
+        if ((std::abs(ax2 + ay2 + az2) & 1) && w > 2) { ax2 += dax; ay2 += day; az2 += daz; }
+        if ((std::abs(bx2 + by2 + bz2) & 1) && h > 2) { bx2 += dbx; by2 += dby; bz2 += dbz; }
+        if ((std::abs(cx2 + cy2 + cz2) & 1) && d > 2) { cx2 += dcx; cy2 += dcy; cz2 += dcz; }
+        if (2 * w > 3 * h && 2 * w > 3 * d) {  // long along a: two halves
+            run(x, y, z, ax2, ay2, az2, bx, by, bz, cx, cy, cz);
+            run(x + ax2, y + ay2, z + az2, ax - ax2, ay - ay2, az - az2, bx, by, bz, cx, cy, cz);
+        } else if (3 * h > 4 * d) {  // keep c whole
+            run(x, y, z, bx2, by2, bz2, cx, cy, cz, ax2, ay2, az2);
+            run(x + bx2, y + by2, z + bz2, ax, ay, az, bx - bx2, by - by2, bz - bz2, cx, cy, cz);
+            run(x + (ax - dax) + (bx2 - dbx), y + (ay - day) + (by2 - dby),
+                z + (az - daz) + (bz2 - dbz), -bx2, -by2, -bz2, cx, cy, cz, -(ax - ax2),
+                -(ay - ay2), -(az - az2));
+        } else if (3 * d > 4 * h) {  // keep b whole
+            run(x, y, z, cx2, cy2, cz2, ax2, ay2, az2, bx, by, bz);
+            run(x + cx2, y + cy2, z + cz2, ax, ay, az, bx, by, bz, cx - cx2, cy - cy2, cz - cz2);
+            run(x + (ax - dax) + (cx2 - dcx), y + (ay - day) + (cy2 - dcy),
+                z + (az - daz) + (cz2 - dcz), -cx2, -cy2, -cz2, -(ax - ax2), -(ay - ay2),
+                -(az - az2), bx, by, bz);
+        } else {  // split all three
+            run(x, y, z, bx2, by2, bz2, cx2, cy2, cz2, ax2, ay2, az2);
+            run(x + bx2, y + by2, z + bz2, cx, cy, cz, ax2, ay2, az2, bx - bx2, by - by2,
+                bz - bz2);
+            run(x + (bx2 - dbx) + (cx - dcx), y + (by2 - dby) + (cy - dcy),
+                z + (bz2 - dbz) + (cz - dcz), ax, ay, az, -bx2, -by2, -bz2, -(cx - cx2),
+                -(cy - cy2), -(cz - cz2));
+            run(x + (ax - dax) + bx2 + (cx - dcx), y + (ay - day) + by2 + (cy - dcy),
+                z + (az - daz) + bz2 + (cz - dcz), -cx, -cy, -cz, -(ax - ax2), -(ay - ay2),
+                -(az - az2), bx - bx2, by - by2, bz - bz2);
+            run(x + (ax - dax) + (bx2 - dbx), y + (ay - day) + (by2 - dby),
+                z + (az - daz) + (bz2 - dbz), -bx2, -by2, -bz2, cx2, cy2, cz2, -(ax - ax2),
+                -(ay - ay2), -(az - az2));
+        }
+    }
+};
+
+// Hex local edges: axis and the corner (dx,dy,dz) the edge starts from
+// (Geometry::Constants<CUBE>::Edges, fem/geom.cpp:1020-1025: every edge runs
+// from its lower to its higher corner along one axis).
+const int kEdgeAxis[12] = {0, 1, 0, 1, 0, 1, 0, 1, 2, 2, 2, 2};
+const int kEdgeStart[12][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 0}, {0, 0, 1}, {1, 0, 1},
+                               {0, 1, 1}, {0, 0, 1}, {0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+const int kEdgeVerts[12][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}, {4, 5}, {5, 6},
+                               {7, 6}, {4, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+// Hex local faces: normal axis, offset along it, and the vertex cycle
+// (FaceVert, fem/geom.cpp:1030-1035).
+const int kFaceAxis[6] = {2, 1, 0, 1, 0, 2};
+const int kFaceOff[6] = {0, 0, 1, 1, 0, 1};
+const int kFaceVerts[6][4] = {{3, 2, 1, 0}, {0, 1, 5, 4}, {1, 2, 6, 5},
+                              {2, 3, 7, 6}, {3, 0, 4, 7}, {4, 5, 6, 7}};
+
+// Mesh::GetQuadOrientation (mesh/mesh.cpp:7586-7630).
+int quad_orientation(const int *base, const int *test) {
+    int i = 0;
+    while (i < 4 && test[i] != base[0]) ++i;
+    return (test[(i + 1) % 4] == base[1]) ? 2 * i : 2 * i + 1;
+}
+
+}  // namespace
+
+void Mesh::build_topology(bool sfc) {
+    ne = (int64_t)nx * ny * nz;
+    nv = (int64_t)(nx + 1) * (ny + 1) * (nz + 1);
+    ecoord.resize(3 * ne);
+    if (sfc) {
+        std::vector<int> c;
+        c.reserve(3 * ne);
+        Gilbert g{c};
+        if (nx >= ny && nx >= nz) g.run(0, 0, 0, nx, 0, 0, 0, ny, 0, 0, 0, nz);
+        else if (ny >= nx && ny >= nz) g.run(0, 0, 0, 0, ny, 0, nx, 0, 0, 0, 0, nz);
+        else g.run(0, 0, 0, 0, 0, nz, nx, 0, 0, 0, ny, 0);
+        H1PA_CHECK((int64_t)c.size() == 3 * ne, H1PA_EINVAL, "space-filling curve size mismatch");
+        std::copy(c.begin(), c.end(), ecoord.begin());
+    } else {
+        int64_t k = 0;
+        for (int z = 0; z < nz; ++z)
+            for (int y = 0; y < ny; ++y)
+                for (int x = 0; x < nx; ++x, ++k) {
+                    ecoord[3 * k] = x; ecoord[3 * k + 1] = y; ecoord[3 * k + 2] = z;
+                }
+    }
+    const int64_t sx = nx + 1, sxy = (int64_t)(nx + 1) * (ny + 1);
+    auto vtx = [&](int64_t x, int64_t y, int64_t z) { return (int)(x + sx * y + sxy * z); };
+    ev.resize(8 * ne);
+    ee.resize(12 * ne);
+    eeo.resize(12 * ne);
+    ef.resize(6 * ne);
+    efo.resize(6 * ne);
+    static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                     {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+    for (int64_t e = 0; e < ne; ++e) {
+        const int x = ecoord[3 * e], y = ecoord[3 * e + 1], z = ecoord[3 * e + 2];
+        for (int v = 0; v < 8; ++v)
+            ev[8 * e + v] = vtx(x + corner[v][0], y + corner[v][1], z + corner[v][2]);
+    }
+    // Structured edge address: axis a, start vertex (x,y,z); sized per axis.
+    const int64_t n_ax[3] = {(int64_t)nx * (ny + 1) * (nz + 1), (int64_t)(nx + 1) * ny * (nz + 1),
+                             (int64_t)(nx + 1) * (ny + 1) * nz};
+    const int64_t ext[3][3] = {{nx, ny + 1, nz + 1}, {nx + 1, ny, nz + 1}, {nx + 1, ny + 1, nz}};
+    std::vector<int> eid[3];
+    for (int a = 0; a < 3; ++a) eid[a].assign(n_ax[a], -1);
+    int next = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        const int x = ecoord[3 * e], y = ecoord[3 * e + 1], z = ecoord[3 * e + 2];
+        for (int j = 0; j < 12; ++j) {
+            const int a = kEdgeAxis[j];
+            const int64_t ix = x + kEdgeStart[j][0], iy = y + kEdgeStart[j][1], iz = z + kEdgeStart[j][2];
+            const int64_t s = ix + ext[a][0] * (iy + ext[a][1] * iz);
+            int &id = eid[a][s];
+            if (id < 0) id = next++;
+            ee[12 * e + j] = id;
+            const int va = ev[8 * e + kEdgeVerts[j][0]], vb = ev[8 * e + kEdgeVerts[j][1]];
+            eeo[12 * e + j] = va < vb ? 1 : -1;
+        }
+    }
+    nedges = next;
+    // Structured face address: normal axis a, lower corner (x,y,z).
+    const int64_t f_ax[3] = {(int64_t)(nx + 1) * ny * nz, (int64_t)nx * (ny + 1) * nz,
+                             (int64_t)nx * ny * (nz + 1)};
+    const int64_t fext[3][3] = {{nx + 1, ny, nz}, {nx, ny + 1, nz}, {nx, ny, nz + 1}};
+    std::vector<int> fid[3];
+    std::vector<int64_t> first;  // (element * 6 + local face) that numbered the face
+    for (int a = 0; a < 3; ++a) fid[a].assign(f_ax[a], -1);
+    first.reserve(3 * ne + 3 * (int64_t)nx * ny);
+    next = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+        const int x = ecoord[3 * e], y = ecoord[3 * e + 1], z = ecoord[3 * e + 2];
+        for (int j = 0; j < 6; ++j) {
+            const int a = kFaceAxis[j];
+            int64_t c[3] = {x, y, z};
+            c[a] += kFaceOff[j];
+            const int64_t s = c[0] + fext[a][0] * (c[1] + fext[a][1] * c[2]);
+            int &id = fid[a][s];
+            if (id < 0) {
+                id = next++;
+                first.push_back(6 * e + j);
+                efo[6 * e + j] = 0;
+            } else {
+                const int64_t e1 = first[id] / 6;
+                const int j1 = (int)(first[id] % 6);
+                int base[4], test[4];
+                for (int k = 0; k < 4; ++k) {
+                    base[k] = ev[8 * e1 + kFaceVerts[j1][k]];
+                    test[k] = ev[8 * e + kFaceVerts[j][k]];
+                }
+                efo[6 * e + j] = quad_orientation(base, test);
+            }
+            ef[6 * e + j] = id;
+        }
+    }
+    nfaces = next;
+    xyz.resize(3 * nv);
+    int64_t v = 0;
+    for (int z = 0; z <= nz; ++z)
+        for (int y = 0; y <= ny; ++y)
+            for (int x = 0; x <= nx; ++x, ++v) {
+                xyz[3 * v] = (double)x / nx;
+                xyz[3 * v + 1] = (double)y / ny;
+                xyz[3 * v + 2] = (double)z / nz;
+            }
+}
+
+void Mesh::ensure_device(hipStream_t s) {
+    if (d_topo.p) return;
+    // [ne][29]: 8 vertices, 12 edges, 6 faces, 3 grid coordinates
+    std::vector<int> topo(29 * ne);
+    std::vector<int8_t> ori(18 * ne);
+    for (int64_t e = 0; e < ne; ++e) {
+        int *t = topo.data() + 29 * e;
+        for (int k = 0; k < 8; ++k) t[k] = ev[8 * e + k];
+        for (int k = 0; k < 12; ++k) {
+            t[8 + k] = ee[12 * e + k];
+            ori[18 * e + k] = (int8_t)eeo[12 * e + k];
+        }
+        for (int k = 0; k < 6; ++k) {
+            t[20 + k] = ef[6 * e + k];
+            ori[18 * e + 12 + k] = (int8_t)efo[6 * e + k];
+        }
+        for (int k = 0; k < 3; ++k) t[26 + k] = ecoord[3 * e + k];
+    }
+    d_topo.upload(topo.data(), topo.size(), s);
+    d_ori.upload(ori.data(), ori.size(), s);
+    H1PA_HIP(hipStreamSynchronize(s));
+}
+
+void Mesh::ensure_nodes(hipStream_t s) {
+    if (nodes_valid) return;
+    const int64_t Dg = node_order + 1;
+    d_nodes.alloc(Dg * Dg * Dg * 3 * ne);
+    ensure_device(s);
+    if (node_order == 1 && curve_amp == 0.0) kern::nodes_trilinear(*this, d_nodes.p, s);
+    else kern::nodes_curved(*this, node_order, curve_amp, d_nodes.p, s);
+    nodes_valid = true;
+}
+
+namespace kern {
+
+namespace {
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+}  // namespace
+
+void perturb_vertices(Mesh &m, double amp, uint64_t seed) {
+    // Interior vertices move by U(-amp h, amp h) per coordinate; the uniform
+    // draw for coordinate c of vertex v is splitmix64(seed ^ splitmix64(3v+c)).
+    const double h[3] = {1.0 / m.nx, 1.0 / m.ny, 1.0 / m.nz};
+    int64_t v = 0;
+    for (int z = 0; z <= m.nz; ++z)
+        for (int y = 0; y <= m.ny; ++y)
+            for (int x = 0; x <= m.nx; ++x, ++v) {
+                if (x == 0 || y == 0 || z == 0 || x == m.nx || y == m.ny || z == m.nz) continue;
+                for (int c = 0; c < 3; ++c) {
+                    const uint64_t r = splitmix64(seed ^ splitmix64(3ULL * (uint64_t)v + c));
+                    const double u = (double)(r >> 11) * (1.0 / 9007199254740992.0);
+                    m.xyz[3 * v + c] += amp * h[c] * (2.0 * u - 1.0);
+                }
+            }
+    m.nodes_valid = false;
+}
+
+}  // namespace kern
+}  // namespace h1pa

@@ -1,0 +1,377 @@
+// restriction.hip -- ElementRestriction on the device.
+//
+//  * gather map: one thread per E-DOF evaluates MFEM's element DOF numbering
+//    (vertex / edge / face / bubble blocks, orientation permutations) straight
+//    from the compact element topology, already permuted to lexicographic
+//    order -- FiniteElementSpace::GetElementDofs (fem/fespace.cpp:3428-3535)
+//    fused with the dof_map step of the ElementRestriction constructor
+//    (fem/restriction.cpp:60-98).
+//  * transpose table: integer count -> exclusive scan -> slot fill -> per-row
+//    sort.  The slot fill uses integer atomics, the per-row sort makes the
+//    result independent of their order, so offsets/indices come out
+//    bit-identical to MFEM's sequential build (restriction.cpp:73-113).
+//  * E->L: one thread per L-DOF sums its E-entries in table order -- no
+//    floating-point atomics, deterministic (restriction.cpp:159-228).
+#include "internal.hpp"
+
+namespace h1pa {
+namespace kern {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+__device__ inline int quad_dof_ord(int ori, int o, int pm1) {
+    // H1_FECollection::QuadDofOrd (fem/fe_coll.cpp:1920-1936)
+    const int pm2 = pm1 - 1, i = o % pm1, j = o / pm1;
+    switch (ori & 7) {
+        case 0: return i + j * pm1;
+        case 1: return j + i * pm1;
+        case 2: return j + (pm2 - i) * pm1;
+        case 3: return (pm2 - i) + j * pm1;
+        case 4: return (pm2 - i) + (pm2 - j) * pm1;
+        case 5: return (pm2 - j) + (pm2 - i) * pm1;
+        case 6: return (pm2 - j) + i * pm1;
+        default: return i + (pm2 - j) * pm1;
+    }
+}
+
+__global__ void k_build_gather(int64_t ne, int p, int dof, const int *__restrict__ topo,
+                               const int8_t *__restrict__ ori, const int *__restrict__ dof_map,
+                               int64_t nvdofs, int64_t nedofs, int64_t nfdofs,
+                               int *__restrict__ elem_dof, int *__restrict__ gather_map) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ne * dof) return;
+    const int64_t e = i / dof;
+    const int lex = (int)(i - e * dof);
+    const int n = dof_map[lex];
+    const int pm1 = p - 1, nf = pm1 * pm1;
+    const int *t = topo + 29 * e;
+    const int8_t *o = ori + 18 * e;
+    int64_t g;
+    if (n < 8) {
+        g = t[n];
+    } else if (n < 8 + 12 * pm1) {
+        const int k = n - 8, ed = k / pm1, j = k % pm1;
+        g = nvdofs + (int64_t)t[8 + ed] * pm1 + (o[ed] > 0 ? j : pm1 - 1 - j);
+    } else if (n < 8 + 12 * pm1 + 6 * nf) {
+        const int k = n - 8 - 12 * pm1, f = k / nf, j = k % nf;
+        g = nvdofs + nedofs + (int64_t)t[20 + f] * nf + quad_dof_ord(o[12 + f], j, pm1);
+    } else {
+        g = nvdofs + nedofs + nfdofs + e * (int64_t)nf * pm1 + (n - (8 + 12 * pm1 + 6 * nf));
+    }
+    gather_map[i] = (int)g;
+    elem_dof[e * dof + n] = (int)g;
+}
+
+__global__ void k_gather_from_table(int64_t ne, int dof, const int *__restrict__ elem_dof,
+                                    const int *__restrict__ dof_map, int *__restrict__ gmap) {
+    // restriction.cpp:91-110 (signed entries: -1-i encodes a sign flip)
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ne * dof) return;
+    const int64_t e = i / dof;
+    const int d = (int)(i - e * dof);
+    const int sdid = dof_map ? dof_map[d] : 0;
+    const int did = dof_map ? (sdid >= 0 ? sdid : -1 - sdid) : d;
+    const int sgid = elem_dof[dof * e + did];
+    const int gid = sgid >= 0 ? sgid : -1 - sgid;
+    const bool plus = (sgid >= 0 && sdid >= 0) || (sgid < 0 && sdid < 0);
+    gmap[i] = plus ? gid : -1 - gid;
+}
+
+__global__ void k_count(int64_t n, const int *__restrict__ gmap, int *__restrict__ cnt) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = gmap[i];
+    atomicAdd(&cnt[(g >= 0 ? g : -1 - g) + 1], 1);
+}
+
+__global__ void k_fill(int64_t n, const int *__restrict__ gmap, int *__restrict__ cursor,
+                       int *__restrict__ indices) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int g = gmap[i];
+    const bool plus = g >= 0;
+    const int gid = plus ? g : -1 - g;
+    const int pos = atomicAdd(&cursor[gid], 1);
+    indices[pos] = plus ? (int)i : -1 - (int)i;
+}
+
+__global__ void k_sort_rows(int64_t ndofs, const int *__restrict__ offsets, int *__restrict__ idx) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    const int b = offsets[i], e = offsets[i + 1];
+    auto key = [](int v) { return v >= 0 ? v : -1 - v; };
+    for (int a = b + 1; a < e; ++a) {
+        const int v = idx[a], kv = key(v);
+        int c = a - 1;
+        while (c >= b && key(idx[c]) > kv) {
+            idx[c + 1] = idx[c];
+            --c;
+        }
+        idx[c + 1] = v;
+    }
+}
+
+// ---- inclusive scan of int32 (three passes) --------------------------------
+constexpr int kScanThreads = 256, kScanItems = 16, kScanTile = kScanThreads * kScanItems;
+
+__device__ inline int block_inclusive_scan(int v, int *sh) {
+    // Hillis-Steele over 256 threads in LDS.
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int add = t >= off ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += add;
+        __syncthreads();
+    }
+    return sh[t];
+}
+
+__global__ void k_scan_tiles(int *__restrict__ a, int64_t n, int *__restrict__ tile_sums) {
+    __shared__ int sh[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int loc[kScanItems];
+    int s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        s += (i < n) ? a[i] : 0;
+        loc[k] = s;
+    }
+    const int incl = block_inclusive_scan(s, sh);
+    const int excl = incl - s;
+    for (int k = 0; k < kScanItems; ++k) {
+        const int64_t i = base + k;
+        if (i < n) a[i] = loc[k] + excl;
+    }
+    if (threadIdx.x == kScanThreads - 1) tile_sums[blockIdx.x] = incl;
+}
+
+__global__ void k_scan_sums(int *__restrict__ sums, int64_t n) {
+    // one block: sequential carry across chunks of 256
+    __shared__ int sh[kScanThreads];
+    int carry = 0;
+    for (int64_t c = 0; c < n; c += kScanThreads) {
+        const int64_t i = c + threadIdx.x;
+        const int v = i < n ? sums[i] : 0;
+        const int incl = block_inclusive_scan(v, sh);
+        if (i < n) sums[i] = incl + carry;
+        __syncthreads();
+        carry += sh[kScanThreads - 1];
+        __syncthreads();
+    }
+}
+
+__global__ void k_scan_add(int *__restrict__ a, int64_t n, const int *__restrict__ tile_sums) {
+    if (blockIdx.x == 0) return;
+    const int add = tile_sums[blockIdx.x - 1];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+        const int64_t i = base + k;
+        if (i < n) a[i] += add;
+    }
+}
+
+void inclusive_scan(int *a, int64_t n, hipStream_t s) {
+    if (n <= 0) return;
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    DevBuf<int> sums(tiles);
+    hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, a, n, sums.p);
+    if (tiles > 1) {
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, sums.p, tiles);
+        hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, a, n, sums.p);
+    }
+    H1PA_HIP(hipGetLastError());
+    H1PA_HIP(hipStreamSynchronize(s));  // `sums` is freed on return
+}
+
+__global__ void k_boundary_mark(int64_t ne, int p, int nx, int ny, int nz,
+                                const int *__restrict__ topo, const int *__restrict__ gmap,
+                                uint8_t *__restrict__ mask) {
+    const int D = p + 1, dof = D * D * D;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ne * dof) return;
+    const int64_t e = i / dof;
+    const int lex = (int)(i - e * dof);
+    const int a = lex % D, b = (lex / D) % D, c = lex / (D * D);
+    const int *t = topo + 29 * e;
+    const int x = t[26], y = t[27], z = t[28];
+    const bool on = (x == 0 && a == 0) || (x == nx - 1 && a == p) || (y == 0 && b == 0) ||
+                    (y == ny - 1 && b == p) || (z == 0 && c == 0) || (z == nz - 1 && c == p);
+    if (on) {
+        const int g = gmap[i];
+        mask[g >= 0 ? g : -1 - g] = 1;
+    }
+}
+
+__global__ void k_mask_to_int(int64_t n, const uint8_t *__restrict__ m, int *__restrict__ o) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) o[i] = m[i] ? 1 : 0;
+}
+
+__global__ void k_compact(int64_t n, const uint8_t *__restrict__ m, const int *__restrict__ incl,
+                          int *__restrict__ list) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n && m[i]) list[incl[i] - 1] = (int)i;
+}
+
+__global__ void k_list_to_mask(int64_t nl, const int *__restrict__ list, uint8_t *__restrict__ m,
+                               int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nl) return;
+    const int j = list[i];
+    if (j >= 0 && j < n) m[j] = 1;  // out-of-range entries are ignored
+}
+
+// ---- L <-> E ---------------------------------------------------------------
+__global__ void k_restrict_mult(int64_t nE, const int *__restrict__ gmap,
+                                const double *__restrict__ x, double *__restrict__ y, bool abs_) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nE) return;
+    const int g = gmap[i];
+    const bool plus = g >= 0;
+    const double v = x[plus ? g : -1 - g];
+    y[i] = (plus || abs_) ? v : -v;
+}
+
+__global__ void k_restrict_mult_t(int64_t ndofs, const int *__restrict__ offsets,
+                                  const int *__restrict__ indices, const double *__restrict__ xe,
+                                  double *__restrict__ y, int mode) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    const int b = offsets[i], e = offsets[i + 1];
+    double v = 0.0;
+    if (mode == 3) {
+        if (e > b) {
+            const int ix = indices[e - 1];
+            v = ix >= 0 ? xe[ix] : -xe[-1 - ix];
+        }
+    } else {
+        for (int j = b; j < e; ++j) {
+            const int ix = indices[j];
+            if (ix >= 0) v += xe[ix];
+            else v += (mode == 2) ? xe[-1 - ix] : -xe[-1 - ix];
+        }
+    }
+    if (mode == 1) y[i] += v;
+    else y[i] = v;
+}
+
+__global__ void k_scatter_constrained(int64_t ndofs, const int *__restrict__ offsets,
+                                      const int *__restrict__ indices,
+                                      const double *__restrict__ xe,
+                                      const uint8_t *__restrict__ mask,
+                                      const double *__restrict__ xin, double *__restrict__ y) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= ndofs) return;
+    if (mask && mask[i]) {
+        y[i] = xin[i];
+        return;
+    }
+    const int b = offsets[i], e = offsets[i + 1];
+    double v = 0.0;
+    for (int j = b; j < e; ++j) {
+        const int ix = indices[j];
+        if (ix >= 0) v += xe[ix];
+        else v += -xe[-1 - ix];
+    }
+    y[i] = v;
+}
+
+}  // namespace
+
+void build_gather_map(const Mesh &m, int p, int64_t nvdofs, int64_t nedofs, int64_t nfdofs,
+                      int *elem_dof, int *gather_map, hipStream_t s) {
+    const int D = p + 1, dof = D * D * D;
+    std::vector<int> dm(dof);
+    h1_dof_map(p, dm.data());
+    DevBuf<int> d_dm;
+    d_dm.upload(dm.data(), dof, s);
+    const int64_t n = m.ne * dof;
+    hipLaunchKernelGGL(k_build_gather, dim3(nblocks(n)), dim3(kBlock), 0, s, m.ne, p, dof,
+                       m.d_topo.p, m.d_ori.p, d_dm.p, nvdofs, nedofs, nfdofs, elem_dof, gather_map);
+    H1PA_HIP(hipGetLastError());
+    H1PA_HIP(hipStreamSynchronize(s));
+}
+
+void gather_from_table(int64_t ne, int dof, const int *elem_dof, const int *dof_map, int *gmap,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_from_table, dim3(nblocks(ne * dof)), dim3(kBlock), 0, s, ne, dof,
+                       elem_dof, dof_map, gmap);
+    H1PA_HIP(hipGetLastError());
+}
+
+void build_transpose(int64_t ne, int dof, int64_t ndofs, const int *gmap, int *offsets,
+                     int *indices, hipStream_t s) {
+    const int64_t n = ne * dof;
+    H1PA_HIP(hipMemsetAsync(offsets, 0, sizeof(int) * (ndofs + 1), s));
+    hipLaunchKernelGGL(k_count, dim3(nblocks(n)), dim3(kBlock), 0, s, n, gmap, offsets);
+    H1PA_HIP(hipGetLastError());
+    inclusive_scan(offsets + 1, ndofs, s);  // offsets[i] = start of row i
+    DevBuf<int> cursor(ndofs + 1);
+    H1PA_HIP(hipMemcpyAsync(cursor.p, offsets, sizeof(int) * (ndofs + 1), hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kBlock), 0, s, n, gmap, cursor.p, indices);
+    hipLaunchKernelGGL(k_sort_rows, dim3(nblocks(ndofs)), dim3(kBlock), 0, s, ndofs, offsets,
+                       indices);
+    H1PA_HIP(hipGetLastError());
+    H1PA_HIP(hipStreamSynchronize(s));
+}
+
+void boundary_mask(const Mesh &m, int p, const int *gmap, uint8_t *mask, int64_t ndofs,
+                   hipStream_t s) {
+    const int D = p + 1;
+    const int64_t n = m.ne * D * D * D;
+    H1PA_HIP(hipMemsetAsync(mask, 0, ndofs, s));
+    hipLaunchKernelGGL(k_boundary_mark, dim3(nblocks(n)), dim3(kBlock), 0, s, m.ne, p, m.nx, m.ny,
+                       m.nz, m.d_topo.p, gmap, mask);
+    H1PA_HIP(hipGetLastError());
+}
+
+int64_t mask_to_list(const uint8_t *mask, int64_t n, DevBuf<int> &list, hipStream_t s) {
+    DevBuf<int> incl(n);
+    hipLaunchKernelGGL(k_mask_to_int, dim3(nblocks(n)), dim3(kBlock), 0, s, n, mask, incl.p);
+    H1PA_HIP(hipGetLastError());
+    inclusive_scan(incl.p, n, s);
+    int total = 0;
+    H1PA_HIP(hipMemcpyAsync(&total, incl.p + n - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    H1PA_HIP(hipStreamSynchronize(s));
+    list.alloc(total > 0 ? total : 1);
+    hipLaunchKernelGGL(k_compact, dim3(nblocks(n)), dim3(kBlock), 0, s, n, mask, incl.p, list.p);
+    H1PA_HIP(hipGetLastError());
+    H1PA_HIP(hipStreamSynchronize(s));
+    return total;
+}
+
+void list_to_mask(const int *list, int64_t nl, uint8_t *mask, int64_t n, hipStream_t s) {
+    H1PA_HIP(hipMemsetAsync(mask, 0, n, s));
+    if (nl > 0)
+        hipLaunchKernelGGL(k_list_to_mask, dim3(nblocks(nl)), dim3(kBlock), 0, s, nl, list, mask, n);
+    H1PA_HIP(hipGetLastError());
+}
+
+void restrict_mult(int64_t nE, const int *gmap, const double *x, double *y, bool abs_,
+                   hipStream_t s) {
+    hipLaunchKernelGGL(k_restrict_mult, dim3(nblocks(nE)), dim3(kBlock), 0, s, nE, gmap, x, y, abs_);
+    H1PA_HIP(hipGetLastError());
+}
+
+void restrict_mult_transpose(int64_t ndofs, const int *offsets, const int *indices,
+                             const double *xe, double *y, int mode, hipStream_t s) {
+    hipLaunchKernelGGL(k_restrict_mult_t, dim3(nblocks(ndofs)), dim3(kBlock), 0, s, ndofs, offsets,
+                       indices, xe, y, mode);
+    H1PA_HIP(hipGetLastError());
+}
+
+void scatter_constrained(int64_t ndofs, const int *offsets, const int *indices, const double *xe,
+                         const uint8_t *mask, const double *xin, double *y, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_constrained, dim3(nblocks(ndofs)), dim3(kBlock), 0, s, ndofs,
+                       offsets, indices, xe, mask, xin, y);
+    H1PA_HIP(hipGetLastError());
+}
+
+}  // namespace kern
+}  // namespace h1pa

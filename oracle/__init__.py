@@ -1,0 +1,419 @@
+"""Python view of the CPU oracle (oracle/h1pa_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg are the only users.  The product (mfem_amd/) never imports
+this module.
+
+The oracle restates MFEM's H1 partial-assembly path on the CPU; every C
+function cites the MFEM file:line it follows.  `Problem` strings the pieces
+together the way MFEM's ex1-style driver does (mesh -> H1 space ->
+ElementRestriction -> geometric factors -> PA setup -> apply / diagonal /
+constrained Jacobi-PCG).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libh1pa_oracle.so")
+_lib = None
+
+_i = C.c_int
+_d = C.c_double
+_pi = C.POINTER(C.c_int)
+_pd = C.POINTER(C.c_double)
+
+
+class _Mesh(C.Structure):
+    _fields_ = [("nx", _i), ("ny", _i), ("nz", _i), ("nv", _i), ("ne", _i),
+                ("nedges", _i), ("nfaces", _i), ("ev", _pi), ("ee", _pi),
+                ("eeo", _pi), ("ef", _pi), ("efo", _pi), ("f_elem1", _pi)]
+
+
+class _PaOp(C.Structure):
+    _fields_ = [("D1", _i), ("Q1", _i), ("ne", _i), ("ndofs", _i),
+                ("gather_map", _pi), ("offsets", _pi), ("indices", _pi),
+                ("B", _pd), ("G", _pd), ("Dd", _pd), ("Mv", _pd),
+                ("n_ess", _i), ("ess", _pi)]
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "libh1pa_oracle.so"],
+                       check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        L = _lib
+        L.orc_mesh_cartesian.argtypes = [_i, _i, _i, _i, C.POINTER(_Mesh)]
+        L.orc_mesh_free.argtypes = [C.POINTER(_Mesh)]
+        L.orc_grid_sfc3d.argtypes = [_i, _i, _i, _pi]
+        L.orc_mesh_vertices.argtypes = [_i, _i, _i, _pd]
+        L.orc_perturb_vertices.argtypes = [_i, _i, _i, _d, C.c_ulonglong, _pd]
+        L.orc_h1_elem_dofs.argtypes = [C.POINTER(_Mesh), _i, _pi]
+        L.orc_h1_elem_dofs.restype = _i
+        L.orc_h1_dof_map.argtypes = [_i, _pi]
+        L.orc_h1_boundary_dofs.argtypes = [C.POINTER(_Mesh), _i, _pi, _pi, _i]
+        L.orc_h1_boundary_dofs.restype = _i
+        L.orc_restriction_build.argtypes = [_i, _i, _i, _pi, _pi, _pi, _pi, _pi]
+        L.orc_restriction_mult.argtypes = [_i, _i, _pi, _pd, _pd]
+        L.orc_restriction_mult_transpose.argtypes = [_i, _i, _pi, _pi, _pd, _pd, _i]
+        L.orc_gauss_legendre.argtypes = [_i, _pd, _pd]
+        L.orc_gauss_lobatto.argtypes = [_i, _pd, _pd]
+        L.orc_basis.argtypes = [_i, _i, _pd, _pd]
+        L.orc_basis_at.argtypes = [_i, _i, _pd, _pd, _pd]
+        L.orc_tensor_weights.argtypes = [_i, _pd]
+        L.orc_nodes_trilinear.argtypes = [C.POINTER(_Mesh), _pd, _pd]
+        L.orc_nodes_curved.argtypes = [C.POINTER(_Mesh), _pd, _i, _d, _pd]
+        L.orc_geom.argtypes = [_i, _i, _i, _pd, _pd, _pd]
+        L.orc_coeff_sincos.argtypes = [_i, _i, _pd, _pd]
+        L.orc_diffusion_setup.argtypes = [_i, _i, _pd, _pd, _pd, _i, _pd]
+        L.orc_mass_setup.argtypes = [_i, _i, _pd, _pd, _pd, _i, _pd]
+        L.orc_diffusion_apply.argtypes = [_i, _i, _i, _pd, _pd, _pd, _pd, _pd]
+        L.orc_mass_apply.argtypes = [_i, _i, _i, _pd, _pd, _pd, _pd]
+        L.orc_diffusion_diag.argtypes = [_i, _i, _i, _pd, _pd, _pd, _pd]
+        L.orc_mass_diag.argtypes = [_i, _i, _i, _pd, _pd, _pd]
+        L.orc_pa_mult.argtypes = [C.POINTER(_PaOp), _pd, _pd]
+        L.orc_pa_diagonal.argtypes = [C.POINTER(_PaOp), _pd]
+        L.orc_constrained_mult.argtypes = [C.POINTER(_PaOp), _pd, _pd]
+        L.orc_form_linear_system.argtypes = [C.POINTER(_PaOp), _pd, _pd, _pd, _pd]
+        L.orc_pcg.argtypes = [C.POINTER(_PaOp), _pd, _pd, _pd, _d, _d, _i, _pi, _pd, _pd]
+        L.orc_pcg.restype = _i
+        L.orc_domain_lf.argtypes = [_i, _i, _i, _pd, _d, _pd]
+        L.orc_num_threads.restype = _i
+    return _lib
+
+
+def _ip(a):
+    return a.ctypes.data_as(_pi)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_pd)
+
+
+def _ia(n):
+    return np.zeros(int(n), dtype=np.int32)
+
+
+def _da(n):
+    return np.zeros(int(n), dtype=np.float64)
+
+
+# ---------------------------------------------------------------------------
+# mesh / space / restriction
+# ---------------------------------------------------------------------------
+def grid_sfc3d(nx, ny, nz):
+    c = _ia(3 * nx * ny * nz)
+    lib().orc_grid_sfc3d(nx, ny, nz, _ip(c))
+    return c.reshape(-1, 3)
+
+
+class CartesianMesh:
+    """Mesh::MakeCartesian3D(nx, ny, nz, HEXAHEDRON, 1, 1, 1, sfc) topology."""
+
+    def __init__(self, nx, ny, nz, sfc=True):
+        m = _Mesh()
+        if lib().orc_mesh_cartesian(nx, ny, nz, 1 if sfc else 0, C.byref(m)) != 0:
+            raise MemoryError("orc_mesh_cartesian")
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nv, self.ne = m.nv, m.ne
+        self.nedges, self.nfaces = m.nedges, m.nfaces
+        ne = m.ne
+        self.ev = np.ctypeslib.as_array(m.ev, (ne * 8,)).copy().reshape(ne, 8)
+        self.ee = np.ctypeslib.as_array(m.ee, (ne * 12,)).copy().reshape(ne, 12)
+        self.eeo = np.ctypeslib.as_array(m.eeo, (ne * 12,)).copy().reshape(ne, 12)
+        self.ef = np.ctypeslib.as_array(m.ef, (ne * 6,)).copy().reshape(ne, 6)
+        self.efo = np.ctypeslib.as_array(m.efo, (ne * 6,)).copy().reshape(ne, 6)
+        self.f_elem1 = np.ctypeslib.as_array(m.f_elem1, (m.nfaces,)).copy()
+        self._m = m
+
+    def __del__(self):
+        try:
+            lib().orc_mesh_free(C.byref(self._m))
+        except Exception:
+            pass
+
+    def vertices(self, perturb=0.0, seed=0):
+        xyz = _da(3 * self.nv)
+        lib().orc_mesh_vertices(self.nx, self.ny, self.nz, _dp(xyz))
+        if perturb:
+            lib().orc_perturb_vertices(self.nx, self.ny, self.nz, float(perturb),
+                                       C.c_ulonglong(seed), _dp(xyz))
+        return xyz.reshape(-1, 3)
+
+    def elem_dofs(self, p):
+        nd = (p + 1) ** 3
+        ed = _ia(self.ne * nd)
+        ndofs = lib().orc_h1_elem_dofs(C.byref(self._m), p, _ip(ed))
+        return ed.reshape(self.ne, nd), ndofs
+
+    def boundary_dofs(self, p, elem_dof):
+        ed = np.ascontiguousarray(elem_dof, dtype=np.int32).ravel()
+        n = lib().orc_h1_boundary_dofs(C.byref(self._m), p, _ip(ed), _ip(_ia(1)), 0)
+        ess = _ia(n)
+        lib().orc_h1_boundary_dofs(C.byref(self._m), p, _ip(ed), _ip(ess), n)
+        return ess
+
+    def nodes_trilinear(self, xyz):
+        X = _da(8 * 3 * self.ne)
+        lib().orc_nodes_trilinear(C.byref(self._m), _dp(np.ascontiguousarray(xyz).ravel()), _dp(X))
+        return X
+
+    def nodes_curved(self, xyz, pg, amp):
+        X = _da((pg + 1) ** 3 * 3 * self.ne)
+        lib().orc_nodes_curved(C.byref(self._m), _dp(np.ascontiguousarray(xyz).ravel()), pg,
+                               float(amp), _dp(X))
+        return X
+
+
+def dof_map(p):
+    dm = _ia((p + 1) ** 3)
+    lib().orc_h1_dof_map(p, _ip(dm))
+    return dm
+
+
+def restriction(elem_dof, dmap, ndofs):
+    ed = np.ascontiguousarray(elem_dof, dtype=np.int32).ravel()
+    ne, nd = elem_dof.shape
+    g, o, ix = _ia(ne * nd), _ia(ndofs + 1), _ia(ne * nd)
+    lib().orc_restriction_build(ne, nd, ndofs, _ip(ed), _ip(dmap) if dmap is not None else None,
+                                _ip(g), _ip(o), _ip(ix))
+    return g, o, ix
+
+
+def restriction_mult(gather_map, nd, x):
+    y = _da(gather_map.size)
+    lib().orc_restriction_mult(gather_map.size // nd, nd, _ip(gather_map),
+                               _dp(np.ascontiguousarray(x, dtype=np.float64)), _dp(y))
+    return y
+
+
+def restriction_mult_transpose(offsets, indices, nd, xe, mode=0, y=None):
+    ndofs = offsets.size - 1
+    y = _da(ndofs) if y is None else y.copy()
+    lib().orc_restriction_mult_transpose(ndofs, nd, _ip(offsets), _ip(indices),
+                                         _dp(np.ascontiguousarray(xe, dtype=np.float64)), _dp(y),
+                                         mode)
+    return y
+
+
+# ---------------------------------------------------------------------------
+# quadrature / basis / geometry / setup / kernels
+# ---------------------------------------------------------------------------
+def gauss_legendre(n):
+    x, w = _da(n), _da(n)
+    lib().orc_gauss_legendre(n, _dp(x), _dp(w))
+    return x, w
+
+
+def gauss_lobatto(n):
+    x, w = _da(n), _da(n)
+    lib().orc_gauss_lobatto(n, _dp(x), _dp(w))
+    return x, w
+
+
+def basis(p, Q):
+    """B, G as (Q, D) arrays: B[q, d] = l_d(x_q) (MFEM: B[q + Q*d])."""
+    B, G = _da(Q * (p + 1)), _da(Q * (p + 1))
+    lib().orc_basis(p, Q, _dp(B), _dp(G))
+    return B.reshape(p + 1, Q).T.copy(), G.reshape(p + 1, Q).T.copy()
+
+
+def basis_at(p, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = x.size
+    B, G = _da(n * (p + 1)), _da(n * (p + 1))
+    lib().orc_basis_at(p, n, _dp(x), _dp(B), _dp(G))
+    return B.reshape(p + 1, n).T.copy(), G.reshape(p + 1, n).T.copy()
+
+
+def _flat_BG(B, G):
+    # (Q, D) -> MFEM column-major B[q + Q*d]
+    return (np.ascontiguousarray(B.T).ravel(), np.ascontiguousarray(G.T).ravel())
+
+
+def tensor_weights(Q):
+    W = _da(Q ** 3)
+    lib().orc_tensor_weights(Q, _dp(W))
+    return W
+
+
+def geom(ne, pg, Q, X):
+    nq = Q ** 3
+    J, P = _da(nq * 9 * ne), _da(nq * 3 * ne)
+    lib().orc_geom(ne, pg, Q, _dp(X), _dp(J), _dp(P))
+    return J, P
+
+
+def coeff_sincos(Q, ne, P):
+    C_ = _da(Q ** 3 * ne)
+    lib().orc_coeff_sincos(Q ** 3, ne, _dp(P), _dp(C_))
+    return C_
+
+
+def diffusion_setup(Q, ne, W, J, coef):
+    coef = np.atleast_1d(np.asarray(coef, dtype=np.float64))
+    Dd = _da(Q ** 3 * 6 * ne)
+    lib().orc_diffusion_setup(Q, ne, _dp(W), _dp(J), _dp(coef), coef.size, _dp(Dd))
+    return Dd
+
+
+def mass_setup(Q, ne, W, J, coef):
+    coef = np.atleast_1d(np.asarray(coef, dtype=np.float64))
+    Mv = _da(Q ** 3 * ne)
+    lib().orc_mass_setup(Q, ne, _dp(W), _dp(J), _dp(coef), coef.size, _dp(Mv))
+    return Mv
+
+
+def diffusion_apply(p, Q, ne, B, G, Dd, xe, ye=None):
+    Bf, Gf = _flat_BG(B, G)
+    ye = _da(xe.size) if ye is None else ye.copy()
+    lib().orc_diffusion_apply(p + 1, Q, ne, _dp(Bf), _dp(Gf), _dp(Dd),
+                              _dp(np.ascontiguousarray(xe)), _dp(ye))
+    return ye
+
+
+def mass_apply(p, Q, ne, B, Mv, xe, ye=None):
+    Bf, _ = _flat_BG(B, B)
+    ye = _da(xe.size) if ye is None else ye.copy()
+    lib().orc_mass_apply(p + 1, Q, ne, _dp(Bf), _dp(Mv), _dp(np.ascontiguousarray(xe)), _dp(ye))
+    return ye
+
+
+def diffusion_diag(p, Q, ne, B, G, Dd):
+    Bf, Gf = _flat_BG(B, G)
+    ye = _da((p + 1) ** 3 * ne)
+    lib().orc_diffusion_diag(p + 1, Q, ne, _dp(Bf), _dp(Gf), _dp(Dd), _dp(ye))
+    return ye
+
+
+def mass_diag(p, Q, ne, B, Mv):
+    Bf, _ = _flat_BG(B, B)
+    ye = _da((p + 1) ** 3 * ne)
+    lib().orc_mass_diag(p + 1, Q, ne, _dp(Bf), _dp(Mv), _dp(ye))
+    return ye
+
+
+def domain_lf(p, ne, pg, X, f=1.0):
+    bE = _da((p + 1) ** 3 * ne)
+    lib().orc_domain_lf(p, ne, pg, _dp(X), float(f), _dp(bE))
+    return bE
+
+
+def num_threads():
+    return lib().orc_num_threads()
+
+
+# ---------------------------------------------------------------------------
+# a whole problem, MFEM ex1-style
+# ---------------------------------------------------------------------------
+class Problem:
+    """H1 order-p PA problem on an n^3 (or nx*ny*nz) Cartesian hex mesh.
+
+    perturb: interior vertex noise amplitude as a fraction of h (cfg2: 0.2).
+    curve:   amplitude of the cfg3 curving map (geometry of order p) or 0.
+    coeff:   'one' | 'sincos' | ndarray of per-element constants.
+    integrators: subset of {'diffusion', 'mass'}.
+    """
+
+    def __init__(self, n, p, Q=None, perturb=0.0, seed=0, curve=0.0, coeff="one",
+                 integrators=("diffusion",), sfc=True, dims=None):
+        nx, ny, nz = dims if dims is not None else (n, n, n)
+        self.p, self.D = p, p + 1
+        self.Q = Q if Q is not None else p + 2
+        self.mesh = CartesianMesh(nx, ny, nz, sfc=sfc)
+        m = self.mesh
+        self.ne = m.ne
+        self.xyz = m.vertices(perturb, seed)
+        self.elem_dof, self.ndofs = m.elem_dofs(p)
+        self.dof_map = dof_map(p)
+        self.gather_map, self.offsets, self.indices = restriction(self.elem_dof, self.dof_map,
+                                                                  self.ndofs)
+        self.ess = m.boundary_dofs(p, self.elem_dof)
+        self.B, self.G = basis(p, self.Q)
+        self.W = tensor_weights(self.Q)
+        if curve:
+            self.pg = p
+            self.X = m.nodes_curved(self.xyz, p, curve)
+        else:
+            self.pg = 1
+            self.X = m.nodes_trilinear(self.xyz)
+        self.J, self.P = geom(self.ne, self.pg, self.Q, self.X)
+        nq = self.Q ** 3
+        if isinstance(coeff, str):
+            if coeff == "one":
+                self.C = np.ones(1)
+            elif coeff == "sincos":
+                self.C = coeff_sincos(self.Q, self.ne, self.P)
+            else:
+                raise ValueError(coeff)
+        else:
+            pe = np.asarray(coeff, dtype=np.float64)
+            assert pe.size == self.ne
+            self.C = np.repeat(pe, nq)
+        self.Dd = diffusion_setup(self.Q, self.ne, self.W, self.J, self.C) \
+            if "diffusion" in integrators else None
+        self.Mv = mass_setup(self.Q, self.ne, self.W, self.J, self.C) \
+            if "mass" in integrators else None
+        Bf, Gf = _flat_BG(self.B, self.G)
+        self._Bf, self._Gf = Bf, Gf
+        op = _PaOp()
+        op.D1, op.Q1, op.ne, op.ndofs = self.D, self.Q, self.ne, self.ndofs
+        op.gather_map, op.offsets, op.indices = (_ip(self.gather_map), _ip(self.offsets),
+                                                 _ip(self.indices))
+        op.B, op.G = _dp(Bf), _dp(Gf)
+        op.Dd = _dp(self.Dd) if self.Dd is not None else None
+        op.Mv = _dp(self.Mv) if self.Mv is not None else None
+        op.n_ess, op.ess = int(self.ess.size), _ip(self.ess)
+        self._op = op
+
+    def mult(self, x):
+        y = _da(self.ndofs)
+        lib().orc_pa_mult(C.byref(self._op), _dp(np.ascontiguousarray(x, dtype=np.float64)),
+                          _dp(y))
+        return y
+
+    def constrained_mult(self, x):
+        y = _da(self.ndofs)
+        lib().orc_constrained_mult(C.byref(self._op),
+                                   _dp(np.ascontiguousarray(x, dtype=np.float64)), _dp(y))
+        return y
+
+    def diagonal(self):
+        d = _da(self.ndofs)
+        lib().orc_pa_diagonal(C.byref(self._op), _dp(d))
+        return d
+
+    def rhs(self, f=1.0):
+        bE = domain_lf(self.p, self.ne, self.pg, self.X, f)
+        return restriction_mult_transpose(self.offsets, self.indices, self.D ** 3, bE, 0)
+
+    def form_linear_system(self, x, b):
+        x = np.ascontiguousarray(x, dtype=np.float64).copy()
+        X, Bv = _da(self.ndofs), _da(self.ndofs)
+        lib().orc_form_linear_system(C.byref(self._op), _dp(x),
+                                     _dp(np.ascontiguousarray(b, dtype=np.float64)), _dp(X),
+                                     _dp(Bv))
+        return X, Bv
+
+    def pcg(self, b, rel_tol=1e-12, abs_tol=0.0, max_iter=2000, diag=None):
+        diag = self.diagonal() if diag is None else diag
+        x = _da(self.ndofs)
+        it, fn, n0 = _i(0), _d(0), _d(0)
+        conv = lib().orc_pcg(C.byref(self._op), _dp(diag),
+                             _dp(np.ascontiguousarray(b, dtype=np.float64)), _dp(x),
+                             rel_tol, abs_tol, max_iter, C.byref(it), C.byref(fn), C.byref(n0))
+        return x, dict(converged=bool(conv), iterations=it.value, final_norm=fn.value,
+                       initial_norm=n0.value)
