@@ -168,7 +168,7 @@ int h1pa_form_assemble_diagonal(h1pa_form *form, double *d_diag, void *stream);
 /* Device pointers of the PA data: diffusion D[q1d^3][6][ne] (comps 11,21,31,
  * 22,32,33) and mass v[q1d^3][ne]; NULL where the integrator is absent. */
 int h1pa_form_pa_data(const h1pa_form *form, const double **d_diffusion, const double **d_mass);
-/* info[0..2] = q1d, integrators, ndofs */
+/* info[0..3] = q1d, integrators, ndofs, selected diffusion-apply variant (-1: none) */
 int h1pa_form_info(const h1pa_form *form, int64_t *info);
 /* Benchmark helper (no MFEM counterpart): `steps` back-to-back h1pa_form_mult
  * calls with HIP events recorded on `stream` around each launch.  Returns the

@@ -113,6 +113,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is not built; run `python -m mfem_amd.build`")
+        # torch ships its own HIP runtime; load it first so that libh1pa.so
+        # binds to the same libamdhip64 instance (two runtimes in one
+        # process make every hipMalloc of the second one fail).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
@@ -592,7 +599,7 @@ class BilinearForm:
         dd, dm = C.c_void_p(), C.c_void_p()
         f = self._forms[0]
         _check(lib().h1pa_form_pa_data(f._h, C.byref(dd), C.byref(dm)))
-        info = (C.c_int64 * 3)()
+        info = (C.c_int64 * 4)()
         _check(lib().h1pa_form_info(f._h, info))
         q = int(info[0])
         nq = q ** 3 * self.fes.ne

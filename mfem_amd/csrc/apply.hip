@@ -47,8 +47,8 @@ __device__ __forceinline__ double load_x(const int *__restrict__ gmap,
     return g >= 0 ? v : -v;
 }
 
-template <int D, int Q>
-__global__ void __launch_bounds__(nb_for(Q, 3 * D * Q * Q) * Q * Q)
+template <int D, int Q, int NB, bool PF>
+__global__ void __launch_bounds__(NB * Q * Q)
 k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict__ Gm,
                   const double *__restrict__ pa, const int *__restrict__ gmap,
                   const uint8_t *__restrict__ mask, const double *__restrict__ x,
@@ -56,7 +56,6 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DDQ = D * D * Q, DQQ = D * Q * Q;
     constexpr int BUF = (2 * DDQ > 3 * DQQ) ? 2 * DDQ : 3 * DQQ;
-    constexpr int NB = nb_for(Q, 3 * DQQ);
     __shared__ double smem[NB * BUF];
     const int le = threadIdx.x / QQ;
     const int t = threadIdx.x - le * QQ;
@@ -67,14 +66,15 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
 #define B_(q, d) Bm[(q) + Q * (d)]
 #define G_(q, d) Gm[(q) + Q * (d)]
 
-    // PA data of column (a,b): 6 comps x Q points, issued first.
-    double Dq[6][Q];
-    if (valid) {
-        const double *p = pa + (int64_t)QQQ * 6 * e + a + Q * b;
+    // PA data of column (a,b): 6 comps x Q points.  PF: all issued at entry
+    // so their latency overlaps the x/y stages; otherwise read per qz.
+    const double *pcol = pa + (int64_t)QQQ * 6 * (valid ? e : 0) + a + Q * b;
+    double Dq[6][PF ? Q : 1];
+    if (PF && valid) {
 #pragma unroll
         for (int c = 0; c < 6; ++c)
 #pragma unroll
-            for (int qz = 0; qz < Q; ++qz) Dq[c][qz] = p[QQQ * c + QQ * qz];
+            for (int qz = 0; qz < (PF ? Q : 1); ++qz) Dq[c][qz] = pcol[QQQ * c + QQ * qz];
     }
 
     // ---- forward x: thread (dy, dz) = (a, b)
@@ -152,9 +152,13 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
                     g1 += B_(qz, dz) * z1[dz];
                     g2 += G_(qz, dz) * z2[dz];
                 }
-                const double O0 = Dq[0][qz] * g0 + Dq[1][qz] * g1 + Dq[2][qz] * g2;
-                const double O1 = Dq[1][qz] * g0 + Dq[3][qz] * g1 + Dq[4][qz] * g2;
-                const double O2 = Dq[2][qz] * g0 + Dq[4][qz] * g1 + Dq[5][qz] * g2;
+                double d6[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c)
+                    d6[c] = PF ? Dq[c][PF ? qz : 0] : pcol[QQQ * c + QQ * qz];
+                const double O0 = d6[0] * g0 + d6[1] * g1 + d6[2] * g2;
+                const double O1 = d6[1] * g0 + d6[3] * g1 + d6[4] * g2;
+                const double O2 = d6[2] * g0 + d6[4] * g1 + d6[5] * g2;
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) {
                     T0[dz] += B_(qz, dz) * O0;
@@ -488,14 +492,31 @@ k_mass_diag(int64_t ne, const double *__restrict__ Bm, const double *__restrict_
 }
 
 // ---- dispatch over (D, Q) -----------------------------------------------------
-template <int D, int Q>
-void launch_diffusion(int64_t ne, const double *B, const double *G, const double *pa,
-                      const int *gmap, const uint8_t *mask, const double *x, double *ye,
-                      bool acc, hipStream_t s) {
-    constexpr int NB = nb_for(Q, 3 * D * Q * Q);
+template <int D, int Q, int NB, bool PF>
+void launch_diffusion_v(int64_t ne, const double *B, const double *G, const double *pa,
+                        const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                        bool acc, hipStream_t s) {
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
-    hipLaunchKernelGGL((k_diffusion_apply<D, Q>), dim3(grid), dim3(NB * Q * Q), 0, s, ne, B, G, pa,
-                       gmap, mask, x, ye, acc ? 1 : 0);
+    hipLaunchKernelGGL((k_diffusion_apply<D, Q, NB, PF>), dim3(grid), dim3(NB * Q * Q), 0, s, ne, B,
+                       G, pa, gmap, mask, x, ye, acc ? 1 : 0);
+}
+
+// Variants of the diffusion apply: (elements per block, prefetch).  Every
+// variant performs the same arithmetic per element in the same order, so
+// they give bit-identical results and only differ in speed.
+constexpr int kDiffVariants = 4;
+template <int D, int Q>
+void launch_diffusion(int variant, int64_t ne, const double *B, const double *G,
+                      const double *pa, const int *gmap, const uint8_t *mask, const double *x,
+                      double *ye, bool acc, hipStream_t s) {
+    constexpr int NBmax = nb_for(Q, 3 * D * Q * Q);
+    constexpr int NBhalf = NBmax / 2 > 0 ? NBmax / 2 : 1;
+    switch (variant) {
+        case 1: launch_diffusion_v<D, Q, NBmax, false>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
+        case 2: launch_diffusion_v<D, Q, NBhalf, true>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
+        case 3: launch_diffusion_v<D, Q, NBhalf, false>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
+        default: launch_diffusion_v<D, Q, NBmax, true>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
+    }
 }
 template <int D, int Q>
 void launch_mass(int64_t ne, const double *B, const double *pa, const int *gmap,
@@ -527,14 +548,35 @@ void launch_mass_diag(int64_t ne, const double *B, const double *pa, double *ye,
 
 bool apply_supported(int p, int q1d) { return p >= 1 && p <= 8 && (q1d == p + 1 || q1d == p + 2); }
 
+namespace {
+// Selected diffusion variant per (D, Q); filled by the autotuner.
+int g_diff_variant[11][11] = {};
+}  // namespace
+
+int diffusion_variant_count() { return kDiffVariants; }
+void set_diffusion_variant(int p, int q1d, int v) {
+    if (p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10) g_diff_variant[p + 1][q1d] = v;
+}
+int get_diffusion_variant(int p, int q1d) {
+    return (p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10) ? g_diff_variant[p + 1][q1d] : 0;
+}
+
 void diffusion_apply(int p, int q1d, int64_t ne, const double *B, const double *G,
                      const double *pa, const int *gmap, const uint8_t *mask, const double *x,
                      double *ye, bool accumulate, hipStream_t s) {
+    diffusion_apply_variant(get_diffusion_variant(p, q1d), p, q1d, ne, B, G, pa, gmap, mask, x, ye,
+                            accumulate, s);
+}
+
+void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *B,
+                             const double *G, const double *pa, const int *gmap,
+                             const uint8_t *mask, const double *x, double *ye, bool accumulate,
+                             hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
-        launch_diffusion<DD, QQ>(ne, B, G, pa, gmap, mask, x, ye, accumulate, s);          \
+        launch_diffusion<DD, QQ>(variant, ne, B, G, pa, gmap, mask, x, ye, accumulate, s); \
         H1PA_HIP(hipGetLastError());                                                       \
         return;                                                                            \
     }

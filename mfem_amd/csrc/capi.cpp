@@ -2,6 +2,7 @@
 //
 // Every entry point catches h1pa::Error (and std::exception) and turns it into
 // a status code plus a thread-local message; nothing throws across the ABI.
+#include <cstdlib>
 #include <cstring>
 #include <exception>
 #include <new>
@@ -369,6 +370,54 @@ int h1pa_form_destroy(h1pa_form *form) {
     API_END
 }
 
+}  // extern "C"
+
+namespace {
+// Per-(p, q1d) choice among the diffusion-apply variants, made once per
+// process on the first form large enough to time reliably.  All variants
+// compute bit-identical results.  H1PA_DIFF_VARIANT=<v> forces a variant,
+// H1PA_AUTOTUNE=0 keeps the default.
+bool g_tuned[10][11] = {};
+
+void autotune_diffusion(Form &f, hipStream_t s) {
+    Space &sp = *f.space;
+    const int p = sp.p, Q = f.q1d;
+    if (const char *v = std::getenv("H1PA_DIFF_VARIANT")) {
+        kern::set_diffusion_variant(p, Q, std::atoi(v));
+        return;
+    }
+    const char *at = std::getenv("H1PA_AUTOTUNE");
+    if ((at && at[0] == '0') || g_tuned[p][Q] || sp.ne < 20000) return;
+    DevBuf<double> x(sp.ndofs);
+    H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * sp.ndofs, s));
+    hipEvent_t e0, e1;
+    H1PA_HIP(hipEventCreate(&e0));
+    H1PA_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v = 0; v < kern::diffusion_variant_count(); ++v) {
+        for (int w = 0; w < 2; ++w)
+            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.B.p, f.G.p, f.pa_diff.p,
+                                          sp.restr.gather_map.p, nullptr, x.p, f.ework0.p, false, s);
+        H1PA_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 5; ++r)
+            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.B.p, f.G.p, f.pa_diff.p,
+                                          sp.restr.gather_map.p, nullptr, x.p, f.ework0.p, false, s);
+        H1PA_HIP(hipEventRecord(e1, s));
+        H1PA_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        H1PA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) { best_ms = ms; best = v; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    kern::set_diffusion_variant(p, Q, best);
+    g_tuned[p][Q] = true;
+}
+}  // namespace
+
+extern "C" {
+
 int h1pa_form_assemble(h1pa_form *form, void *stream) {
     API_BEGIN
     REQUIRE(form, "form: null");
@@ -391,6 +440,7 @@ int h1pa_form_assemble(h1pa_form *form, void *stream) {
     f.ework0.alloc(sp.ne * sp.dof);
     H1PA_HIP(hipStreamSynchronize(s));
     f.assembled = true;
+    if (f.desc.integrators & H1PA_INTEG_DIFFUSION) autotune_diffusion(f, s);
     API_END
 }
 
@@ -526,6 +576,9 @@ int h1pa_form_info(const h1pa_form *form, int64_t *info) {
     info[0] = form->f.q1d;
     info[1] = form->f.desc.integrators;
     info[2] = form->f.space->ndofs;
+    info[3] = (form->f.desc.integrators & H1PA_INTEG_DIFFUSION)
+                  ? kern::get_diffusion_variant(form->f.space->p, form->f.q1d)
+                  : -1;
     API_END
 }
 

@@ -172,6 +172,14 @@ void diffusion_diagonal(int p, int q1d, int64_t ne, const double *B, const doubl
 void mass_diagonal(int p, int q1d, int64_t ne, const double *B, const double *pa, double *ye,
                    bool accumulate, hipStream_t s);
 bool apply_supported(int p, int q1d);
+// Speed variants of the diffusion apply (bit-identical results).
+int diffusion_variant_count();
+void set_diffusion_variant(int p, int q1d, int variant);
+int get_diffusion_variant(int p, int q1d);
+void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *B,
+                             const double *G, const double *pa, const int *gmap,
+                             const uint8_t *mask, const double *x, double *ye, bool accumulate,
+                             hipStream_t s);
 void jacobi_dinv(int64_t n, const double *diag, const uint8_t *mask, double *dinv, hipStream_t s);
 void form_x(int64_t n, const uint8_t *mask, const double *x, double *X, hipStream_t s);
 void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, const double *z,
