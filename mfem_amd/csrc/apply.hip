@@ -14,12 +14,15 @@
 //  * z-lines: thread (qx,qy) keeps the z-line in registers, so the z
 //    contraction, the pointwise 3x3 product and the backward z contraction
 //    never touch LDS;
-//  * the PA data of a thread's (qx,qy) column is loaded at kernel entry, so
-//    its HBM latency overlaps the x/y stages;
-//  * B and G are read with uniform addresses (scalar loads, SGPR operands of
-//    v_fma_f64);
+//  * B and G enter the FMAs as scalar (SGPR) operands.  The contracted index
+//    is unrolled, the output-line loop is not: each iteration needs one row
+//    (or column) of B and G -- at most 2*10 doubles, loaded with wide scalar
+//    loads from a contiguous copy -- so the SGPR file never spills;
 //  * one LDS buffer of max(2 D^2 Q, 3 D Q^2) doubles per element, reused
 //    across the stages with a barrier between the reads and the writes.
+//
+// basis layout (4*Q*D doubles): [ B(q,d) q-fastest | G q-fastest |
+//                                 B d-fastest      | G d-fastest ].
 #include "internal.hpp"
 
 namespace h1pa {
@@ -47,35 +50,26 @@ __device__ __forceinline__ double load_x(const int *__restrict__ gmap,
     return g >= 0 ? v : -v;
 }
 
-template <int D, int Q, int NB, bool PF>
+// U: unroll factor of the output-line loops (1 or 2).
+template <int D, int Q, int NB, int U>
 __global__ void __launch_bounds__(NB * Q * Q)
-k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict__ Gm,
-                  const double *__restrict__ pa, const int *__restrict__ gmap,
-                  const uint8_t *__restrict__ mask, const double *__restrict__ x,
-                  double *__restrict__ ye, int accumulate) {
+k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
+                  const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
+                  const double *__restrict__ x, double *__restrict__ ye, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DDQ = D * D * Q, DQQ = D * Q * Q;
     constexpr int BUF = (2 * DDQ > 3 * DQQ) ? 2 * DDQ : 3 * DQQ;
     __shared__ double smem[NB * BUF];
+    const double *__restrict__ Bq = basis;              // B[q + Q d]
+    const double *__restrict__ Gq = basis + Q * D;
+    const double *__restrict__ Bd = basis + 2 * Q * D;  // B[d + D q]
+    const double *__restrict__ Gd = basis + 3 * Q * D;
     const int le = threadIdx.x / QQ;
     const int t = threadIdx.x - le * QQ;
     const int a = t % Q, b = t / Q;
     const int64_t e = (int64_t)blockIdx.x * NB + le;
     const bool valid = e < ne;
     double *s = smem + le * BUF;
-#define B_(q, d) Bm[(q) + Q * (d)]
-#define G_(q, d) Gm[(q) + Q * (d)]
-
-    // PA data of column (a,b): 6 comps x Q points.  PF: all issued at entry
-    // so their latency overlaps the x/y stages; otherwise read per qz.
-    const double *pcol = pa + (int64_t)QQQ * 6 * (valid ? e : 0) + a + Q * b;
-    double Dq[6][PF ? Q : 1];
-    if (PF && valid) {
-#pragma unroll
-        for (int c = 0; c < 6; ++c)
-#pragma unroll
-            for (int qz = 0; qz < (PF ? Q : 1); ++qz) Dq[c][qz] = pcol[QQQ * c + QQ * qz];
-    }
 
     // ---- forward x: thread (dy, dz) = (a, b)
     if (valid && a < D && b < D) {
@@ -83,16 +77,18 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
         const int64_t base = (int64_t)DDD * e + D * (a + D * b);
 #pragma unroll
         for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
-#pragma unroll
+        double *out = s + Q * (a + D * b);
+#pragma unroll U
         for (int qx = 0; qx < Q; ++qx) {
+            const double *br = Bd + D * qx, *gr = Gd + D * qx;
             double bu = 0.0, gu = 0.0;
 #pragma unroll
             for (int dx = 0; dx < D; ++dx) {
-                bu += B_(qx, dx) * u[dx];
-                gu += G_(qx, dx) * u[dx];
+                bu = fma(br[dx], u[dx], bu);
+                gu = fma(gr[dx], u[dx], gu);
             }
-            s[qx + Q * (a + D * b)] = bu;
-            s[DDQ + qx + Q * (a + D * b)] = gu;
+            out[qx] = bu;
+            out[DDQ + qx] = gu;
         }
     }
     __syncthreads();
@@ -110,18 +106,20 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
         }
         __syncthreads();
         if (act) {
-#pragma unroll
+#pragma unroll U
             for (int qy = 0; qy < Q; ++qy) {
+                const double *br = Bd + D * qy, *gr = Gd + D * qy;
                 double bb = 0.0, gb = 0.0, bg = 0.0;
 #pragma unroll
                 for (int dy = 0; dy < D; ++dy) {
-                    bb += B_(qy, dy) * r0[dy];
-                    gb += G_(qy, dy) * r0[dy];
-                    bg += B_(qy, dy) * r1[dy];
+                    bb = fma(br[dy], r0[dy], bb);
+                    gb = fma(gr[dy], r0[dy], gb);
+                    bg = fma(br[dy], r1[dy], bg);
                 }
-                s[0 * DQQ + a + Q * (qy + Q * b)] = bg;  // -> d/dx
-                s[1 * DQQ + a + Q * (qy + Q * b)] = gb;  // -> d/dy
-                s[2 * DQQ + a + Q * (qy + Q * b)] = bb;  // -> d/dz
+                double *o = s + a + Q * (qy + Q * b);
+                o[0 * DQQ] = bg;  // -> d/dx
+                o[1 * DQQ] = gb;  // -> d/dy
+                o[2 * DQQ] = bb;  // -> d/dz
             }
         }
     }
@@ -133,44 +131,47 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
         if (valid) {
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) {
-                z0[dz] = s[0 * DQQ + a + Q * (b + Q * dz)];
-                z1[dz] = s[1 * DQQ + a + Q * (b + Q * dz)];
-                z2[dz] = s[2 * DQQ + a + Q * (b + Q * dz)];
+                const double *i = s + a + Q * (b + Q * dz);
+                z0[dz] = i[0 * DQQ];
+                z1[dz] = i[1 * DQQ];
+                z2[dz] = i[2 * DQQ];
             }
         }
         __syncthreads();
         if (valid) {
+            const double *pcol = pa + (int64_t)QQQ * 6 * e + a + Q * b;
             double T0[D], T1[D], T2[D];
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) T0[dz] = T1[dz] = T2[dz] = 0.0;
-#pragma unroll
+#pragma unroll U
             for (int qz = 0; qz < Q; ++qz) {
+                const double *br = Bd + D * qz, *gr = Gd + D * qz;
+                const double *pq = pcol + QQ * qz;
+                const double d11 = pq[0 * QQQ], d21 = pq[1 * QQQ], d31 = pq[2 * QQQ];
+                const double d22 = pq[3 * QQQ], d32 = pq[4 * QQQ], d33 = pq[5 * QQQ];
                 double g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) {
-                    g0 += B_(qz, dz) * z0[dz];
-                    g1 += B_(qz, dz) * z1[dz];
-                    g2 += G_(qz, dz) * z2[dz];
+                    g0 = fma(br[dz], z0[dz], g0);
+                    g1 = fma(br[dz], z1[dz], g1);
+                    g2 = fma(gr[dz], z2[dz], g2);
                 }
-                double d6[6];
-#pragma unroll
-                for (int c = 0; c < 6; ++c)
-                    d6[c] = PF ? Dq[c][PF ? qz : 0] : pcol[QQQ * c + QQ * qz];
-                const double O0 = d6[0] * g0 + d6[1] * g1 + d6[2] * g2;
-                const double O1 = d6[1] * g0 + d6[3] * g1 + d6[4] * g2;
-                const double O2 = d6[2] * g0 + d6[4] * g1 + d6[5] * g2;
+                const double O0 = fma(d11, g0, fma(d21, g1, d31 * g2));
+                const double O1 = fma(d21, g0, fma(d22, g1, d32 * g2));
+                const double O2 = fma(d31, g0, fma(d32, g1, d33 * g2));
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) {
-                    T0[dz] += B_(qz, dz) * O0;
-                    T1[dz] += B_(qz, dz) * O1;
-                    T2[dz] += G_(qz, dz) * O2;
+                    T0[dz] = fma(br[dz], O0, T0[dz]);
+                    T1[dz] = fma(br[dz], O1, T1[dz]);
+                    T2[dz] = fma(gr[dz], O2, T2[dz]);
                 }
             }
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) {
-                s[0 * DQQ + a + Q * (b + Q * dz)] = T0[dz];
-                s[1 * DQQ + a + Q * (b + Q * dz)] = T1[dz];
-                s[2 * DQQ + a + Q * (b + Q * dz)] = T2[dz];
+                double *o = s + a + Q * (b + Q * dz);
+                o[0 * DQQ] = T0[dz];
+                o[1 * DQQ] = T1[dz];
+                o[2 * DQQ] = T2[dz];
             }
         }
     }
@@ -183,20 +184,23 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
         if (act) {
 #pragma unroll
             for (int qy = 0; qy < Q; ++qy) {
-                t0[qy] = s[0 * DQQ + a + Q * (qy + Q * b)];
-                t1[qy] = s[1 * DQQ + a + Q * (qy + Q * b)];
-                t2[qy] = s[2 * DQQ + a + Q * (qy + Q * b)];
+                const double *i = s + a + Q * (qy + Q * b);
+                t0[qy] = i[0 * DQQ];
+                t1[qy] = i[1 * DQQ];
+                t2[qy] = i[2 * DQQ];
             }
         }
         __syncthreads();
         if (act) {
-#pragma unroll
+#pragma unroll U
             for (int dy = 0; dy < D; ++dy) {
+                const double *bc = Bq + Q * dy, *gc = Gq + Q * dy;
                 double sa = 0.0, sb = 0.0;
 #pragma unroll
                 for (int qy = 0; qy < Q; ++qy) {
-                    sa += B_(qy, dy) * t0[qy];
-                    sb += G_(qy, dy) * t1[qy] + B_(qy, dy) * t2[qy];
+                    sa = fma(bc[qy], t0[qy], sa);
+                    sb = fma(gc[qy], t1[qy], sb);
+                    sb = fma(bc[qy], t2[qy], sb);
                 }
                 s[a + Q * (dy + D * b)] = sa;
                 s[DDQ + a + Q * (dy + D * b)] = sb;
@@ -208,59 +212,57 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ Bm, const double *__res
     // ---- backward x: thread (dy, dz) = (a, b)
     if (valid && a < D && b < D) {
         double sa[Q], sb[Q];
+        const double *in = s + Q * (a + D * b);
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) {
-            sa[qx] = s[qx + Q * (a + D * b)];
-            sb[qx] = s[DDQ + qx + Q * (a + D * b)];
+            sa[qx] = in[qx];
+            sb[qx] = in[DDQ + qx];
         }
         double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
+#pragma unroll U
         for (int dx = 0; dx < D; ++dx) {
+            const double *bc = Bq + Q * dx, *gc = Gq + Q * dx;
             double v = 0.0;
 #pragma unroll
-            for (int qx = 0; qx < Q; ++qx) v += G_(qx, dx) * sa[qx] + B_(qx, dx) * sb[qx];
+            for (int qx = 0; qx < Q; ++qx) {
+                v = fma(gc[qx], sa[qx], v);
+                v = fma(bc[qx], sb[qx], v);
+            }
             if (accumulate) yp[dx] += v;
             else yp[dx] = v;
         }
     }
-#undef B_
-#undef G_
 }
 
-template <int D, int Q>
-__global__ void __launch_bounds__(nb_for(Q, D * Q * Q) * Q * Q)
-k_mass_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict__ pa,
+template <int D, int Q, int NB>
+__global__ void __launch_bounds__(NB * Q * Q)
+k_mass_apply(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
              const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
              const double *__restrict__ x, double *__restrict__ ye, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
-    constexpr int DDQ = D * D * Q, DQQ = D * Q * Q;
+    constexpr int DQQ = D * Q * Q;
     constexpr int BUF = DQQ;
-    constexpr int NB = nb_for(Q, DQQ);
     __shared__ double smem[NB * BUF];
+    const double *__restrict__ Bq = basis;
+    const double *__restrict__ Bd = basis + 2 * Q * D;
     const int le = threadIdx.x / QQ;
     const int t = threadIdx.x - le * QQ;
     const int a = t % Q, b = t / Q;
     const int64_t e = (int64_t)blockIdx.x * NB + le;
     const bool valid = e < ne;
     double *s = smem + le * BUF;
-#define B_(q, d) Bm[(q) + Q * (d)]
-    double vq[Q];
-    if (valid) {
-        const double *p = pa + (int64_t)QQQ * e + a + Q * b;
-#pragma unroll
-        for (int qz = 0; qz < Q; ++qz) vq[qz] = p[QQ * qz];
-    }
     if (valid && a < D && b < D) {
         double u[D];
         const int64_t base = (int64_t)DDD * e + D * (a + D * b);
 #pragma unroll
         for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
-#pragma unroll
+#pragma unroll 1
         for (int qx = 0; qx < Q; ++qx) {
-            double bu = 0.0;
+            const double *br = Bd + D * qx;
+            double v = 0.0;
 #pragma unroll
-            for (int dx = 0; dx < D; ++dx) bu += B_(qx, dx) * u[dx];
-            s[qx + Q * (a + D * b)] = bu;
+            for (int dx = 0; dx < D; ++dx) v = fma(br[dx], u[dx], v);
+            s[qx + Q * (a + D * b)] = v;
         }
     }
     __syncthreads();
@@ -272,11 +274,12 @@ k_mass_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict
             for (int dy = 0; dy < D; ++dy) r[dy] = s[a + Q * (dy + D * b)];
         __syncthreads();
         if (act)
-#pragma unroll
+#pragma unroll 1
             for (int qy = 0; qy < Q; ++qy) {
+                const double *br = Bd + D * qy;
                 double v = 0.0;
 #pragma unroll
-                for (int dy = 0; dy < D; ++dy) v += B_(qy, dy) * r[dy];
+                for (int dy = 0; dy < D; ++dy) v = fma(br[dy], r[dy], v);
                 s[a + Q * (qy + Q * b)] = v;
             }
     }
@@ -288,17 +291,19 @@ k_mass_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict
             for (int dz = 0; dz < D; ++dz) z[dz] = s[a + Q * (b + Q * dz)];
         __syncthreads();
         if (valid) {
+            const double *pcol = pa + (int64_t)QQQ * e + a + Q * b;
             double T[D];
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) T[dz] = 0.0;
-#pragma unroll
+#pragma unroll 1
             for (int qz = 0; qz < Q; ++qz) {
+                const double *br = Bd + D * qz;
                 double v = 0.0;
 #pragma unroll
-                for (int dz = 0; dz < D; ++dz) v += B_(qz, dz) * z[dz];
-                v *= vq[qz];
+                for (int dz = 0; dz < D; ++dz) v = fma(br[dz], z[dz], v);
+                v *= pcol[QQ * qz];
 #pragma unroll
-                for (int dz = 0; dz < D; ++dz) T[dz] += B_(qz, dz) * v;
+                for (int dz = 0; dz < D; ++dz) T[dz] = fma(br[dz], v, T[dz]);
             }
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) s[a + Q * (b + Q * dz)] = T[dz];
@@ -313,11 +318,12 @@ k_mass_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict
             for (int qy = 0; qy < Q; ++qy) tq[qy] = s[a + Q * (qy + Q * b)];
         __syncthreads();
         if (act)
-#pragma unroll
+#pragma unroll 1
             for (int dy = 0; dy < D; ++dy) {
+                const double *bc = Bq + Q * dy;
                 double v = 0.0;
 #pragma unroll
-                for (int qy = 0; qy < Q; ++qy) v += B_(qy, dy) * tq[qy];
+                for (int qy = 0; qy < Q; ++qy) v = fma(bc[qy], tq[qy], v);
                 s[a + Q * (dy + D * b)] = v;
             }
     }
@@ -327,17 +333,16 @@ k_mass_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) sx[qx] = s[qx + Q * (a + D * b)];
         double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
+#pragma unroll 1
         for (int dx = 0; dx < D; ++dx) {
+            const double *bc = Bq + Q * dx;
             double v = 0.0;
 #pragma unroll
-            for (int qx = 0; qx < Q; ++qx) v += B_(qx, dx) * sx[qx];
+            for (int qx = 0; qx < Q; ++qx) v = fma(bc[qx], sx[qx], v);
             if (accumulate) yp[dx] += v;
             else yp[dx] = v;
         }
     }
-#undef B_
-    (void)DDQ;
 }
 
 // Diagonal of the element diffusion matrices (PADiffusionDiagonal3D,
@@ -345,15 +350,17 @@ k_mass_apply(int64_t ne, const double *__restrict__ Bm, const double *__restrict
 // the 1D factor in x: GG (O11), BB (O22, O33, 2 O23), GB (2 O12, 2 O13).
 template <int D, int Q>
 __global__ void __launch_bounds__(Q * Q)
-k_diffusion_diag(int64_t ne, const double *__restrict__ Bm, const double *__restrict__ Gm,
-                 const double *__restrict__ pa, double *__restrict__ ye, int accumulate) {
+k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
+                 double *__restrict__ ye, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DQQ = D * Q * Q, DDQ = D * D * Q;
     __shared__ double s[6 * DQQ];
+    const double *__restrict__ Bq = basis;
+    const double *__restrict__ Gq = basis + Q * D;
+    const double *__restrict__ Bd = basis + 2 * Q * D;
+    const double *__restrict__ Gd = basis + 3 * Q * D;
     const int a = threadIdx.x % Q, b = threadIdx.x / Q;
     const int64_t e = blockIdx.x;
-#define B_(q, d) Bm[(q) + Q * (d)]
-#define G_(q, d) Gm[(q) + Q * (d)]
     {   // z: thread (qx, qy)
         const double *p = pa + (int64_t)QQQ * 6 * e + a + Q * b;
         double acc[6][D];
@@ -361,21 +368,21 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ Bm, const double *__rest
         for (int k = 0; k < 6; ++k)
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) acc[k][dz] = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int qz = 0; qz < Q; ++qz) {
+            const double *br = Bd + D * qz, *gr = Gd + D * qz;
             const double O11 = p[QQQ * 0 + QQ * qz], O12 = p[QQQ * 1 + QQ * qz],
                          O13 = p[QQQ * 2 + QQ * qz], O22 = p[QQQ * 3 + QQ * qz],
                          O23 = p[QQQ * 4 + QQ * qz], O33 = p[QQQ * 5 + QQ * qz];
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) {
-                const double bb = B_(qz, dz) * B_(qz, dz), gg = G_(qz, dz) * G_(qz, dz),
-                             gb = G_(qz, dz) * B_(qz, dz);
-                acc[0][dz] += bb * O11;  // -> x: GG
-                acc[1][dz] += bb * O22;  // -> x: BB, y: GG
-                acc[2][dz] += gg * O33;  // -> x: BB, y: BB
-                acc[3][dz] += gb * O23;  // -> x: BB, y: GB
-                acc[4][dz] += bb * O12;  // -> x: GB, y: GB
-                acc[5][dz] += gb * O13;  // -> x: GB, y: BB
+                const double bb = br[dz] * br[dz], gg = gr[dz] * gr[dz], gb = gr[dz] * br[dz];
+                acc[0][dz] = fma(bb, O11, acc[0][dz]);  // -> x: GG
+                acc[1][dz] = fma(bb, O22, acc[1][dz]);  // -> x: BB, y: GG
+                acc[2][dz] = fma(gg, O33, acc[2][dz]);  // -> x: BB, y: BB
+                acc[3][dz] = fma(gb, O23, acc[3][dz]);  // -> x: BB, y: GB
+                acc[4][dz] = fma(bb, O12, acc[4][dz]);  // -> x: GB, y: GB
+                acc[5][dz] = fma(gb, O13, acc[5][dz]);  // -> x: GB, y: BB
             }
         }
 #pragma unroll
@@ -389,18 +396,18 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ Bm, const double *__rest
     if (acty) {   // y: thread (qx, dz)
 #pragma unroll
         for (int dy = 0; dy < D; ++dy) ygg[dy] = ybb[dy] = ygb[dy] = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int qy = 0; qy < Q; ++qy) {
+            const double *br = Bd + D * qy, *gr = Gd + D * qy;
             const int id = a + Q * (qy + Q * b);
             const double A11 = s[0 * DQQ + id], A22 = s[1 * DQQ + id], A33 = s[2 * DQQ + id],
                          A23 = s[3 * DQQ + id], A12 = s[4 * DQQ + id], A13 = s[5 * DQQ + id];
 #pragma unroll
             for (int dy = 0; dy < D; ++dy) {
-                const double bb = B_(qy, dy) * B_(qy, dy), gg = G_(qy, dy) * G_(qy, dy),
-                             gb = G_(qy, dy) * B_(qy, dy);
-                ygg[dy] += bb * A11;
-                ybb[dy] += gg * A22 + bb * A33 + 2.0 * gb * A23;
-                ygb[dy] += 2.0 * gb * A12 + 2.0 * bb * A13;
+                const double bb = br[dy] * br[dy], gg = gr[dy] * gr[dy], gb = gr[dy] * br[dy];
+                ygg[dy] = fma(bb, A11, ygg[dy]);
+                ybb[dy] = fma(gg, A22, fma(bb, A33, fma(2.0 * gb, A23, ybb[dy])));
+                ygb[dy] = fma(2.0 * gb, A12, fma(2.0 * bb, A13, ygb[dy]));
             }
         }
     }
@@ -416,44 +423,45 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ Bm, const double *__rest
     __syncthreads();
     if (a < D && b < D) {   // x: thread (dy, dz)
         double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
+#pragma unroll 1
         for (int dx = 0; dx < D; ++dx) {
+            const double *bc = Bq + Q * dx, *gc = Gq + Q * dx;
             double v = 0.0;
 #pragma unroll
             for (int qx = 0; qx < Q; ++qx) {
                 const int id = qx + Q * (a + D * b);
-                v += G_(qx, dx) * G_(qx, dx) * s[0 * DDQ + id] +
-                     B_(qx, dx) * B_(qx, dx) * s[1 * DDQ + id] +
-                     G_(qx, dx) * B_(qx, dx) * s[2 * DDQ + id];
+                v = fma(gc[qx] * gc[qx], s[0 * DDQ + id], v);
+                v = fma(bc[qx] * bc[qx], s[1 * DDQ + id], v);
+                v = fma(gc[qx] * bc[qx], s[2 * DDQ + id], v);
             }
             if (accumulate) yp[dx] += v;
             else yp[dx] = v;
         }
     }
-#undef B_
-#undef G_
 }
 
 template <int D, int Q>
 __global__ void __launch_bounds__(Q * Q)
-k_mass_diag(int64_t ne, const double *__restrict__ Bm, const double *__restrict__ pa,
+k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
             double *__restrict__ ye, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
-    constexpr int DQQ = D * Q * Q, DDQ = D * D * Q;
+    constexpr int DQQ = D * Q * Q;
     __shared__ double s[DQQ];
+    const double *__restrict__ Bq = basis;
+    const double *__restrict__ Bd = basis + 2 * Q * D;
     const int a = threadIdx.x % Q, b = threadIdx.x / Q;
     const int64_t e = blockIdx.x;
-#define B_(q, d) Bm[(q) + Q * (d)]
     {
         const double *p = pa + (int64_t)QQQ * e + a + Q * b;
         double acc[D];
 #pragma unroll
         for (int dz = 0; dz < D; ++dz) acc[dz] = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int qz = 0; qz < Q; ++qz) {
+            const double *br = Bd + D * qz;
             const double v = p[QQ * qz];
 #pragma unroll
-            for (int dz = 0; dz < D; ++dz) acc[dz] += B_(qz, dz) * B_(qz, dz) * v;
+            for (int dz = 0; dz < D; ++dz) acc[dz] = fma(br[dz] * br[dz], v, acc[dz]);
         }
 #pragma unroll
         for (int dz = 0; dz < D; ++dz) s[a + Q * (b + Q * dz)] = acc[dz];
@@ -464,11 +472,12 @@ k_mass_diag(int64_t ne, const double *__restrict__ Bm, const double *__restrict_
     if (acty) {
 #pragma unroll
         for (int dy = 0; dy < D; ++dy) y2[dy] = 0.0;
-#pragma unroll
+#pragma unroll 1
         for (int qy = 0; qy < Q; ++qy) {
+            const double *br = Bd + D * qy;
             const double v = s[a + Q * (qy + Q * b)];
 #pragma unroll
-            for (int dy = 0; dy < D; ++dy) y2[dy] += B_(qy, dy) * B_(qy, dy) * v;
+            for (int dy = 0; dy < D; ++dy) y2[dy] = fma(br[dy] * br[dy], v, y2[dy]);
         }
     }
     __syncthreads();
@@ -478,65 +487,64 @@ k_mass_diag(int64_t ne, const double *__restrict__ Bm, const double *__restrict_
     __syncthreads();
     if (a < D && b < D) {
         double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
+#pragma unroll 1
         for (int dx = 0; dx < D; ++dx) {
+            const double *bc = Bq + Q * dx;
             double v = 0.0;
 #pragma unroll
-            for (int qx = 0; qx < Q; ++qx) v += B_(qx, dx) * B_(qx, dx) * s[qx + Q * (a + D * b)];
+            for (int qx = 0; qx < Q; ++qx) v = fma(bc[qx] * bc[qx], s[qx + Q * (a + D * b)], v);
             if (accumulate) yp[dx] += v;
             else yp[dx] = v;
         }
     }
-#undef B_
-    (void)DDQ;
 }
 
-// ---- dispatch over (D, Q) -----------------------------------------------------
-template <int D, int Q, int NB, bool PF>
-void launch_diffusion_v(int64_t ne, const double *B, const double *G, const double *pa,
-                        const int *gmap, const uint8_t *mask, const double *x, double *ye,
-                        bool acc, hipStream_t s) {
+// ---- dispatch over (D, Q) and variants -------------------------------------
+template <int D, int Q, int NB, int U>
+void launch_diffusion_v(int64_t ne, const double *basis, const double *pa, const int *gmap,
+                        const uint8_t *mask, const double *x, double *ye, bool acc,
+                        hipStream_t s) {
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
-    hipLaunchKernelGGL((k_diffusion_apply<D, Q, NB, PF>), dim3(grid), dim3(NB * Q * Q), 0, s, ne, B,
-                       G, pa, gmap, mask, x, ye, acc ? 1 : 0);
+    hipLaunchKernelGGL((k_diffusion_apply<D, Q, NB, U>), dim3(grid), dim3(NB * Q * Q), 0, s, ne,
+                       basis, pa, gmap, mask, x, ye, acc ? 1 : 0);
 }
 
-// Variants of the diffusion apply: (elements per block, prefetch).  Every
-// variant performs the same arithmetic per element in the same order, so
-// they give bit-identical results and only differ in speed.
+// Variants of the diffusion apply: (elements per block, line-loop unroll).
+// Every variant performs the same arithmetic per element in the same order,
+// so they give bit-identical results and differ only in speed.
 constexpr int kDiffVariants = 4;
 template <int D, int Q>
-void launch_diffusion(int variant, int64_t ne, const double *B, const double *G,
-                      const double *pa, const int *gmap, const uint8_t *mask, const double *x,
-                      double *ye, bool acc, hipStream_t s) {
+void launch_diffusion(int variant, int64_t ne, const double *basis, const double *pa,
+                      const int *gmap, const uint8_t *mask, const double *x, double *ye, bool acc,
+                      hipStream_t s) {
     constexpr int NBmax = nb_for(Q, 3 * D * Q * Q);
     constexpr int NBhalf = NBmax / 2 > 0 ? NBmax / 2 : 1;
     switch (variant) {
-        case 1: launch_diffusion_v<D, Q, NBmax, false>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
-        case 2: launch_diffusion_v<D, Q, NBhalf, true>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
-        case 3: launch_diffusion_v<D, Q, NBhalf, false>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
-        default: launch_diffusion_v<D, Q, NBmax, true>(ne, B, G, pa, gmap, mask, x, ye, acc, s); break;
+        case 1: launch_diffusion_v<D, Q, NBmax, 2>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
+        case 2: launch_diffusion_v<D, Q, NBhalf, 1>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
+        case 3: launch_diffusion_v<D, Q, NBhalf, 2>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
+        default: launch_diffusion_v<D, Q, NBmax, 1>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
     }
 }
 template <int D, int Q>
-void launch_mass(int64_t ne, const double *B, const double *pa, const int *gmap,
+void launch_mass(int64_t ne, const double *basis, const double *pa, const int *gmap,
                  const uint8_t *mask, const double *x, double *ye, bool acc, hipStream_t s) {
     constexpr int NB = nb_for(Q, D * Q * Q);
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
-    hipLaunchKernelGGL((k_mass_apply<D, Q>), dim3(grid), dim3(NB * Q * Q), 0, s, ne, B, pa, gmap,
-                       mask, x, ye, acc ? 1 : 0);
+    hipLaunchKernelGGL((k_mass_apply<D, Q, NB>), dim3(grid), dim3(NB * Q * Q), 0, s, ne, basis, pa,
+                       gmap, mask, x, ye, acc ? 1 : 0);
 }
 template <int D, int Q>
-void launch_diff_diag(int64_t ne, const double *B, const double *G, const double *pa, double *ye,
-                      bool acc, hipStream_t s) {
-    hipLaunchKernelGGL((k_diffusion_diag<D, Q>), dim3((unsigned)ne), dim3(Q * Q), 0, s, ne, B, G,
+void launch_diff_diag(int64_t ne, const double *basis, const double *pa, double *ye, bool acc,
+                      hipStream_t s) {
+    hipLaunchKernelGGL((k_diffusion_diag<D, Q>), dim3((unsigned)ne), dim3(Q * Q), 0, s, ne, basis,
                        pa, ye, acc ? 1 : 0);
 }
 template <int D, int Q>
-void launch_mass_diag(int64_t ne, const double *B, const double *pa, double *ye, bool acc,
+void launch_mass_diag(int64_t ne, const double *basis, const double *pa, double *ye, bool acc,
                       hipStream_t s) {
-    hipLaunchKernelGGL((k_mass_diag<D, Q>), dim3((unsigned)ne), dim3(Q * Q), 0, s, ne, B, pa, ye,
-                       acc ? 1 : 0);
+    hipLaunchKernelGGL((k_mass_diag<D, Q>), dim3((unsigned)ne), dim3(Q * Q), 0, s, ne, basis, pa,
+                       ye, acc ? 1 : 0);
 }
 
 // (D, Q) pairs compiled: Q = D (q = p+1) and Q = D + 1 (q = p+2, MFEM default).
@@ -544,14 +552,12 @@ void launch_mass_diag(int64_t ne, const double *B, const double *pa, double *ye,
     X(2, 2) X(2, 3) X(3, 3) X(3, 4) X(4, 4) X(4, 5) X(5, 5) X(5, 6) X(6, 6) X(6, 7) \
     X(7, 7) X(7, 8) X(8, 8) X(8, 9) X(9, 9) X(9, 10)
 
+// Selected diffusion variant per (D, Q); filled by the autotuner.
+int g_diff_variant[11][11] = {};
+
 }  // namespace
 
 bool apply_supported(int p, int q1d) { return p >= 1 && p <= 8 && (q1d == p + 1 || q1d == p + 2); }
-
-namespace {
-// Selected diffusion variant per (D, Q); filled by the autotuner.
-int g_diff_variant[11][11] = {};
-}  // namespace
 
 int diffusion_variant_count() { return kDiffVariants; }
 void set_diffusion_variant(int p, int q1d, int v) {
@@ -561,37 +567,37 @@ int get_diffusion_variant(int p, int q1d) {
     return (p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10) ? g_diff_variant[p + 1][q1d] : 0;
 }
 
-void diffusion_apply(int p, int q1d, int64_t ne, const double *B, const double *G,
-                     const double *pa, const int *gmap, const uint8_t *mask, const double *x,
-                     double *ye, bool accumulate, hipStream_t s) {
-    diffusion_apply_variant(get_diffusion_variant(p, q1d), p, q1d, ne, B, G, pa, gmap, mask, x, ye,
-                            accumulate, s);
+void diffusion_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
+                     const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                     bool accumulate, hipStream_t s) {
+    diffusion_apply_variant(get_diffusion_variant(p, q1d), p, q1d, ne, basis, pa, gmap, mask, x,
+                            ye, accumulate, s);
 }
 
-void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *B,
-                             const double *G, const double *pa, const int *gmap,
-                             const uint8_t *mask, const double *x, double *ye, bool accumulate,
-                             hipStream_t s) {
+void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *basis,
+                             const double *pa, const int *gmap, const uint8_t *mask,
+                             const double *x, double *ye, bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
-#define X(DD, QQ)                                                                          \
-    if (D == DD && q1d == QQ) {                                                            \
-        launch_diffusion<DD, QQ>(variant, ne, B, G, pa, gmap, mask, x, ye, accumulate, s); \
-        H1PA_HIP(hipGetLastError());                                                       \
-        return;                                                                            \
+#define X(DD, QQ)                                                                           \
+    if (D == DD && q1d == QQ) {                                                             \
+        launch_diffusion<DD, QQ>(variant, ne, basis, pa, gmap, mask, x, ye, accumulate, s); \
+        H1PA_HIP(hipGetLastError());                                                        \
+        return;                                                                             \
     }
     H1PA_FOR_EACH_DQ(X)
 #undef X
     throw Error{H1PA_EUNSUP, "diffusion apply: unsupported (p, q1d)"};
 }
 
-void mass_apply(int p, int q1d, int64_t ne, const double *B, const double *pa, const int *gmap,
-                const uint8_t *mask, const double *x, double *ye, bool accumulate, hipStream_t s) {
+void mass_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
+                const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
-        launch_mass<DD, QQ>(ne, B, pa, gmap, mask, x, ye, accumulate, s);                  \
+        launch_mass<DD, QQ>(ne, basis, pa, gmap, mask, x, ye, accumulate, s);              \
         H1PA_HIP(hipGetLastError());                                                       \
         return;                                                                            \
     }
@@ -600,13 +606,13 @@ void mass_apply(int p, int q1d, int64_t ne, const double *B, const double *pa, c
     throw Error{H1PA_EUNSUP, "mass apply: unsupported (p, q1d)"};
 }
 
-void diffusion_diagonal(int p, int q1d, int64_t ne, const double *B, const double *G,
-                        const double *pa, double *ye, bool accumulate, hipStream_t s) {
+void diffusion_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
+                        double *ye, bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
-        launch_diff_diag<DD, QQ>(ne, B, G, pa, ye, accumulate, s);                         \
+        launch_diff_diag<DD, QQ>(ne, basis, pa, ye, accumulate, s);                        \
         H1PA_HIP(hipGetLastError());                                                       \
         return;                                                                            \
     }
@@ -615,13 +621,13 @@ void diffusion_diagonal(int p, int q1d, int64_t ne, const double *B, const doubl
     throw Error{H1PA_EUNSUP, "diffusion diagonal: unsupported (p, q1d)"};
 }
 
-void mass_diagonal(int p, int q1d, int64_t ne, const double *B, const double *pa, double *ye,
+void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa, double *ye,
                    bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
-        launch_mass_diag<DD, QQ>(ne, B, pa, ye, accumulate, s);                            \
+        launch_mass_diag<DD, QQ>(ne, basis, pa, ye, accumulate, s);                        \
         H1PA_HIP(hipGetLastError());                                                       \
         return;                                                                            \
     }

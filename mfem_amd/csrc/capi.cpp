@@ -397,11 +397,11 @@ void autotune_diffusion(Form &f, hipStream_t s) {
     float best_ms = 1e30f;
     for (int v = 0; v < kern::diffusion_variant_count(); ++v) {
         for (int w = 0; w < 2; ++w)
-            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.B.p, f.G.p, f.pa_diff.p,
+            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
                                           sp.restr.gather_map.p, nullptr, x.p, f.ework0.p, false, s);
         H1PA_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < 5; ++r)
-            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.B.p, f.G.p, f.pa_diff.p,
+            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
                                           sp.restr.gather_map.p, nullptr, x.p, f.ework0.p, false, s);
         H1PA_HIP(hipEventRecord(e1, s));
         H1PA_HIP(hipEventSynchronize(e1));
@@ -429,8 +429,16 @@ int h1pa_form_assemble(h1pa_form *form, void *stream) {
     std::vector<double> xq(Q), wq(Q), B(Q * D), G(Q * D);
     gauss_legendre(Q, xq.data(), wq.data());
     lagrange_basis(sp.p, Q, xq.data(), B.data(), G.data());
-    f.B.upload(B.data(), B.size(), s);
-    f.G.upload(G.data(), G.size(), s);
+    // packed basis: B, G with q fastest, then their transposes (d fastest)
+    std::vector<double> pack(4 * Q * D);
+    for (int q = 0; q < Q; ++q)
+        for (int d = 0; d < D; ++d) {
+            pack[q + Q * d] = B[q + Q * d];
+            pack[Q * D + q + Q * d] = G[q + Q * d];
+            pack[2 * Q * D + d + D * q] = B[q + Q * d];
+            pack[3 * Q * D + d + D * q] = G[q + Q * d];
+        }
+    f.basis.upload(pack.data(), pack.size(), s);
     const int64_t nq = (int64_t)Q * Q * Q;
     if (f.desc.integrators & H1PA_INTEG_DIFFUSION) f.pa_diff.alloc(nq * 6 * sp.ne);
     else f.pa_diff.release();
@@ -459,10 +467,10 @@ void element_apply(Form &f, const int *gmap, const uint8_t *mask, const double *
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
     if (diff)
-        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.B.p, f.G.p, f.pa_diff.p, gmap, mask, x, ye,
+        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, gmap, mask, x, ye,
                               accumulate, s);
     if (mass)
-        kern::mass_apply(sp.p, f.q1d, sp.ne, f.B.p, f.pa_mass.p, gmap, mask, x, ye,
+        kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, gmap, mask, x, ye,
                          accumulate || diff, s);
 }
 
@@ -554,9 +562,9 @@ int h1pa_form_assemble_diagonal(h1pa_form *form, double *diag, void *stream) {
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
     if (diff)
-        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.G.p, f.pa_diff.p, f.ework0.p, false, s);
+        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
     if (mass)
-        kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.pa_mass.p, f.ework0.p, diff, s);
+        kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
                                   diag, 2, s);
     API_END
@@ -651,8 +659,8 @@ int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const doubl
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
     if (diff)
-        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.G.p, f.pa_diff.p, f.ework0.p, false, s);
-    if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.B.p, f.pa_mass.p, f.ework0.p, diff, s);
+        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
+    if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
                                   diag.p, 2, s);
     pcg_solve(f, mask, diag.p, b, x, *opts, *stats, s);

@@ -116,8 +116,7 @@ struct Form {
     Space *space = nullptr;
     h1pa_form_desc desc{};
     int q1d = 0;
-    DevBuf<double> B, G;       // [Q][D] MFEM layout B[q + Q*d]
-    DevBuf<double> Bt;         // packed B|G for kernels (see apply kernels)
+    DevBuf<double> basis;      // [B q-fastest | G | B d-fastest | G d-fastest]
     DevBuf<double> pa_diff;    // [Q^3][6][ne]
     DevBuf<double> pa_mass;    // [Q^3][ne]
     DevBuf<double> ework0, ework1;  // E-vector scratch
@@ -161,25 +160,25 @@ void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s);
 
 // E->E: ye = A_E xe (overwrite) or += ; x may be gathered from an L-vector
 // through gmap (gmap != nullptr) with constrained entries masked to zero.
-void diffusion_apply(int p, int q1d, int64_t ne, const double *B, const double *G,
-                     const double *pa, const int *gmap, const uint8_t *mask, const double *x,
-                     double *ye, bool accumulate, hipStream_t s);
-void mass_apply(int p, int q1d, int64_t ne, const double *B, const double *pa,
+// basis = [B q-fastest | G q-fastest | B d-fastest | G d-fastest], 4*Q*D doubles.
+void diffusion_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
+                     const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                     bool accumulate, hipStream_t s);
+void mass_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
                 const int *gmap, const uint8_t *mask, const double *x, double *ye,
                 bool accumulate, hipStream_t s);
-void diffusion_diagonal(int p, int q1d, int64_t ne, const double *B, const double *G,
-                        const double *pa, double *ye, bool accumulate, hipStream_t s);
-void mass_diagonal(int p, int q1d, int64_t ne, const double *B, const double *pa, double *ye,
+void diffusion_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
+                        double *ye, bool accumulate, hipStream_t s);
+void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa, double *ye,
                    bool accumulate, hipStream_t s);
 bool apply_supported(int p, int q1d);
 // Speed variants of the diffusion apply (bit-identical results).
 int diffusion_variant_count();
 void set_diffusion_variant(int p, int q1d, int variant);
 int get_diffusion_variant(int p, int q1d);
-void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *B,
-                             const double *G, const double *pa, const int *gmap,
-                             const uint8_t *mask, const double *x, double *ye, bool accumulate,
-                             hipStream_t s);
+void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *basis,
+                             const double *pa, const int *gmap, const uint8_t *mask,
+                             const double *x, double *ye, bool accumulate, hipStream_t s);
 void jacobi_dinv(int64_t n, const double *diag, const uint8_t *mask, double *dinv, hipStream_t s);
 void form_x(int64_t n, const uint8_t *mask, const double *x, double *X, hipStream_t s);
 void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, const double *z,

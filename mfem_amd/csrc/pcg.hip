@@ -249,10 +249,10 @@ static void pcg_element_apply(Form &f, const uint8_t *mask, const double *d, hip
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
     if (diff)
-        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.B.p, f.G.p, f.pa_diff.p, sp.restr.gather_map.p,
+        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, sp.restr.gather_map.p,
                               mask, d, f.ework0.p, false, s);
     if (mass)
-        kern::mass_apply(sp.p, f.q1d, sp.ne, f.B.p, f.pa_mass.p, sp.restr.gather_map.p, mask, d,
+        kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, sp.restr.gather_map.p, mask, d,
                          f.ework0.p, diff, s);
 }
 
