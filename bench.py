@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="also run p = 1..8 at --sweep-dofs")
     ap.add_argument("--sweep-dofs", type=float, default=4e6)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="run one 1 GiB device copy first (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -131,6 +133,12 @@ def main():
     import mfem_amd as M
     M.lib()
     torch.cuda.init()
+    if args.calib_copy:
+        src = torch.ones(128 * 1024 * 1024, dtype=torch.float64, device="cuda")  # 1 GiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     n = n_for_dofs(args.p, args.dofs)
     print(f"[bench] PA diffusion apply p={args.p} n={n}^3", file=sys.stderr, flush=True)

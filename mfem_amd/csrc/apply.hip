@@ -50,16 +50,41 @@ __device__ __forceinline__ double load_x(const int *__restrict__ gmap,
     return g >= 0 ? v : -v;
 }
 
+// Store the E-DOF (dx, a, b) of element e.  With out.yL != nullptr the
+// interior ("bubble") DOFs, which no other element touches, go straight into
+// the L-vector at yL[bb + e (D-2)^3 + lex interior index] (MFEM's bubble
+// block, fem/fespace.cpp:3518-3527); everything else goes to the E-vector, at
+// its lexicographic position or, with out.l2n, at its native position.
+template <int D>
+__device__ __forceinline__ void store_out(const EOut &out, int64_t e, int a, int b, int dx,
+                                          double v, int accumulate) {
+    constexpr int P = D - 1, NI = D - 2;
+    double *dst;
+    if (out.yL && a >= 1 && a < P && b >= 1 && b < P && dx >= 1 && dx < P) {
+        dst = out.yL + out.bb + e * (int64_t)(NI * NI * NI) + (dx - 1) + NI * ((a - 1) + NI * (b - 1));
+    } else {
+        const int lex = dx + D * (a + D * b);
+        dst = out.ye + (int64_t)(D * D * D) * e + (out.l2n ? out.l2n[lex] : lex);
+    }
+    if (accumulate) *dst += v;
+    else *dst = v;
+}
+
 // U: unroll factor of the output-line loops (1 or 2).
-template <int D, int Q, int NB, int U>
+// PF: the thread's PA-data column (6 x Q doubles) is loaded at kernel entry
+//     so its HBM latency overlaps the x/y stages; the z loop is then fully
+//     unrolled and takes its B/G rows from an LDS copy (the SGPR file cannot
+//     hold all of B and G).
+template <int D, int Q, int NB, int U, bool PF>
 __global__ void __launch_bounds__(NB * Q * Q)
 k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
                   const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
-                  const double *__restrict__ x, double *__restrict__ ye, int accumulate) {
+                  const double *__restrict__ x, const EOut out, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DDQ = D * D * Q, DQQ = D * Q * Q;
     constexpr int BUF = (2 * DDQ > 3 * DQQ) ? 2 * DDQ : 3 * DQQ;
     __shared__ double smem[NB * BUF];
+    __shared__ double sBG[PF ? 2 * Q * D : 1];
     const double *__restrict__ Bq = basis;              // B[q + Q d]
     const double *__restrict__ Gq = basis + Q * D;
     const double *__restrict__ Bd = basis + 2 * Q * D;  // B[d + D q]
@@ -70,6 +95,17 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
     const int64_t e = (int64_t)blockIdx.x * NB + le;
     const bool valid = e < ne;
     double *s = smem + le * BUF;
+    double Dq[6][PF ? Q : 1];
+    if constexpr (PF) {
+        for (int i = threadIdx.x; i < 2 * Q * D; i += NB * QQ) sBG[i] = Bd[i];  // Bd|Gd
+        if (valid) {
+            const double *pcol = pa + (int64_t)QQQ * 6 * e + a + Q * b;
+#pragma unroll
+            for (int c = 0; c < 6; ++c)
+#pragma unroll
+                for (int qz = 0; qz < Q; ++qz) Dq[c][qz] = pcol[QQQ * c + QQ * qz];
+        }
+    }
 
     // ---- forward x: thread (dy, dz) = (a, b)
     if (valid && a < D && b < D) {
@@ -138,7 +174,41 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
             }
         }
         __syncthreads();
-        if (valid) {
+        if (valid && PF) {
+            double T0[D], T1[D], T2[D];
+#pragma unroll
+            for (int dz = 0; dz < D; ++dz) T0[dz] = T1[dz] = T2[dz] = 0.0;
+#pragma unroll
+            for (int qz = 0; qz < Q; ++qz) {
+                const double *br = sBG + D * qz, *gr = sBG + Q * D + D * qz;
+                const double d11 = Dq[0][PF ? qz : 0], d21 = Dq[1][PF ? qz : 0],
+                             d31 = Dq[2][PF ? qz : 0], d22 = Dq[3][PF ? qz : 0],
+                             d32 = Dq[4][PF ? qz : 0], d33 = Dq[5][PF ? qz : 0];
+                double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+                for (int dz = 0; dz < D; ++dz) {
+                    g0 = fma(br[dz], z0[dz], g0);
+                    g1 = fma(br[dz], z1[dz], g1);
+                    g2 = fma(gr[dz], z2[dz], g2);
+                }
+                const double O0 = fma(d11, g0, fma(d21, g1, d31 * g2));
+                const double O1 = fma(d21, g0, fma(d22, g1, d32 * g2));
+                const double O2 = fma(d31, g0, fma(d32, g1, d33 * g2));
+#pragma unroll
+                for (int dz = 0; dz < D; ++dz) {
+                    T0[dz] = fma(br[dz], O0, T0[dz]);
+                    T1[dz] = fma(br[dz], O1, T1[dz]);
+                    T2[dz] = fma(gr[dz], O2, T2[dz]);
+                }
+            }
+#pragma unroll
+            for (int dz = 0; dz < D; ++dz) {
+                double *o = s + a + Q * (b + Q * dz);
+                o[0 * DQQ] = T0[dz];
+                o[1 * DQQ] = T1[dz];
+                o[2 * DQQ] = T2[dz];
+            }
+        } else if (valid) {
             const double *pcol = pa + (int64_t)QQQ * 6 * e + a + Q * b;
             double T0[D], T1[D], T2[D];
 #pragma unroll
@@ -218,7 +288,6 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
             sa[qx] = in[qx];
             sb[qx] = in[DDQ + qx];
         }
-        double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
 #pragma unroll U
         for (int dx = 0; dx < D; ++dx) {
             const double *bc = Bq + Q * dx, *gc = Gq + Q * dx;
@@ -228,8 +297,7 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
                 v = fma(gc[qx], sa[qx], v);
                 v = fma(bc[qx], sb[qx], v);
             }
-            if (accumulate) yp[dx] += v;
-            else yp[dx] = v;
+            store_out<D>(out, e, a, b, dx, v, accumulate);
         }
     }
 }
@@ -238,7 +306,7 @@ template <int D, int Q, int NB>
 __global__ void __launch_bounds__(NB * Q * Q)
 k_mass_apply(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
              const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
-             const double *__restrict__ x, double *__restrict__ ye, int accumulate) {
+             const double *__restrict__ x, const EOut out, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DQQ = D * Q * Q;
     constexpr int BUF = DQQ;
@@ -332,15 +400,13 @@ k_mass_apply(int64_t ne, const double *__restrict__ basis, const double *__restr
         double sx[Q];
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) sx[qx] = s[qx + Q * (a + D * b)];
-        double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
 #pragma unroll 1
         for (int dx = 0; dx < D; ++dx) {
             const double *bc = Bq + Q * dx;
             double v = 0.0;
 #pragma unroll
             for (int qx = 0; qx < Q; ++qx) v = fma(bc[qx], sx[qx], v);
-            if (accumulate) yp[dx] += v;
-            else yp[dx] = v;
+            store_out<D>(out, e, a, b, dx, v, accumulate);
         }
     }
 }
@@ -500,39 +566,43 @@ k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restri
 }
 
 // ---- dispatch over (D, Q) and variants -------------------------------------
-template <int D, int Q, int NB, int U>
+template <int D, int Q, int NB, int U, bool PF>
 void launch_diffusion_v(int64_t ne, const double *basis, const double *pa, const int *gmap,
-                        const uint8_t *mask, const double *x, double *ye, bool acc,
+                        const uint8_t *mask, const double *x, const EOut &out, bool acc,
                         hipStream_t s) {
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
-    hipLaunchKernelGGL((k_diffusion_apply<D, Q, NB, U>), dim3(grid), dim3(NB * Q * Q), 0, s, ne,
-                       basis, pa, gmap, mask, x, ye, acc ? 1 : 0);
+    hipLaunchKernelGGL((k_diffusion_apply<D, Q, NB, U, PF>), dim3(grid), dim3(NB * Q * Q), 0, s,
+                       ne, basis, pa, gmap, mask, x, out, acc ? 1 : 0);
 }
 
-// Variants of the diffusion apply: (elements per block, line-loop unroll).
-// Every variant performs the same arithmetic per element in the same order,
-// so they give bit-identical results and differ only in speed.
-constexpr int kDiffVariants = 4;
+// Variants of the diffusion apply: (elements per block, line-loop unroll,
+// PA-data prefetch).  Every variant performs the same arithmetic per element
+// in the same order, so they give bit-identical results and differ only in
+// speed.
+constexpr int kDiffVariants = 6;
 template <int D, int Q>
 void launch_diffusion(int variant, int64_t ne, const double *basis, const double *pa,
-                      const int *gmap, const uint8_t *mask, const double *x, double *ye, bool acc,
-                      hipStream_t s) {
+                      const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
+                      bool acc, hipStream_t s) {
     constexpr int NBmax = nb_for(Q, 3 * D * Q * Q);
     constexpr int NBhalf = NBmax / 2 > 0 ? NBmax / 2 : 1;
     switch (variant) {
-        case 1: launch_diffusion_v<D, Q, NBmax, 2>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
-        case 2: launch_diffusion_v<D, Q, NBhalf, 1>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
-        case 3: launch_diffusion_v<D, Q, NBhalf, 2>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
-        default: launch_diffusion_v<D, Q, NBmax, 1>(ne, basis, pa, gmap, mask, x, ye, acc, s); break;
+        case 1: launch_diffusion_v<D, Q, NBmax, 2, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 2: launch_diffusion_v<D, Q, NBhalf, 1, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 3: launch_diffusion_v<D, Q, NBhalf, 2, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 4: launch_diffusion_v<D, Q, NBmax, 1, true>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 5: launch_diffusion_v<D, Q, NBhalf, 1, true>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        default: launch_diffusion_v<D, Q, NBmax, 1, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
     }
 }
 template <int D, int Q>
 void launch_mass(int64_t ne, const double *basis, const double *pa, const int *gmap,
-                 const uint8_t *mask, const double *x, double *ye, bool acc, hipStream_t s) {
+                 const uint8_t *mask, const double *x, const EOut &out, bool acc,
+                 hipStream_t s) {
     constexpr int NB = nb_for(Q, D * Q * Q);
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
     hipLaunchKernelGGL((k_mass_apply<D, Q, NB>), dim3(grid), dim3(NB * Q * Q), 0, s, ne, basis, pa,
-                       gmap, mask, x, ye, acc ? 1 : 0);
+                       gmap, mask, x, out, acc ? 1 : 0);
 }
 template <int D, int Q>
 void launch_diff_diag(int64_t ne, const double *basis, const double *pa, double *ye, bool acc,
@@ -568,20 +638,20 @@ int get_diffusion_variant(int p, int q1d) {
 }
 
 void diffusion_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
-                     const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                     const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
                      bool accumulate, hipStream_t s) {
     diffusion_apply_variant(get_diffusion_variant(p, q1d), p, q1d, ne, basis, pa, gmap, mask, x,
-                            ye, accumulate, s);
+                            out, accumulate, s);
 }
 
 void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *basis,
                              const double *pa, const int *gmap, const uint8_t *mask,
-                             const double *x, double *ye, bool accumulate, hipStream_t s) {
+                             const double *x, const EOut &out, bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                           \
     if (D == DD && q1d == QQ) {                                                             \
-        launch_diffusion<DD, QQ>(variant, ne, basis, pa, gmap, mask, x, ye, accumulate, s); \
+        launch_diffusion<DD, QQ>(variant, ne, basis, pa, gmap, mask, x, out, accumulate, s);     \
         H1PA_HIP(hipGetLastError());                                                        \
         return;                                                                             \
     }
@@ -591,13 +661,13 @@ void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const doub
 }
 
 void mass_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
-                const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
                 bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
-        launch_mass<DD, QQ>(ne, basis, pa, gmap, mask, x, ye, accumulate, s);              \
+        launch_mass<DD, QQ>(ne, basis, pa, gmap, mask, x, out, accumulate, s);             \
         H1PA_HIP(hipGetLastError());                                                       \
         return;                                                                            \
     }

@@ -176,6 +176,21 @@ int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space) {
         s.ess_mask.alloc(s.ndofs);
         kern::boundary_mask(m, order, r.gather_map.p, s.ess_mask.p, s.ndofs, st);
         s.n_ess = kern::mask_to_list(s.ess_mask.p, s.ndofs, s.ess, st);
+        // vertex block scattered in first-contribution order; bubble block
+        // (interior DOFs) written directly by the element kernels
+        kern::first_contribution_order(s.ne * s.dof, r.gather_map.p, r.offsets.p, r.indices.p,
+                                       s.nvdofs, r.perm, st);
+        r.nperm = s.nvdofs;
+        r.bubble_begin = s.nvdofs + s.nedofs + s.nfdofs;
+        {
+            std::vector<int> dm(s.dof);
+            h1_dof_map(order, dm.data());  // lex -> native
+            r.lex2nat.upload(dm.data(), dm.size(), st);
+            r.indices_nat.alloc(s.ne * s.dof);
+            kern::native_indices(s.ne * s.dof, (int)s.dof, r.indices.p, r.lex2nat.p,
+                                 r.indices_nat.p, st);
+            H1PA_HIP(hipStreamSynchronize(st));
+        }
         auto *rh = new h1pa_restriction();
         rh->owned_by_space = true;
         sp->r = rh;
@@ -265,6 +280,8 @@ int h1pa_restriction_create(int ne, int dof, int ndofs, const int *h_elem_dof, c
         R.indices.alloc((size_t)ne * dof);
         kern::gather_from_table(ne, dof, ed.p, h_dof_map ? dm.p : nullptr, R.gather_map.p, s);
         kern::build_transpose(ne, dof, ndofs, R.gather_map.p, R.offsets.p, R.indices.p, s);
+        R.nperm = 0;
+        R.bubble_begin = ndofs;
     } catch (...) {
         delete r;
         throw;
@@ -310,6 +327,7 @@ int h1pa_restriction_mult_transpose(const h1pa_restriction *r, const double *x, 
     API_BEGIN
     REQUIRE(r && x && y, "restriction_mult_transpose: null argument");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 0,
+                                  r->R().perm.p, r->R().nperm,
                                   as_stream(stream));
     API_END
 }
@@ -320,6 +338,7 @@ int h1pa_restriction_add_mult_transpose(const h1pa_restriction *r, const double 
     REQUIRE(r && x && y, "restriction_add_mult_transpose: null argument");
     REQUIRE(a == 1.0, "General coefficient case is not yet supported!");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 1,
+                                  r->R().perm.p, r->R().nperm,
                                   as_stream(stream));
     API_END
 }
@@ -329,6 +348,7 @@ int h1pa_restriction_abs_mult_transpose(const h1pa_restriction *r, const double 
     API_BEGIN
     REQUIRE(r && x && y, "restriction_abs_mult_transpose: null argument");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 2,
+                                  r->R().perm.p, r->R().nperm,
                                   as_stream(stream));
     API_END
 }
@@ -338,6 +358,7 @@ int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *
     API_BEGIN
     REQUIRE(r && x && y, "restriction_mult_left_inverse: null argument");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 3,
+                                  r->R().perm.p, r->R().nperm,
                                   as_stream(stream));
     API_END
 }
@@ -395,14 +416,16 @@ void autotune_diffusion(Form &f, hipStream_t s) {
     H1PA_HIP(hipEventCreate(&e1));
     int best = 0;
     float best_ms = 1e30f;
+    kern::EOut plain;
+    plain.ye = f.ework0.p;
     for (int v = 0; v < kern::diffusion_variant_count(); ++v) {
         for (int w = 0; w < 2; ++w)
             kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
-                                          sp.restr.gather_map.p, nullptr, x.p, f.ework0.p, false, s);
+                                          sp.restr.gather_map.p, nullptr, x.p, plain, false, s);
         H1PA_HIP(hipEventRecord(e0, s));
         for (int r = 0; r < 5; ++r)
             kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
-                                          sp.restr.gather_map.p, nullptr, x.p, f.ework0.p, false, s);
+                                          sp.restr.gather_map.p, nullptr, x.p, plain, false, s);
         H1PA_HIP(hipEventRecord(e1, s));
         H1PA_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -461,16 +484,25 @@ void require_assembled(const Form &f) {
 
 // E-part of the operator: ework0 = sum of integrators applied to x.  `gmap`
 // null means x is already an E-vector.
+// yL != nullptr selects the fused layout: bubble DOFs straight into yL and
+// the other E-entries in native order (read back through indices_nat).
 void element_apply(Form &f, const int *gmap, const uint8_t *mask, const double *x, double *ye,
-                   bool accumulate, hipStream_t s) {
+                   double *yL, bool accumulate, hipStream_t s) {
+    kern::EOut out;
+    out.ye = ye;
+    if (yL) {
+        out.yL = yL;
+        out.bb = f.space->restr.bubble_begin;
+        out.l2n = f.space->restr.lex2nat.p;
+    }
     Space &sp = *f.space;
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
     if (diff)
-        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, gmap, mask, x, ye,
+        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, gmap, mask, x, out,
                               accumulate, s);
     if (mass)
-        kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, gmap, mask, x, ye,
+        kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, gmap, mask, x, out,
                          accumulate || diff, s);
 }
 
@@ -496,9 +528,10 @@ int h1pa_form_mult(h1pa_form *form, const double *x, double *y, void *stream) {
     require_assembled(f);
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
-    element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, false, s);
-    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p, y,
-                                  0, s);
+    element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, y, false, s);
+    kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                  sp.restr.indices_nat.p, f.ework0.p, y, 0, sp.restr.perm.p,
+                                  sp.restr.nperm, s);
     API_END
 }
 
@@ -515,10 +548,11 @@ int h1pa_form_mult_timed(h1pa_form *form, const double *x, double *y, int steps,
     try {
         for (int k = 0; k < steps; ++k) {
             H1PA_HIP(hipEventRecord(ev[3 * k], s));
-            element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, false, s);
+            element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, y, false, s);
             H1PA_HIP(hipEventRecord(ev[3 * k + 1], s));
-            kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p,
-                                          f.ework0.p, y, 0, s);
+            kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                          sp.restr.indices_nat.p, f.ework0.p, y, 0,
+                                          sp.restr.perm.p, sp.restr.nperm, s);
             H1PA_HIP(hipEventRecord(ev[3 * k + 2], s));
         }
         H1PA_HIP(hipEventSynchronize(ev.back()));
@@ -548,7 +582,7 @@ int h1pa_form_add_mult_e(h1pa_form *form, const double *xE, double *yE, void *st
     REQUIRE(form && xE && yE, "form_add_mult_e: null argument");
     Form &f = form->f;
     require_assembled(f);
-    element_apply(f, nullptr, nullptr, xE, yE, true, as_stream(stream));
+    element_apply(f, nullptr, nullptr, xE, yE, nullptr, true, as_stream(stream));
     API_END
 }
 
@@ -566,7 +600,7 @@ int h1pa_form_assemble_diagonal(h1pa_form *form, double *diag, void *stream) {
     if (mass)
         kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  diag, 2, s);
+                                  diag, 2, sp.restr.perm.p, sp.restr.nperm, s);
     API_END
 }
 
@@ -599,9 +633,13 @@ int h1pa_form_constrained_mult(h1pa_form *form, const int *d_ess, int64_t n_ess,
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
     const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
-    element_apply(f, sp.restr.gather_map.p, mask, x, f.ework0.p, false, s);
-    kern::scatter_constrained(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p, mask, x,
-                              y, s);
+    // interior DOFs go straight to y unless the constraint list may hold some
+    const bool direct = !mask || mask == sp.ess_mask.p;
+    element_apply(f, sp.restr.gather_map.p, mask, x, f.ework0.p, direct ? y : nullptr, false, s);
+    kern::scatter_constrained(direct ? sp.restr.bubble_begin : sp.ndofs, sp.restr.offsets.p,
+                              direct ? sp.restr.indices_nat.p : sp.restr.indices.p, f.ework0.p,
+                              mask, x, y, sp.restr.perm.p,
+                              sp.restr.nperm, s);
     API_END
 }
 
@@ -624,9 +662,10 @@ int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess, co
     }
     DevBuf<double> z(sp.ndofs);
     kern::form_x(sp.ndofs, mask, x, X, s);  // X = x on ess, 0 elsewhere (= w)
-    element_apply(f, sp.restr.gather_map.p, nullptr, X, f.ework0.p, false, s);
-    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  z.p, 0, s);
+    element_apply(f, sp.restr.gather_map.p, nullptr, X, f.ework0.p, z.p, false, s);
+    kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                  sp.restr.indices_nat.p, f.ework0.p, z.p, 0, sp.restr.perm.p,
+                                  sp.restr.nperm, s);
     kern::form_b(sp.ndofs, mask, X, b, z.p, B, s);
     H1PA_HIP(hipStreamSynchronize(s));
     API_END
@@ -640,7 +679,8 @@ int h1pa_space_domain_lf(h1pa_space *space, double fval, double *b, void *stream
     sp.mesh->ensure_nodes(s);
     DevBuf<double> bE(sp.ne * sp.dof);
     kern::domain_lf(*sp.mesh, sp.p, fval, bE.p, s);
-    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, bE.p, b, 0, s);
+    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, bE.p, b, 0,
+                                  sp.restr.perm.p, sp.restr.nperm, s);
     H1PA_HIP(hipStreamSynchronize(s));
     API_END
 }
@@ -662,7 +702,7 @@ int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const doubl
         kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
     if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  diag.p, 2, s);
+                                  diag.p, 2, sp.restr.perm.p, sp.restr.nperm, s);
     pcg_solve(f, mask, diag.p, b, x, *opts, *stats, s);
     if (!stats->converged) {
         set_error("PCG: No convergence!");

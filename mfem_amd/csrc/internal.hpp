@@ -96,8 +96,19 @@ struct Mesh {
 struct Restriction {
     int64_t ne = 0, dof = 0, ndofs = 0;
     DevBuf<int> gather_map, offsets, indices;
-    // transpose table with the trailing (last) entry per row, for the left inverse
-    void build_transpose(hipStream_t s);  // from gather_map
+    // Scatter order of the first `nperm` L-DOFs (the vertex block): sorted by
+    // their first contributing E-index, so that a wave of the E->L pass reads
+    // E-entries of neighbouring elements.  Empty: natural order.
+    DevBuf<int> perm;
+    int64_t nperm = 0;
+    // First L-DOF of the bubble block (interior DOFs, one E-entry each) or
+    // ndofs when the numbering has no such block.
+    int64_t bubble_begin = 0;
+    // Internal E-vector layout of the fused operator: per element in native
+    // (vertex, edge, face, bubble) order, so the entries of one edge or face
+    // are contiguous for the E->L pass.  lex2nat[lex] = native position;
+    // indices_nat = indices re-addressed to that layout (same order).
+    DevBuf<int> lex2nat, indices_nat;
 };
 
 struct Space {
@@ -142,13 +153,21 @@ void list_to_mask(const int *list, int64_t nlist, uint8_t *mask, int64_t n, hipS
 
 void restrict_mult(int64_t nE, const int *gmap, const double *x, double *y, bool abs_,
                    hipStream_t s);
+// indices_nat[j] = indices[j] with E-index e*dof + lex moved to e*dof + lex2nat[lex].
+void native_indices(int64_t nE, int dof, const int *indices, const int *lex2nat, int *out,
+                    hipStream_t s);
+// First-contribution order of the L-DOFs [0, nv): perm[k] = k-th vertex DOF.
+void first_contribution_order(int64_t nE, const int *gmap, const int *offsets,
+                              const int *indices, int64_t nv, DevBuf<int> &perm, hipStream_t s);
+// E->L over rows [0, nrows), rows t < nperm taken as perm[t].
 // mode 0 set, 1 add, 2 abs set, 3 left inverse
-void restrict_mult_transpose(int64_t ndofs, const int *offsets, const int *indices,
-                             const double *xe, double *y, int mode, hipStream_t s);
+void restrict_mult_transpose(int64_t nrows, const int *offsets, const int *indices,
+                             const double *xe, double *y, int mode, const int *perm,
+                             int64_t nperm, hipStream_t s);
 // E->L set with constraint handling: y[i] = mask[i] ? xin[i] : sum (DIAG_ONE)
-void scatter_constrained(int64_t ndofs, const int *offsets, const int *indices,
+void scatter_constrained(int64_t nrows, const int *offsets, const int *indices,
                          const double *xe, const uint8_t *mask, const double *xin, double *y,
-                         hipStream_t s);
+                         const int *perm, int64_t nperm, hipStream_t s);
 
 void perturb_vertices(Mesh &m, double amp, uint64_t seed);  // host
 void nodes_trilinear(const Mesh &m, double *d_nodes, hipStream_t s);
@@ -161,11 +180,20 @@ void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s);
 // E->E: ye = A_E xe (overwrite) or += ; x may be gathered from an L-vector
 // through gmap (gmap != nullptr) with constrained entries masked to zero.
 // basis = [B q-fastest | G q-fastest | B d-fastest | G d-fastest], 4*Q*D doubles.
+// Output of an element kernel: the E-vector ye (lexicographic, or native
+// order when l2n != nullptr) and, when yL != nullptr, interior (bubble) DOFs
+// written straight to the L-vector yL at MFEM's bubble block starting at bb.
+struct EOut {
+    double *ye = nullptr;
+    double *yL = nullptr;
+    int64_t bb = 0;
+    const int *l2n = nullptr;
+};
 void diffusion_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
-                     const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                     const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
                      bool accumulate, hipStream_t s);
 void mass_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
-                const int *gmap, const uint8_t *mask, const double *x, double *ye,
+                const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
                 bool accumulate, hipStream_t s);
 void diffusion_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
                         double *ye, bool accumulate, hipStream_t s);
@@ -178,7 +206,7 @@ void set_diffusion_variant(int p, int q1d, int variant);
 int get_diffusion_variant(int p, int q1d);
 void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const double *basis,
                              const double *pa, const int *gmap, const uint8_t *mask,
-                             const double *x, double *ye, bool accumulate, hipStream_t s);
+                             const double *x, const EOut &out, bool accumulate, hipStream_t s);
 void jacobi_dinv(int64_t n, const double *diag, const uint8_t *mask, double *dinv, hipStream_t s);
 void form_x(int64_t n, const uint8_t *mask, const double *x, double *X, hipStream_t s);
 void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, const double *z,

@@ -127,12 +127,13 @@ __global__ void __launch_bounds__(kThreads)
 k_pcg_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
               const double *__restrict__ ye, const uint8_t *__restrict__ mask,
               const double *__restrict__ d, double *__restrict__ z, double *__restrict__ partials,
-              PcgState *st) {
+              PcgState *st, const int *__restrict__ perm, int64_t nperm) {
     __shared__ double sh[kThreads];
     if (st->done) return;
     double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads) {
+    for (int64_t t = blockIdx.x * (int64_t)kThreads + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = t < nperm ? perm[t] : t;
         double v;
         if (mask && mask[i]) {
             v = d[i];
@@ -248,12 +249,15 @@ static void pcg_element_apply(Form &f, const uint8_t *mask, const double *d, hip
     Space &sp = *f.space;
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
+    kern::EOut out;  // E-vector in native order, read back through indices_nat
+    out.ye = f.ework0.p;
+    out.l2n = sp.restr.lex2nat.p;
     if (diff)
         kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, sp.restr.gather_map.p,
-                              mask, d, f.ework0.p, false, s);
+                              mask, d, out, false, s);
     if (mass)
         kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, sp.restr.gather_map.p, mask, d,
-                         f.ework0.p, diff, s);
+                         out, diff, s);
 }
 
 void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
@@ -275,8 +279,8 @@ void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b
     auto apply = [&]() {
         pcg_element_apply(f, mask, d.p, s);
         hipLaunchKernelGGL(kern::k_pcg_scatter, dim3(nb), dim3(kern::kThreads), 0, s, n,
-                           sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p, mask, d.p, z.p,
-                           partials.p, st.p);
+                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask, d.p, z.p,
+                           partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
     };
     apply();
     H1PA_HIP(hipGetLastError());

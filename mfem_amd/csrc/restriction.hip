@@ -238,11 +238,13 @@ __global__ void k_restrict_mult(int64_t nE, const int *__restrict__ gmap,
     y[i] = (plus || abs_) ? v : -v;
 }
 
-__global__ void k_restrict_mult_t(int64_t ndofs, const int *__restrict__ offsets,
+__global__ void k_restrict_mult_t(int64_t nrows, const int *__restrict__ offsets,
                                   const int *__restrict__ indices, const double *__restrict__ xe,
-                                  double *__restrict__ y, int mode) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= ndofs) return;
+                                  double *__restrict__ y, int mode, const int *__restrict__ perm,
+                                  int64_t nperm) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const int64_t i = t < nperm ? perm[t] : t;
     const int b = offsets[i], e = offsets[i + 1];
     double v = 0.0;
     if (mode == 3) {
@@ -261,13 +263,15 @@ __global__ void k_restrict_mult_t(int64_t ndofs, const int *__restrict__ offsets
     else y[i] = v;
 }
 
-__global__ void k_scatter_constrained(int64_t ndofs, const int *__restrict__ offsets,
+__global__ void k_scatter_constrained(int64_t nrows, const int *__restrict__ offsets,
                                       const int *__restrict__ indices,
                                       const double *__restrict__ xe,
                                       const uint8_t *__restrict__ mask,
-                                      const double *__restrict__ xin, double *__restrict__ y) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= ndofs) return;
+                                      const double *__restrict__ xin, double *__restrict__ y,
+                                      const int *__restrict__ perm, int64_t nperm) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= nrows) return;
+    const int64_t i = t < nperm ? perm[t] : t;
     if (mask && mask[i]) {
         y[i] = xin[i];
         return;
@@ -280,6 +284,44 @@ __global__ void k_scatter_constrained(int64_t ndofs, const int *__restrict__ off
         else v += -xe[-1 - ix];
     }
     y[i] = v;
+}
+
+__global__ void k_native_indices(int64_t nE, int dof, const int *__restrict__ indices,
+                                 const int *__restrict__ l2n, int *__restrict__ out) {
+    const int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (j >= nE) return;
+    const int v = indices[j];
+    const int lid = v >= 0 ? v : -1 - v;
+    const int e = lid / dof, l = lid - e * dof;
+    const int n = e * dof + l2n[l];
+    out[j] = v >= 0 ? n : -1 - n;
+}
+
+// flag[i] = 1 if E-index i is the first contribution of an L-DOF below nv
+__global__ void k_first_flag(int64_t nE, const int *__restrict__ gmap,
+                             const int *__restrict__ offsets, const int *__restrict__ indices,
+                             int64_t nv, int *__restrict__ flag) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nE) return;
+    const int g = gmap[i];
+    const int gid = g >= 0 ? g : -1 - g;
+    int f = 0;
+    if (gid < nv) {
+        const int first = indices[offsets[gid]];
+        f = ((first >= 0 ? first : -1 - first) == i) ? 1 : 0;
+    }
+    flag[i] = f;
+}
+
+__global__ void k_first_place(int64_t nE, const int *__restrict__ gmap,
+                              const int *__restrict__ incl, int *__restrict__ perm) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nE) return;
+    const int prev = i > 0 ? incl[i - 1] : 0;
+    if (incl[i] != prev) {
+        const int g = gmap[i];
+        perm[prev] = g >= 0 ? g : -1 - g;
+    }
 }
 
 }  // namespace
@@ -359,17 +401,46 @@ void restrict_mult(int64_t nE, const int *gmap, const double *x, double *y, bool
     H1PA_HIP(hipGetLastError());
 }
 
-void restrict_mult_transpose(int64_t ndofs, const int *offsets, const int *indices,
-                             const double *xe, double *y, int mode, hipStream_t s) {
-    hipLaunchKernelGGL(k_restrict_mult_t, dim3(nblocks(ndofs)), dim3(kBlock), 0, s, ndofs, offsets,
-                       indices, xe, y, mode);
+void native_indices(int64_t nE, int dof, const int *indices, const int *lex2nat, int *out,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_native_indices, dim3(nblocks(nE)), dim3(kBlock), 0, s, nE, dof, indices,
+                       lex2nat, out);
     H1PA_HIP(hipGetLastError());
 }
 
-void scatter_constrained(int64_t ndofs, const int *offsets, const int *indices, const double *xe,
-                         const uint8_t *mask, const double *xin, double *y, hipStream_t s) {
-    hipLaunchKernelGGL(k_scatter_constrained, dim3(nblocks(ndofs)), dim3(kBlock), 0, s, ndofs,
-                       offsets, indices, xe, mask, xin, y);
+void first_contribution_order(int64_t nE, const int *gmap, const int *offsets,
+                              const int *indices, int64_t nv, DevBuf<int> &perm, hipStream_t s) {
+    DevBuf<int> incl(nE);
+    hipLaunchKernelGGL(k_first_flag, dim3(nblocks(nE)), dim3(kBlock), 0, s, nE, gmap, offsets,
+                       indices, nv, incl.p);
+    H1PA_HIP(hipGetLastError());
+    inclusive_scan(incl.p, nE, s);
+    int total = 0;
+    H1PA_HIP(hipMemcpyAsync(&total, incl.p + nE - 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    H1PA_HIP(hipStreamSynchronize(s));
+    H1PA_CHECK(total == nv, H1PA_EINVAL, "first-contribution order: every vertex DOF needs an entry");
+    perm.alloc(nv > 0 ? nv : 1);
+    hipLaunchKernelGGL(k_first_place, dim3(nblocks(nE)), dim3(kBlock), 0, s, nE, gmap, incl.p,
+                       perm.p);
+    H1PA_HIP(hipGetLastError());
+    H1PA_HIP(hipStreamSynchronize(s));
+}
+
+void restrict_mult_transpose(int64_t nrows, const int *offsets, const int *indices,
+                             const double *xe, double *y, int mode, const int *perm,
+                             int64_t nperm, hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_restrict_mult_t, dim3(nblocks(nrows)), dim3(kBlock), 0, s, nrows, offsets,
+                       indices, xe, y, mode, perm, perm ? nperm : 0);
+    H1PA_HIP(hipGetLastError());
+}
+
+void scatter_constrained(int64_t nrows, const int *offsets, const int *indices, const double *xe,
+                         const uint8_t *mask, const double *xin, double *y, const int *perm,
+                         int64_t nperm, hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_scatter_constrained, dim3(nblocks(nrows)), dim3(kBlock), 0, s, nrows,
+                       offsets, indices, xe, mask, xin, y, perm, perm ? nperm : 0);
     H1PA_HIP(hipGetLastError());
 }
 
