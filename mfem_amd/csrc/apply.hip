@@ -71,11 +71,11 @@ __device__ __forceinline__ void store_out(const EOut &out, int64_t e, int a, int
 }
 
 // U: unroll factor of the output-line loops (1 or 2).
-// PF: the thread's PA-data column (6 x Q doubles) is loaded at kernel entry
-//     so its HBM latency overlaps the x/y stages; the z loop is then fully
-//     unrolled and takes its B/G rows from an LDS copy (the SGPR file cannot
-//     hold all of B and G).
-template <int D, int Q, int NB, int U, bool PF>
+// K: PA-data pipeline depth.  0: each z-plane's 6 values are loaded in its
+//    own iteration.  K > 0: the first K planes are loaded at kernel entry
+//    (their latency hides behind the x/y stages) and plane qz + K is issued
+//    while plane qz is computed, from a K x 6 register ring.
+template <int D, int Q, int NB, int U, int K>
 __global__ void __launch_bounds__(NB * Q * Q)
 k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
                   const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
@@ -84,7 +84,6 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
     constexpr int DDQ = D * D * Q, DQQ = D * Q * Q;
     constexpr int BUF = (2 * DDQ > 3 * DQQ) ? 2 * DDQ : 3 * DQQ;
     __shared__ double smem[NB * BUF];
-    __shared__ double sBG[PF ? 2 * Q * D : 1];
     const double *__restrict__ Bq = basis;              // B[q + Q d]
     const double *__restrict__ Gq = basis + Q * D;
     const double *__restrict__ Bd = basis + 2 * Q * D;  // B[d + D q]
@@ -95,15 +94,15 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
     const int64_t e = (int64_t)blockIdx.x * NB + le;
     const bool valid = e < ne;
     double *s = smem + le * BUF;
-    double Dq[6][PF ? Q : 1];
-    if constexpr (PF) {
-        for (int i = threadIdx.x; i < 2 * Q * D; i += NB * QQ) sBG[i] = Bd[i];  // Bd|Gd
+    constexpr int KK = K < Q ? K : Q;
+    const double *pcol = pa + (int64_t)QQQ * 6 * (valid ? e : 0) + a + Q * b;
+    double ring[KK > 0 ? KK : 1][6];
+    if constexpr (KK > 0) {
         if (valid) {
-            const double *pcol = pa + (int64_t)QQQ * 6 * e + a + Q * b;
 #pragma unroll
-            for (int c = 0; c < 6; ++c)
+            for (int j = 0; j < KK; ++j)
 #pragma unroll
-                for (int qz = 0; qz < Q; ++qz) Dq[c][qz] = pcol[QQQ * c + QQ * qz];
+                for (int c = 0; c < 6; ++c) ring[j][c] = pcol[QQQ * c + QQ * j];
         }
     }
 
@@ -174,31 +173,41 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
             }
         }
         __syncthreads();
-        if (valid && PF) {
+        if (valid && KK > 0) {
             double T0[D], T1[D], T2[D];
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) T0[dz] = T1[dz] = T2[dz] = 0.0;
+#pragma unroll 1
+            for (int qb = 0; qb < Q; qb += (KK > 0 ? KK : 1)) {
 #pragma unroll
-            for (int qz = 0; qz < Q; ++qz) {
-                const double *br = sBG + D * qz, *gr = sBG + Q * D + D * qz;
-                const double d11 = Dq[0][PF ? qz : 0], d21 = Dq[1][PF ? qz : 0],
-                             d31 = Dq[2][PF ? qz : 0], d22 = Dq[3][PF ? qz : 0],
-                             d32 = Dq[4][PF ? qz : 0], d33 = Dq[5][PF ? qz : 0];
-                double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+                for (int j = 0; j < (KK > 0 ? KK : 1); ++j) {
+                    const int qz = qb + j;
+                    if (qz < Q) {
+                        const double *br = Bd + D * qz, *gr = Gd + D * qz;
+                        const double d11 = ring[j][0], d21 = ring[j][1], d31 = ring[j][2];
+                        const double d22 = ring[j][3], d32 = ring[j][4], d33 = ring[j][5];
+                        if (qz + KK < Q) {
 #pragma unroll
-                for (int dz = 0; dz < D; ++dz) {
-                    g0 = fma(br[dz], z0[dz], g0);
-                    g1 = fma(br[dz], z1[dz], g1);
-                    g2 = fma(gr[dz], z2[dz], g2);
-                }
-                const double O0 = fma(d11, g0, fma(d21, g1, d31 * g2));
-                const double O1 = fma(d21, g0, fma(d22, g1, d32 * g2));
-                const double O2 = fma(d31, g0, fma(d32, g1, d33 * g2));
+                            for (int c = 0; c < 6; ++c)
+                                ring[j][c] = pcol[QQQ * c + QQ * (qz + KK)];
+                        }
+                        double g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
-                for (int dz = 0; dz < D; ++dz) {
-                    T0[dz] = fma(br[dz], O0, T0[dz]);
-                    T1[dz] = fma(br[dz], O1, T1[dz]);
-                    T2[dz] = fma(gr[dz], O2, T2[dz]);
+                        for (int dz = 0; dz < D; ++dz) {
+                            g0 = fma(br[dz], z0[dz], g0);
+                            g1 = fma(br[dz], z1[dz], g1);
+                            g2 = fma(gr[dz], z2[dz], g2);
+                        }
+                        const double O0 = fma(d11, g0, fma(d21, g1, d31 * g2));
+                        const double O1 = fma(d21, g0, fma(d22, g1, d32 * g2));
+                        const double O2 = fma(d31, g0, fma(d32, g1, d33 * g2));
+#pragma unroll
+                        for (int dz = 0; dz < D; ++dz) {
+                            T0[dz] = fma(br[dz], O0, T0[dz]);
+                            T1[dz] = fma(br[dz], O1, T1[dz]);
+                            T2[dz] = fma(gr[dz], O2, T2[dz]);
+                        }
+                    }
                 }
             }
 #pragma unroll
@@ -209,7 +218,6 @@ k_diffusion_apply(int64_t ne, const double *__restrict__ basis, const double *__
                 o[2 * DQQ] = T2[dz];
             }
         } else if (valid) {
-            const double *pcol = pa + (int64_t)QQQ * 6 * e + a + Q * b;
             double T0[D], T1[D], T2[D];
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) T0[dz] = T1[dz] = T2[dz] = 0.0;
@@ -566,17 +574,17 @@ k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restri
 }
 
 // ---- dispatch over (D, Q) and variants -------------------------------------
-template <int D, int Q, int NB, int U, bool PF>
+template <int D, int Q, int NB, int U, int K>
 void launch_diffusion_v(int64_t ne, const double *basis, const double *pa, const int *gmap,
                         const uint8_t *mask, const double *x, const EOut &out, bool acc,
                         hipStream_t s) {
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
-    hipLaunchKernelGGL((k_diffusion_apply<D, Q, NB, U, PF>), dim3(grid), dim3(NB * Q * Q), 0, s,
+    hipLaunchKernelGGL((k_diffusion_apply<D, Q, NB, U, K>), dim3(grid), dim3(NB * Q * Q), 0, s,
                        ne, basis, pa, gmap, mask, x, out, acc ? 1 : 0);
 }
 
 // Variants of the diffusion apply: (elements per block, line-loop unroll,
-// PA-data prefetch).  Every variant performs the same arithmetic per element
+// PA-data pipeline depth).  Every variant performs the same arithmetic per element
 // in the same order, so they give bit-identical results and differ only in
 // speed.
 constexpr int kDiffVariants = 6;
@@ -587,12 +595,12 @@ void launch_diffusion(int variant, int64_t ne, const double *basis, const double
     constexpr int NBmax = nb_for(Q, 3 * D * Q * Q);
     constexpr int NBhalf = NBmax / 2 > 0 ? NBmax / 2 : 1;
     switch (variant) {
-        case 1: launch_diffusion_v<D, Q, NBmax, 2, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
-        case 2: launch_diffusion_v<D, Q, NBhalf, 1, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
-        case 3: launch_diffusion_v<D, Q, NBhalf, 2, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
-        case 4: launch_diffusion_v<D, Q, NBmax, 1, true>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
-        case 5: launch_diffusion_v<D, Q, NBhalf, 1, true>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
-        default: launch_diffusion_v<D, Q, NBmax, 1, false>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 1: launch_diffusion_v<D, Q, NBmax, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 2: launch_diffusion_v<D, Q, NBhalf, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 3: launch_diffusion_v<D, Q, NBhalf, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 4: launch_diffusion_v<D, Q, NBmax, 1, 2>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 5: launch_diffusion_v<D, Q, NBmax, 1, 3>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        default: launch_diffusion_v<D, Q, NBmax, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
     }
 }
 template <int D, int Q>
