@@ -209,6 +209,99 @@ int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const doubl
                    double *d_x, const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats,
                    void *stream);
 
+/* ---- mixed precision ------------------------------------------------------
+ * No MFEM counterpart: MFEM's precision is one build-wide real_t
+ * (config/config.hpp:41-48) and it has no mixed-precision PA or PCG.  The
+ * operator data of the reduced-precision applies are shadows of the FP64 PA
+ * data made by h1pa_form_assemble's output (FP64 -> FP32 round-to-nearest-even,
+ * BF16 from that FP32 value, round-to-nearest-even). */
+enum {
+    H1PA_PREC_FP64 = 0,     /* FP64 PA data, vectors and arithmetic                  */
+    H1PA_PREC_FP32 = 1,     /* FP32 PA data, basis, vectors and arithmetic           */
+    H1PA_PREC_BF16_PA = 2   /* BF16-stored PA data, FP32 basis/vectors/arithmetic    */
+};
+/* y = A x in reduced precision (precision FP32 or BF16_PA): FP32 L-vectors
+ * on the device, same operator as h1pa_form_mult. */
+int h1pa_form_mult_f32(h1pa_form *form, int precision, const float *d_x, float *d_y,
+                       void *stream);
+typedef struct {
+    int inner_precision;   /* H1PA_PREC_FP32 or H1PA_PREC_BF16_PA                    */
+    double rel_tol;        /* stop: (B r, r) <= rel_tol^2 (B b, b) on the FP64 true
+                              residual r = b - A x, B = Jacobi (CGSolver's measure)  */
+    double delta;          /* refinement trigger: a step is taken whenever the inner
+                              residual's B-norm has fallen by this factor (0 < delta < 1) */
+    int max_outer;         /* refinement steps                                       */
+    int max_inner;         /* inner iterations in total                              */
+} h1pa_ir_opts;
+typedef struct {
+    int converged;
+    int outer_iterations;  /* refinement steps taken                                 */
+    int inner_iterations;  /* inner CG iterations                                    */
+    double final_norm;     /* sqrt((B r, r)) of the FP64 true residual               */
+    double rel_residual;   /* ||b - A x||_2 / ||b||_2 with the FP64 operator          */
+    double seconds;
+} h1pa_ir_stats;
+/* Mixed-precision solve of the DIAG_ONE ConstrainedOperator system of the ess
+ * list: an inner Jacobi-CG with FP32 vectors and the FP32 or BF16-stored PA
+ * operator, wrapped in FP64 iterative refinement by reliable updates: at each
+ * refinement step x += xs (FP64), r = b - A x (FP64 operator) and the inner
+ * residual is replaced by (float) r while the search direction is kept.
+ * d_b and d_x are FP64 L-vectors; b's ess entries are the values x takes
+ * there (as after h1pa_form_linear_system). */
+int h1pa_ir_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *d_b,
+                  double *d_x, const h1pa_ir_opts *opts, h1pa_ir_stats *stats, void *stream);
+
+/* ---- batched solves -------------------------------------------------------
+ * No MFEM counterpart (MFEM runs one CGSolver::Mult at a time,
+ * linalg/solvers.cpp:869-1050).  A batch advances many independent
+ * Jacobi-PCG (or mixed-precision refinement) solves together: one replay of
+ * a captured HIP graph runs `iters_per_replay` iterations of every running
+ * solve, with one element-kernel launch per (p, q1d) group over all solves'
+ * elements.  Each solve's arithmetic is independent of the others, so a
+ * solve gives bitwise the same result alone and in any mix.  The batch owns
+ * its own stream; it is not re-entrant.  Forms must be diffusion-only with
+ * q1d = p + 2; each form, d_b and d_x must stay valid until the solve has
+ * been returned by h1pa_batch_poll. */
+typedef struct h1pa_batch h1pa_batch;
+typedef struct {
+    int precision;         /* H1PA_PREC_FP64: plain PCG (CGSolver semantics);
+                              H1PA_PREC_FP32 / BF16_PA: as h1pa_ir_solve       */
+    int iters_per_replay;  /* iterations per graph replay (1..256)              */
+    int use_graph;         /* 1: capture and replay a HIP graph; 0: plain launches */
+    int autotune;          /* 1: time the element-kernel variants per group      */
+    double delta;          /* mixed: refinement trigger (as h1pa_ir_opts.delta)  */
+    int max_outer;         /* mixed: refinement steps                            */
+    int max_inner;         /* mixed: inner iterations in total                   */
+} h1pa_batch_opts;
+typedef struct {
+    int converged;
+    int iterations;        /* PCG iterations (mixed: inner iterations, summed)  */
+    int outer_iterations;  /* mixed: refinement steps; FP64: 0                  */
+    double final_norm;     /* FP64: sqrt((B r, r)); mixed: ||b - A x||_2        */
+    double rel_residual;   /* mixed: ||b - A x||_2 / ||b||_2; FP64: -1          */
+} h1pa_batch_result;
+int h1pa_batch_create(const h1pa_batch_opts *opts, h1pa_batch **batch);
+int h1pa_batch_destroy(h1pa_batch *batch);
+/* Add the solve of the DIAG_ONE constrained system (ess list) A x = b.
+ * opts->rel_tol / abs_tol / max_iter are the CGSolver settings in FP64 mode;
+ * in mixed mode opts->rel_tol is h1pa_ir_opts.rel_tol (abs_tol, max_iter
+ * unused: the batch's max_outer / max_inner apply).
+ * `tag` is returned by h1pa_batch_poll; *slot receives the slot index. */
+int h1pa_batch_add(h1pa_batch *batch, h1pa_form *form, const int *d_ess, int64_t n_ess,
+                   const double *d_b, double *d_x, const h1pa_pcg_opts *opts, int64_t tag,
+                   int *slot);
+/* Run up to max_replays replays, stopping early when no solve is running;
+ * *n_running receives the number of solves still running. */
+int h1pa_batch_run(h1pa_batch *batch, int max_replays, int *n_running);
+/* Return (and retire) up to `cap` finished solves: tags[i], results[i]. */
+int h1pa_batch_poll(h1pa_batch *batch, int cap, int64_t *tags, h1pa_batch_result *results,
+                    int *n);
+/* info[0..5] = solves added and not yet polled, solves running, device bytes
+ * held by the batch's vectors, replays, graph captures, (p, q1d) groups. */
+int h1pa_batch_info(const h1pa_batch *batch, int64_t *info);
+/* Device bytes held by a form, its space and its mesh (admission control). */
+int h1pa_form_device_bytes(const h1pa_form *form, int64_t *bytes);
+
 #ifdef __cplusplus
 }
 #endif

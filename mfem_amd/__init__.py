@@ -60,6 +60,29 @@ class PcgStats(C.Structure):
                 ("final_norm", _d), ("seconds", _d)]
 
 
+class IrOpts(C.Structure):
+    _fields_ = [("inner_precision", _i), ("rel_tol", _d), ("delta", _d),
+                ("max_outer", _i), ("max_inner", _i)]
+
+
+class IrStats(C.Structure):
+    _fields_ = [("converged", _i), ("outer_iterations", _i), ("inner_iterations", _i),
+                ("final_norm", _d), ("rel_residual", _d), ("seconds", _d)]
+
+
+PREC_FP64, PREC_FP32, PREC_BF16_PA = 0, 1, 2
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("precision", _i), ("iters_per_replay", _i), ("use_graph", _i), ("autotune", _i),
+                ("delta", _d), ("max_outer", _i), ("max_inner", _i)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("converged", _i), ("iterations", _i), ("outer_iterations", _i),
+                ("final_norm", _d), ("rel_residual", _d)]
+
+
 # name -> (restype, argtypes); every entry point of include/h1pa.h
 SIGNATURES = {
     "h1pa_last_error": (C.c_char_p, []),
@@ -104,6 +127,17 @@ SIGNATURES = {
     "h1pa_space_domain_lf": (_i, [_vp, _d, _vp, _vp]),
     "h1pa_pcg_solve": (_i, [_vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                             _vp]),
+    "h1pa_form_mult_f32": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "h1pa_batch_create": (_i, [C.POINTER(BatchOpts), C.POINTER(_vp)]),
+    "h1pa_batch_destroy": (_i, [_vp]),
+    "h1pa_batch_add": (_i, [_vp, _vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), _i64,
+                            C.POINTER(_i)]),
+    "h1pa_batch_run": (_i, [_vp, _i, C.POINTER(_i)]),
+    "h1pa_batch_poll": (_i, [_vp, _i, _pi64, C.POINTER(BatchResult), C.POINTER(_i)]),
+    "h1pa_batch_info": (_i, [_vp, _pi64]),
+    "h1pa_form_device_bytes": (_i, [_vp, _pi64]),
+    "h1pa_ir_solve": (_i, [_vp, _vp, _i64, _vp, _vp, C.POINTER(IrOpts), C.POINTER(IrStats),
+                           _vp]),
 }
 
 
@@ -127,6 +161,17 @@ def lib():
             fn.argtypes = args
         _lib = L
     return _lib
+
+
+def _destroy(fn_name, h):
+    """Release a handle; a no-op once the library is gone (interpreter exit)."""
+    L = _lib
+    if L is None or not h:
+        return
+    try:
+        getattr(L, fn_name)(h)
+    except Exception:  # noqa: BLE001 - destructors must not raise
+        pass
 
 
 def _check(status):
@@ -229,10 +274,9 @@ class Mesh:
                                                 C.byref(h)))
         return Mesh(h, nx, ny, nz)
 
-    def __del__(self):
-        if getattr(self, "_h", None):
-            lib().h1pa_mesh_destroy(self._h)
-            self._h = None
+    def __del__(self, _d=_destroy):
+        _d("h1pa_mesh_destroy", getattr(self, "_h", None))
+        self._h = None
 
     def _counts(self):
         c = (C.c_int64 * 4)()
@@ -323,10 +367,10 @@ class ElementRestriction:
                                              C.byref(h)))
         return ElementRestriction(h, ne, dof, int(ndofs))
 
-    def __del__(self):
-        if getattr(self, "_owned", False) and getattr(self, "_h", None):
-            lib().h1pa_restriction_destroy(self._h)
-            self._h = None
+    def __del__(self, _d=_destroy):
+        if getattr(self, "_owned", False):
+            _d("h1pa_restriction_destroy", getattr(self, "_h", None))
+        self._h = None
 
     def Height(self):
         return self.ne * self.dof
@@ -389,11 +433,10 @@ class FiniteElementSpace:
         _check(lib().h1pa_space_restriction(h, C.byref(r)))
         self._restr = ElementRestriction(r, self.ne, self.dof, self.ndofs, owner=self, owned=False)
 
-    def __del__(self):
-        if getattr(self, "_h", None):
-            self._restr = None
-            lib().h1pa_space_destroy(self._h)
-            self._h = None
+    def __del__(self, _d=_destroy):
+        self._restr = None
+        _d("h1pa_space_destroy", getattr(self, "_h", None))
+        self._h = None
 
     def GetNE(self):
         return self.ne
@@ -508,10 +551,9 @@ class _PAForm:
         _check(lib().h1pa_form_create(fes._h, C.byref(desc), C.byref(h)))
         self._h = h
 
-    def __del__(self):
-        if getattr(self, "_h", None):
-            lib().h1pa_form_destroy(self._h)
-            self._h = None
+    def __del__(self, _d=_destroy):
+        _d("h1pa_form_destroy", getattr(self, "_h", None))
+        self._h = None
 
 
 class BilinearForm:
@@ -581,6 +623,14 @@ class BilinearForm:
                                           C.byref(te), C.byref(ts), _stream()))
         return tot.value, te.value, ts.value
 
+    def MultLowPrecision(self, x, y, precision=PREC_FP32):
+        """y = A x with FP32 vectors and FP32 (PREC_FP32) or BF16-stored
+        (PREC_BF16_PA) PA data; no MFEM counterpart."""
+        n = self.fes.ndofs
+        _check_vec(x, n, "float32"); _check_vec(y, n, "float32")
+        _check(lib().h1pa_form_mult_f32(self._single()._h, int(precision), _ptr(x), _ptr(y),
+                                        _stream()))
+
     def AddMultPA_E(self, xE, yE):
         """yE += A_E xE on E-vectors (BilinearFormIntegrator::AddMultPA)."""
         for f in self._forms:
@@ -605,6 +655,15 @@ class BilinearForm:
         nq = q ** 3 * self.fes.ne
         return (_wrap_device_double(dd.value, 6 * nq) if dd.value else None,
                 _wrap_device_double(dm.value, nq) if dm.value else None)
+
+    def DeviceBytes(self):
+        """Device bytes held by the form(s), the space and the mesh."""
+        tot = 0
+        for f in self._forms:
+            v = C.c_int64()
+            _check(lib().h1pa_form_device_bytes(f._h, C.byref(v)))
+            tot += v.value
+        return tot
 
     def _single(self):
         if len(self._forms) != 1:
@@ -702,6 +761,126 @@ class CGSolver:
 
     def GetConverged(self):
         return bool(self.stats.converged)
+
+
+class IRSolver:
+    """Mixed-precision solve (no MFEM counterpart): inner Jacobi-CG with FP32
+    vectors and FP32 or BF16-stored PA data, FP64 iterative refinement by
+    reliable updates (FP64 residual and solution update, search direction
+    kept).  Stops on CGSolver's measure (B r, r) <= rel_tol^2 (B b, b) taken
+    on the FP64 true residual.  Same Set*/Mult shape as CGSolver."""
+
+    def __init__(self, inner_precision=PREC_FP32):
+        self.inner_precision = int(inner_precision)
+        self.rel_tol, self.delta = 1e-10, 0.1
+        self.max_outer, self.max_inner = 200, 100000
+        self.op = None
+        self.stats = IrStats()
+
+    def SetRelTol(self, t):
+        self.rel_tol = float(t)
+
+    def SetDelta(self, t):
+        """Refinement trigger: a step whenever the inner residual fell by t."""
+        self.delta = float(t)
+
+    def SetMaxIter(self, outer, inner=None):
+        self.max_outer = int(outer)
+        if inner is not None:
+            self.max_inner = int(inner)
+
+    def SetOperator(self, op):
+        if not isinstance(op, ConstrainedOperator):
+            raise H1paError(H1PA_EUNSUP, "the operator must come from FormLinearSystem")
+        self.op = op
+
+    def SetPreconditioner(self, prec):
+        if not isinstance(prec, OperatorJacobiSmoother):
+            raise H1paError(H1PA_EUNSUP, "only OperatorJacobiSmoother is provided")
+
+    def Mult(self, b, x):
+        if self.op is None:
+            raise H1paError(H1PA_EINVAL, "SetOperator first")
+        f = self.op.form._single()
+        o = IrOpts(self.inner_precision, self.rel_tol, self.delta, self.max_outer,
+                   self.max_inner)
+        rc = lib().h1pa_ir_solve(f._h, _ptr(self.op.ess), self.op.ess.numel(), _ptr(b), _ptr(x),
+                                 C.byref(o), C.byref(self.stats), _stream())
+        if rc not in (H1PA_OK, H1PA_ENOTCONV):
+            _check(rc)
+
+    def GetConverged(self):
+        return bool(self.stats.converged)
+
+
+class BatchSolver:
+    """Many independent Jacobi-PCG (or mixed-precision) solves advanced
+    together on the device; no MFEM counterpart.  Each added solve keeps the
+    CGSolver settings it was added with; Run advances all running solves by
+    `iters_per_replay` iterations per replay of a captured HIP graph; Poll
+    returns and retires the finished ones.  The operator's form and the B, X
+    tensors are kept alive here until the solve is polled."""
+
+    def __init__(self, precision=PREC_FP64, iters_per_replay=8, use_graph=True, autotune=True,
+                 delta=0.1, max_outer=200, max_inner=100000):
+        o = BatchOpts(int(precision), int(iters_per_replay), int(bool(use_graph)),
+                      int(bool(autotune)), float(delta), int(max_outer), int(max_inner))
+        h = C.c_void_p()
+        _check(lib().h1pa_batch_create(C.byref(o), C.byref(h)))
+        self._h = h
+        self._keep = {}
+        self._next_tag = 0
+
+    def __del__(self, _d=_destroy):
+        _d("h1pa_batch_destroy", getattr(self, "_h", None))
+        self._h = None
+
+    def Add(self, A, B, X, rel_tol=1e-12, abs_tol=0.0, max_iter=10000, tag=None):
+        """Add the solve A X = B (A from FormLinearSystem); returns its tag."""
+        if not isinstance(A, ConstrainedOperator):
+            raise H1paError(H1PA_EUNSUP, "the operator must come from FormLinearSystem")
+        n = A.Height()
+        _check_vec(B, n); _check_vec(X, n)
+        if tag is None:
+            tag = self._next_tag
+        self._next_tag = max(self._next_tag, int(tag)) + 1
+        f = A.form._single()
+        o = PcgOpts(float(rel_tol), float(abs_tol), int(max_iter), 8)
+        slot = C.c_int()
+        _check(lib().h1pa_batch_add(self._h, f._h, _ptr(A.ess), A.ess.numel(), _ptr(B), _ptr(X),
+                                    C.byref(o), int(tag), C.byref(slot)))
+        self._keep[int(tag)] = (A, B, X, f)
+        return int(tag)
+
+    def Run(self, max_replays=1 << 30):
+        """Advance until nothing runs or max_replays replays; returns the
+        number of solves still running."""
+        n = C.c_int()
+        _check(lib().h1pa_batch_run(self._h, int(max_replays), C.byref(n)))
+        return n.value
+
+    def Poll(self, cap=4096):
+        """[(tag, {converged, iterations, outer_iterations, final_norm,
+        rel_residual})] for the solves that finished since the last call."""
+        tags = (C.c_int64 * cap)()
+        res = (BatchResult * cap)()
+        n = C.c_int()
+        _check(lib().h1pa_batch_poll(self._h, int(cap), tags, res, C.byref(n)))
+        out = []
+        for i in range(n.value):
+            r = res[i]
+            out.append((int(tags[i]), dict(converged=bool(r.converged), iterations=r.iterations,
+                                           outer_iterations=r.outer_iterations,
+                                           final_norm=r.final_norm,
+                                           rel_residual=r.rel_residual)))
+            self._keep.pop(int(tags[i]), None)
+        return out
+
+    def Info(self):
+        info = (C.c_int64 * 6)()
+        _check(lib().h1pa_batch_info(self._h, info))
+        return dict(live=info[0], running=info[1], bytes=info[2], replays=info[3],
+                    captures=info[4], groups=info[5])
 
 
 class LinearForm:

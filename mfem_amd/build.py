@@ -19,7 +19,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libh1pa.so")
 
 SOURCES = ["basis.cpp", "mesh.cpp", "capi.cpp", "restriction.hip", "geometry.hip",
-           "apply.hip", "pcg.hip"]
+           "apply.hip", "pcg.hip", "batch.hip"]
 
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",

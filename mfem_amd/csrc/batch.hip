@@ -1,0 +1,1029 @@
+// batch.hip -- many independent solves advanced together.
+//
+// No MFEM counterpart: MFEM solves one system per CGSolver::Mult
+// (linalg/solvers.cpp:869-1050).  Here every live solve ("slot") owns its
+// vectors and a small device state machine; one replay of a captured HIP
+// graph advances all of them by K iterations:
+//
+//   init      slots in INIT: x = 0, r = b, z = dinv r, d = z, (z, r)
+//   K x {     element kernels, one launch per (p, Q) group, over all slots'
+//               elements (a work list of (slot, first element) blocks)
+//             scatter   z = A d (DIAG_ONE rows copy d), den = (d, z)
+//             update    x += a d, r -= a z, z = dinv r, (r, z), stop test
+//             [mixed: refinement step of the slots whose inner residual
+//              fell by delta: x += xs, r64 = b - A x in FP64, r = (float) r64]
+//             dir       d = z + b d                                       }
+//
+// FP64 mode is MFEM's CGSolver on each slot.  Mixed mode is an FP32 inner CG
+// (FP32 or BF16-stored PA data) with FP64 reliable updates, stopping on
+// CGSolver's measure taken on the FP64 true residual.  Every kernel skips,
+// block-uniformly, the slots that are not in the phase it serves.  Dot
+// products are reduced per slot over fixed chunks of that slot's DOFs, and
+// the last chunk to finish sums the chunk partials in index order, so a
+// slot's iterates do not depend on which other slots share the batch: a
+// solve gives bitwise the same result alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "apply_body.hpp"
+
+namespace h1pa {
+
+enum : int {
+    PH_INNER = 0,   // iterations running
+    PH_RU = 1,      // mixed: FP64 refinement step due this iteration
+    PH_DONE = 2,    // finished (converged or not)
+    PH_INIT = 4,    // freshly added
+    PH_EMPTY = 5    // slot retired
+};
+
+struct SlotState {
+    double nom, nom0, den, betanom, beta, r0;  // PCG scalars (mixed: nom = inner (r, z))
+    double rho_ref, rho64, rho64_0, r2, b2;    // mixed: refinement bookkeeping
+    int iter, in_loop, max_iter, converged, final_iter;
+    int phase, outer;
+    unsigned counter;
+};
+
+struct SlotDesc {
+    const void *pa;        // inner PA data (FP64, FP32 or BF16)
+    const double *pa64;    // FP64 PA data (refinement residual)
+    const int *gmap, *offsets, *indices, *perm, *l2n;
+    const uint8_t *mask;
+    const double *b;
+    double *x;
+    void *r, *z, *d, *dinv, *xi, *ye;  // inner vectors (type T of the batch); xi: FP64
+                                       // mode the solution x, mixed the update xs
+    double *r64, *dinv64, *ye64;       // mixed: FP64 residual, Jacobi, E-vector
+    int64_t ne, n, nperm;
+    int part0, nchunks;
+    double rel_tol, abs_tol;           // CGSolver settings (mixed: rel_tol on the FP64 measure)
+    double delta;                      // mixed: refinement trigger
+    int max_outer;
+    int pad_;
+};
+
+struct VecTask {
+    int slot, chunk;
+};
+struct ElemTask {
+    int slot;
+    int e0;
+};
+
+namespace kern {
+namespace {
+
+constexpr int kT = 256;          // threads per vector block
+constexpr int kPer = 4;          // DOFs per thread per chunk
+constexpr int kChunk = kT * kPer;
+
+__device__ inline double bsum(double v, double *sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kT / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Writes the chunk partial(s); in the last chunk of the slot to finish,
+// returns true with the slot totals (fixed order) in tot[0..NV).
+template <int NV>
+__device__ inline bool slot_reduce(const double (&part)[NV], const SlotDesc &sd, SlotState *st,
+                                   int chunk, double *partials, double *sh, double (&tot)[NV]) {
+    __shared__ int last;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k) partials[(int64_t)NV * (sd.part0 + chunk) + k] = part[k];
+        __threadfence();
+        const unsigned prev = atomicAdd(&st->counter, 1u);
+        last = (prev == (unsigned)sd.nchunks - 1u);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    const volatile double *vp = partials + (int64_t)NV * sd.part0;
+    for (int k = 0; k < NV; ++k) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < sd.nchunks; i += kT) v += vp[(int64_t)NV * i + k];
+        tot[k] = bsum(v, sh);
+    }
+    if (threadIdx.x == 0) st->counter = 0u;
+    return true;
+}
+
+template <typename T>
+__device__ inline T *vp(void *p) { return reinterpret_cast<T *>(p); }
+
+// The DOFs of chunk tk.chunk that this thread owns (kPer of them, kT apart).
+#define H1PA_CHUNK_LOOP(i)                                                                 \
+    for (int64_t i = (int64_t)tk.chunk * kChunk + threadIdx.x,                             \
+                 i##_end = min(sd.n, (int64_t)(tk.chunk + 1) * kChunk);                    \
+         i < i##_end; i += kT)
+
+// ---- FP64 mode: CGSolver on each slot ---------------------------------------
+__global__ void __launch_bounds__(kT)
+kb_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+        SlotState *__restrict__ states, double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    SlotState *st = states + tk.slot;
+    if (st->phase != PH_INIT) return;
+    double *r = vp<double>(sd.r), *z = vp<double>(sd.z), *d = vp<double>(sd.d);
+    double *x = sd.x;
+    const double *dinv = vp<double>(sd.dinv);
+    double acc = 0.0;
+    H1PA_CHUNK_LOOP(i) {
+        const double ri = sd.b[i];
+        const double zi = dinv[i] * ri;
+        r[i] = ri;
+        x[i] = 0.0;
+        z[i] = zi;
+        d[i] = zi;
+        acc += zi * ri;
+    }
+    double part[1] = {bsum(acc, sh)}, tot[1];
+    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
+        const double nom = tot[0];
+        st->nom = st->nom0 = st->betanom = nom;
+        st->iter = 1;
+        st->in_loop = 0;
+        st->final_iter = 0;
+        double r0 = nom * sd.rel_tol * sd.rel_tol;
+        if (sd.abs_tol * sd.abs_tol > r0) r0 = sd.abs_tol * sd.abs_tol;
+        st->r0 = r0;
+        int next = PH_INNER;
+        if (nom < 0.0) {
+            next = PH_DONE;
+        } else if (nom <= r0) {
+            st->converged = 1;
+            next = PH_DONE;
+        }
+        st->phase = next;
+    }
+}
+
+// ---- mixed mode: FP32 inner CG with FP64 reliable updates ---------------------
+__global__ void __launch_bounds__(kT)
+kbm_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+         SlotState *__restrict__ states, double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    SlotState *st = states + tk.slot;
+    if (st->phase != PH_INIT) return;
+    float *r = vp<float>(sd.r), *z = vp<float>(sd.z), *d = vp<float>(sd.d);
+    float *xs = vp<float>(sd.xi);
+    const float *dinv = vp<float>(sd.dinv);
+    double acc[3] = {0.0, 0.0, 0.0};
+    H1PA_CHUNK_LOOP(i) {
+        const double bi = sd.b[i];
+        sd.x[i] = 0.0;
+        sd.r64[i] = bi;
+        const float ri = (float)bi;
+        const float zi = dinv[i] * ri;
+        r[i] = ri;
+        xs[i] = 0.f;
+        z[i] = zi;
+        d[i] = zi;
+        acc[0] += (double)ri * (double)zi;
+        acc[1] += bi * sd.dinv64[i] * bi;
+        acc[2] += bi * bi;
+    }
+    double part[3] = {bsum(acc[0], sh), bsum(acc[1], sh), bsum(acc[2], sh)}, tot[3];
+    if (slot_reduce<3>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
+        st->nom = st->rho_ref = tot[0];
+        st->rho64 = st->rho64_0 = tot[1];
+        st->r2 = st->b2 = tot[2];
+        st->iter = 0;
+        st->outer = 0;
+        st->converged = 0;
+        if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
+            st->converged = 1;
+            st->phase = PH_DONE;
+        } else {
+            st->phase = PH_INNER;
+        }
+    }
+}
+
+// Element kernel over a group's work list.  RU = true: the FP64 residual
+// apply (x -> ye64) of mixed slots in PH_RU; otherwise d -> ye for slots in
+// PH_INNER.  force = 1 ignores the phase (timing only).
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, bool RU>
+__global__ void __launch_bounds__(NB * Q * Q)
+kb_diffusion(const ElemTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+             const SlotState *__restrict__ states, const T *__restrict__ basis, int force) {
+    const ElemTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    if (!force && states[tk.slot].phase != (RU ? PH_RU : PH_INNER)) return;
+    __shared__ T smem[NB * diffusion_buf<D, Q>()];
+    EOutT<T> out;
+    out.l2n = sd.l2n;
+    if constexpr (RU) {
+        out.ye = reinterpret_cast<T *>(sd.ye64);
+        diffusion_elements<T, TP, D, Q, NB, U, K>(tk.e0, sd.ne, basis,
+                                                  reinterpret_cast<const TP *>(sd.pa64), sd.gmap,
+                                                  sd.mask, reinterpret_cast<const T *>(sd.x), out,
+                                                  0, smem);
+    } else {
+        out.ye = vp<T>(sd.ye);
+        diffusion_elements<T, TP, D, Q, NB, U, K>(tk.e0, sd.ne, basis,
+                                                  reinterpret_cast<const TP *>(sd.pa), sd.gmap,
+                                                  sd.mask, vp<T>(sd.d), out, 0, smem);
+    }
+}
+
+// z = A d for PH_INNER slots (DIAG_ONE rows copy d); den = (d, z).
+template <typename T, bool MIXED>
+__global__ void __launch_bounds__(kT)
+kb_scatter(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+           SlotState *__restrict__ states, double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    SlotState *st = states + tk.slot;
+    if (st->phase != PH_INNER) return;
+    const T *ye = vp<T>(sd.ye), *d = vp<T>(sd.d);
+    T *z = vp<T>(sd.z);
+    double acc = 0.0;
+    const int64_t i0 = (int64_t)tk.chunk * kChunk;
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t t = i0 + k * kT + threadIdx.x;
+        if (t >= sd.n) break;
+        const int64_t i = t < sd.nperm ? sd.perm[t] : t;
+        T v;
+        if (sd.mask && sd.mask[i]) {
+            v = d[i];
+        } else {
+            v = T(0);
+            for (int j = sd.offsets[i]; j < sd.offsets[i + 1]; ++j) {
+                const int ix = sd.indices[j];
+                v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
+            }
+        }
+        z[i] = v;
+        acc += (double)d[i] * (double)v;
+    }
+    double part[1] = {bsum(acc, sh)}, tot[1];
+    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
+        st->den = tot[0];
+        if (tot[0] == 0.0) {
+            if (MIXED) {
+                st->phase = PH_RU;  // inner residual is zero: check the true one
+            } else {            // CGSolver: den == 0 ends the loop
+                st->final_iter = st->in_loop ? st->iter : 0;
+                st->phase = PH_DONE;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kT)
+kb_update(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+          SlotState *__restrict__ states, double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    SlotState *st = states + tk.slot;
+    if (st->phase != PH_INNER) return;
+    const double alpha = st->nom / st->den;
+    double *x = sd.x, *r = vp<double>(sd.r), *z = vp<double>(sd.z);
+    const double *d = vp<double>(sd.d), *dinv = vp<double>(sd.dinv);
+    double acc = 0.0;
+    H1PA_CHUNK_LOOP(i) {
+        x[i] += alpha * d[i];
+        const double ri = r[i] - alpha * z[i];
+        r[i] = ri;
+        const double zi = dinv[i] * ri;
+        z[i] = zi;
+        acc += ri * zi;
+    }
+    double part[1] = {bsum(acc, sh)}, tot[1];
+    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
+        const double betanom = tot[0];
+        const int i = st->iter;
+        st->in_loop = 1;
+        st->betanom = betanom;
+        if (betanom < 0.0) {
+            st->final_iter = i;
+            st->phase = PH_DONE;
+        } else if (betanom <= st->r0) {
+            st->converged = 1;
+            st->final_iter = i;
+            st->phase = PH_DONE;
+        } else if (i + 1 > st->max_iter) {
+            st->final_iter = st->max_iter;
+            st->phase = PH_DONE;
+        } else {
+            st->beta = betanom / st->nom;
+            st->nom = betanom;
+            st->iter = i + 1;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kT)
+kbm_update(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+           SlotState *__restrict__ states, double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    SlotState *st = states + tk.slot;
+    if (st->phase != PH_INNER) return;
+    const float alpha = (float)(st->nom / st->den);
+    float *xs = vp<float>(sd.xi), *r = vp<float>(sd.r), *z = vp<float>(sd.z);
+    const float *d = vp<float>(sd.d), *dinv = vp<float>(sd.dinv);
+    double acc = 0.0;
+    H1PA_CHUNK_LOOP(i) {
+        xs[i] += alpha * d[i];
+        const float ri = r[i] - alpha * z[i];
+        r[i] = ri;
+        const float zi = dinv[i] * ri;
+        z[i] = zi;
+        acc += (double)ri * (double)zi;
+    }
+    double part[1] = {bsum(acc, sh)}, tot[1];
+    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
+        const double rho = tot[0];
+        st->iter += 1;
+        const double tol2 = sd.rel_tol * sd.rel_tol;
+        if (rho <= sd.delta * sd.delta * st->rho_ref || rho <= tol2 * st->rho64_0 ||
+            st->iter >= st->max_iter) {
+            st->phase = PH_RU;
+        } else {
+            st->beta = rho / st->nom;
+            st->nom = rho;
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kT)
+kb_dir(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+       const SlotState *__restrict__ states) {
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    const SlotState *st = states + tk.slot;
+    if (st->phase != PH_INNER) return;
+    const T beta = (T)st->beta;
+    const T *z = vp<T>(sd.z);
+    T *d = vp<T>(sd.d);
+    H1PA_CHUNK_LOOP(i) d[i] = z[i] + beta * d[i];
+}
+
+// Refinement step, part 1 (PH_RU): x += xs, xs = 0.
+__global__ void __launch_bounds__(kT)
+kbm_ru_x(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+         const SlotState *__restrict__ states) {
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    if (states[tk.slot].phase != PH_RU) return;
+    float *xs = vp<float>(sd.xi);
+    H1PA_CHUNK_LOOP(i) {
+        sd.x[i] += (double)xs[i];
+        xs[i] = 0.f;
+    }
+}
+
+// Refinement step, part 2 (PH_RU, after the FP64 element kernel on x):
+// r64 = b - A x, (B r64, r64), ||r64||^2, r = (float) r64, z = dinv r, (r, z).
+__global__ void __launch_bounds__(kT)
+kbm_ru_res(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
+           SlotState *__restrict__ states, double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    const VecTask tk = tasks[blockIdx.x];
+    const SlotDesc &sd = desc[tk.slot];
+    SlotState *st = states + tk.slot;
+    if (st->phase != PH_RU) return;
+    float *r = vp<float>(sd.r), *z = vp<float>(sd.z);
+    const float *dinv = vp<float>(sd.dinv);
+    double acc[3] = {0.0, 0.0, 0.0};
+    const int64_t i0 = (int64_t)tk.chunk * kChunk;
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t t = i0 + k * kT + threadIdx.x;
+        if (t >= sd.n) break;
+        const int64_t i = t < sd.nperm ? sd.perm[t] : t;
+        double y;
+        if (sd.mask && sd.mask[i]) {
+            y = sd.x[i];
+        } else {
+            y = 0.0;
+            for (int j = sd.offsets[i]; j < sd.offsets[i + 1]; ++j) {
+                const int ix = sd.indices[j];
+                y += ix >= 0 ? sd.ye64[ix] : -sd.ye64[-1 - ix];
+            }
+        }
+        const double ri = sd.b[i] - y;
+        sd.r64[i] = ri;
+        const float rf = (float)ri;
+        const float zf = dinv[i] * rf;
+        r[i] = rf;
+        z[i] = zf;
+        acc[0] += (double)rf * (double)zf;
+        acc[1] += ri * sd.dinv64[i] * ri;
+        acc[2] += ri * ri;
+    }
+    double part[3] = {bsum(acc[0], sh), bsum(acc[1], sh), bsum(acc[2], sh)}, tot[3];
+    if (slot_reduce<3>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
+        st->outer += 1;
+        st->rho64 = tot[1];
+        st->r2 = tot[2];
+        if (tot[1] <= sd.rel_tol * sd.rel_tol * st->rho64_0) {
+            st->converged = 1;
+            st->phase = PH_DONE;
+        } else if (st->outer >= sd.max_outer || st->iter >= st->max_iter) {
+            st->phase = PH_DONE;
+        } else {
+            st->beta = st->nom > 0.0 ? tot[0] / st->nom : 0.0;
+            st->nom = tot[0];
+            st->rho_ref = tot[0];
+            st->phase = PH_INNER;
+        }
+    }
+}
+
+// ---- element-kernel dispatch over (D, Q, variant) --------------------------
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, bool RU>
+void launch_kb(const ElemTask *tasks, int ntasks, const SlotDesc *desc, const SlotState *st,
+               const T *basis, int force, hipStream_t s) {
+    hipLaunchKernelGGL((kb_diffusion<T, TP, D, Q, NB, U, K, RU>), dim3((unsigned)ntasks),
+                       dim3(NB * Q * Q), 0, s, tasks, desc, st, basis, force);
+}
+
+template <int D, int Q>
+constexpr int nb_max() { return nb_for(Q, 3 * D * Q * Q); }
+template <int D, int Q>
+constexpr int nb_half() { return nb_max<D, Q>() / 2 > 0 ? nb_max<D, Q>() / 2 : 1; }
+
+// Elements per block of variant v (same table as the single-problem kernels).
+template <int D, int Q>
+int nb_of(int v) { return (v == 2 || v == 3) ? nb_half<D, Q>() : nb_max<D, Q>(); }
+
+template <typename T, typename TP, int D, int Q, bool RU>
+void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc *desc,
+                       const SlotState *st, const T *basis, int force, hipStream_t s) {
+    constexpr int NM = nb_max<D, Q>(), NH = nb_half<D, Q>();
+    if constexpr (sizeof(T) == 8) {
+        switch (v) {
+            case 1: launch_kb<T, TP, D, Q, NM, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 2: launch_kb<T, TP, D, Q, NH, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 3: launch_kb<T, TP, D, Q, NH, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 4: launch_kb<T, TP, D, Q, NM, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 5: launch_kb<T, TP, D, Q, NM, 1, 3, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+        }
+    } else {
+        (void)NH;
+        switch (v) {
+            case 1: launch_kb<T, TP, D, Q, NM, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 4: launch_kb<T, TP, D, Q, NM, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+        }
+    }
+}
+
+}  // namespace
+}  // namespace kern
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+// Variants valid for the inner type (FP32 applies have 0, 1, 4).
+bool variant_ok(int prec, int v) {
+    return prec == H1PA_PREC_FP64 ? (v >= 0 && v < 6) : (v == 0 || v == 1 || v == 4);
+}
+
+#define H1PA_BATCH_DQ(X) X(2, 3) X(3, 4) X(4, 5) X(5, 6) X(6, 7) X(7, 8) X(8, 9) X(9, 10)
+
+int nb_of_dq(int D, int Q, int v) {
+#define X(DD, QQ) \
+    if (D == DD && Q == QQ) return kern::nb_of<DD, QQ>(v);
+    H1PA_BATCH_DQ(X)
+#undef X
+    throw Error{H1PA_EUNSUP, "batch: unsupported (p, q1d)"};
+}
+
+void launch_group(int prec, bool ru, int D, int Q, int v, const ElemTask *tasks, int ntasks,
+                  const SlotDesc *desc, const SlotState *st, const void *basis, int force,
+                  hipStream_t s) {
+    if (ntasks <= 0) return;
+#define X(DD, QQ)                                                                              \
+    if (D == DD && Q == QQ) {                                                                  \
+        if (ru || prec == H1PA_PREC_FP64) {                                                    \
+            if (ru)                                                                            \
+                kern::launch_kb_variant<double, double, DD, QQ, true>(                         \
+                    v, tasks, ntasks, desc, st, (const double *)basis, force, s);              \
+            else                                                                               \
+                kern::launch_kb_variant<double, double, DD, QQ, false>(                        \
+                    v, tasks, ntasks, desc, st, (const double *)basis, force, s);              \
+        } else if (prec == H1PA_PREC_FP32) {                                                   \
+            kern::launch_kb_variant<float, float, DD, QQ, false>(v, tasks, ntasks, desc, st,   \
+                                                                 (const float *)basis, force, s); \
+        } else {                                                                               \
+            kern::launch_kb_variant<float, bf16_t, DD, QQ, false>(                             \
+                v, tasks, ntasks, desc, st, (const float *)basis, force, s);                   \
+        }                                                                                      \
+        H1PA_HIP(hipGetLastError());                                                           \
+        return;                                                                                \
+    }
+    H1PA_BATCH_DQ(X)
+#undef X
+    throw Error{H1PA_EUNSUP, "batch: unsupported (p, q1d)"};
+}
+
+}  // namespace
+
+struct BatchSlot {
+    h1pa_form *form = nullptr;
+    int64_t tag = 0;
+    int group = -1;
+    int64_t n = 0, ne = 0;
+    DevBuf<uint8_t> mask;
+    DevBuf<double> v64[5];  // FP64 mode: r z d dinv ye;  mixed: r64 dinv64 ye64
+    DevBuf<float> v32[5];   // mixed: r z d dinv xs
+    SlotDesc desc{};
+    size_t bytes = 0;
+    bool live = false;      // added and not yet polled
+};
+
+struct BatchGroup {
+    int p = 0, q1d = 0, variant = 0, nb = 1;
+    // the group's own copies of the basis (forms may be destroyed before the
+    // batch is): FP64, and FP32 for a mixed batch
+    DevBuf<double> basis64_own;
+    DevBuf<float> basis32_own;
+    const void *basis = nullptr;      // inner basis (type of the batch)
+    const double *basis64 = nullptr;  // FP64 basis (refinement residual)
+    DevBuf<ElemTask> tasks;
+    int ntasks = 0;
+    bool tuned = false;
+};
+
+}  // namespace h1pa
+
+struct h1pa_batch {
+    h1pa_batch_opts o{};
+    hipStream_t stream = nullptr;
+    std::vector<std::unique_ptr<h1pa::BatchSlot>> slots;
+    std::vector<h1pa::BatchGroup> groups;
+    h1pa::DevBuf<h1pa::SlotDesc> d_desc;
+    h1pa::DevBuf<h1pa::SlotState> d_state;
+    std::vector<h1pa::SlotState> h_state;
+    h1pa::DevBuf<double> partials;
+    h1pa::DevBuf<h1pa::VecTask> vtasks;
+    int nvtasks = 0;
+    bool dirty = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t replays = 0, captures = 0;
+
+    ~h1pa_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        slots.clear();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace h1pa {
+using Batch = h1pa_batch;
+
+namespace {
+
+bool mixed_mode(const Batch &B) { return B.o.precision != H1PA_PREC_FP64; }
+
+int group_for(Batch &B, Form &f) {
+    const int p = f.space->p;
+    for (size_t g = 0; g < B.groups.size(); ++g)
+        if (B.groups[g].p == p && B.groups[g].q1d == f.q1d) return (int)g;
+    BatchGroup G;
+    G.p = p;
+    G.q1d = f.q1d;
+    G.variant = kern::get_diffusion_variant(p, f.q1d);
+    if (!variant_ok(B.o.precision, G.variant)) G.variant = 0;
+    G.nb = nb_of_dq(p + 1, f.q1d, G.variant);
+    B.groups.push_back(std::move(G));
+    return (int)B.groups.size() - 1;
+}
+
+bool running_phase(int ph) { return ph == PH_INIT || ph == PH_INNER || ph == PH_RU; }
+
+void drop_graph(Batch &B) {
+    if (B.exec) {
+        H1PA_HIP(hipGraphExecDestroy(B.exec));
+        B.exec = nullptr;
+    }
+    if (B.graph) {
+        H1PA_HIP(hipGraphDestroy(B.graph));
+        B.graph = nullptr;
+    }
+}
+
+void element_tasks(const Batch &B, int g, int nb, std::vector<ElemTask> &out) {
+    out.clear();
+    for (size_t k = 0; k < B.slots.size(); ++k) {
+        const BatchSlot &sl = *B.slots[k];
+        if (sl.group != g || !sl.live || !running_phase(B.h_state[k].phase)) continue;
+        for (int64_t e = 0; e < sl.ne; e += nb) out.push_back({(int)k, (int)e});
+    }
+}
+
+// Rebuild descriptors, work lists and partial slices for the running slots;
+// the captured graph is dropped.
+void rebuild(Batch &B) {
+    drop_graph(B);
+    const int ns = (int)B.slots.size();
+    std::vector<SlotDesc> desc(ns);
+    std::vector<VecTask> vt;
+    int part = 0;
+    for (int k = 0; k < ns; ++k) {
+        BatchSlot &sl = *B.slots[k];
+        desc[k] = sl.desc;
+        desc[k].part0 = 0;
+        desc[k].nchunks = 0;
+        if (!sl.live || !running_phase(B.h_state[k].phase)) continue;
+        const int nch = (int)((sl.n + kern::kChunk - 1) / kern::kChunk);
+        desc[k].part0 = part;
+        desc[k].nchunks = nch;
+        part += nch;
+        for (int c = 0; c < nch; ++c) vt.push_back({k, c});
+    }
+    for (int k = 0; k < ns; ++k) B.slots[k]->desc = desc[k];
+    if (B.d_desc.n < (size_t)ns) B.d_desc.alloc(std::max<size_t>(ns, 2 * B.d_desc.n));
+    B.d_desc.upload(desc.data(), ns, B.stream);
+    if (B.partials.n < 3 * (size_t)part + 3) B.partials.alloc(3 * (size_t)part + 3);
+    B.nvtasks = (int)vt.size();
+    if (B.vtasks.n < vt.size()) B.vtasks.alloc(vt.size());
+    B.vtasks.upload(vt.data(), vt.size(), B.stream);
+    std::vector<ElemTask> et;
+    for (size_t g = 0; g < B.groups.size(); ++g) {
+        BatchGroup &G = B.groups[g];
+        element_tasks(B, (int)g, G.nb, et);
+        G.ntasks = (int)et.size();
+        if (G.tasks.n < et.size()) G.tasks.alloc(et.size());
+        G.tasks.upload(et.data(), et.size(), B.stream);
+    }
+    H1PA_HIP(hipStreamSynchronize(B.stream));
+    B.dirty = false;
+}
+
+// Pick each group's element-kernel variant by timing every candidate on the
+// group's current slots (phase ignored; the E-vectors written are scratch
+// here).  Variants with other elements-per-block get their own work list.
+void autotune(Batch &B) {
+    if (!B.o.autotune) return;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    H1PA_HIP(hipEventCreate(&e0));
+    H1PA_HIP(hipEventCreate(&e1));
+    std::vector<ElemTask> et;
+    DevBuf<ElemTask> tmp;
+    try {
+        for (size_t g = 0; g < B.groups.size(); ++g) {
+            BatchGroup &G = B.groups[g];
+            if (G.tuned || G.ntasks < 256) continue;
+            G.tuned = true;
+            float best = 1e30f;
+            int bestv = G.variant;
+            for (int v = 0; v < 6; ++v) {
+                if (!variant_ok(B.o.precision, v)) continue;
+                const int nb = nb_of_dq(G.p + 1, G.q1d, v);
+                element_tasks(B, (int)g, nb, et);
+                tmp.upload(et.data(), et.size(), B.stream);
+                const int nt = (int)et.size();
+                for (int rep = 0; rep < 2; ++rep)
+                    launch_group(B.o.precision, false, G.p + 1, G.q1d, v, tmp.p, nt, B.d_desc.p,
+                                 B.d_state.p, G.basis, 1, B.stream);
+                H1PA_HIP(hipEventRecord(e0, B.stream));
+                for (int rep = 0; rep < 5; ++rep)
+                    launch_group(B.o.precision, false, G.p + 1, G.q1d, v, tmp.p, nt, B.d_desc.p,
+                                 B.d_state.p, G.basis, 1, B.stream);
+                H1PA_HIP(hipEventRecord(e1, B.stream));
+                H1PA_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                H1PA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (ms < best) {
+                    best = ms;
+                    bestv = v;
+                }
+            }
+            if (bestv != G.variant) {
+                G.variant = bestv;
+                G.nb = nb_of_dq(G.p + 1, G.q1d, bestv);
+                B.dirty = true;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void enqueue_replay(Batch &B, hipStream_t s) {
+    const unsigned nv = (unsigned)B.nvtasks;
+    if (nv == 0) return;
+    const int prec = B.o.precision;
+    const dim3 grid(nv), blk(kern::kT);
+    if (!mixed_mode(B)) {
+        hipLaunchKernelGGL(kern::kb_init, grid, blk, 0, s, B.vtasks.p, B.d_desc.p, B.d_state.p,
+                           B.partials.p);
+        for (int it = 0; it < B.o.iters_per_replay; ++it) {
+            for (BatchGroup &G : B.groups)
+                launch_group(prec, false, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks,
+                             B.d_desc.p, B.d_state.p, G.basis, 0, s);
+            hipLaunchKernelGGL((kern::kb_scatter<double, false>), grid, blk, 0, s, B.vtasks.p,
+                               B.d_desc.p, B.d_state.p, B.partials.p);
+            hipLaunchKernelGGL(kern::kb_update, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
+                               B.d_state.p, B.partials.p);
+            hipLaunchKernelGGL(kern::kb_dir<double>, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
+                               B.d_state.p);
+        }
+    } else {
+        hipLaunchKernelGGL(kern::kbm_init, grid, blk, 0, s, B.vtasks.p, B.d_desc.p, B.d_state.p,
+                           B.partials.p);
+        for (int it = 0; it < B.o.iters_per_replay; ++it) {
+            for (BatchGroup &G : B.groups)
+                launch_group(prec, false, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks,
+                             B.d_desc.p, B.d_state.p, G.basis, 0, s);
+            hipLaunchKernelGGL((kern::kb_scatter<float, true>), grid, blk, 0, s, B.vtasks.p,
+                               B.d_desc.p, B.d_state.p, B.partials.p);
+            hipLaunchKernelGGL(kern::kbm_update, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
+                               B.d_state.p, B.partials.p);
+            hipLaunchKernelGGL(kern::kbm_ru_x, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
+                               B.d_state.p);
+            for (BatchGroup &G : B.groups)
+                launch_group(prec, true, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks,
+                             B.d_desc.p, B.d_state.p, G.basis64, 0, s);
+            hipLaunchKernelGGL(kern::kbm_ru_res, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
+                               B.d_state.p, B.partials.p);
+            hipLaunchKernelGGL(kern::kb_dir<float>, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
+                               B.d_state.p);
+        }
+    }
+    H1PA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void batch_destroy(Batch *B) { delete B; }
+
+hipStream_t batch_stream(const Batch &B) { return B.stream; }
+
+Batch *batch_create(const h1pa_batch_opts &o) {
+    H1PA_CHECK(o.precision == H1PA_PREC_FP64 || o.precision == H1PA_PREC_FP32 ||
+                   o.precision == H1PA_PREC_BF16_PA,
+               H1PA_EINVAL, "batch: unknown precision");
+    H1PA_CHECK(o.iters_per_replay >= 1 && o.iters_per_replay <= 256, H1PA_EINVAL,
+               "batch: iters_per_replay must be in [1, 256]");
+    if (o.precision != H1PA_PREC_FP64)
+        H1PA_CHECK(o.max_outer >= 1 && o.max_inner >= 1 && o.delta > 0.0 && o.delta < 1.0,
+                   H1PA_EINVAL,
+                   "batch: mixed precision needs max_outer, max_inner >= 1 and 0 < delta < 1");
+    std::unique_ptr<Batch> B(new Batch);
+    B->o = o;
+    H1PA_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
+    return B.release();
+}
+
+int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *b, double *x,
+              const h1pa_pcg_opts &po, int64_t tag) {
+    Form &f = form->f;
+    Space &sp = *f.space;
+    H1PA_CHECK(f.desc.integrators == H1PA_INTEG_DIFFUSION, H1PA_EUNSUP,
+               "batch: diffusion-only forms are supported");
+    H1PA_CHECK(f.q1d == sp.p + 2, H1PA_EUNSUP, "batch: q1d must be p + 2");
+    H1PA_CHECK(po.max_iter >= 1, H1PA_EINVAL, "batch: max_iter must be >= 1");
+    hipStream_t s = B.stream;
+    const bool mixed = mixed_mode(B);
+    if (mixed) ensure_low_precision(f, B.o.precision, s);
+    std::unique_ptr<BatchSlot> sl(new BatchSlot);
+    sl->form = form;
+    sl->tag = tag;
+    sl->n = sp.ndofs;
+    sl->ne = sp.ne;
+    sl->group = group_for(B, f);
+    BatchGroup &G = B.groups[sl->group];
+    if (!G.basis64_own.p) {
+        G.basis64_own.alloc(f.basis.n);
+        H1PA_HIP(hipMemcpyAsync(G.basis64_own.p, f.basis.p, f.basis.n * sizeof(double),
+                                hipMemcpyDeviceToDevice, s));
+        G.basis64 = G.basis64_own.p;
+        G.basis = G.basis64_own.p;
+        if (mixed) {
+            G.basis32_own.alloc(f.basis32.n);
+            H1PA_HIP(hipMemcpyAsync(G.basis32_own.p, f.basis32.p, f.basis32.n * sizeof(float),
+                                    hipMemcpyDeviceToDevice, s));
+            G.basis = G.basis32_own.p;
+        }
+    }
+    const int64_t n = sp.ndofs, nE = sp.ne * sp.dof;
+    if (mask_src) {
+        sl->mask.alloc(n);
+        H1PA_HIP(hipMemcpyAsync(sl->mask.p, mask_src, n, hipMemcpyDeviceToDevice, s));
+    }
+    DevBuf<double> diag(n);
+    operator_diagonal(f, diag.p, s);
+    SlotDesc &d = sl->desc;
+    d.gmap = sp.restr.gather_map.p;
+    d.offsets = sp.restr.offsets.p;
+    d.indices = sp.restr.indices_nat.p;
+    d.perm = sp.restr.perm.p;
+    d.nperm = sp.restr.nperm;
+    d.l2n = sp.restr.lex2nat.p;
+    d.mask = sl->mask.p;
+    d.b = b;
+    d.x = x;
+    d.ne = sp.ne;
+    d.n = n;
+    d.pa64 = f.pa_diff.p;
+    d.rel_tol = po.rel_tol;
+    d.abs_tol = po.abs_tol;
+    size_t bytes = sl->mask.n;
+    if (!mixed) {
+        for (int k = 0; k < 4; ++k) sl->v64[k].alloc(n);
+        sl->v64[4].alloc(nE);
+        d.r = sl->v64[0].p;
+        d.z = sl->v64[1].p;
+        d.d = sl->v64[2].p;
+        d.dinv = sl->v64[3].p;
+        d.ye = sl->v64[4].p;
+        d.xi = x;
+        d.pa = f.pa_diff.p;
+        kern::jacobi_dinv(n, diag.p, sl->mask.p, sl->v64[3].p, s);
+        bytes += (4 * n + nE) * 8;
+    } else {
+        for (int k = 0; k < 5; ++k) sl->v32[k].alloc(n);
+        sl->v64[0].alloc(n);
+        sl->v64[1].alloc(n);
+        sl->v64[2].alloc(nE);
+        d.r = sl->v32[0].p;
+        d.z = sl->v32[1].p;
+        d.d = sl->v32[2].p;
+        d.dinv = sl->v32[3].p;
+        d.xi = sl->v32[4].p;
+        d.r64 = sl->v64[0].p;
+        d.dinv64 = sl->v64[1].p;
+        // the FP32 E-vector lives in the FP64 one: they are never live together
+        d.ye = sl->v64[2].p;
+        d.ye64 = sl->v64[2].p;
+        d.pa = f.pa_diff_lp.p;
+        d.delta = B.o.delta;
+        d.max_outer = B.o.max_outer;
+        kern::jacobi_dinv(n, diag.p, sl->mask.p, sl->v64[1].p, s);
+        kern::jacobi_dinv_f32(n, diag.p, sl->mask.p, sl->v32[3].p, s);
+        bytes += 5 * n * 4 + 2 * n * 8 + nE * 8;
+    }
+    sl->bytes = bytes;
+    sl->live = true;
+    SlotState st{};
+    st.phase = PH_INIT;
+    st.max_iter = mixed ? B.o.max_inner : po.max_iter;
+    // reuse a retired position if there is one
+    int k = -1;
+    for (size_t i = 0; i < B.slots.size(); ++i)
+        if (!B.slots[i]->live && B.h_state[i].phase == PH_EMPTY) {
+            k = (int)i;
+            break;
+        }
+    if (k < 0) {
+        k = (int)B.slots.size();
+        if (B.d_state.n < (size_t)k + 1) {
+            DevBuf<SlotState> nst(std::max<size_t>(64, 2 * ((size_t)k + 1)));
+            if (k > 0)
+                H1PA_HIP(hipMemcpyAsync(nst.p, B.d_state.p, sizeof(SlotState) * k,
+                                        hipMemcpyDeviceToDevice, s));
+            H1PA_HIP(hipStreamSynchronize(s));
+            B.d_state = std::move(nst);
+        }
+        B.slots.push_back(nullptr);
+        B.h_state.push_back(SlotState{});
+    }
+    B.slots[k] = std::move(sl);
+    B.h_state[k] = st;
+    H1PA_HIP(hipMemcpyAsync(B.d_state.p + k, &B.h_state[k], sizeof(SlotState),
+                            hipMemcpyHostToDevice, s));
+    H1PA_HIP(hipStreamSynchronize(s));
+    B.dirty = true;
+    return k;
+}
+
+// Runs up to `max_replays` replays (stops early once no slot is running).
+// Returns the number of slots still running.
+int batch_run(Batch &B, int max_replays) {
+    hipStream_t s = B.stream;
+    auto running = [&]() {
+        int live = 0;
+        for (size_t k = 0; k < B.slots.size(); ++k)
+            if (B.slots[k]->live && running_phase(B.h_state[k].phase)) ++live;
+        return live;
+    };
+    for (int rp = 0; rp < max_replays; ++rp) {
+        if (running() == 0) break;
+        // compact once a third of the work list belongs to finished slots
+        if (!B.dirty) {
+            int64_t dead = 0, all = 0;
+            for (size_t k = 0; k < B.slots.size(); ++k) {
+                const int nch = B.slots[k]->desc.nchunks;
+                all += nch;
+                if (!running_phase(B.h_state[k].phase)) dead += nch;
+            }
+            if (all > 0 && 3 * dead > all) B.dirty = true;
+        }
+        if (B.dirty) {
+            rebuild(B);
+            autotune(B);
+            if (B.dirty) rebuild(B);  // a variant with another block size was chosen
+        }
+        if (B.o.use_graph) {
+            if (!B.exec) {
+                H1PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                try {
+                    enqueue_replay(B, s);
+                } catch (...) {
+                    hipGraph_t g = nullptr;
+                    (void)hipStreamEndCapture(s, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    throw;
+                }
+                H1PA_HIP(hipStreamEndCapture(s, &B.graph));
+                H1PA_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+                ++B.captures;
+            }
+            H1PA_HIP(hipGraphLaunch(B.exec, s));
+        } else {
+            enqueue_replay(B, s);
+        }
+        ++B.replays;
+        H1PA_HIP(hipMemcpyAsync(B.h_state.data(), B.d_state.p, sizeof(SlotState) * B.slots.size(),
+                                hipMemcpyDeviceToHost, s));
+        H1PA_HIP(hipStreamSynchronize(s));
+    }
+    return running();
+}
+
+// Finished slots not yet reported: fills up to `cap` results and retires them.
+int batch_poll(Batch &B, int cap, int64_t *tags, h1pa_batch_result *res) {
+    int n = 0;
+    const bool mixed = mixed_mode(B);
+    for (size_t k = 0; k < B.slots.size() && n < cap; ++k) {
+        BatchSlot &sl = *B.slots[k];
+        const SlotState &st = B.h_state[k];
+        if (!sl.live || st.phase != PH_DONE) continue;
+        if (tags) tags[n] = sl.tag;
+        if (res) {
+            h1pa_batch_result &r = res[n];
+            r.converged = st.converged;
+            r.iterations = mixed ? st.iter : st.final_iter;
+            r.outer_iterations = mixed ? st.outer : 0;
+            r.final_norm = mixed ? std::sqrt(std::max(st.rho64, 0.0))
+                                 : (st.betanom >= 0 ? std::sqrt(st.betanom) : st.betanom);
+            r.rel_residual = mixed ? (st.b2 > 0 ? std::sqrt(st.r2 / st.b2) : 0.0) : -1.0;
+        }
+        ++n;
+        // retire: free the slot's vectors, keep its position
+        sl.live = false;
+        for (auto &v : sl.v64) v.release();
+        for (auto &v : sl.v32) v.release();
+        sl.mask.release();
+        sl.bytes = 0;
+        B.h_state[k].phase = PH_EMPTY;
+        H1PA_HIP(hipMemcpyAsync(B.d_state.p + k, &B.h_state[k], sizeof(SlotState),
+                                hipMemcpyHostToDevice, B.stream));
+    }
+    if (n) H1PA_HIP(hipStreamSynchronize(B.stream));
+    return n;
+}
+
+void batch_info(const Batch &B, int64_t *info) {
+    int64_t live = 0, running = 0, bytes = 0;
+    for (size_t k = 0; k < B.slots.size(); ++k) {
+        const BatchSlot &sl = *B.slots[k];
+        if (!sl.live) continue;
+        ++live;
+        if (running_phase(B.h_state[k].phase)) ++running;
+        bytes += (int64_t)sl.bytes;
+    }
+    info[0] = live;
+    info[1] = running;
+    info[2] = bytes;
+    info[3] = B.replays;
+    info[4] = B.captures;
+    info[5] = (int64_t)B.groups.size();
+}
+
+}  // namespace h1pa

@@ -509,14 +509,41 @@ void element_apply(Form &f, const int *gmap, const uint8_t *mask, const double *
 const uint8_t *constraint_mask(Form &f, const int *d_ess, int64_t n_ess, hipStream_t s) {
     Space &sp = *f.space;
     if (d_ess == sp.ess.p && n_ess == sp.n_ess) return sp.ess_mask.p;
+    // rebuilt on every call: the caller may have changed the list in place
     if (!f.cmask.p) f.cmask.alloc(sp.ndofs);
-    if (f.cmask_src != d_ess || f.cmask_n != n_ess) {
-        kern::list_to_mask(d_ess, n_ess, f.cmask.p, sp.ndofs, s);
-        f.cmask_src = d_ess;
-        f.cmask_n = n_ess;
-    }
+    kern::list_to_mask(d_ess, n_ess, f.cmask.p, sp.ndofs, s);
     return f.cmask.p;
 }
+
+}  // namespace
+
+// Jacobi diagonal of the unconstrained operator into diag (L-vector).
+void h1pa::operator_diagonal(Form &f, double *diag, hipStream_t s) {
+    Space &sp = *f.space;
+    const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
+    const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
+    if (diff)
+        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
+    if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
+    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
+                                  diag, 2, sp.restr.perm.p, sp.restr.nperm, s);
+}
+
+int64_t h1pa::form_device_bytes(const Form &f) {
+    const Space &sp = *f.space;
+    const Mesh &m = *sp.mesh;
+    int64_t b = 0;
+    b += f.basis.n * 8 + f.pa_diff.n * 8 + f.pa_mass.n * 8 + f.ework0.n * 8 + f.ework1.n * 8;
+    b += f.lwork0.n * 8 + f.lwork1.n * 8 + f.cmask.n + f.basis32.n * 4;
+    b += f.pa_diff_lp.n * 4 + f.pa_mass_lp.n * 4 + f.ework32.n * 4;
+    const Restriction &r = sp.restr;
+    b += (r.gather_map.n + r.offsets.n + r.indices.n + r.perm.n + r.lex2nat.n + r.indices_nat.n) * 4;
+    b += (sp.elem_dof.n + sp.ess.n) * 4 + sp.ess_mask.n;
+    b += m.d_nodes.n * 8 + m.d_topo.n * 4 + m.d_ori.n;
+    return b;
+}
+
+namespace {
 }  // namespace
 
 extern "C" {
@@ -696,18 +723,109 @@ int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const doubl
     hipStream_t s = as_stream(stream);
     const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
     DevBuf<double> diag(sp.ndofs);
-    const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
-    const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
-    if (diff)
-        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
-    if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
-    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  diag.p, 2, sp.restr.perm.p, sp.restr.nperm, s);
+    operator_diagonal(f, diag.p, s);
     pcg_solve(f, mask, diag.p, b, x, *opts, *stats, s);
     if (!stats->converged) {
         set_error("PCG: No convergence!");
         return H1PA_ENOTCONV;
     }
+    API_END
+}
+
+int h1pa_form_mult_f32(h1pa_form *form, int precision, const float *x, float *y, void *stream) {
+    API_BEGIN
+    REQUIRE(form && x && y, "form_mult_f32: null argument");
+    REQUIRE(precision == H1PA_PREC_FP32 || precision == H1PA_PREC_BF16_PA,
+            "form_mult_f32: precision must be H1PA_PREC_FP32 or H1PA_PREC_BF16_PA");
+    Form &f = form->f;
+    require_assembled(f);
+    H1PA_CHECK(f.q1d == f.space->p + 2, H1PA_EUNSUP,
+               "form_mult_f32: reduced-precision applies support q1d = p + 2 only");
+    form_mult_lp(f, precision, x, y, as_stream(stream));
+    API_END
+}
+
+int h1pa_ir_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *b, double *x,
+                  const h1pa_ir_opts *opts, h1pa_ir_stats *stats, void *stream) {
+    API_BEGIN
+    REQUIRE(form && b && x && opts && stats && (n_ess == 0 || d_ess), "ir_solve: null argument");
+    REQUIRE(opts->inner_precision == H1PA_PREC_FP32 || opts->inner_precision == H1PA_PREC_BF16_PA,
+            "ir_solve: inner_precision must be H1PA_PREC_FP32 or H1PA_PREC_BF16_PA");
+    REQUIRE(opts->max_outer >= 1 && opts->max_inner >= 1,
+            "ir_solve: max_outer and max_inner must be >= 1");
+    REQUIRE(opts->rel_tol > 0.0 && opts->delta > 0.0 && opts->delta < 1.0,
+            "ir_solve: need rel_tol > 0 and 0 < delta < 1");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    H1PA_CHECK(f.q1d == sp.p + 2, H1PA_EUNSUP,
+               "ir_solve: reduced-precision applies support q1d = p + 2 only");
+    hipStream_t s = as_stream(stream);
+    const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
+    DevBuf<double> diag(sp.ndofs);
+    operator_diagonal(f, diag.p, s);
+    ir_solve(f, mask, diag.p, b, x, *opts, *stats, s);
+    if (!stats->converged) {
+        set_error("IR: No convergence!");
+        return H1PA_ENOTCONV;
+    }
+    API_END
+}
+
+int h1pa_batch_create(const h1pa_batch_opts *opts, h1pa_batch **batch) {
+    API_BEGIN
+    REQUIRE(opts && batch, "batch_create: null argument");
+    *batch = batch_create(*opts);
+    API_END
+}
+
+int h1pa_batch_destroy(h1pa_batch *batch) {
+    API_BEGIN
+    batch_destroy(batch);
+    API_END
+}
+
+int h1pa_batch_add(h1pa_batch *batch, h1pa_form *form, const int *d_ess, int64_t n_ess,
+                   const double *b, double *x, const h1pa_pcg_opts *opts, int64_t tag,
+                   int *slot) {
+    API_BEGIN
+    REQUIRE(batch && form && b && x && opts && (n_ess == 0 || d_ess), "batch_add: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    hipStream_t s = batch_stream(*batch);
+    const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
+    const int k = batch_add(*batch, form, mask, b, x, *opts, tag);
+    if (slot) *slot = k;
+    API_END
+}
+
+int h1pa_batch_run(h1pa_batch *batch, int max_replays, int *n_running) {
+    API_BEGIN
+    REQUIRE(batch && max_replays >= 0, "batch_run: bad argument");
+    const int r = batch_run(*batch, max_replays);
+    if (n_running) *n_running = r;
+    API_END
+}
+
+int h1pa_batch_poll(h1pa_batch *batch, int cap, int64_t *tags, h1pa_batch_result *results,
+                    int *n) {
+    API_BEGIN
+    REQUIRE(batch && n && cap >= 0, "batch_poll: bad argument");
+    *n = batch_poll(*batch, cap, tags, results);
+    API_END
+}
+
+int h1pa_batch_info(const h1pa_batch *batch, int64_t *info) {
+    API_BEGIN
+    REQUIRE(batch && info, "batch_info: null argument");
+    batch_info(*batch, info);
+    API_END
+}
+
+int h1pa_form_device_bytes(const h1pa_form *form, int64_t *bytes) {
+    API_BEGIN
+    REQUIRE(form && bytes, "form_device_bytes: null argument");
+    *bytes = form_device_bytes(form->f);
     API_END
 }
 

@@ -30,6 +30,11 @@ struct Error {
 
 inline hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
 
+// bfloat16 storage: the upper 16 bits of an IEEE float.
+struct bf16_t {
+    uint16_t bits;
+};
+
 // ---- device buffer ---------------------------------------------------------
 template <typename T>
 struct DevBuf {
@@ -132,10 +137,14 @@ struct Form {
     DevBuf<double> pa_mass;    // [Q^3][ne]
     DevBuf<double> ework0, ework1;  // E-vector scratch
     DevBuf<double> lwork0, lwork1;  // L-vector scratch
-    DevBuf<uint8_t> cmask;          // constraint mask for the current ess list
-    const int *cmask_src = nullptr;
-    int64_t cmask_n = -1;
+    DevBuf<uint8_t> cmask;          // constraint mask of the last ess list passed in
     bool assembled = false;
+    // Reduced-precision shadows for the inner solver (built on first use from
+    // the FP64 data): basis in FP32, PA data in FP32 or BF16 (lp_kind 0 / 1).
+    DevBuf<float> basis32;
+    DevBuf<uint32_t> pa_diff_lp, pa_mass_lp;  // raw storage, 4-byte aligned
+    int lp_kind = -1;
+    DevBuf<float> ework32;
 };
 
 // ---- kernel launchers (HIP translation units) ------------------------------
@@ -183,12 +192,23 @@ void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s);
 // Output of an element kernel: the E-vector ye (lexicographic, or native
 // order when l2n != nullptr) and, when yL != nullptr, interior (bubble) DOFs
 // written straight to the L-vector yL at MFEM's bubble block starting at bb.
-struct EOut {
-    double *ye = nullptr;
-    double *yL = nullptr;
+template <typename T>
+struct EOutT {
+    T *ye = nullptr;
+    T *yL = nullptr;
     int64_t bb = 0;
     const int *l2n = nullptr;
+    const int *stop = nullptr;  // device flag: when nonzero the launch does nothing
 };
+using EOut = EOutT<double>;
+// Reduced-precision applies: FP32 vectors/arithmetic, PA data as FP32
+// (pa_kind 0) or BF16 (pa_kind 1, FP32 accumulation).
+void diffusion_apply_lp(int p, int q1d, int64_t ne, const float *basis, const void *pa,
+                        int pa_kind, const int *gmap, const uint8_t *mask, const float *x,
+                        const EOutT<float> &out, bool accumulate, hipStream_t s);
+void mass_apply_lp(int p, int q1d, int64_t ne, const float *basis, const void *pa, int pa_kind,
+                   const int *gmap, const uint8_t *mask, const float *x, const EOutT<float> &out,
+                   bool accumulate, hipStream_t s);
 void diffusion_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
                      const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
                      bool accumulate, hipStream_t s);
@@ -208,6 +228,8 @@ void diffusion_apply_variant(int variant, int p, int q1d, int64_t ne, const doub
                              const double *pa, const int *gmap, const uint8_t *mask,
                              const double *x, const EOut &out, bool accumulate, hipStream_t s);
 void jacobi_dinv(int64_t n, const double *diag, const uint8_t *mask, double *dinv, hipStream_t s);
+void jacobi_dinv_f32(int64_t n, const double *diag, const uint8_t *mask, float *dinv,
+                     hipStream_t s);
 void form_x(int64_t n, const uint8_t *mask, const double *x, double *X, hipStream_t s);
 void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, const double *z,
             double *B, hipStream_t s);
@@ -215,6 +237,28 @@ void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, co
 
 void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
                const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s);
+// Builds f's reduced-precision shadows for `precision` (H1PA_PREC_FP32 or
+// H1PA_PREC_BF16_PA) unless they already match.
+void ensure_low_precision(Form &f, int precision, hipStream_t s);
+void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
+              const h1pa_ir_opts &o, h1pa_ir_stats &stats, hipStream_t s);
+// Reduced-precision L->L apply of the unconstrained operator (FP32 vectors).
+void form_mult_lp(Form &f, int precision, const float *x, float *y, hipStream_t s);
+// Jacobi diagonal of the unconstrained operator (L-vector, set).
+void operator_diagonal(Form &f, double *diag, hipStream_t s);
+
+// ---- batched solves (batch.hip) --------------------------------------------
+h1pa_batch *batch_create(const h1pa_batch_opts &o);
+void batch_destroy(h1pa_batch *B);
+hipStream_t batch_stream(const h1pa_batch &B);
+// mask: the solve's constraint mask (copied) or null; returns the slot index
+int batch_add(h1pa_batch &B, h1pa_form *form, const uint8_t *mask, const double *b, double *x,
+              const h1pa_pcg_opts &po, int64_t tag);
+int batch_run(h1pa_batch &B, int max_replays);
+int batch_poll(h1pa_batch &B, int cap, int64_t *tags, h1pa_batch_result *res);
+void batch_info(const h1pa_batch &B, int64_t *info);
+// Device bytes held by a form, its space and its mesh.
+int64_t form_device_bytes(const Form &f);
 
 }  // namespace h1pa
 

@@ -1,4 +1,5 @@
-// pcg.hip -- Jacobi-preconditioned CG on the device, MFEM CGSolver semantics.
+// pcg.hip -- Jacobi-preconditioned CG on the device, MFEM CGSolver semantics,
+// and the mixed-precision iterative-refinement solver built on it.
 //
 // CGSolver::Mult (linalg/solvers.cpp:869-1050) with OperatorJacobiSmoother
 // (:401-453) on the DIAG_ONE ConstrainedOperator (linalg/operator.cpp:601-662).
@@ -11,7 +12,8 @@
 //   apply  : element kernel on d (constrained entries read as 0)
 //   scatter: z = A d (DIAG_ONE rows copy d), den = (d, z)
 // Dot products reduce deterministically: fixed grid, fixed in-block tree,
-// and the last block to finish sums the block partials in index order.
+// and the last block to finish sums the block partials in index order.  The
+// vector type T is double or float; scalars and dot accumulation are double.
 #include <chrono>
 #include <cmath>
 
@@ -76,29 +78,30 @@ __device__ inline bool reduce_partials(double part, double *partials, unsigned *
     return true;
 }
 
+template <typename T>
 __global__ void k_jacobi_dinv(int64_t n, const double *__restrict__ diag,
-                              const uint8_t *__restrict__ mask, double *__restrict__ dinv) {
+                              const uint8_t *__restrict__ mask, T *__restrict__ dinv) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
-    dinv[i] = (mask && mask[i]) ? 1.0 : 1.0 / diag[i];
+    dinv[i] = (T)((mask && mask[i]) ? 1.0 : 1.0 / diag[i]);
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_init(int64_t n, const double *__restrict__ b, const double *__restrict__ dinv,
-           double *__restrict__ x, double *__restrict__ r, double *__restrict__ z,
-           double *__restrict__ d, double *__restrict__ partials, PcgState *st, double rel_tol,
-           double abs_tol) {
+k_pcg_init(int64_t n, const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ x,
+           T *__restrict__ r, T *__restrict__ z, T *__restrict__ d, double *__restrict__ partials,
+           PcgState *st, double rel_tol, double abs_tol) {
     __shared__ double sh[kThreads];
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        const double ri = b[i];
-        const double zi = dinv[i] * ri;
+        const T ri = b[i];
+        const T zi = dinv[i] * ri;
         r[i] = ri;
-        x[i] = 0.0;
+        x[i] = T(0);
         z[i] = zi;
         d[i] = zi;
-        acc += zi * ri;
+        acc += (double)zi * (double)ri;
     }
     const double part = block_sum(acc, sh);
     double nom;
@@ -123,29 +126,30 @@ k_pcg_init(int64_t n, const double *__restrict__ b, const double *__restrict__ d
 }
 
 // z = A d from the element output, DIAG_ONE rows copy d; den = (d, z).
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_pcg_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
-              const double *__restrict__ ye, const uint8_t *__restrict__ mask,
-              const double *__restrict__ d, double *__restrict__ z, double *__restrict__ partials,
-              PcgState *st, const int *__restrict__ perm, int64_t nperm) {
+              const T *__restrict__ ye, const uint8_t *__restrict__ mask, const T *__restrict__ d,
+              T *__restrict__ z, double *__restrict__ partials, PcgState *st,
+              const int *__restrict__ perm, int64_t nperm) {
     __shared__ double sh[kThreads];
     if (st->done) return;
     double acc = 0.0;
     for (int64_t t = blockIdx.x * (int64_t)kThreads + threadIdx.x; t < n;
          t += (int64_t)gridDim.x * kThreads) {
         const int64_t i = t < nperm ? perm[t] : t;
-        double v;
+        T v;
         if (mask && mask[i]) {
             v = d[i];
         } else {
-            v = 0.0;
+            v = T(0);
             for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
                 const int ix = indices[j];
                 v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
             }
         }
         z[i] = v;
-        acc += d[i] * v;
+        acc += (double)d[i] * (double)v;
     }
     const double part = block_sum(acc, sh);
     double den;
@@ -158,23 +162,24 @@ k_pcg_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict_
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_update(int64_t n, const double *__restrict__ dinv, double *__restrict__ x,
-             double *__restrict__ r, double *__restrict__ z, const double *__restrict__ d,
-             double *__restrict__ partials, PcgState *st) {
+k_pcg_update(int64_t n, const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
+             T *__restrict__ z, const T *__restrict__ d, double *__restrict__ partials,
+             PcgState *st) {
     __shared__ double sh[kThreads];
     if (st->done) return;
-    const double alpha = st->nom / st->den;
+    const T alpha = (T)(st->nom / st->den);
     double acc = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        const double di = d[i];
+        const T di = d[i];
         x[i] += alpha * di;
-        const double ri = r[i] - alpha * z[i];
+        const T ri = r[i] - alpha * z[i];
         r[i] = ri;
-        const double zi = dinv[i] * ri;
+        const T zi = dinv[i] * ri;
         z[i] = zi;
-        acc += ri * zi;
+        acc += (double)ri * (double)zi;
     }
     const double part = block_sum(acc, sh);
     double betanom;
@@ -200,10 +205,277 @@ k_pcg_update(int64_t n, const double *__restrict__ dinv, double *__restrict__ x,
     }
 }
 
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_dir(int64_t n, const double *__restrict__ z, double *__restrict__ d, const PcgState *st) {
+k_pcg_dir(int64_t n, const T *__restrict__ z, T *__restrict__ d, const PcgState *st) {
     if (st->done) return;
-    const double beta = st->beta;
+    const T beta = (T)st->beta;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        d[i] = z[i] + beta * d[i];
+}
+
+__global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+// FP64 -> BF16: round to FP32 (nearest even), then keep the upper 16 bits
+// with round-to-nearest-even on the dropped half (NaN stays quiet NaN).
+__global__ void k_to_bf16(int64_t n, const double *__restrict__ a, uint16_t *__restrict__ b) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float f = (float)a[i];
+    const uint32_t u = __float_as_uint(f);
+    uint16_t h;
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu))
+        h = (uint16_t)((u >> 16) | 0x40u);
+    else
+        h = (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+    b[i] = h;
+}
+
+// E->L set over all rows in reduced precision (no constraints).
+template <typename T>
+__global__ void k_scatter_set(int64_t n, const int *__restrict__ offsets,
+                              const int *__restrict__ indices, const T *__restrict__ ye,
+                              T *__restrict__ y, const int *__restrict__ perm, int64_t nperm) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = t < nperm ? perm[t] : t;
+    T v = T(0);
+    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+        const int ix = indices[j];
+        v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
+    }
+    y[i] = v;
+}
+
+// ---- mixed precision: CG with reliable updates -----------------------------
+// The FP32 inner CG runs on its recursive residual r and accumulates its
+// solution in xs.  Whenever the preconditioned norm (r, z) has fallen by
+// delta^2 since the last refinement (or reaches the stop level, or the
+// iteration limit), a refinement step replaces it with FP64 quantities:
+// x += xs, xs = 0, r64 = b - A x (FP64 operator), r = (float) r64 -- and the
+// search direction d is kept, so the Krylov space is not restarted.  The
+// stop test is MFEM's (B r, r) <= rel_tol^2 (B b, b) evaluated on the FP64
+// true residual.
+struct IrState {
+    double rho, rho_ref, den, beta;  // inner CG, sums in FP64
+    double rho64, rho64_0, r2, b2;   // FP64 true residual
+    double tol2, delta2;
+    int iter, outer, max_inner, max_outer;
+    int ru;   // 1: refinement step due this iteration
+    int nru;  // !ru: skip flag of the FP64 element kernel
+    int done, converged;
+    unsigned counter;
+};
+
+template <int NV>
+__device__ inline bool reduce_n(const double (&part)[NV], double *partials, unsigned *counter,
+                                double *sh, double (&tot)[NV]) {
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < NV; ++k) partials[(int64_t)k * gridDim.x + blockIdx.x] = part[k];
+        __threadfence();
+        const unsigned prev = atomicAdd(counter, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return false;
+    __threadfence();
+    const volatile double *vp = partials;
+    for (int k = 0; k < NV; ++k) {
+        double v = 0.0;
+        for (unsigned i = threadIdx.x; i < gridDim.x; i += kThreads) v += vp[(int64_t)k * gridDim.x + i];
+        tot[k] = block_sum(v, sh);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *counter = 0u;
+    return true;
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_ir_init(int64_t n, const double *__restrict__ b, const double *__restrict__ dinv64,
+          const float *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r64,
+          float *__restrict__ r, float *__restrict__ xs, float *__restrict__ z,
+          float *__restrict__ d, double *__restrict__ partials, IrState *st) {
+    __shared__ double sh[kThreads];
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        const double bi = b[i];
+        x[i] = 0.0;
+        r64[i] = bi;
+        const float ri = (float)bi;
+        const float zi = dinv[i] * ri;
+        r[i] = ri;
+        xs[i] = 0.f;
+        z[i] = zi;
+        d[i] = zi;
+        acc[0] += (double)ri * (double)zi;
+        acc[1] += bi * dinv64[i] * bi;
+        acc[2] += bi * bi;
+    }
+    double part[3], tot[3];
+    for (int k = 0; k < 3; ++k) {
+        part[k] = block_sum(acc[k], sh);
+        __syncthreads();
+    }
+    if (reduce_n<3>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) {
+        st->rho = st->rho_ref = tot[0];
+        st->rho64 = st->rho64_0 = tot[1];
+        st->r2 = st->b2 = tot[2];
+        st->iter = 0;
+        st->outer = 0;
+        st->ru = 0;
+        st->nru = 1;
+        st->converged = 0;
+        st->done = 0;
+        if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
+            st->done = 1;
+            st->converged = 1;
+        }
+    }
+}
+
+// q = A d (FP32; DIAG_ONE rows copy d), den = (d, q).
+__global__ void __launch_bounds__(kThreads)
+k_ir_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
+             const float *__restrict__ ye, const uint8_t *__restrict__ mask,
+             const float *__restrict__ d, float *__restrict__ q, double *__restrict__ partials,
+             IrState *st, const int *__restrict__ perm, int64_t nperm) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    double acc = 0.0;
+    for (int64_t t = blockIdx.x * (int64_t)kThreads + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = t < nperm ? perm[t] : t;
+        float v;
+        if (mask && mask[i]) {
+            v = d[i];
+        } else {
+            v = 0.f;
+            for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+                const int ix = indices[j];
+                v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
+            }
+        }
+        q[i] = v;
+        acc += (double)d[i] * (double)v;
+    }
+    double part[1] = {block_sum(acc, sh)}, tot[1];
+    if (reduce_n<1>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) st->den = tot[0];
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_ir_update(int64_t n, const float *__restrict__ dinv, float *__restrict__ xs,
+            float *__restrict__ r, float *__restrict__ z, const float *__restrict__ q,
+            const float *__restrict__ d, double *__restrict__ partials, IrState *st) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    const double den = st->den;
+    const float alpha = den != 0.0 ? (float)(st->rho / den) : 0.f;
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        xs[i] += alpha * d[i];
+        const float ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        const float zi = dinv[i] * ri;
+        z[i] = zi;
+        acc += (double)ri * (double)zi;
+    }
+    double part[1] = {block_sum(acc, sh)}, tot[1];
+    if (reduce_n<1>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) {
+        const double rho_new = tot[0];
+        st->iter += 1;
+        const bool trigger = den == 0.0 || rho_new <= st->delta2 * st->rho_ref ||
+                             rho_new <= st->tol2 * st->rho64_0 || st->iter >= st->max_inner;
+        if (trigger) {
+            st->ru = 1;
+            st->nru = 0;
+        } else {
+            st->beta = rho_new / st->rho;
+            st->rho = rho_new;
+        }
+    }
+}
+
+// Refinement step, part 1: x += xs, xs = 0.
+__global__ void k_ir_ru_x(int64_t n, float *__restrict__ xs, double *__restrict__ x,
+                          const IrState *st) {
+    if (!st->ru) return;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] += (double)xs[i];
+    xs[i] = 0.f;
+}
+
+// Refinement step, part 2 (after the FP64 element kernel on x):
+// r64 = b - A x, (B r64, r64), ||r64||^2, r = (float) r64, z = dinv r, (r, z).
+__global__ void __launch_bounds__(kThreads)
+k_ir_ru_res(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
+            const double *__restrict__ ye64, const uint8_t *__restrict__ mask,
+            const double *__restrict__ b, const double *__restrict__ x,
+            const double *__restrict__ dinv64, const float *__restrict__ dinv,
+            double *__restrict__ r64, float *__restrict__ r, float *__restrict__ z,
+            double *__restrict__ partials, IrState *st, const int *__restrict__ perm,
+            int64_t nperm) {
+    __shared__ double sh[kThreads];
+    if (!st->ru) return;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int64_t t = blockIdx.x * (int64_t)kThreads + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * kThreads) {
+        const int64_t i = t < nperm ? perm[t] : t;
+        double y;
+        if (mask && mask[i]) {
+            y = x[i];
+        } else {
+            y = 0.0;
+            for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
+                const int ix = indices[j];
+                y += ix >= 0 ? ye64[ix] : -ye64[-1 - ix];
+            }
+        }
+        const double ri = b[i] - y;
+        r64[i] = ri;
+        const float rf = (float)ri;
+        const float zf = dinv[i] * rf;
+        r[i] = rf;
+        z[i] = zf;
+        acc[0] += (double)rf * (double)zf;
+        acc[1] += ri * dinv64[i] * ri;
+        acc[2] += ri * ri;
+    }
+    double part[3], tot[3];
+    for (int k = 0; k < 3; ++k) {
+        part[k] = block_sum(acc[k], sh);
+        __syncthreads();
+    }
+    if (reduce_n<3>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) {
+        st->outer += 1;
+        st->rho64 = tot[1];
+        st->r2 = tot[2];
+        st->ru = 0;
+        st->nru = 1;
+        if (tot[1] <= st->tol2 * st->rho64_0) {
+            st->done = 1;
+            st->converged = 1;
+        } else if (st->outer >= st->max_outer || st->iter >= st->max_inner) {
+            st->done = 1;
+        } else {
+            st->beta = st->rho > 0.0 ? tot[0] / st->rho : 0.0;
+            st->rho = tot[0];
+            st->rho_ref = tot[0];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_ir_dir(int64_t n, const float *__restrict__ z, float *__restrict__ d, const IrState *st) {
+    if (st->done) return;
+    const float beta = (float)st->beta;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads)
         d[i] = z[i] + beta * d[i];
@@ -225,8 +497,15 @@ __global__ void k_form_b(int64_t n, const uint8_t *__restrict__ mask, const doub
 }  // namespace
 
 void jacobi_dinv(int64_t n, const double *diag, const uint8_t *mask, double *dinv, hipStream_t s) {
-    hipLaunchKernelGGL(k_jacobi_dinv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, diag,
-                       mask, dinv);
+    hipLaunchKernelGGL(k_jacobi_dinv<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                       diag, mask, dinv);
+    H1PA_HIP(hipGetLastError());
+}
+
+void jacobi_dinv_f32(int64_t n, const double *diag, const uint8_t *mask, float *dinv,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(k_jacobi_dinv<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                       diag, mask, dinv);
     H1PA_HIP(hipGetLastError());
 }
 
@@ -244,50 +523,71 @@ void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, co
 
 }  // namespace kern
 
-// Element part of the constrained apply into f.ework0 (d -> A_E d).
-static void pcg_element_apply(Form &f, const uint8_t *mask, const double *d, hipStream_t s) {
+namespace {
+
+// Element part of the constrained apply, d -> A_E d, E-vector in native
+// order (read back through indices_nat).  Stops early once `stop` is set.
+template <typename T>
+void pcg_element_apply(Form &f, int prec, const uint8_t *mask, const T *d, T *ye, const int *stop,
+                       hipStream_t s) {
     Space &sp = *f.space;
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
-    kern::EOut out;  // E-vector in native order, read back through indices_nat
-    out.ye = f.ework0.p;
+    kern::EOutT<T> out;
+    out.ye = ye;
     out.l2n = sp.restr.lex2nat.p;
-    if (diff)
-        kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, sp.restr.gather_map.p,
-                              mask, d, out, false, s);
-    if (mass)
-        kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, sp.restr.gather_map.p, mask, d,
-                         out, diff, s);
+    out.stop = stop;
+    if constexpr (sizeof(T) == 8) {
+        (void)prec;
+        if (diff)
+            kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p,
+                                  sp.restr.gather_map.p, mask, d, out, false, s);
+        if (mass)
+            kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, sp.restr.gather_map.p,
+                             mask, d, out, diff, s);
+    } else {
+        const int kind = prec == H1PA_PREC_BF16_PA ? 1 : 0;
+        if (diff)
+            kern::diffusion_apply_lp(sp.p, f.q1d, sp.ne, f.basis32.p, f.pa_diff_lp.p, kind,
+                                     sp.restr.gather_map.p, mask, d, out, false, s);
+        if (mass)
+            kern::mass_apply_lp(sp.p, f.q1d, sp.ne, f.basis32.p, f.pa_mass_lp.p, kind,
+                                sp.restr.gather_map.p, mask, d, out, diff, s);
+    }
 }
 
-void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
-               const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s) {
+// Jacobi-PCG on the constrained operator with vectors of type T; dinv in T.
+template <typename T>
+void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, T *x, T *ye,
+             const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s) {
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
-    DevBuf<double> r(n), z(n), d(n), dinv(n), partials(kern::kMaxBlocks);
+    DevBuf<T> r(n), z(n), d(n);
+    DevBuf<double> partials(kern::kMaxBlocks);
     DevBuf<PcgState> st(1);
     PcgState h{};
     h.max_iter = o.max_iter;
     H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
-    kern::jacobi_dinv(n, diag, mask, dinv.p, s);
     const unsigned nb = kern::vec_blocks(n);
     const int check = o.check_every > 0 ? o.check_every : 8;
+    const int *stop = &st.p->done;
 
     auto t0 = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(kern::k_pcg_init, dim3(nb), dim3(kern::kThreads), 0, s, n, b, dinv.p, x, r.p,
-                       z.p, d.p, partials.p, st.p, o.rel_tol, o.abs_tol);
+    hipLaunchKernelGGL(kern::k_pcg_init<T>, dim3(nb), dim3(kern::kThreads), 0, s, n, b, dinv, x,
+                       r.p, z.p, d.p, partials.p, st.p, o.rel_tol, o.abs_tol);
     auto apply = [&]() {
-        pcg_element_apply(f, mask, d.p, s);
-        hipLaunchKernelGGL(kern::k_pcg_scatter, dim3(nb), dim3(kern::kThreads), 0, s, n,
-                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask, d.p, z.p,
+        pcg_element_apply<T>(f, prec, mask, d.p, ye, stop, s);
+        hipLaunchKernelGGL(kern::k_pcg_scatter<T>, dim3(nb), dim3(kern::kThreads), 0, s, n,
+                           sp.restr.offsets.p, sp.restr.indices_nat.p, ye, mask, d.p, z.p,
                            partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
     };
     apply();
     H1PA_HIP(hipGetLastError());
     for (int it = 1; it <= o.max_iter; ++it) {
-        hipLaunchKernelGGL(kern::k_pcg_update, dim3(nb), dim3(kern::kThreads), 0, s, n, dinv.p, x,
+        hipLaunchKernelGGL(kern::k_pcg_update<T>, dim3(nb), dim3(kern::kThreads), 0, s, n, dinv, x,
                            r.p, z.p, d.p, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_pcg_dir, dim3(nb), dim3(kern::kThreads), 0, s, n, z.p, d.p, st.p);
+        hipLaunchKernelGGL(kern::k_pcg_dir<T>, dim3(nb), dim3(kern::kThreads), 0, s, n, z.p, d.p,
+                           st.p);
         apply();
         H1PA_HIP(hipGetLastError());
         if (it % check == 0 || it == o.max_iter) {
@@ -303,6 +603,124 @@ void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b
     stats.iterations = h.done ? h.final_iter : o.max_iter;
     stats.initial_norm = h.nom0 >= 0.0 ? std::sqrt(h.nom0) : h.nom0;
     stats.final_norm = h.betanom >= 0.0 ? std::sqrt(h.betanom) : h.betanom;
+    stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // namespace
+
+void ensure_low_precision(Form &f, int precision, hipStream_t s) {
+    const int kind = precision == H1PA_PREC_BF16_PA ? 1 : 0;
+    if (f.lp_kind == kind) return;
+    Space &sp = *f.space;
+    auto convert = [&](const DevBuf<double> &src, DevBuf<uint32_t> &dst) {
+        if (!src.p) {
+            dst.release();
+            return;
+        }
+        const int64_t n = (int64_t)src.n;
+        const unsigned g = (unsigned)((n + 255) / 256);
+        if (kind == 0) {
+            dst.alloc((size_t)n);
+            hipLaunchKernelGGL(kern::k_to_float, dim3(g), dim3(256), 0, s, n, src.p,
+                               reinterpret_cast<float *>(dst.p));
+        } else {
+            dst.alloc((size_t)(n + 1) / 2);
+            hipLaunchKernelGGL(kern::k_to_bf16, dim3(g), dim3(256), 0, s, n, src.p,
+                               reinterpret_cast<uint16_t *>(dst.p));
+        }
+        H1PA_HIP(hipGetLastError());
+    };
+    if (!f.basis32.p) {
+        const int64_t nb = (int64_t)f.basis.n;
+        f.basis32.alloc((size_t)nb);
+        hipLaunchKernelGGL(kern::k_to_float, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, s,
+                           nb, f.basis.p, f.basis32.p);
+        H1PA_HIP(hipGetLastError());
+    }
+    f.lp_kind = -1;
+    convert(f.pa_diff, f.pa_diff_lp);
+    convert(f.pa_mass, f.pa_mass_lp);
+    if (f.ework32.n < (size_t)(sp.ne * sp.dof)) f.ework32.alloc((size_t)(sp.ne * sp.dof));
+    f.lp_kind = kind;
+}
+
+void form_mult_lp(Form &f, int precision, const float *x, float *y, hipStream_t s) {
+    ensure_low_precision(f, precision, s);
+    Space &sp = *f.space;
+    pcg_element_apply<float>(f, precision, nullptr, x, f.ework32.p, nullptr, s);
+    hipLaunchKernelGGL(kern::k_scatter_set<float>, dim3((unsigned)((sp.ndofs + 255) / 256)),
+                       dim3(256), 0, s, sp.ndofs, sp.restr.offsets.p, sp.restr.indices_nat.p,
+                       f.ework32.p, y, sp.restr.perm.p, sp.restr.nperm);
+    H1PA_HIP(hipGetLastError());
+}
+
+void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
+               const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s) {
+    Space &sp = *f.space;
+    DevBuf<double> dinv(sp.ndofs);
+    kern::jacobi_dinv(sp.ndofs, diag, mask, dinv.p, s);
+    pcg_run<double>(f, H1PA_PREC_FP64, mask, dinv.p, b, x, f.ework0.p, o, stats, s);
+}
+
+void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
+              const h1pa_ir_opts &o, h1pa_ir_stats &stats, hipStream_t s) {
+    Space &sp = *f.space;
+    const int64_t n = sp.ndofs;
+    const unsigned nb256 = (unsigned)((n + 255) / 256);
+    const unsigned nb = kern::vec_blocks(n);
+    ensure_low_precision(f, o.inner_precision, s);
+    DevBuf<float> dinv(n), r(n), xs(n), z(n), d(n), q(n);
+    DevBuf<double> dinv64(n), r64(n);
+    DevBuf<double> partials(3 * (size_t)kern::kMaxBlocks);
+    DevBuf<kern::IrState> st(1);
+    hipLaunchKernelGGL(kern::k_jacobi_dinv<float>, dim3(nb256), dim3(256), 0, s, n, diag, mask,
+                       dinv.p);
+    hipLaunchKernelGGL(kern::k_jacobi_dinv<double>, dim3(nb256), dim3(256), 0, s, n, diag, mask,
+                       dinv64.p);
+    kern::IrState h{};
+    h.tol2 = o.rel_tol * o.rel_tol;
+    h.delta2 = o.delta * o.delta;
+    h.max_inner = o.max_inner;
+    h.max_outer = o.max_outer;
+    H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    const int *done = &st.p->done;
+    const int *nru = &st.p->nru;
+    auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(kern::k_ir_init, dim3(nb), dim3(kern::kThreads), 0, s, n, b, dinv64.p,
+                       dinv.p, x, r64.p, r.p, xs.p, z.p, d.p, partials.p, st.p);
+    H1PA_HIP(hipGetLastError());
+    const int limit = o.max_inner + o.max_outer + 1;
+    for (int it = 1; it <= limit; ++it) {
+        // q = A d with the reduced-precision operator
+        pcg_element_apply<float>(f, o.inner_precision, mask, d.p, f.ework32.p, done, s);
+        hipLaunchKernelGGL(kern::k_ir_scatter, dim3(nb), dim3(kern::kThreads), 0, s, n,
+                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework32.p, mask, d.p,
+                           q.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_ir_update, dim3(nb), dim3(kern::kThreads), 0, s, n, dinv.p,
+                           xs.p, r.p, z.p, q.p, d.p, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_ir_ru_x, dim3(nb256), dim3(256), 0, s, n, xs.p, x, st.p);
+        pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, x, f.ework0.p, nru, s);
+        hipLaunchKernelGGL(kern::k_ir_ru_res, dim3(nb), dim3(kern::kThreads), 0, s, n,
+                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask, b, x,
+                           dinv64.p, dinv.p, r64.p, r.p, z.p, partials.p, st.p, sp.restr.perm.p,
+                           sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_ir_dir, dim3(nb), dim3(kern::kThreads), 0, s, n, z.p, d.p,
+                           st.p);
+        H1PA_HIP(hipGetLastError());
+        if (it % 8 == 0 || it == limit) {
+            H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            H1PA_HIP(hipStreamSynchronize(s));
+            if (h.done) break;
+        }
+    }
+    H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    H1PA_HIP(hipStreamSynchronize(s));
+    auto t1 = std::chrono::steady_clock::now();
+    stats.converged = h.converged;
+    stats.outer_iterations = h.outer;
+    stats.inner_iterations = h.iter;
+    stats.final_norm = std::sqrt(std::max(h.rho64, 0.0));
+    stats.rel_residual = h.b2 > 0 ? std::sqrt(h.r2 / h.b2) : 0.0;
     stats.seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 

@@ -316,6 +316,19 @@ def num_threads():
     return lib().orc_num_threads()
 
 
+def round_bf16(a):
+    """FP64 -> FP32 (round to nearest even) -> BF16 (round to nearest even on
+    the dropped 16 bits), returned as FP64 values.  The storage rule of the
+    reduced-precision PA data (no MFEM counterpart)."""
+    f = np.ascontiguousarray(a, dtype=np.float64).astype(np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+    nan = np.isnan(f)
+    out = r.astype(np.uint32).view(np.float32).astype(np.float64)
+    out[nan] = np.nan
+    return out
+
+
 # ---------------------------------------------------------------------------
 # a whole problem, MFEM ex1-style
 # ---------------------------------------------------------------------------
@@ -378,6 +391,17 @@ class Problem:
         op.Mv = _dp(self.Mv) if self.Mv is not None else None
         op.n_ess, op.ess = int(self.ess.size), _ip(self.ess)
         self._op = op
+
+    def replace_pa(self, Dd=None, Mv=None):
+        """Swap in other PA data of the same shape (e.g. rounded copies)."""
+        if Dd is not None:
+            assert self.Dd is not None and Dd.size == self.Dd.size
+            self.Dd = np.ascontiguousarray(Dd, dtype=np.float64)
+            self._op.Dd = _dp(self.Dd)
+        if Mv is not None:
+            assert self.Mv is not None and Mv.size == self.Mv.size
+            self.Mv = np.ascontiguousarray(Mv, dtype=np.float64)
+            self._op.Mv = _dp(self.Mv)
 
     def mult(self, x):
         y = _da(self.ndofs)

@@ -60,3 +60,14 @@ def test_large_topology_is_fast(h1pa):
     m = h1pa.Mesh.MakeCartesian3D(64, 64, 64)
     assert m.GetNE() == 64 ** 3
     assert time.time() - t0 < 5.0
+
+
+def test_mixed_precision_argument_errors(h1pa):
+    """Argument checks run before any device work, so they hold without a GPU."""
+    lib = h1pa.lib()
+    o = h1pa.IrOpts(h1pa.PREC_FP32, 1e-10, 1e-4, 10, 100)
+    st = h1pa.IrStats()
+    assert lib.h1pa_ir_solve(None, None, 0, None, None, ctypes.byref(o), ctypes.byref(st),
+                             None) == h1pa.H1PA_EINVAL
+    assert b"null argument" in lib.h1pa_last_error()
+    assert lib.h1pa_form_mult_f32(None, h1pa.PREC_FP32, None, None, None) == h1pa.H1PA_EINVAL
