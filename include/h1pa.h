@@ -186,6 +186,11 @@ int h1pa_form_constrained_mult(h1pa_form *form, const int *d_ess, int64_t n_ess,
 int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess,
                             const double *d_x, const double *d_b, double *d_X, double *d_B,
                             void *stream);
+/* Vector::Randomize (linalg/vector.cpp:955-966) on the device with a
+ * counter-based generator: d_x[i] = lo + (hi - lo) u_i, u_i =
+ * (splitmix64(seed ^ splitmix64(i)) >> 11) 2^-53, reproducible for any n. */
+int h1pa_vector_randomize(double *d_x, int64_t n, uint64_t seed, double lo, double hi,
+                          void *stream);
 /* DomainLFIntegrator(ConstantCoefficient(f)) assembled into an L-vector with
  * q1d = p+1 (fem/lininteg.cpp:51-79, LinearForm::Assemble). */
 int h1pa_space_domain_lf(h1pa_space *space, double f, double *d_b, void *stream);
@@ -286,10 +291,12 @@ int h1pa_batch_destroy(h1pa_batch *batch);
  * opts->rel_tol / abs_tol / max_iter are the CGSolver settings in FP64 mode;
  * in mixed mode opts->rel_tol is h1pa_ir_opts.rel_tol (abs_tol, max_iter
  * unused: the batch's max_outer / max_inner apply).
- * `tag` is returned by h1pa_batch_poll; *slot receives the slot index. */
+ * `tag` is returned by h1pa_batch_poll; *slot receives the slot index.
+ * `stream` is the stream that assembled the form and wrote d_b (the batch's
+ * own stream waits for the work queued there so far). */
 int h1pa_batch_add(h1pa_batch *batch, h1pa_form *form, const int *d_ess, int64_t n_ess,
                    const double *d_b, double *d_x, const h1pa_pcg_opts *opts, int64_t tag,
-                   int *slot);
+                   int *slot, void *stream);
 /* Run up to max_replays replays, stopping early when no solve is running;
  * *n_running receives the number of solves still running. */
 int h1pa_batch_run(h1pa_batch *batch, int max_replays, int *n_running);

@@ -131,7 +131,8 @@ SIGNATURES = {
     "h1pa_batch_create": (_i, [C.POINTER(BatchOpts), C.POINTER(_vp)]),
     "h1pa_batch_destroy": (_i, [_vp]),
     "h1pa_batch_add": (_i, [_vp, _vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), _i64,
-                            C.POINTER(_i)]),
+                            C.POINTER(_i), _vp]),
+    "h1pa_vector_randomize": (_i, [_vp, _i64, C.c_uint64, _d, _d, _vp]),
     "h1pa_batch_run": (_i, [_vp, _i, C.POINTER(_i)]),
     "h1pa_batch_poll": (_i, [_vp, _i, _pi64, C.POINTER(BatchResult), C.POINTER(_i)]),
     "h1pa_batch_info": (_i, [_vp, _pi64]),
@@ -813,6 +814,14 @@ class IRSolver:
         return bool(self.stats.converged)
 
 
+def Randomize(x, seed, lo=0.0, hi=1.0):
+    """Vector::Randomize on the device (linalg/vector.cpp:955-966), with a
+    counter-based generator: x[i] = lo + (hi - lo) u_i, reproducible."""
+    _check_vec(x, x.numel())
+    _check(lib().h1pa_vector_randomize(_ptr(x), x.numel(), int(seed) & ((1 << 64) - 1),
+                                       float(lo), float(hi), _stream()))
+
+
 class BatchSolver:
     """Many independent Jacobi-PCG (or mixed-precision) solves advanced
     together on the device; no MFEM counterpart.  Each added solve keeps the
@@ -848,7 +857,7 @@ class BatchSolver:
         o = PcgOpts(float(rel_tol), float(abs_tol), int(max_iter), 8)
         slot = C.c_int()
         _check(lib().h1pa_batch_add(self._h, f._h, _ptr(A.ess), A.ess.numel(), _ptr(B), _ptr(X),
-                                    C.byref(o), int(tag), C.byref(slot)))
+                                    C.byref(o), int(tag), C.byref(slot), _stream()))
         self._keep[int(tag)] = (A, B, X, f)
         return int(tag)
 

@@ -17,6 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libh1pa.so")
+SERVE = os.path.join(HERE, "h1pa_serve")
 
 SOURCES = ["basis.cpp", "mesh.cpp", "capi.cpp", "restriction.hip", "geometry.hip",
            "apply.hip", "pcg.hip", "batch.hip"]
@@ -55,8 +56,16 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the serving driver: a native executable over the C ABI
+    tmp = SERVE + ".tmp"
+    cmd = [_hipcc(), "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "serve_main.cpp"),
+           "-o", tmp, f"-L{HERE}", "-lh1pa", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"h1pa_serve build failed:\n{r.stderr}")
+    os.replace(tmp, SERVE)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {SERVE}")
     return LIB
 
 

@@ -18,10 +18,11 @@
 // (FP32 or BF16-stored PA data) with FP64 reliable updates, stopping on
 // CGSolver's measure taken on the FP64 true residual.  Every kernel skips,
 // block-uniformly, the slots that are not in the phase it serves.  Dot
-// products are reduced per slot over fixed chunks of that slot's DOFs, and
-// the last chunk to finish sums the chunk partials in index order, so a
-// slot's iterates do not depend on which other slots share the batch: a
-// solve gives bitwise the same result alone and in any mix.
+// products are reduced per slot over fixed chunks of that slot's DOFs: each
+// chunk's block stores its partial, and a finalize kernel (one block per
+// running slot) sums them in index order and applies the step's scalar
+// logic.  So a slot's iterates do not depend on which other slots share the
+// batch: a solve gives bitwise the same result alone and in any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,7 +48,6 @@ struct SlotState {
     double rho_ref, rho64, rho64_0, r2, b2;    // mixed: refinement bookkeeping
     int iter, in_loop, max_iter, converged, final_iter;
     int phase, outer;
-    unsigned counter;
 };
 
 struct SlotDesc {
@@ -95,29 +95,28 @@ __device__ inline double bsum(double v, double *sh) {
     return r;
 }
 
-// Writes the chunk partial(s); in the last chunk of the slot to finish,
-// returns true with the slot totals (fixed order) in tot[0..NV).
+// partials[NV (part0 + chunk) + k] = block sum of acc[k]: plain stores, no
+// fence (a device-scope fence writes back and invalidates the L2 here); the
+// finalize kernels read them after the kernel boundary.
 template <int NV>
-__device__ inline bool slot_reduce(const double (&part)[NV], const SlotDesc &sd, SlotState *st,
-                                   int chunk, double *partials, double *sh, double (&tot)[NV]) {
-    __shared__ int last;
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; ++k) partials[(int64_t)NV * (sd.part0 + chunk) + k] = part[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(&st->counter, 1u);
-        last = (prev == (unsigned)sd.nchunks - 1u);
+__device__ inline void store_chunk(const double (&acc)[NV], const SlotDesc &sd, int chunk,
+                                   double *partials, double *sh) {
+    for (int k = 0; k < NV; ++k) {
+        const double v = bsum(acc[k], sh);
+        if (threadIdx.x == 0) partials[(int64_t)NV * (sd.part0 + chunk) + k] = v;
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    const volatile double *vp = partials + (int64_t)NV * sd.part0;
+}
+
+// Finalize block: the slot's totals, chunk partials summed in index order.
+template <int NV>
+__device__ inline void slot_totals(const SlotDesc &sd, const double *partials, double *sh,
+                                   double (&tot)[NV]) {
+    const double *p = partials + (int64_t)NV * sd.part0;
     for (int k = 0; k < NV; ++k) {
         double v = 0.0;
-        for (int i = threadIdx.x; i < sd.nchunks; i += kT) v += vp[(int64_t)NV * i + k];
+        for (int i = threadIdx.x; i < sd.nchunks; i += kT) v += p[(int64_t)NV * i + k];
         tot[k] = bsum(v, sh);
     }
-    if (threadIdx.x == 0) st->counter = 0u;
-    return true;
 }
 
 template <typename T>
@@ -129,19 +128,31 @@ __device__ inline T *vp(void *p) { return reinterpret_cast<T *>(p); }
                  i##_end = min(sd.n, (int64_t)(tk.chunk + 1) * kChunk);                    \
          i < i##_end; i += kT)
 
+// Producer kernels run one block per (slot, chunk) work item and skip the
+// slots not in the phase they serve; each finalize kernel runs one block per
+// running slot (runlist) and applies the scalar logic of that step.
+#define H1PA_TASK_PROLOGUE(PHASE_OK)                                              \
+    const VecTask tk = tasks[blockIdx.x];                                         \
+    const SlotDesc &sd = desc[tk.slot];                                           \
+    const int ph_ = states[tk.slot].phase;                                        \
+    if (!(PHASE_OK)) return;
+#define H1PA_FIN_PROLOGUE(PHASE_OK)                                               \
+    const int slot = runlist[blockIdx.x];                                         \
+    const SlotDesc &sd = desc[slot];                                              \
+    SlotState *st = states + slot;                                                \
+    const int ph_ = st->phase;                                                    \
+    if (!(PHASE_OK)) return;
+
 // ---- FP64 mode: CGSolver on each slot ---------------------------------------
 __global__ void __launch_bounds__(kT)
 kb_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
-        SlotState *__restrict__ states, double *__restrict__ partials) {
+        const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    SlotState *st = states + tk.slot;
-    if (st->phase != PH_INIT) return;
+    H1PA_TASK_PROLOGUE(ph_ == PH_INIT)
     double *r = vp<double>(sd.r), *z = vp<double>(sd.z), *d = vp<double>(sd.d);
     double *x = sd.x;
     const double *dinv = vp<double>(sd.dinv);
-    double acc = 0.0;
+    double acc[1] = {0.0};
     H1PA_CHUNK_LOOP(i) {
         const double ri = sd.b[i];
         const double zi = dinv[i] * ri;
@@ -149,38 +160,43 @@ kb_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
         x[i] = 0.0;
         z[i] = zi;
         d[i] = zi;
-        acc += zi * ri;
+        acc[0] += zi * ri;
     }
-    double part[1] = {bsum(acc, sh)}, tot[1];
-    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
-        const double nom = tot[0];
-        st->nom = st->nom0 = st->betanom = nom;
-        st->iter = 1;
-        st->in_loop = 0;
-        st->final_iter = 0;
-        double r0 = nom * sd.rel_tol * sd.rel_tol;
-        if (sd.abs_tol * sd.abs_tol > r0) r0 = sd.abs_tol * sd.abs_tol;
-        st->r0 = r0;
-        int next = PH_INNER;
-        if (nom < 0.0) {
-            next = PH_DONE;
-        } else if (nom <= r0) {
-            st->converged = 1;
-            next = PH_DONE;
-        }
-        st->phase = next;
+    store_chunk<1>(acc, sd, tk.chunk, partials, sh);
+}
+
+__global__ void __launch_bounds__(kT)
+kb_fin_init(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
+            SlotState *__restrict__ states, const double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    H1PA_FIN_PROLOGUE(ph_ == PH_INIT)
+    double tot[1];
+    slot_totals<1>(sd, partials, sh, tot);
+    if (threadIdx.x != 0) return;
+    const double nom = tot[0];
+    st->nom = st->nom0 = st->betanom = nom;
+    st->iter = 1;
+    st->in_loop = 0;
+    st->final_iter = 0;
+    double r0 = nom * sd.rel_tol * sd.rel_tol;
+    if (sd.abs_tol * sd.abs_tol > r0) r0 = sd.abs_tol * sd.abs_tol;
+    st->r0 = r0;
+    int next = PH_INNER;
+    if (nom < 0.0) {
+        next = PH_DONE;
+    } else if (nom <= r0) {
+        st->converged = 1;
+        next = PH_DONE;
     }
+    st->phase = next;
 }
 
 // ---- mixed mode: FP32 inner CG with FP64 reliable updates ---------------------
 __global__ void __launch_bounds__(kT)
 kbm_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
-         SlotState *__restrict__ states, double *__restrict__ partials) {
+         const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    SlotState *st = states + tk.slot;
-    if (st->phase != PH_INIT) return;
+    H1PA_TASK_PROLOGUE(ph_ == PH_INIT)
     float *r = vp<float>(sd.r), *z = vp<float>(sd.z), *d = vp<float>(sd.d);
     float *xs = vp<float>(sd.xi);
     const float *dinv = vp<float>(sd.dinv);
@@ -199,20 +215,28 @@ kbm_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
         acc[1] += bi * sd.dinv64[i] * bi;
         acc[2] += bi * bi;
     }
-    double part[3] = {bsum(acc[0], sh), bsum(acc[1], sh), bsum(acc[2], sh)}, tot[3];
-    if (slot_reduce<3>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
-        st->nom = st->rho_ref = tot[0];
-        st->rho64 = st->rho64_0 = tot[1];
-        st->r2 = st->b2 = tot[2];
-        st->iter = 0;
-        st->outer = 0;
-        st->converged = 0;
-        if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
-            st->converged = 1;
-            st->phase = PH_DONE;
-        } else {
-            st->phase = PH_INNER;
-        }
+    store_chunk<3>(acc, sd, tk.chunk, partials, sh);
+}
+
+__global__ void __launch_bounds__(kT)
+kbm_fin_init(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
+             SlotState *__restrict__ states, const double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    H1PA_FIN_PROLOGUE(ph_ == PH_INIT)
+    double tot[3];
+    slot_totals<3>(sd, partials, sh, tot);
+    if (threadIdx.x != 0) return;
+    st->nom = st->rho_ref = tot[0];
+    st->rho64 = st->rho64_0 = tot[1];
+    st->r2 = st->b2 = tot[2];
+    st->iter = 0;
+    st->outer = 0;
+    st->converged = 0;
+    if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
+        st->converged = 1;
+        st->phase = PH_DONE;
+    } else {
+        st->phase = PH_INNER;
     }
 }
 
@@ -243,23 +267,17 @@ kb_diffusion(const ElemTask *__restrict__ tasks, const SlotDesc *__restrict__ de
     }
 }
 
-// z = A d for PH_INNER slots (DIAG_ONE rows copy d); den = (d, z).
-template <typename T, bool MIXED>
+// z = A d for PH_INNER slots (DIAG_ONE rows copy d); partials of (d, z).
+template <typename T>
 __global__ void __launch_bounds__(kT)
 kb_scatter(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
-           SlotState *__restrict__ states, double *__restrict__ partials) {
+           const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    SlotState *st = states + tk.slot;
-    if (st->phase != PH_INNER) return;
+    H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
     const T *ye = vp<T>(sd.ye), *d = vp<T>(sd.d);
     T *z = vp<T>(sd.z);
-    double acc = 0.0;
-    const int64_t i0 = (int64_t)tk.chunk * kChunk;
-    for (int k = 0; k < kPer; ++k) {
-        const int64_t t = i0 + k * kT + threadIdx.x;
-        if (t >= sd.n) break;
+    double acc[1] = {0.0};
+    H1PA_CHUNK_LOOP(t) {
         const int64_t i = t < sd.nperm ? sd.perm[t] : t;
         T v;
         if (sd.mask && sd.mask[i]) {
@@ -272,98 +290,119 @@ kb_scatter(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
             }
         }
         z[i] = v;
-        acc += (double)d[i] * (double)v;
+        acc[0] += (double)d[i] * (double)v;
     }
-    double part[1] = {bsum(acc, sh)}, tot[1];
-    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
-        st->den = tot[0];
-        if (tot[0] == 0.0) {
-            if (MIXED) {
-                st->phase = PH_RU;  // inner residual is zero: check the true one
-            } else {            // CGSolver: den == 0 ends the loop
-                st->final_iter = st->in_loop ? st->iter : 0;
-                st->phase = PH_DONE;
-            }
+    store_chunk<1>(acc, sd, tk.chunk, partials, sh);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(kT)
+kb_fin_den(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
+           SlotState *__restrict__ states, const double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    H1PA_FIN_PROLOGUE(ph_ == PH_INNER)
+    double tot[1];
+    slot_totals<1>(sd, partials, sh, tot);
+    if (threadIdx.x != 0) return;
+    st->den = tot[0];
+    if (tot[0] == 0.0) {
+        if (MIXED) {
+            st->phase = PH_RU;  // inner residual is zero: check the true one
+        } else {            // CGSolver: den == 0 ends the loop
+            st->final_iter = st->in_loop ? st->iter : 0;
+            st->phase = PH_DONE;
         }
     }
 }
 
 __global__ void __launch_bounds__(kT)
 kb_update(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
-          SlotState *__restrict__ states, double *__restrict__ partials) {
+          const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    SlotState *st = states + tk.slot;
-    if (st->phase != PH_INNER) return;
+    H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
+    const SlotState *st = states + tk.slot;
     const double alpha = st->nom / st->den;
     double *x = sd.x, *r = vp<double>(sd.r), *z = vp<double>(sd.z);
     const double *d = vp<double>(sd.d), *dinv = vp<double>(sd.dinv);
-    double acc = 0.0;
+    double acc[1] = {0.0};
     H1PA_CHUNK_LOOP(i) {
         x[i] += alpha * d[i];
         const double ri = r[i] - alpha * z[i];
         r[i] = ri;
         const double zi = dinv[i] * ri;
         z[i] = zi;
-        acc += ri * zi;
+        acc[0] += ri * zi;
     }
-    double part[1] = {bsum(acc, sh)}, tot[1];
-    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
-        const double betanom = tot[0];
-        const int i = st->iter;
-        st->in_loop = 1;
-        st->betanom = betanom;
-        if (betanom < 0.0) {
-            st->final_iter = i;
-            st->phase = PH_DONE;
-        } else if (betanom <= st->r0) {
-            st->converged = 1;
-            st->final_iter = i;
-            st->phase = PH_DONE;
-        } else if (i + 1 > st->max_iter) {
-            st->final_iter = st->max_iter;
-            st->phase = PH_DONE;
-        } else {
-            st->beta = betanom / st->nom;
-            st->nom = betanom;
-            st->iter = i + 1;
-        }
+    store_chunk<1>(acc, sd, tk.chunk, partials, sh);
+}
+
+__global__ void __launch_bounds__(kT)
+kb_fin_update(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
+              SlotState *__restrict__ states, const double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    H1PA_FIN_PROLOGUE(ph_ == PH_INNER)
+    double tot[1];
+    slot_totals<1>(sd, partials, sh, tot);
+    if (threadIdx.x != 0) return;
+    const double betanom = tot[0];
+    const int i = st->iter;
+    st->in_loop = 1;
+    st->betanom = betanom;
+    if (betanom < 0.0) {
+        st->final_iter = i;
+        st->phase = PH_DONE;
+    } else if (betanom <= st->r0) {
+        st->converged = 1;
+        st->final_iter = i;
+        st->phase = PH_DONE;
+    } else if (i + 1 > st->max_iter) {
+        st->final_iter = st->max_iter;
+        st->phase = PH_DONE;
+    } else {
+        st->beta = betanom / st->nom;
+        st->nom = betanom;
+        st->iter = i + 1;
     }
 }
 
 __global__ void __launch_bounds__(kT)
 kbm_update(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
-           SlotState *__restrict__ states, double *__restrict__ partials) {
+           const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    SlotState *st = states + tk.slot;
-    if (st->phase != PH_INNER) return;
+    H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
+    const SlotState *st = states + tk.slot;
     const float alpha = (float)(st->nom / st->den);
     float *xs = vp<float>(sd.xi), *r = vp<float>(sd.r), *z = vp<float>(sd.z);
     const float *d = vp<float>(sd.d), *dinv = vp<float>(sd.dinv);
-    double acc = 0.0;
+    double acc[1] = {0.0};
     H1PA_CHUNK_LOOP(i) {
         xs[i] += alpha * d[i];
         const float ri = r[i] - alpha * z[i];
         r[i] = ri;
         const float zi = dinv[i] * ri;
         z[i] = zi;
-        acc += (double)ri * (double)zi;
+        acc[0] += (double)ri * (double)zi;
     }
-    double part[1] = {bsum(acc, sh)}, tot[1];
-    if (slot_reduce<1>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
-        const double rho = tot[0];
-        st->iter += 1;
-        const double tol2 = sd.rel_tol * sd.rel_tol;
-        if (rho <= sd.delta * sd.delta * st->rho_ref || rho <= tol2 * st->rho64_0 ||
-            st->iter >= st->max_iter) {
-            st->phase = PH_RU;
-        } else {
-            st->beta = rho / st->nom;
-            st->nom = rho;
-        }
+    store_chunk<1>(acc, sd, tk.chunk, partials, sh);
+}
+
+__global__ void __launch_bounds__(kT)
+kbm_fin_update(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
+               SlotState *__restrict__ states, const double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    H1PA_FIN_PROLOGUE(ph_ == PH_INNER)
+    double tot[1];
+    slot_totals<1>(sd, partials, sh, tot);
+    if (threadIdx.x != 0) return;
+    const double rho = tot[0];
+    st->iter += 1;
+    const double tol2 = sd.rel_tol * sd.rel_tol;
+    if (rho <= sd.delta * sd.delta * st->rho_ref || rho <= tol2 * st->rho64_0 ||
+        st->iter >= st->max_iter) {
+        st->phase = PH_RU;
+    } else {
+        st->beta = rho / st->nom;
+        st->nom = rho;
     }
 }
 
@@ -371,11 +410,8 @@ template <typename T>
 __global__ void __launch_bounds__(kT)
 kb_dir(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
        const SlotState *__restrict__ states) {
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    const SlotState *st = states + tk.slot;
-    if (st->phase != PH_INNER) return;
-    const T beta = (T)st->beta;
+    H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
+    const T beta = (T)states[tk.slot].beta;
     const T *z = vp<T>(sd.z);
     T *d = vp<T>(sd.d);
     H1PA_CHUNK_LOOP(i) d[i] = z[i] + beta * d[i];
@@ -385,9 +421,7 @@ kb_dir(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
 __global__ void __launch_bounds__(kT)
 kbm_ru_x(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
          const SlotState *__restrict__ states) {
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    if (states[tk.slot].phase != PH_RU) return;
+    H1PA_TASK_PROLOGUE(ph_ == PH_RU)
     float *xs = vp<float>(sd.xi);
     H1PA_CHUNK_LOOP(i) {
         sd.x[i] += (double)xs[i];
@@ -396,22 +430,17 @@ kbm_ru_x(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
 }
 
 // Refinement step, part 2 (PH_RU, after the FP64 element kernel on x):
-// r64 = b - A x, (B r64, r64), ||r64||^2, r = (float) r64, z = dinv r, (r, z).
+// r64 = b - A x, r = (float) r64, z = dinv r; partials of (r, z), (B r64, r64)
+// and ||r64||^2.
 __global__ void __launch_bounds__(kT)
 kbm_ru_res(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
-           SlotState *__restrict__ states, double *__restrict__ partials) {
+           const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
-    const VecTask tk = tasks[blockIdx.x];
-    const SlotDesc &sd = desc[tk.slot];
-    SlotState *st = states + tk.slot;
-    if (st->phase != PH_RU) return;
+    H1PA_TASK_PROLOGUE(ph_ == PH_RU)
     float *r = vp<float>(sd.r), *z = vp<float>(sd.z);
     const float *dinv = vp<float>(sd.dinv);
     double acc[3] = {0.0, 0.0, 0.0};
-    const int64_t i0 = (int64_t)tk.chunk * kChunk;
-    for (int k = 0; k < kPer; ++k) {
-        const int64_t t = i0 + k * kT + threadIdx.x;
-        if (t >= sd.n) break;
+    H1PA_CHUNK_LOOP(t) {
         const int64_t i = t < sd.nperm ? sd.perm[t] : t;
         double y;
         if (sd.mask && sd.mask[i]) {
@@ -433,22 +462,30 @@ kbm_ru_res(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
         acc[1] += ri * sd.dinv64[i] * ri;
         acc[2] += ri * ri;
     }
-    double part[3] = {bsum(acc[0], sh), bsum(acc[1], sh), bsum(acc[2], sh)}, tot[3];
-    if (slot_reduce<3>(part, sd, st, tk.chunk, partials, sh, tot) && threadIdx.x == 0) {
-        st->outer += 1;
-        st->rho64 = tot[1];
-        st->r2 = tot[2];
-        if (tot[1] <= sd.rel_tol * sd.rel_tol * st->rho64_0) {
-            st->converged = 1;
-            st->phase = PH_DONE;
-        } else if (st->outer >= sd.max_outer || st->iter >= st->max_iter) {
-            st->phase = PH_DONE;
-        } else {
-            st->beta = st->nom > 0.0 ? tot[0] / st->nom : 0.0;
-            st->nom = tot[0];
-            st->rho_ref = tot[0];
-            st->phase = PH_INNER;
-        }
+    store_chunk<3>(acc, sd, tk.chunk, partials, sh);
+}
+
+__global__ void __launch_bounds__(kT)
+kbm_fin_ru(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
+           SlotState *__restrict__ states, const double *__restrict__ partials) {
+    __shared__ double sh[kT];
+    H1PA_FIN_PROLOGUE(ph_ == PH_RU)
+    double tot[3];
+    slot_totals<3>(sd, partials, sh, tot);
+    if (threadIdx.x != 0) return;
+    st->outer += 1;
+    st->rho64 = tot[1];
+    st->r2 = tot[2];
+    if (tot[1] <= sd.rel_tol * sd.rel_tol * st->rho64_0) {
+        st->converged = 1;
+        st->phase = PH_DONE;
+    } else if (st->outer >= sd.max_outer || st->iter >= st->max_iter) {
+        st->phase = PH_DONE;
+    } else {
+        st->beta = st->nom > 0.0 ? tot[0] / st->nom : 0.0;
+        st->nom = tot[0];
+        st->rho_ref = tot[0];
+        st->phase = PH_INNER;
     }
 }
 
@@ -584,6 +621,8 @@ struct h1pa_batch {
     h1pa::DevBuf<double> partials;
     h1pa::DevBuf<h1pa::VecTask> vtasks;
     int nvtasks = 0;
+    h1pa::DevBuf<int> runlist;  // slots with work items, one finalize block each
+    int nrun = 0;
     bool dirty = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -648,6 +687,7 @@ void rebuild(Batch &B) {
     const int ns = (int)B.slots.size();
     std::vector<SlotDesc> desc(ns);
     std::vector<VecTask> vt;
+    std::vector<int> run;
     int part = 0;
     for (int k = 0; k < ns; ++k) {
         BatchSlot &sl = *B.slots[k];
@@ -660,6 +700,7 @@ void rebuild(Batch &B) {
         desc[k].nchunks = nch;
         part += nch;
         for (int c = 0; c < nch; ++c) vt.push_back({k, c});
+        run.push_back(k);
     }
     for (int k = 0; k < ns; ++k) B.slots[k]->desc = desc[k];
     if (B.d_desc.n < (size_t)ns) B.d_desc.alloc(std::max<size_t>(ns, 2 * B.d_desc.n));
@@ -668,6 +709,9 @@ void rebuild(Batch &B) {
     B.nvtasks = (int)vt.size();
     if (B.vtasks.n < vt.size()) B.vtasks.alloc(vt.size());
     B.vtasks.upload(vt.data(), vt.size(), B.stream);
+    B.nrun = (int)run.size();
+    if (B.runlist.n < run.size()) B.runlist.alloc(run.size());
+    B.runlist.upload(run.data(), run.size(), B.stream);
     std::vector<ElemTask> et;
     for (size_t g = 0; g < B.groups.size(); ++g) {
         BatchGroup &G = B.groups[g];
@@ -738,41 +782,42 @@ void enqueue_replay(Batch &B, hipStream_t s) {
     const unsigned nv = (unsigned)B.nvtasks;
     if (nv == 0) return;
     const int prec = B.o.precision;
-    const dim3 grid(nv), blk(kern::kT);
+    const dim3 grid(nv), blk(kern::kT), fin((unsigned)B.nrun);
+    const VecTask *vt = B.vtasks.p;
+    const int *rl = B.runlist.p;
+    const SlotDesc *ds = B.d_desc.p;
+    SlotState *st = B.d_state.p;
+    double *pt = B.partials.p;
+    auto elements = [&](bool ru) {
+        for (BatchGroup &G : B.groups)
+            launch_group(prec, ru, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks, ds, st,
+                         ru ? (const void *)G.basis64 : G.basis, 0, s);
+    };
     if (!mixed_mode(B)) {
-        hipLaunchKernelGGL(kern::kb_init, grid, blk, 0, s, B.vtasks.p, B.d_desc.p, B.d_state.p,
-                           B.partials.p);
+        hipLaunchKernelGGL(kern::kb_init, grid, blk, 0, s, vt, ds, st, pt);
+        hipLaunchKernelGGL(kern::kb_fin_init, fin, blk, 0, s, rl, ds, st, pt);
         for (int it = 0; it < B.o.iters_per_replay; ++it) {
-            for (BatchGroup &G : B.groups)
-                launch_group(prec, false, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks,
-                             B.d_desc.p, B.d_state.p, G.basis, 0, s);
-            hipLaunchKernelGGL((kern::kb_scatter<double, false>), grid, blk, 0, s, B.vtasks.p,
-                               B.d_desc.p, B.d_state.p, B.partials.p);
-            hipLaunchKernelGGL(kern::kb_update, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
-                               B.d_state.p, B.partials.p);
-            hipLaunchKernelGGL(kern::kb_dir<double>, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
-                               B.d_state.p);
+            elements(false);
+            hipLaunchKernelGGL(kern::kb_scatter<double>, grid, blk, 0, s, vt, ds, st, pt);
+            hipLaunchKernelGGL(kern::kb_fin_den<false>, fin, blk, 0, s, rl, ds, st, pt);
+            hipLaunchKernelGGL(kern::kb_update, grid, blk, 0, s, vt, ds, st, pt);
+            hipLaunchKernelGGL(kern::kb_fin_update, fin, blk, 0, s, rl, ds, st, pt);
+            hipLaunchKernelGGL(kern::kb_dir<double>, grid, blk, 0, s, vt, ds, st);
         }
     } else {
-        hipLaunchKernelGGL(kern::kbm_init, grid, blk, 0, s, B.vtasks.p, B.d_desc.p, B.d_state.p,
-                           B.partials.p);
+        hipLaunchKernelGGL(kern::kbm_init, grid, blk, 0, s, vt, ds, st, pt);
+        hipLaunchKernelGGL(kern::kbm_fin_init, fin, blk, 0, s, rl, ds, st, pt);
         for (int it = 0; it < B.o.iters_per_replay; ++it) {
-            for (BatchGroup &G : B.groups)
-                launch_group(prec, false, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks,
-                             B.d_desc.p, B.d_state.p, G.basis, 0, s);
-            hipLaunchKernelGGL((kern::kb_scatter<float, true>), grid, blk, 0, s, B.vtasks.p,
-                               B.d_desc.p, B.d_state.p, B.partials.p);
-            hipLaunchKernelGGL(kern::kbm_update, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
-                               B.d_state.p, B.partials.p);
-            hipLaunchKernelGGL(kern::kbm_ru_x, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
-                               B.d_state.p);
-            for (BatchGroup &G : B.groups)
-                launch_group(prec, true, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks,
-                             B.d_desc.p, B.d_state.p, G.basis64, 0, s);
-            hipLaunchKernelGGL(kern::kbm_ru_res, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
-                               B.d_state.p, B.partials.p);
-            hipLaunchKernelGGL(kern::kb_dir<float>, grid, blk, 0, s, B.vtasks.p, B.d_desc.p,
-                               B.d_state.p);
+            elements(false);
+            hipLaunchKernelGGL(kern::kb_scatter<float>, grid, blk, 0, s, vt, ds, st, pt);
+            hipLaunchKernelGGL(kern::kb_fin_den<true>, fin, blk, 0, s, rl, ds, st, pt);
+            hipLaunchKernelGGL(kern::kbm_update, grid, blk, 0, s, vt, ds, st, pt);
+            hipLaunchKernelGGL(kern::kbm_fin_update, fin, blk, 0, s, rl, ds, st, pt);
+            hipLaunchKernelGGL(kern::kbm_ru_x, grid, blk, 0, s, vt, ds, st);
+            elements(true);
+            hipLaunchKernelGGL(kern::kbm_ru_res, grid, blk, 0, s, vt, ds, st, pt);
+            hipLaunchKernelGGL(kern::kbm_fin_ru, fin, blk, 0, s, rl, ds, st, pt);
+            hipLaunchKernelGGL(kern::kb_dir<float>, grid, blk, 0, s, vt, ds, st);
         }
     }
     H1PA_HIP(hipGetLastError());

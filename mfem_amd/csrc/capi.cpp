@@ -698,6 +698,14 @@ int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess, co
     API_END
 }
 
+int h1pa_vector_randomize(double *x, int64_t n, uint64_t seed, double lo, double hi,
+                          void *stream) {
+    API_BEGIN
+    REQUIRE(n >= 0 && (n == 0 || x), "vector_randomize: bad argument");
+    kern::randomize(n, x, seed, lo, hi, as_stream(stream));
+    API_END
+}
+
 int h1pa_space_domain_lf(h1pa_space *space, double fval, double *b, void *stream) {
     API_BEGIN
     REQUIRE(space && b, "space_domain_lf: null argument");
@@ -787,12 +795,20 @@ int h1pa_batch_destroy(h1pa_batch *batch) {
 
 int h1pa_batch_add(h1pa_batch *batch, h1pa_form *form, const int *d_ess, int64_t n_ess,
                    const double *b, double *x, const h1pa_pcg_opts *opts, int64_t tag,
-                   int *slot) {
+                   int *slot, void *stream) {
     API_BEGIN
     REQUIRE(batch && form && b && x && opts && (n_ess == 0 || d_ess), "batch_add: null argument");
     Form &f = form->f;
     require_assembled(f);
     hipStream_t s = batch_stream(*batch);
+    {   // order the batch stream after the caller's work on `stream`
+        hipEvent_t ev;
+        H1PA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        hipError_t e1 = hipEventRecord(ev, as_stream(stream));
+        hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(s, ev, 0) : e1;
+        (void)hipEventDestroy(ev);
+        H1PA_HIP(e2);
+    }
     const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
     const int k = batch_add(*batch, form, mask, b, x, *opts, tag);
     if (slot) *slot = k;

@@ -42,7 +42,7 @@ struct Fourier8 {           // g(x) = sum_m a cos(2 pi k_m . x + phi_m)
     double a;
 };
 
-inline uint64_t splitmix64(uint64_t x) {
+__host__ __device__ inline uint64_t splitmix64(uint64_t x) {
     x += 0x9E3779B97F4A7C15ULL;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
     x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
@@ -65,6 +65,16 @@ Fourier8 make_fourier(uint64_t seed) {
     }
     f.a = 0.5 / std::sqrt(8.0);
     return f;
+}
+
+// x[i] = lo + (hi - lo) u_i, u_i = (splitmix64(seed ^ splitmix64(i)) >> 11) 2^-53.
+__global__ void k_randomize(int64_t n, double *__restrict__ x, uint64_t seed, double lo,
+                            double hi) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double u = (double)(splitmix64(seed ^ splitmix64((uint64_t)i)) >> 11) *
+                     (1.0 / 9007199254740992.0);
+    x[i] = lo + (hi - lo) * u;
 }
 
 // ---- nodes -------------------------------------------------------------------
@@ -286,6 +296,13 @@ void pa_setup(const Mesh &m, const Form &f, hipStream_t s) {
     hipLaunchKernelGGL(k_pa_setup, dim3(nblocks(n)), dim3(kBlock), 0, s, m.ne, gb, m.d_nodes.p,
                        f.desc.coeff_kind, f.desc.coeff_value, f.desc.d_coeff, fm, f.pa_diff.p,
                        f.pa_mass.p);
+    H1PA_HIP(hipGetLastError());
+}
+
+void randomize(int64_t n, double *x, uint64_t seed, double lo, double hi, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_randomize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, x, seed,
+                       lo, hi);
     H1PA_HIP(hipGetLastError());
 }
 

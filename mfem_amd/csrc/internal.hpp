@@ -185,6 +185,7 @@ void geometric_factors(const Mesh &m, int q1d, double *X, double *J, double *det
                        hipStream_t s);
 void pa_setup(const Mesh &m, const Form &f, hipStream_t s);
 void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s);
+void randomize(int64_t n, double *x, uint64_t seed, double lo, double hi, hipStream_t s);
 
 // E->E: ye = A_E xe (overwrite) or += ; x may be gathered from an L-vector
 // through gmap (gmap != nullptr) with constrained entries masked to zero.

@@ -1,19 +1,24 @@
 // pcg.hip -- Jacobi-preconditioned CG on the device, MFEM CGSolver semantics,
-// and the mixed-precision iterative-refinement solver built on it.
+// and the mixed-precision solver built on it.
 //
 // CGSolver::Mult (linalg/solvers.cpp:869-1050) with OperatorJacobiSmoother
 // (:401-453) on the DIAG_ONE ConstrainedOperator (linalg/operator.cpp:601-662).
 // MFEM runs ~12 vector passes and two host round-trips (Dot -> device sync ->
-// host sum, general/reducers.hpp:575-584) per iteration.  Here one iteration
-// is four launches with every scalar kept on the device:
+// host sum, general/reducers.hpp:575-584) per iteration.  Here every scalar
+// stays on the device; one iteration is
 //   update : alpha = nom/den; x += alpha d; r -= alpha z; z = dinv r;
-//            betanom = (r, z); convergence test; beta = betanom/nom
+//            partials of (r, z)                     -> fin_update: betanom,
+//            convergence test, beta = betanom/nom
 //   dir    : d = z + beta d
 //   apply  : element kernel on d (constrained entries read as 0)
-//   scatter: z = A d (DIAG_ONE rows copy d), den = (d, z)
-// Dot products reduce deterministically: fixed grid, fixed in-block tree,
-// and the last block to finish sums the block partials in index order.  The
-// vector type T is double or float; scalars and dot accumulation are double.
+//   scatter: z = A d (DIAG_ONE rows copy d), partials of (d, z) -> fin_den
+// Dot products: each block reduces its chunk of kChunk entries in a fixed
+// tree and stores the partial; a one-block finalize kernel sums the partials
+// in index order.  No block waits for or fences against another (a
+// device-scope fence writes back and invalidates the whole L2 on this GPU),
+// and the result does not depend on timing.  The vector type T is double or
+// float; scalars and dot accumulation are double.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 
@@ -29,23 +34,27 @@ struct PcgState {
     int done;
     int converged;
     int final_iter;
-    unsigned counter;
 };
 
 namespace kern {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxBlocks = 1024;
+constexpr int kPer = 4;                      // entries per thread per chunk
+constexpr int kChunk = kThreads * kPer;      // entries per block
 
-inline unsigned vec_blocks(int64_t n) {
-    int64_t b = (n + kThreads - 1) / kThreads;
-    if (b > kMaxBlocks) b = kMaxBlocks;
-    if (b < 1) b = 1;
-    return (unsigned)b;
+inline unsigned chunks(int64_t n) {
+    const int64_t b = (n + kChunk - 1) / kChunk;
+    return (unsigned)std::max<int64_t>(b, 1);
 }
 
-// Block sum in a fixed tree order; result valid in thread 0.
+// The entries of this block's chunk owned by this thread (kPer, kThreads apart).
+#define H1PA_CHUNK(i, n)                                                                       \
+    for (int64_t i = (int64_t)blockIdx.x * kChunk + threadIdx.x,                               \
+                 i##_end = min((int64_t)(n), (int64_t)(blockIdx.x + 1) * kChunk);              \
+         i < i##_end; i += kThreads)
+
+// Block sum in a fixed tree order, returned to every thread.
 __device__ inline double block_sum(double v, double *sh) {
     sh[threadIdx.x] = v;
     __syncthreads();
@@ -53,29 +62,29 @@ __device__ inline double block_sum(double v, double *sh) {
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    return sh[0];
+    const double r = sh[0];
+    __syncthreads();
+    return r;
 }
 
-// Writes this block's partial; returns true in the one block that finishes
-// last, which then holds the fixed-order total in `total` (thread 0 valid).
-__device__ inline bool reduce_partials(double part, double *partials, unsigned *counter,
-                                       double *sh, double &total) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x] = part;
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
+// partials[k][blockIdx.x] = block sum of acc[k]
+template <int NV>
+__device__ inline void store_partials(const double (&acc)[NV], double *partials, double *sh) {
+    for (int k = 0; k < NV; ++k) {
+        const double v = block_sum(acc[k], sh);
+        if (threadIdx.x == 0) partials[(int64_t)k * gridDim.x + blockIdx.x] = v;
     }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    const volatile double *vp = partials;
-    double v = 0.0;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += kThreads) v += vp[i];
-    total = block_sum(v, sh);
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
+}
+
+// One block: tot[k] = sum over b < nb of partials[k][b], in a fixed order.
+template <int NV>
+__device__ inline void sum_partials(const double *partials, int nb, double *sh,
+                                    double (&tot)[NV]) {
+    for (int k = 0; k < NV; ++k) {
+        double v = 0.0;
+        for (int b = threadIdx.x; b < nb; b += kThreads) v += partials[(int64_t)k * nb + b];
+        tot[k] = block_sum(v, sh);
+    }
 }
 
 template <typename T>
@@ -86,57 +95,62 @@ __global__ void k_jacobi_dinv(int64_t n, const double *__restrict__ diag,
     dinv[i] = (T)((mask && mask[i]) ? 1.0 : 1.0 / diag[i]);
 }
 
+// ---- FP64 / FP32 Jacobi-PCG -------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_pcg_init(int64_t n, const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ x,
-           T *__restrict__ r, T *__restrict__ z, T *__restrict__ d, double *__restrict__ partials,
-           PcgState *st, double rel_tol, double abs_tol) {
+           T *__restrict__ r, T *__restrict__ z, T *__restrict__ d, double *__restrict__ partials) {
     __shared__ double sh[kThreads];
-    double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads) {
+    double acc[1] = {0.0};
+    H1PA_CHUNK(i, n) {
         const T ri = b[i];
         const T zi = dinv[i] * ri;
         r[i] = ri;
         x[i] = T(0);
         z[i] = zi;
         d[i] = zi;
-        acc += (double)zi * (double)ri;
+        acc[0] += (double)zi * (double)ri;
     }
-    const double part = block_sum(acc, sh);
-    double nom;
-    if (reduce_partials(part, partials, &st->counter, sh, nom) && threadIdx.x == 0) {
-        st->nom = st->nom0 = nom;
-        st->iter = 1;
-        st->in_loop = 0;
-        double r0 = nom * rel_tol * rel_tol;
-        if (abs_tol * abs_tol > r0) r0 = abs_tol * abs_tol;
-        st->r0 = r0;
-        st->final_iter = 0;
-        st->converged = 0;
-        st->done = 0;
-        st->betanom = nom;
-        if (nom < 0.0) {
-            st->done = 1;
-        } else if (nom <= r0) {
-            st->done = 1;
-            st->converged = 1;
-        }
+    store_partials<1>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_pcg_fin_init(int nb, const double *__restrict__ partials, PcgState *st, double rel_tol,
+               double abs_tol) {
+    __shared__ double sh[kThreads];
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x != 0) return;
+    const double nom = tot[0];
+    st->nom = st->nom0 = nom;
+    st->iter = 1;
+    st->in_loop = 0;
+    double r0 = nom * rel_tol * rel_tol;
+    if (abs_tol * abs_tol > r0) r0 = abs_tol * abs_tol;
+    st->r0 = r0;
+    st->final_iter = 0;
+    st->converged = 0;
+    st->done = 0;
+    st->betanom = nom;
+    if (nom < 0.0) {
+        st->done = 1;
+    } else if (nom <= r0) {
+        st->done = 1;
+        st->converged = 1;
     }
 }
 
-// z = A d from the element output, DIAG_ONE rows copy d; den = (d, z).
+// z = A d from the element output, DIAG_ONE rows copy d; partials of (d, z).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_pcg_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
               const T *__restrict__ ye, const uint8_t *__restrict__ mask, const T *__restrict__ d,
-              T *__restrict__ z, double *__restrict__ partials, PcgState *st,
+              T *__restrict__ z, double *__restrict__ partials, const PcgState *st,
               const int *__restrict__ perm, int64_t nperm) {
     __shared__ double sh[kThreads];
     if (st->done) return;
-    double acc = 0.0;
-    for (int64_t t = blockIdx.x * (int64_t)kThreads + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * kThreads) {
+    double acc[1] = {0.0};
+    H1PA_CHUNK(t, n) {
         const int64_t i = t < nperm ? perm[t] : t;
         T v;
         if (mask && mask[i]) {
@@ -149,16 +163,22 @@ k_pcg_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict_
             }
         }
         z[i] = v;
-        acc += (double)d[i] * (double)v;
+        acc[0] += (double)d[i] * (double)v;
     }
-    const double part = block_sum(acc, sh);
-    double den;
-    if (reduce_partials(part, partials, &st->counter, sh, den) && threadIdx.x == 0) {
-        st->den = den;
-        if (den == 0.0) {  // CGSolver: den <= 0 and den == 0 ends the loop
-            st->done = 1;
-            st->final_iter = st->in_loop ? st->iter : 0;
-        }
+    store_partials<1>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_pcg_fin_den(int nb, const double *__restrict__ partials, PcgState *st) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x != 0) return;
+    st->den = tot[0];
+    if (tot[0] == 0.0) {  // CGSolver: den <= 0 and den == 0 ends the loop
+        st->done = 1;
+        st->final_iter = st->in_loop ? st->iter : 0;
     }
 }
 
@@ -166,42 +186,48 @@ template <typename T>
 __global__ void __launch_bounds__(kThreads)
 k_pcg_update(int64_t n, const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
              T *__restrict__ z, const T *__restrict__ d, double *__restrict__ partials,
-             PcgState *st) {
+             const PcgState *st) {
     __shared__ double sh[kThreads];
     if (st->done) return;
     const T alpha = (T)(st->nom / st->den);
-    double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads) {
+    double acc[1] = {0.0};
+    H1PA_CHUNK(i, n) {
         const T di = d[i];
         x[i] += alpha * di;
         const T ri = r[i] - alpha * z[i];
         r[i] = ri;
         const T zi = dinv[i] * ri;
         z[i] = zi;
-        acc += (double)ri * (double)zi;
+        acc[0] += (double)ri * (double)zi;
     }
-    const double part = block_sum(acc, sh);
-    double betanom;
-    if (reduce_partials(part, partials, &st->counter, sh, betanom) && threadIdx.x == 0) {
-        const int i = st->iter;
-        st->in_loop = 1;
-        st->betanom = betanom;
-        if (betanom < 0.0) {
-            st->done = 1;
-            st->final_iter = i;
-        } else if (betanom <= st->r0) {
-            st->done = 1;
-            st->converged = 1;
-            st->final_iter = i;
-        } else if (i + 1 > st->max_iter) {
-            st->done = 1;
-            st->final_iter = st->max_iter;
-        } else {
-            st->beta = betanom / st->nom;
-            st->nom = betanom;
-            st->iter = i + 1;
-        }
+    store_partials<1>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_pcg_fin_update(int nb, const double *__restrict__ partials, PcgState *st) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x != 0) return;
+    const double betanom = tot[0];
+    const int i = st->iter;
+    st->in_loop = 1;
+    st->betanom = betanom;
+    if (betanom < 0.0) {
+        st->done = 1;
+        st->final_iter = i;
+    } else if (betanom <= st->r0) {
+        st->done = 1;
+        st->converged = 1;
+        st->final_iter = i;
+    } else if (i + 1 > st->max_iter) {
+        st->done = 1;
+        st->final_iter = st->max_iter;
+    } else {
+        st->beta = betanom / st->nom;
+        st->nom = betanom;
+        st->iter = i + 1;
     }
 }
 
@@ -210,9 +236,7 @@ __global__ void __launch_bounds__(kThreads)
 k_pcg_dir(int64_t n, const T *__restrict__ z, T *__restrict__ d, const PcgState *st) {
     if (st->done) return;
     const T beta = (T)st->beta;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads)
-        d[i] = z[i] + beta * d[i];
+    H1PA_CHUNK(i, n) d[i] = z[i] + beta * d[i];
 }
 
 __global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
@@ -268,42 +292,16 @@ struct IrState {
     int ru;   // 1: refinement step due this iteration
     int nru;  // !ru: skip flag of the FP64 element kernel
     int done, converged;
-    unsigned counter;
 };
-
-template <int NV>
-__device__ inline bool reduce_n(const double (&part)[NV], double *partials, unsigned *counter,
-                                double *sh, double (&tot)[NV]) {
-    __shared__ bool last;
-    if (threadIdx.x == 0) {
-        for (int k = 0; k < NV; ++k) partials[(int64_t)k * gridDim.x + blockIdx.x] = part[k];
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        last = (prev == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    const volatile double *vp = partials;
-    for (int k = 0; k < NV; ++k) {
-        double v = 0.0;
-        for (unsigned i = threadIdx.x; i < gridDim.x; i += kThreads) v += vp[(int64_t)k * gridDim.x + i];
-        tot[k] = block_sum(v, sh);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *counter = 0u;
-    return true;
-}
 
 __global__ void __launch_bounds__(kThreads)
 k_ir_init(int64_t n, const double *__restrict__ b, const double *__restrict__ dinv64,
           const float *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r64,
           float *__restrict__ r, float *__restrict__ xs, float *__restrict__ z,
-          float *__restrict__ d, double *__restrict__ partials, IrState *st) {
+          float *__restrict__ d, double *__restrict__ partials) {
     __shared__ double sh[kThreads];
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads) {
+    H1PA_CHUNK(i, n) {
         const double bi = b[i];
         x[i] = 0.0;
         r64[i] = bi;
@@ -317,39 +315,40 @@ k_ir_init(int64_t n, const double *__restrict__ b, const double *__restrict__ di
         acc[1] += bi * dinv64[i] * bi;
         acc[2] += bi * bi;
     }
-    double part[3], tot[3];
-    for (int k = 0; k < 3; ++k) {
-        part[k] = block_sum(acc[k], sh);
-        __syncthreads();
-    }
-    if (reduce_n<3>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) {
-        st->rho = st->rho_ref = tot[0];
-        st->rho64 = st->rho64_0 = tot[1];
-        st->r2 = st->b2 = tot[2];
-        st->iter = 0;
-        st->outer = 0;
-        st->ru = 0;
-        st->nru = 1;
-        st->converged = 0;
-        st->done = 0;
-        if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
-            st->done = 1;
-            st->converged = 1;
-        }
+    store_partials<3>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_ir_fin_init(int nb, const double *__restrict__ partials, IrState *st) {
+    __shared__ double sh[kThreads];
+    double tot[3];
+    sum_partials<3>(partials, nb, sh, tot);
+    if (threadIdx.x != 0) return;
+    st->rho = st->rho_ref = tot[0];
+    st->rho64 = st->rho64_0 = tot[1];
+    st->r2 = st->b2 = tot[2];
+    st->iter = 0;
+    st->outer = 0;
+    st->ru = 0;
+    st->nru = 1;
+    st->converged = 0;
+    st->done = 0;
+    if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
+        st->done = 1;
+        st->converged = 1;
     }
 }
 
-// q = A d (FP32; DIAG_ONE rows copy d), den = (d, q).
+// q = A d (FP32; DIAG_ONE rows copy d), partials of (d, q).
 __global__ void __launch_bounds__(kThreads)
 k_ir_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
              const float *__restrict__ ye, const uint8_t *__restrict__ mask,
              const float *__restrict__ d, float *__restrict__ q, double *__restrict__ partials,
-             IrState *st, const int *__restrict__ perm, int64_t nperm) {
+             const IrState *st, const int *__restrict__ perm, int64_t nperm) {
     __shared__ double sh[kThreads];
     if (st->done) return;
-    double acc = 0.0;
-    for (int64_t t = blockIdx.x * (int64_t)kThreads + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * kThreads) {
+    double acc[1] = {0.0};
+    H1PA_CHUNK(t, n) {
         const int64_t i = t < nperm ? perm[t] : t;
         float v;
         if (mask && mask[i]) {
@@ -362,71 +361,86 @@ k_ir_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__
             }
         }
         q[i] = v;
-        acc += (double)d[i] * (double)v;
+        acc[0] += (double)d[i] * (double)v;
     }
-    double part[1] = {block_sum(acc, sh)}, tot[1];
-    if (reduce_n<1>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) st->den = tot[0];
+    store_partials<1>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_ir_fin_den(int nb, const double *__restrict__ partials, IrState *st) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x == 0) st->den = tot[0];
 }
 
 __global__ void __launch_bounds__(kThreads)
 k_ir_update(int64_t n, const float *__restrict__ dinv, float *__restrict__ xs,
             float *__restrict__ r, float *__restrict__ z, const float *__restrict__ q,
-            const float *__restrict__ d, double *__restrict__ partials, IrState *st) {
+            const float *__restrict__ d, double *__restrict__ partials, const IrState *st) {
     __shared__ double sh[kThreads];
     if (st->done) return;
     const double den = st->den;
     const float alpha = den != 0.0 ? (float)(st->rho / den) : 0.f;
-    double acc = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads) {
+    double acc[1] = {0.0};
+    H1PA_CHUNK(i, n) {
         xs[i] += alpha * d[i];
         const float ri = r[i] - alpha * q[i];
         r[i] = ri;
         const float zi = dinv[i] * ri;
         z[i] = zi;
-        acc += (double)ri * (double)zi;
+        acc[0] += (double)ri * (double)zi;
     }
-    double part[1] = {block_sum(acc, sh)}, tot[1];
-    if (reduce_n<1>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) {
-        const double rho_new = tot[0];
-        st->iter += 1;
-        const bool trigger = den == 0.0 || rho_new <= st->delta2 * st->rho_ref ||
-                             rho_new <= st->tol2 * st->rho64_0 || st->iter >= st->max_inner;
-        if (trigger) {
-            st->ru = 1;
-            st->nru = 0;
-        } else {
-            st->beta = rho_new / st->rho;
-            st->rho = rho_new;
-        }
+    store_partials<1>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_ir_fin_update(int nb, const double *__restrict__ partials, IrState *st) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x != 0) return;
+    const double rho_new = tot[0];
+    st->iter += 1;
+    const bool trigger = st->den == 0.0 || rho_new <= st->delta2 * st->rho_ref ||
+                         rho_new <= st->tol2 * st->rho64_0 || st->iter >= st->max_inner;
+    if (trigger) {
+        st->ru = 1;
+        st->nru = 0;
+    } else {
+        st->beta = rho_new / st->rho;
+        st->rho = rho_new;
     }
 }
 
-// Refinement step, part 1: x += xs, xs = 0.
-__global__ void k_ir_ru_x(int64_t n, float *__restrict__ xs, double *__restrict__ x,
-                          const IrState *st) {
+// Refinement step, part 1: x += xs, xs = 0 (grid-stride; cheap when skipped).
+__global__ void __launch_bounds__(kThreads)
+k_ir_ru_x(int64_t n, float *__restrict__ xs, double *__restrict__ x, const IrState *st) {
     if (!st->ru) return;
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    x[i] += (double)xs[i];
-    xs[i] = 0.f;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        x[i] += (double)xs[i];
+        xs[i] = 0.f;
+    }
 }
 
 // Refinement step, part 2 (after the FP64 element kernel on x):
-// r64 = b - A x, (B r64, r64), ||r64||^2, r = (float) r64, z = dinv r, (r, z).
+// r64 = b - A x; partials of (B r64, r64), ||r64||^2 and, with
+// r = (float) r64, z = dinv r, of (r, z).
 __global__ void __launch_bounds__(kThreads)
 k_ir_ru_res(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
             const double *__restrict__ ye64, const uint8_t *__restrict__ mask,
             const double *__restrict__ b, const double *__restrict__ x,
             const double *__restrict__ dinv64, const float *__restrict__ dinv,
             double *__restrict__ r64, float *__restrict__ r, float *__restrict__ z,
-            double *__restrict__ partials, IrState *st, const int *__restrict__ perm,
+            double *__restrict__ partials, const IrState *st, const int *__restrict__ perm,
             int64_t nperm) {
     __shared__ double sh[kThreads];
     if (!st->ru) return;
     double acc[3] = {0.0, 0.0, 0.0};
-    for (int64_t t = blockIdx.x * (int64_t)kThreads + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * kThreads) {
+    H1PA_CHUNK(t, n) {
         const int64_t i = t < nperm ? perm[t] : t;
         double y;
         if (mask && mask[i]) {
@@ -448,27 +462,30 @@ k_ir_ru_res(int64_t n, const int *__restrict__ offsets, const int *__restrict__ 
         acc[1] += ri * dinv64[i] * ri;
         acc[2] += ri * ri;
     }
-    double part[3], tot[3];
-    for (int k = 0; k < 3; ++k) {
-        part[k] = block_sum(acc[k], sh);
-        __syncthreads();
-    }
-    if (reduce_n<3>(part, partials, &st->counter, sh, tot) && threadIdx.x == 0) {
-        st->outer += 1;
-        st->rho64 = tot[1];
-        st->r2 = tot[2];
-        st->ru = 0;
-        st->nru = 1;
-        if (tot[1] <= st->tol2 * st->rho64_0) {
-            st->done = 1;
-            st->converged = 1;
-        } else if (st->outer >= st->max_outer || st->iter >= st->max_inner) {
-            st->done = 1;
-        } else {
-            st->beta = st->rho > 0.0 ? tot[0] / st->rho : 0.0;
-            st->rho = tot[0];
-            st->rho_ref = tot[0];
-        }
+    store_partials<3>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_ir_fin_ru(int nb, const double *__restrict__ partials, IrState *st) {
+    __shared__ double sh[kThreads];
+    if (!st->ru) return;
+    double tot[3];
+    sum_partials<3>(partials, nb, sh, tot);
+    if (threadIdx.x != 0) return;
+    st->outer += 1;
+    st->rho64 = tot[1];
+    st->r2 = tot[2];
+    st->ru = 0;
+    st->nru = 1;
+    if (tot[1] <= st->tol2 * st->rho64_0) {
+        st->done = 1;
+        st->converged = 1;
+    } else if (st->outer >= st->max_outer || st->iter >= st->max_inner) {
+        st->done = 1;
+    } else {
+        st->beta = st->rho > 0.0 ? tot[0] / st->rho : 0.0;
+        st->rho = tot[0];
+        st->rho_ref = tot[0];
     }
 }
 
@@ -476,9 +493,7 @@ __global__ void __launch_bounds__(kThreads)
 k_ir_dir(int64_t n, const float *__restrict__ z, float *__restrict__ d, const IrState *st) {
     if (st->done) return;
     const float beta = (float)st->beta;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads)
-        d[i] = z[i] + beta * d[i];
+    H1PA_CHUNK(i, n) d[i] = z[i] + beta * d[i];
 }
 
 __global__ void k_form_x(int64_t n, const uint8_t *__restrict__ mask, const double *__restrict__ x,
@@ -563,31 +578,35 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
     DevBuf<T> r(n), z(n), d(n);
-    DevBuf<double> partials(kern::kMaxBlocks);
+    const unsigned nb = kern::chunks(n);
+    DevBuf<double> partials(nb);
     DevBuf<PcgState> st(1);
     PcgState h{};
     h.max_iter = o.max_iter;
     H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
-    const unsigned nb = kern::vec_blocks(n);
     const int check = o.check_every > 0 ? o.check_every : 8;
     const int *stop = &st.p->done;
+    const dim3 grid(nb), blk(kern::kThreads), one(1);
 
     auto t0 = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(kern::k_pcg_init<T>, dim3(nb), dim3(kern::kThreads), 0, s, n, b, dinv, x,
-                       r.p, z.p, d.p, partials.p, st.p, o.rel_tol, o.abs_tol);
+    hipLaunchKernelGGL(kern::k_pcg_init<T>, grid, blk, 0, s, n, b, dinv, x, r.p, z.p, d.p,
+                       partials.p);
+    hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p,
+                       o.rel_tol, o.abs_tol);
     auto apply = [&]() {
         pcg_element_apply<T>(f, prec, mask, d.p, ye, stop, s);
-        hipLaunchKernelGGL(kern::k_pcg_scatter<T>, dim3(nb), dim3(kern::kThreads), 0, s, n,
-                           sp.restr.offsets.p, sp.restr.indices_nat.p, ye, mask, d.p, z.p,
-                           partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_pcg_scatter<T>, grid, blk, 0, s, n, sp.restr.offsets.p,
+                           sp.restr.indices_nat.p, ye, mask, d.p, z.p, partials.p, st.p,
+                           sp.restr.perm.p, sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
     };
     apply();
     H1PA_HIP(hipGetLastError());
     for (int it = 1; it <= o.max_iter; ++it) {
-        hipLaunchKernelGGL(kern::k_pcg_update<T>, dim3(nb), dim3(kern::kThreads), 0, s, n, dinv, x,
-                           r.p, z.p, d.p, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_pcg_dir<T>, dim3(nb), dim3(kern::kThreads), 0, s, n, z.p, d.p,
-                           st.p);
+        hipLaunchKernelGGL(kern::k_pcg_update<T>, grid, blk, 0, s, n, dinv, x, r.p, z.p, d.p,
+                           partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_pcg_dir<T>, grid, blk, 0, s, n, z.p, d.p, st.p);
         apply();
         H1PA_HIP(hipGetLastError());
         if (it % check == 0 || it == o.max_iter) {
@@ -667,11 +686,12 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
     const unsigned nb256 = (unsigned)((n + 255) / 256);
-    const unsigned nb = kern::vec_blocks(n);
+    const unsigned nb = kern::chunks(n);
+    const unsigned nbx = std::min<unsigned>(nb, 1024);
     ensure_low_precision(f, o.inner_precision, s);
     DevBuf<float> dinv(n), r(n), xs(n), z(n), d(n), q(n);
     DevBuf<double> dinv64(n), r64(n);
-    DevBuf<double> partials(3 * (size_t)kern::kMaxBlocks);
+    DevBuf<double> partials(3 * (size_t)nb);
     DevBuf<kern::IrState> st(1);
     hipLaunchKernelGGL(kern::k_jacobi_dinv<float>, dim3(nb256), dim3(256), 0, s, n, diag, mask,
                        dinv.p);
@@ -685,27 +705,30 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
     const int *done = &st.p->done;
     const int *nru = &st.p->nru;
+    const dim3 grid(nb), blk(kern::kThreads), one(1);
     auto t0 = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(kern::k_ir_init, dim3(nb), dim3(kern::kThreads), 0, s, n, b, dinv64.p,
-                       dinv.p, x, r64.p, r.p, xs.p, z.p, d.p, partials.p, st.p);
+    hipLaunchKernelGGL(kern::k_ir_init, grid, blk, 0, s, n, b, dinv64.p, dinv.p, x, r64.p, r.p,
+                       xs.p, z.p, d.p, partials.p);
+    hipLaunchKernelGGL(kern::k_ir_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p);
     H1PA_HIP(hipGetLastError());
     const int limit = o.max_inner + o.max_outer + 1;
     for (int it = 1; it <= limit; ++it) {
         // q = A d with the reduced-precision operator
         pcg_element_apply<float>(f, o.inner_precision, mask, d.p, f.ework32.p, done, s);
-        hipLaunchKernelGGL(kern::k_ir_scatter, dim3(nb), dim3(kern::kThreads), 0, s, n,
-                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework32.p, mask, d.p,
-                           q.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
-        hipLaunchKernelGGL(kern::k_ir_update, dim3(nb), dim3(kern::kThreads), 0, s, n, dinv.p,
-                           xs.p, r.p, z.p, q.p, d.p, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_ir_ru_x, dim3(nb256), dim3(256), 0, s, n, xs.p, x, st.p);
+        hipLaunchKernelGGL(kern::k_ir_scatter, grid, blk, 0, s, n, sp.restr.offsets.p,
+                           sp.restr.indices_nat.p, f.ework32.p, mask, d.p, q.p, partials.p, st.p,
+                           sp.restr.perm.p, sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_ir_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_ir_update, grid, blk, 0, s, n, dinv.p, xs.p, r.p, z.p, q.p,
+                           d.p, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_ir_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_ir_ru_x, dim3(nbx), blk, 0, s, n, xs.p, x, st.p);
         pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, x, f.ework0.p, nru, s);
-        hipLaunchKernelGGL(kern::k_ir_ru_res, dim3(nb), dim3(kern::kThreads), 0, s, n,
-                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask, b, x,
-                           dinv64.p, dinv.p, r64.p, r.p, z.p, partials.p, st.p, sp.restr.perm.p,
-                           sp.restr.nperm);
-        hipLaunchKernelGGL(kern::k_ir_dir, dim3(nb), dim3(kern::kThreads), 0, s, n, z.p, d.p,
-                           st.p);
+        hipLaunchKernelGGL(kern::k_ir_ru_res, grid, blk, 0, s, n, sp.restr.offsets.p,
+                           sp.restr.indices_nat.p, f.ework0.p, mask, b, x, dinv64.p, dinv.p,
+                           r64.p, r.p, z.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_ir_fin_ru, one, blk, 0, s, (int)nb, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_ir_dir, grid, blk, 0, s, n, z.p, d.p, st.p);
         H1PA_HIP(hipGetLastError());
         if (it % 8 == 0 || it == limit) {
             H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));

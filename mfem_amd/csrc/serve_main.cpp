@@ -1,0 +1,382 @@
+// serve_main.cpp -- h1pa_serve: a request loop that serves Poisson solves on
+// one GPU through the C ABI (include/h1pa.h) only.  No MFEM counterpart.
+//
+// Requests arrive as a Poisson process; each names an n^3 Cartesian hex mesh
+// (n in [nmin, nmax], P(n) proportional to 1/n), an order p (uniform in
+// [pmin, pmax]), a coefficient seed (k = exp of 8 seeded Fourier modes) and
+// an RHS seed (uniform [-1, 1]); interior vertices are perturbed by 0.15 h,
+// the boundary is homogeneous Dirichlet, the stop test is CGSolver's
+// relative measure at rel_tol.
+//
+//   --mode batch   continuous batching: arrived requests are set up and
+//                  admitted while the HBM budget allows; all admitted solves
+//                  advance together (h1pa_batch_run, one graph replay per
+//                  loop turn); finished solves are retired and their memory
+//                  released.
+//   --mode serial  plain FP64 GPU PCG: requests are set up and solved one at
+//                  a time, in arrival order (h1pa_pcg_solve).
+//
+// Prints one JSON line: solves/s over the whole run, latency percentiles
+// (arrival to result), and the request mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <random>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/h1pa.h"
+
+namespace {
+
+struct Opts {
+    std::string mode = "batch";
+    int precision = H1PA_PREC_BF16_PA;
+    double rate = 200.0;      // requests per second
+    double seconds = 2.0;     // arrival window
+    double budget_gb = 200.0;
+    int nmin = 6, nmax = 32, pmin = 1, pmax = 8;
+    double rel_tol = 1e-10;
+    int iters_per_replay = 8;
+    uint64_t seed = 1;
+    int max_requests = 1 << 30;
+};
+
+struct Request {
+    int id = 0;
+    double t_arrive = 0.0;
+    int n = 0, p = 0;
+    uint64_t coeff_seed = 0, rhs_seed = 0;
+    double t_done = -1.0;
+    int iterations = 0;
+    int converged = 0;
+};
+
+struct Live {
+    h1pa_mesh *mesh = nullptr;
+    h1pa_space *space = nullptr;
+    h1pa_form *form = nullptr;
+    double *b = nullptr, *x = nullptr, *B = nullptr, *X = nullptr;
+    int64_t bytes = 0;
+};
+
+void check(int rc, const char *what) {
+    if (rc != H1PA_OK) {
+        std::fprintf(stderr, "h1pa_serve: %s failed (%d): %s\n", what, rc, h1pa_last_error());
+        std::exit(1);
+    }
+}
+
+void hcheck(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "h1pa_serve: %s: %s\n", what, hipGetErrorString(e));
+        std::exit(1);
+    }
+}
+
+// Device bytes a request will hold once admitted (form, space, mesh, batch
+// slot and the four request vectors); used before it is built.
+int64_t estimate_bytes(int n, int p, int precision, bool batch) {
+    const int64_t ne = (int64_t)n * n * n, D = p + 1, Q = p + 2;
+    const int64_t N = (int64_t)(n * p + 1) * (n * p + 1) * (n * p + 1);
+    const int64_t D3 = D * D * D, Q3 = Q * Q * Q;
+    int64_t b = 0;
+    b += ne * 6 * Q3 * 8;                    // PA data
+    b += ne * D3 * 8;                        // form E scratch
+    b += ne * D3 * 4 * 4 + N * 4 * 2;        // gather, indices, indices_nat, elem_dof, offsets, perm
+    b += ne * (29 * 4 + 18 + 8 * 3 * 8);     // topology, orientations, nodes
+    b += N;                                  // boundary mask
+    b += 4 * N * 8;                          // b, x, B, X
+    if (batch) {
+        if (precision == H1PA_PREC_FP64) {
+            b += 4 * N * 8 + ne * D3 * 8 + N;
+        } else {
+            const int sd = precision == H1PA_PREC_BF16_PA ? 2 : 4;
+            b += ne * 6 * Q3 * sd + ne * D3 * 4;     // shadow PA data, FP32 E scratch
+            b += 5 * N * 4 + 2 * N * 8 + ne * D3 * 8 + N;
+        }
+    } else {
+        b += 3 * N * 8 + 2 * N * 8;          // PCG vectors and the diagonal
+    }
+    return b;
+}
+
+double now_s(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+std::vector<Request> make_requests(const Opts &o) {
+    std::mt19937_64 rng(o.seed);
+    std::exponential_distribution<double> gap(o.rate);
+    // P(n) proportional to 1/n on [nmin, nmax]
+    std::vector<double> w;
+    for (int n = o.nmin; n <= o.nmax; ++n) w.push_back(1.0 / n);
+    std::discrete_distribution<int> pick_n(w.begin(), w.end());
+    std::uniform_int_distribution<int> pick_p(o.pmin, o.pmax);
+    std::vector<Request> r;
+    double t = 0.0;
+    for (int i = 0; i < o.max_requests; ++i) {
+        t += gap(rng);
+        if (t >= o.seconds) break;
+        Request q;
+        q.id = i;
+        q.t_arrive = t;
+        q.n = o.nmin + pick_n(rng);
+        q.p = pick_p(rng);
+        q.coeff_seed = rng();
+        q.rhs_seed = rng();
+        r.push_back(q);
+    }
+    return r;
+}
+
+// Mesh, space, form (assembled), RHS and the constrained system B, X.
+Live setup(const Request &q, hipStream_t s) {
+    Live L;
+    check(h1pa_mesh_create_cartesian(q.n, q.n, q.n, 1, &L.mesh), "mesh");
+    check(h1pa_mesh_perturb(L.mesh, 0.15, q.coeff_seed), "perturb");
+    check(h1pa_space_create(L.mesh, q.p, &L.space), "space");
+    h1pa_form_desc d{};
+    d.integrators = H1PA_INTEG_DIFFUSION;
+    d.coeff_kind = H1PA_COEFF_FOURIER;
+    d.coeff_seed = q.coeff_seed;
+    check(h1pa_form_create(L.space, &d, &L.form), "form");
+    check(h1pa_form_assemble(L.form, s), "assemble");
+    int64_t info[4];
+    check(h1pa_space_info(L.space, info), "space_info");
+    const int64_t N = info[3];
+    hcheck(hipMalloc(&L.b, 4 * N * sizeof(double)), "hipMalloc vectors");
+    L.x = L.b + N;
+    L.B = L.x + N;
+    L.X = L.B + N;
+    check(h1pa_vector_randomize(L.b, N, q.rhs_seed, -1.0, 1.0, s), "randomize");
+    hcheck(hipMemsetAsync(L.x, 0, N * sizeof(double), s), "memset");
+    const int *ess;
+    int64_t nEss;
+    check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+    check(h1pa_form_linear_system(L.form, ess, nEss, L.x, L.b, L.X, L.B, s), "linear_system");
+    int64_t fb = 0;
+    check(h1pa_form_device_bytes(L.form, &fb), "device_bytes");
+    L.bytes = fb + 4 * N * 8;
+    return L;
+}
+
+void release(Live &L) {
+    if (L.form) h1pa_form_destroy(L.form);
+    if (L.space) h1pa_space_destroy(L.space);
+    if (L.mesh) h1pa_mesh_destroy(L.mesh);
+    if (L.b) (void)hipFree(L.b);
+    L = Live{};
+}
+
+double pct(std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    const double pos = q * (v.size() - 1);
+    const size_t i = (size_t)pos;
+    const double f = pos - i;
+    return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+}
+
+Opts parse(int argc, char **argv) {
+    Opts o;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto val = [&]() -> std::string {
+            if (i + 1 >= argc) {
+                std::fprintf(stderr, "missing value for %s\n", a.c_str());
+                std::exit(2);
+            }
+            return argv[++i];
+        };
+        if (a == "--mode") o.mode = val();
+        else if (a == "--precision") {
+            const std::string p = val();
+            o.precision = p == "fp64" ? H1PA_PREC_FP64 : p == "fp32" ? H1PA_PREC_FP32
+                                                                     : H1PA_PREC_BF16_PA;
+        } else if (a == "--rate") o.rate = std::atof(val().c_str());
+        else if (a == "--seconds") o.seconds = std::atof(val().c_str());
+        else if (a == "--budget-gb") o.budget_gb = std::atof(val().c_str());
+        else if (a == "--nmin") o.nmin = std::atoi(val().c_str());
+        else if (a == "--nmax") o.nmax = std::atoi(val().c_str());
+        else if (a == "--pmin") o.pmin = std::atoi(val().c_str());
+        else if (a == "--pmax") o.pmax = std::atoi(val().c_str());
+        else if (a == "--rel-tol") o.rel_tol = std::atof(val().c_str());
+        else if (a == "--iters-per-replay") o.iters_per_replay = std::atoi(val().c_str());
+        else if (a == "--seed") o.seed = std::strtoull(val().c_str(), nullptr, 10);
+        else if (a == "--max-requests") o.max_requests = std::atoi(val().c_str());
+        else {
+            std::fprintf(stderr, "unknown option %s\n", a.c_str());
+            std::exit(2);
+        }
+    }
+    if (o.mode != "batch" && o.mode != "serial") {
+        std::fprintf(stderr, "--mode must be batch or serial\n");
+        std::exit(2);
+    }
+    if (o.nmin < 1 || o.nmax < o.nmin || o.pmin < 1 || o.pmax > 8 || o.pmax < o.pmin ||
+        o.rate <= 0 || o.seconds <= 0) {
+        std::fprintf(stderr, "bad request-mix options\n");
+        std::exit(2);
+    }
+    return o;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    const Opts o = parse(argc, argv);
+    std::vector<Request> reqs = make_requests(o);
+    const int R = (int)reqs.size();
+    hipStream_t s;
+    hcheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+    const int64_t budget = (int64_t)(o.budget_gb * 1e9);
+    const bool batch = o.mode == "batch";
+    const int prec = batch ? o.precision : H1PA_PREC_FP64;
+
+    // warm-up: one small solve (first-use costs, code object loading)
+    {
+        Request w;
+        w.n = 4;
+        w.p = 2;
+        Live L = setup(w, s);
+        const int *ess;
+        int64_t nEss;
+        check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+        h1pa_pcg_opts po{o.rel_tol, 0.0, 10000, 8};
+        h1pa_pcg_stats st{};
+        check(h1pa_pcg_solve(L.form, ess, nEss, L.B, L.X, &po, &st, s), "warmup");
+        hcheck(hipStreamSynchronize(s), "sync");
+        release(L);
+    }
+
+    h1pa_batch *B = nullptr;
+    if (batch) {
+        h1pa_batch_opts bo{};
+        bo.precision = prec;
+        bo.iters_per_replay = o.iters_per_replay;
+        bo.use_graph = 1;
+        bo.autotune = 1;
+        bo.delta = 0.1;
+        bo.max_outer = 1000;
+        bo.max_inner = 200000;
+        check(h1pa_batch_create(&bo, &B), "batch_create");
+    }
+
+    std::map<int64_t, Live> live;
+    std::map<int64_t, int64_t> live_est;
+    std::deque<int> queue;
+    int64_t used = 0, peak_used = 0;
+    int next = 0, done = 0, peak_live = 0;
+    const auto t0 = std::chrono::steady_clock::now();
+    std::vector<int64_t> tags(4096);
+    std::vector<h1pa_batch_result> res(4096);
+    while (done < R) {
+        double t = now_s(t0);
+        while (next < R && reqs[next].t_arrive <= t) queue.push_back(next++);
+        if (batch) {
+            // admit in arrival order while the budget allows
+            while (!queue.empty()) {
+                const Request &q = reqs[queue.front()];
+                const int64_t est = estimate_bytes(q.n, q.p, prec, true);
+                if (!live.empty() && used + est > budget) break;
+                Live L = setup(q, s);
+                const int *ess;
+                int64_t nEss;
+                check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+                h1pa_pcg_opts po{o.rel_tol, 0.0, 200000, 8};
+                int slot = 0;
+                check(h1pa_batch_add(B, L.form, ess, nEss, L.B, L.X, &po, q.id, &slot, s),
+                      "batch_add");
+                const int64_t acc = std::max(est, L.bytes);
+                used += acc;
+                live_est[q.id] = acc;
+                live[q.id] = L;
+                queue.pop_front();
+            }
+            peak_used = std::max(peak_used, used);
+            peak_live = std::max(peak_live, (int)live.size());
+            if (live.empty()) {
+                if (next < R) {
+                    const double wait = reqs[next].t_arrive - now_s(t0);
+                    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+                }
+                continue;
+            }
+            int running = 0;
+            check(h1pa_batch_run(B, 1, &running), "batch_run");
+            int nd = 0;
+            check(h1pa_batch_poll(B, (int)tags.size(), tags.data(), res.data(), &nd), "poll");
+            const double tf = now_s(t0);
+            for (int i = 0; i < nd; ++i) {
+                Request &q = reqs[tags[i]];
+                q.t_done = tf;
+                q.iterations = res[i].iterations;
+                q.converged = res[i].converged;
+                release(live[tags[i]]);
+                live.erase(tags[i]);
+                used -= live_est[tags[i]];
+                live_est.erase(tags[i]);
+                ++done;
+            }
+        } else {
+            if (queue.empty()) {
+                if (next < R) {
+                    const double wait = reqs[next].t_arrive - now_s(t0);
+                    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+                }
+                continue;
+            }
+            Request &q = reqs[queue.front()];
+            queue.pop_front();
+            Live L = setup(q, s);
+            peak_used = std::max(peak_used, L.bytes);
+            peak_live = 1;
+            const int *ess;
+            int64_t nEss;
+            check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+            h1pa_pcg_opts po{o.rel_tol, 0.0, 200000, 8};
+            h1pa_pcg_stats st{};
+            const int rc = h1pa_pcg_solve(L.form, ess, nEss, L.B, L.X, &po, &st, s);
+            if (rc != H1PA_OK && rc != H1PA_ENOTCONV) check(rc, "pcg_solve");
+            hcheck(hipStreamSynchronize(s), "sync");
+            q.t_done = now_s(t0);
+            q.iterations = st.iterations;
+            q.converged = st.converged;
+            release(L);
+            ++done;
+        }
+    }
+    const double makespan = now_s(t0);
+    if (B) h1pa_batch_destroy(B);
+    hcheck(hipStreamDestroy(s), "stream destroy");
+
+    std::vector<double> lat;
+    int64_t dofs = 0, conv = 0, iters = 0;
+    for (const Request &q : reqs) {
+        lat.push_back(q.t_done - q.t_arrive);
+        const int64_t m = (int64_t)q.n * q.p + 1;
+        dofs += m * m * m;
+        conv += q.converged;
+        iters += q.iterations;
+    }
+    std::printf(
+        "{\"mode\": \"%s\", \"precision\": \"%s\", \"rate\": %.1f, \"window_s\": %.2f, "
+        "\"requests\": %d, \"converged\": %lld, \"makespan_s\": %.3f, \"solves_per_s\": %.3f, "
+        "\"latency_p50_s\": %.4f, \"latency_p90_s\": %.4f, \"latency_p99_s\": %.4f, "
+        "\"latency_max_s\": %.4f, \"mean_dofs\": %.0f, \"mean_iterations\": %.1f, "
+        "\"peak_live\": %d, \"peak_bytes\": %lld, \"budget_bytes\": %lld}\n",
+        o.mode.c_str(), prec == H1PA_PREC_FP64 ? "fp64" : prec == H1PA_PREC_FP32 ? "fp32" : "bf16",
+        o.rate, o.seconds, R, (long long)conv, makespan, R / makespan, pct(lat, 0.5),
+        pct(lat, 0.9), pct(lat, 0.99), pct(lat, 1.0), R ? (double)dofs / R : 0.0,
+        R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget);
+    return 0;
+}

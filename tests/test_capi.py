@@ -71,3 +71,19 @@ def test_mixed_precision_argument_errors(h1pa):
                              None) == h1pa.H1PA_EINVAL
     assert b"null argument" in lib.h1pa_last_error()
     assert lib.h1pa_form_mult_f32(None, h1pa.PREC_FP32, None, None, None) == h1pa.H1PA_EINVAL
+
+
+def test_splitmix64_known_answer():
+    """The counter-based generator behind Perturb/Randomize/Fourier seeds is
+    SplitMix64: its first output from state 0 is 0xE220A8397B1DCDAF."""
+    from test_gpu_serve import splitmix64
+    assert splitmix64(0) == 0xE220A8397B1DCDAF
+    assert splitmix64(0x9E3779B97F4A7C15) == 0x6E789E6AA1B965F4
+
+
+def test_serve_driver_rejects_bad_options():
+    import subprocess
+    exe = os.path.join(ROOT, "mfem_amd", "h1pa_serve")
+    for args in (["--mode", "queue"], ["--pmax", "9"], ["--bogus"]):
+        r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=30)
+        assert r.returncode == 2
