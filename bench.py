@@ -1,25 +1,41 @@
-"""Benchmark: PA diffusion operator apply (MDOF/s) on one MI355X.
+"""Benchmark on one MI355X: PA diffusion apply (MDOF/s, the headline value),
+plus the solve legs of the same metric (solves/s and p99 latency of
+mixed-precision batched serving against plain FP64 GPU PCG at 1e-10).
 
-Workload (configuration 2 of the sweep): unit cube, Cartesian hex mesh with
-interior vertices perturbed by U(-0.2h, 0.2h) (seed 1), H1 order p with
-Q = p + 2 Gauss-Legendre points, diffusion coefficient
-k(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y) sin(2 pi z) at the physical quadrature
-points, input x ~ U[-1, 1] over the true DOFs.  A step is one y = A x
-(Operator::Mult: fused gather + element kernel, then the deterministic E->L
-scatter) with x, y and all operator data resident in HBM.
+Legs (all on by default; --legs selects):
+  apply  configuration 2: unit cube, Cartesian hex mesh with interior vertices
+         perturbed by U(-0.2h, 0.2h) (seed 1), H1 order p (default 4) with
+         Q = p + 2 Gauss-Legendre points, k(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y)
+         sin(2 pi z) at the physical quadrature points, x ~ U[-1, 1].  A step is
+         one y = A x (Operator::Mult: fused gather + element kernel, then the
+         deterministic E->L scatter) with x, y and all operator data in HBM.
+  solve  configuration 3: 48^3 mesh curved by x' = x + 0.05 sin(2 pi y)
+         sin(2 pi z) (and cyclic), geometry of order p, per-element log-uniform
+         k in [1e-2, 1e2], RHS U[-1, 1], homogeneous Dirichlet; FP64 Jacobi-PCG
+         against the mixed solve (FP32 / BF16-stored PA inner CG, FP64 reliable
+         updates) at rel_tol 1e-10 (CGSolver's measure), p = 4 and 6.
+  batch  configuration 4: 512 problems, n ~ U[4, 24], p ~ U{2..6}, vertex
+         perturbation 0.15h, k = exp(8 seeded Fourier modes), RHS U[-1, 1]:
+         sequential FP64 CGSolver, FP64 batch and BF16 mixed batch.
+  serve  configuration 5 (shortened arrival window): Poisson arrivals, n in
+         [6, 32] with P(n) ~ 1/n, p ~ U{1..8}, 200 GB HBM budget; the native
+         request loop mfem_amd/h1pa_serve in continuous-batching mixed mode and
+         in serial FP64 mode.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 32e6]
-                    [--sweep] [--no-cpu]
+                    [--legs apply,solve,batch,serve] [--no-cpu] [--sweep]
 
-Prints one JSON line (rank 0).  `roofline` is for the dominant kernel (the
-element kernel); `cpu_baseline` times the CPU oracle (C + OpenMP restatement
-of MFEM's PA path) on a bounded sample on the host cores.
+Prints one JSON line.  `roofline` is for the dominant kernel of the apply
+(the element kernel), timed with HIP events on its launch stream;
+`cpu_baseline` times the CPU oracle (C + OpenMP restatement of MFEM's PA
+path) on a bounded sample on the host cores.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -27,6 +43,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, GB/s
+METRIC = "MDOF/s PA diffusion apply; solves/s + p99 latency vs FP64 PCG at 1e-10 rel res"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def n_for_dofs(p, target):
@@ -54,6 +75,9 @@ def algorithmic_bytes(p, ne, ndofs):
     return apply_, element, scatter
 
 
+# ---------------------------------------------------------------------------
+# apply
+# ---------------------------------------------------------------------------
 def build_problem(M, torch, p, n, seed=1):
     mesh = M.Mesh.MakeCartesian3D(n, n, n)
     mesh.Perturb(0.2, seed)
@@ -69,7 +93,7 @@ def build_problem(M, torch, p, n, seed=1):
     return mesh, fes, a, x, y
 
 
-def run_gpu(M, torch, p, n, steps, warmup):
+def run_apply(M, torch, p, n, steps, warmup):
     mesh, fes, a, x, y = build_problem(M, torch, p, n)
     ne, N = fes.GetNE(), fes.GetNDofs()
     for _ in range(warmup):
@@ -81,14 +105,15 @@ def run_gpu(M, torch, p, n, steps, warmup):
     wall = time.perf_counter() - t0
     ms = wall * 1e3 / steps
     b_apply, b_elem, b_scat = algorithmic_bytes(p, ne, N)
+    variant = a.Info()["variant"]
     return dict(p=p, n=n, ne=ne, ndofs=N, ms_per_step=ms, event_ms_total=tot_ms / steps,
                 ms_element=el_ms, ms_scatter=sc_ms, mdofs=N / (ms * 1e-3) / 1e6,
                 apply_gbs=b_apply / (ms * 1e-3) / 1e9, element_gbs=b_elem / (el_ms * 1e-3) / 1e9,
                 scatter_gbs=b_scat / (sc_ms * 1e-3) / 1e9, bytes_apply=b_apply,
-                bytes_element=b_elem)
+                bytes_element=b_elem, variant=variant)
 
 
-def cpu_baseline(p, target_seconds=15.0):
+def cpu_baseline(p, target_seconds=12.0):
     """The CPU oracle (C + OpenMP restatement of MFEM's PA path) on a bounded
     sample of the same workload: same p, perturbation and coefficient, on a
     smaller mesh; MDOF/s of the L->L apply."""
@@ -103,12 +128,199 @@ def cpu_baseline(p, target_seconds=15.0):
         P.mult(x)
         reps += 1
         el = time.perf_counter() - t0
-        if el > target_seconds or reps >= 50:
+        if el > target_seconds or reps >= 2000:
             break
-    return dict(value=P.ndofs * reps / el / 1e6, unit="MDOF/s", cores=oracle.num_threads(),
-                kind="port",
-                sample=f"oracle PA diffusion apply, p={p}, n={n}^3 ({P.ndofs} DOFs), "
+    return dict(value=round(P.ndofs * reps / el / 1e6, 3), unit="MDOF/s",
+                cores=oracle.num_threads(), kind="port",
+                sample=f"oracle PA diffusion apply (FP64), p={p}, n={n}^3 ({P.ndofs} DOFs), "
                        f"{reps} applies in {el:.1f} s, OpenMP over elements")
+
+
+def traffic_from_profiles(p, n):
+    """HBM bytes per launch of the element kernel measured with rocprofv3 PMC
+    counters (FETCH_SIZE, WRITE_SIZE, scaled by the copy-kernel calibration),
+    when profiles/traffic.json holds this workload; else None."""
+    path = os.path.join(HERE, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rows = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rows:
+        if r.get("p") == p and r.get("n") == n and r.get("kernel") == "element":
+            return r.get("bytes_per_launch")
+    return None
+
+
+# ---------------------------------------------------------------------------
+# solve (configuration 3)
+# ---------------------------------------------------------------------------
+def run_solve(M, torch, p, n=48, seed=7):
+    import numpy as np
+    mesh = M.Mesh.MakeCartesian3D(n, n, n)
+    mesh.SetCurvature(p, 0.05)
+    fes = M.FiniteElementSpace(mesh, M.H1_FECollection(p, 3))
+    ne, N = fes.GetNE(), fes.GetNDofs()
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    k = torch.exp(torch.rand(ne, dtype=torch.float64, device="cuda", generator=g)
+                  * (np.log(1e2) - np.log(1e-2)) + np.log(1e-2))
+    a = M.BilinearForm(fes)
+    a.SetAssemblyLevel(M.AssemblyLevel.PARTIAL)
+    a.AddDomainIntegrator(M.DiffusionIntegrator(M.PWConstCoefficient(k)))
+    a.Assemble()
+    b = torch.rand(N, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    ess = fes.GetBoundaryTrueDofs()
+    x0 = torch.zeros(N, dtype=torch.float64, device="cuda")
+    A, X, B = a.FormLinearSystem(ess, x0, b)
+    y = torch.empty_like(B)
+
+    def true_res(Xs):
+        A.Mult(Xs, y)
+        torch.cuda.synchronize()
+        return (torch.linalg.norm(B - y) / torch.linalg.norm(B)).item()
+
+    def fp64(max_iter):
+        cg = M.CGSolver()
+        cg.SetRelTol(1e-10)
+        cg.SetMaxIter(max_iter)
+        cg.SetOperator(A)
+        cg.SetPreconditioner(M.OperatorJacobiSmoother(a, ess))
+        Xs = X.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cg.Mult(B, Xs)
+        torch.cuda.synchronize()
+        return cg, Xs, time.perf_counter() - t0
+
+    def mixed(prec, max_inner):
+        ir = M.IRSolver(prec)
+        ir.SetRelTol(1e-10)
+        ir.SetDelta(0.1)
+        ir.SetMaxIter(1000, max_inner)
+        ir.SetOperator(A)
+        Xs = X.clone()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ir.Mult(B, Xs)
+        torch.cuda.synchronize()
+        return ir, Xs, time.perf_counter() - t0
+
+    fp64(20)  # first-use costs
+    cg, X64, t64 = fp64(100000)
+    out = dict(p=p, n=n, dofs=N, fp64=dict(seconds=round(t64, 4), iterations=cg.GetNumIterations(),
+                                          converged=cg.GetConverged(),
+                                          true_rel_residual=true_res(X64)))
+    for prec, name in ((M.PREC_FP32, "fp32"), (M.PREC_BF16_PA, "bf16")):
+        mixed(prec, 20)
+        ir, Xi, ti = mixed(prec, 100000)
+        err = (torch.linalg.norm(Xi - X64) / torch.linalg.norm(X64)).item()
+        out[name] = dict(seconds=round(ti, 4), iterations=ir.stats.inner_iterations,
+                         refinements=ir.stats.outer_iterations, converged=ir.GetConverged(),
+                         true_rel_residual=true_res(Xi), rel_diff_vs_fp64=err,
+                         speedup_vs_fp64=round(t64 / ti, 3))
+    return out
+
+
+# ---------------------------------------------------------------------------
+# batch (configuration 4)
+# ---------------------------------------------------------------------------
+def run_batch(M, torch, nprob=512, seed=2024):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    ns, ps = rng.integers(4, 25, nprob), rng.integers(2, 7, nprob)
+    probs = []
+    t0 = time.perf_counter()
+    for i in range(nprob):
+        n, p = int(ns[i]), int(ps[i])
+        mesh = M.Mesh.MakeCartesian3D(n, n, n)
+        mesh.Perturb(0.15, i)
+        fes = M.FiniteElementSpace(mesh, M.H1_FECollection(p, 3))
+        a = M.BilinearForm(fes)
+        a.SetAssemblyLevel(M.AssemblyLevel.PARTIAL)
+        a.AddDomainIntegrator(M.DiffusionIntegrator(M.FourierCoefficient(i)))
+        a.Assemble()
+        N = fes.GetNDofs()
+        b = torch.empty(N, dtype=torch.float64, device="cuda")
+        M.Randomize(b, 1000 + i, -1.0, 1.0)
+        ess = fes.GetBoundaryTrueDofs()
+        A, X, B = a.FormLinearSystem(ess, torch.zeros(N, dtype=torch.float64, device="cuda"), b)
+        probs.append((mesh, fes, a, A, X, B, N))
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    total = sum(pr[6] for pr in probs)
+    log(f"batch: {nprob} problems, {total / 1e6:.1f}M DOFs set up in {setup:.1f} s")
+
+    def sequential():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        its, xs = [], []
+        for pr in probs:
+            cg = M.CGSolver()
+            cg.SetRelTol(1e-10)
+            cg.SetMaxIter(100000)
+            cg.SetOperator(pr[3])
+            cg.SetPreconditioner(M.OperatorJacobiSmoother(pr[2], pr[3].ess))
+            x = pr[4].clone()
+            cg.Mult(pr[5], x)
+            its.append(cg.GetNumIterations())
+            xs.append(x)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, its, xs
+
+    def batched(prec):
+        bs = M.BatchSolver(prec, iters_per_replay=8)
+        xs = []
+        for i, pr in enumerate(probs):
+            x = pr[4].clone()
+            xs.append(x)
+            bs.Add(pr[3], pr[5], x, rel_tol=1e-10, max_iter=100000, tag=i)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        left = bs.Run()
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        res = dict(bs.Poll())
+        assert left == 0
+        return t, [res[i] for i in range(nprob)], xs, bs.Info()
+
+    t_seq, its, x_seq = sequential()
+    t_b64, r64, x_b64, _ = batched(M.PREC_FP64)
+    t_bm, rm, x_bm, info = batched(M.PREC_BF16_PA)
+    diff = max((torch.linalg.norm(a_ - b_) / torch.linalg.norm(b_)).item()
+               for a_, b_ in zip(x_bm, x_seq))
+    return dict(
+        problems=nprob, total_dofs=total, setup_seconds=round(setup, 2),
+        sequential_fp64=dict(seconds=round(t_seq, 3), solves_per_s=round(nprob / t_seq, 2),
+                             iterations_mean=float(np.mean(its)), iterations_max=int(max(its))),
+        batch_fp64=dict(seconds=round(t_b64, 3), solves_per_s=round(nprob / t_b64, 2),
+                        converged=sum(r["converged"] for r in r64)),
+        batch_bf16=dict(seconds=round(t_bm, 3), solves_per_s=round(nprob / t_bm, 2),
+                        converged=sum(r["converged"] for r in rm),
+                        iterations_mean=float(np.mean([r["iterations"] for r in rm])),
+                        max_rel_diff_vs_fp64=diff, replays=info["replays"],
+                        graph_captures=info["captures"]),
+        mixed_batch_vs_sequential_fp64=round(t_seq / t_bm, 3))
+
+
+# ---------------------------------------------------------------------------
+# serve (configuration 5, shortened window)
+# ---------------------------------------------------------------------------
+def run_serve(rate, seconds, seed=1):
+    exe = os.path.join(HERE, "mfem_amd", "h1pa_serve")
+    if not os.path.exists(exe):
+        raise RuntimeError("mfem_amd/h1pa_serve is not built (run __graft_entry__.build())")
+    out = {}
+    for mode, prec in (("batch", "bf16"), ("serial", "fp64")):
+        cmd = [exe, "--mode", mode, "--precision", prec, "--rate", str(rate), "--seconds",
+               str(seconds), "--seed", str(seed)]
+        log("serve: " + " ".join(cmd[1:]))
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(f"h1pa_serve failed: {r.stderr[-2000:]}")
+        out[f"{mode}_{prec}"] = json.loads(r.stdout.strip().splitlines()[-1])
+    b, s = out["batch_bf16"], out["serial_fp64"]
+    out["solves_per_s_ratio"] = round(b["solves_per_s"] / s["solves_per_s"], 3)
+    return out
 
 
 def main():
@@ -118,8 +330,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--dofs", type=float, default=32e6)
+    ap.add_argument("--legs", default="apply,solve,batch,serve")
     ap.add_argument("--sweep", action="store_true", help="also run p = 1..8 at --sweep-dofs")
     ap.add_argument("--sweep-dofs", type=float, default=4e6)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="run one 1 GiB device copy first (scales rocprof FETCH/WRITE_SIZE)")
@@ -128,6 +344,7 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(one process, one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    legs = set(args.legs.split(","))
 
     import torch
     import mfem_amd as M
@@ -141,26 +358,41 @@ def main():
         del src, dst
 
     n = n_for_dofs(args.p, args.dofs)
-    print(f"[bench] PA diffusion apply p={args.p} n={n}^3", file=sys.stderr, flush=True)
-    r = run_gpu(M, torch, args.p, n, args.steps, args.warmup)
-    print(f"[bench] {r}", file=sys.stderr, flush=True)
+    log(f"apply: PA diffusion p={args.p} n={n}^3")
+    r = run_apply(M, torch, args.p, n, args.steps, args.warmup)
+    log(str(r))
+    torch.cuda.empty_cache()
     sweep = []
     if args.sweep:
         for p in range(1, 9):
             ns = n_for_dofs(p, args.sweep_dofs)
-            print(f"[bench] sweep p={p} n={ns}^3", file=sys.stderr, flush=True)
-            s = run_gpu(M, torch, p, ns, max(10, args.steps // 2), 3)
+            log(f"sweep p={p} n={ns}^3")
+            s = run_apply(M, torch, p, ns, max(10, args.steps // 2), 3)
             sweep.append({k: s[k] for k in ("p", "n", "ndofs", "mdofs", "apply_gbs",
                                              "element_gbs", "ms_per_step")})
             torch.cuda.empty_cache()
 
+    extra = {}
+    if "solve" in legs:
+        extra["solve"] = []
+        for p in (4, 6):
+            log(f"solve: configuration 3, p={p}")
+            extra["solve"].append(run_solve(M, torch, p))
+            torch.cuda.empty_cache()
+    if "batch" in legs:
+        log(f"batch: configuration 4, {args.batch_problems} problems")
+        extra["batch"] = run_batch(M, torch, args.batch_problems)
+        torch.cuda.empty_cache()
+    if "serve" in legs:
+        extra["serve"] = run_serve(args.serve_rate, args.serve_seconds)
+
     cpu = None
     if not args.no_cpu:
-        print("[bench] cpu baseline (oracle)", file=sys.stderr, flush=True)
+        log("cpu baseline (oracle)")
         cpu = cpu_baseline(args.p)
 
     line = {
-        "metric": "MDOF/s PA diffusion apply",
+        "metric": METRIC,
         "value": round(r["mdofs"], 2),
         "unit": "MDOF/s",
         "n_gpus": 1,
@@ -179,6 +411,7 @@ def main():
             "apply_gbs_algorithmic": round(r["apply_gbs"], 1),
             "apply_roofline_frac": round(r["apply_gbs"] / HBM_PEAK_GBS, 4),
             "scatter_ms": round(r["ms_scatter"], 4),
+            "element_variant": r["variant"],
         },
         "roofline": {
             "bound": "hbm",
@@ -186,13 +419,20 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(r["element_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": f"k_diffusion_apply<{args.p + 1},{args.p + 2}>",
+            "traffic": traffic_from_profiles(args.p, n),
+            "kernel": f"k_diffusion_apply<double,double,{args.p + 1},{args.p + 2},...>",
             "kernel_ms": round(r["ms_element"], 4),
             "bytes_per_launch": r["bytes_element"],
         },
         "cpu_baseline": cpu,
     }
+    line.update(extra)
+    if "serve" in extra:
+        s = extra["serve"]
+        line["solves_per_s"] = s["batch_bf16"]["solves_per_s"]
+        line["p99_latency_s"] = s["batch_bf16"]["latency_p99_s"]
+        line["fp64_pcg_solves_per_s"] = s["serial_fp64"]["solves_per_s"]
+        line["fp64_pcg_p99_latency_s"] = s["serial_fp64"]["latency_p99_s"]
     if sweep:
         line["sweep"] = sweep
     print(json.dumps(line), flush=True)

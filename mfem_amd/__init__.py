@@ -657,6 +657,13 @@ class BilinearForm:
         return (_wrap_device_double(dd.value, 6 * nq) if dd.value else None,
                 _wrap_device_double(dm.value, nq) if dm.value else None)
 
+    def Info(self):
+        """q1d, integrator bitmask, ndofs and the selected diffusion-apply
+        variant (-1: none) of the first form."""
+        info = (C.c_int64 * 4)()
+        _check(lib().h1pa_form_info(self._forms[0]._h, info))
+        return dict(q1d=info[0], integrators=info[1], ndofs=info[2], variant=info[3])
+
     def DeviceBytes(self):
         """Device bytes held by the form(s), the space and the mesh."""
         tot = 0
