@@ -291,15 +291,21 @@ int h1pa_batch_destroy(h1pa_batch *batch);
  * opts->rel_tol / abs_tol / max_iter are the CGSolver settings in FP64 mode;
  * in mixed mode opts->rel_tol is h1pa_ir_opts.rel_tol (abs_tol, max_iter
  * unused: the batch's max_outer / max_inner apply).
- * `tag` is returned by h1pa_batch_poll; *slot receives the slot index.
- * `stream` is the stream that assembled the form and wrote d_b (the batch's
- * own stream waits for the work queued there so far). */
+ * `tag` is returned by h1pa_batch_poll; *slot receives the solve's index
+ * among those added since the last launch.  The solve's setup (Jacobi
+ * diagonal, reduced-precision data) runs on `stream`, after the work already
+ * queued there (the form's assembly, d_b); the batch waits for it. */
 int h1pa_batch_add(h1pa_batch *batch, h1pa_form *form, const int *d_ess, int64_t n_ess,
                    const double *d_b, double *d_x, const h1pa_pcg_opts *opts, int64_t tag,
                    int *slot, void *stream);
 /* Run up to max_replays replays, stopping early when no solve is running;
  * *n_running receives the number of solves still running. */
 int h1pa_batch_run(h1pa_batch *batch, int max_replays, int *n_running);
+/* Asynchronous form of one replay: launch returns at once (*launched = 0 when
+ * nothing runs); h1pa_batch_add may be called while it is in flight (the new
+ * solve joins at the next launch); wait blocks until the replay is done. */
+int h1pa_batch_launch(h1pa_batch *batch, int *launched);
+int h1pa_batch_wait(h1pa_batch *batch, int *n_running);
 /* Return (and retire) up to `cap` finished solves: tags[i], results[i]. */
 int h1pa_batch_poll(h1pa_batch *batch, int cap, int64_t *tags, h1pa_batch_result *results,
                     int *n);

@@ -134,6 +134,8 @@ SIGNATURES = {
                             C.POINTER(_i), _vp]),
     "h1pa_vector_randomize": (_i, [_vp, _i64, C.c_uint64, _d, _d, _vp]),
     "h1pa_batch_run": (_i, [_vp, _i, C.POINTER(_i)]),
+    "h1pa_batch_launch": (_i, [_vp, C.POINTER(_i)]),
+    "h1pa_batch_wait": (_i, [_vp, C.POINTER(_i)]),
     "h1pa_batch_poll": (_i, [_vp, _i, _pi64, C.POINTER(BatchResult), C.POINTER(_i)]),
     "h1pa_batch_info": (_i, [_vp, _pi64]),
     "h1pa_form_device_bytes": (_i, [_vp, _pi64]),
@@ -873,6 +875,18 @@ class BatchSolver:
         number of solves still running."""
         n = C.c_int()
         _check(lib().h1pa_batch_run(self._h, int(max_replays), C.byref(n)))
+        return n.value
+
+    def Launch(self):
+        """Start one replay without waiting; False when nothing runs."""
+        v = C.c_int()
+        _check(lib().h1pa_batch_launch(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def Wait(self):
+        """Wait for the replay in flight; returns the number still running."""
+        n = C.c_int()
+        _check(lib().h1pa_batch_wait(self._h, C.byref(n)))
         return n.value
 
     def Poll(self, cap=4096):

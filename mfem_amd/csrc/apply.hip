@@ -37,8 +37,8 @@ k_diffusion_apply(int64_t ne, const T *__restrict__ basis, const TP *__restrict_
                   const T *__restrict__ x, const EOutT<T> out, int accumulate) {
     if (out.stop && *out.stop) return;  // solver already finished (block-uniform)
     __shared__ T smem[NB * diffusion_buf<D, Q>()];
-    diffusion_elements<T, TP, D, Q, NB, U, K>((int64_t)blockIdx.x * NB, ne, basis, pa, gmap, mask,
-                                              x, out, accumulate, smem);
+    diffusion_elements<T, TP, D, Q, NB, U, K>(xcd_block(blockIdx.x, gridDim.x) * NB, ne, basis, pa,
+                                              gmap, mask, x, out, accumulate, smem);
 }
 
 template <typename T, typename TP, int D, int Q, int NB>
@@ -48,8 +48,8 @@ k_mass_apply(int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
              const T *__restrict__ x, const EOutT<T> out, int accumulate) {
     if (out.stop && *out.stop) return;
     __shared__ T smem[NB * D * Q * Q];
-    mass_elements<T, TP, D, Q, NB>((int64_t)blockIdx.x * NB, ne, basis, pa, gmap, mask, x, out,
-                                   accumulate, smem);
+    mass_elements<T, TP, D, Q, NB>(xcd_block(blockIdx.x, gridDim.x) * NB, ne, basis, pa, gmap, mask,
+                                   x, out, accumulate, smem);
 }
 
 // Diagonal of the element diffusion matrices (PADiffusionDiagonal3D,
@@ -281,6 +281,8 @@ void launch_mass_diag(int64_t ne, const double *basis, const double *pa, double 
 
 // Selected diffusion variant per (D, Q); filled by the autotuner.
 int g_diff_variant[11][11] = {};
+// Reduced-precision applies: per PA storage kind (0 FP32, 1 BF16) and (D, Q).
+int g_diff_variant_lp[2][11][11] = {};
 
 }  // namespace
 
@@ -366,12 +368,29 @@ void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double
 #define H1PA_FOR_EACH_DQ_DEFAULT(X) \
     X(2, 3) X(3, 4) X(4, 5) X(5, 6) X(6, 7) X(7, 8) X(8, 9) X(9, 10)
 
+void set_diffusion_variant_lp(int pa_kind, int p, int q1d, int v) {
+    if ((pa_kind == 0 || pa_kind == 1) && p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10)
+        g_diff_variant_lp[pa_kind][p + 1][q1d] = v;
+}
+int get_diffusion_variant_lp(int pa_kind, int p, int q1d) {
+    return ((pa_kind == 0 || pa_kind == 1) && p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10)
+               ? g_diff_variant_lp[pa_kind][p + 1][q1d]
+               : 0;
+}
+
 void diffusion_apply_lp(int p, int q1d, int64_t ne, const float *basis, const void *pa,
                         int pa_kind, const int *gmap, const uint8_t *mask, const float *x,
                         const EOutT<float> &out, bool accumulate, hipStream_t s) {
+    diffusion_apply_lp_variant(get_diffusion_variant_lp(pa_kind, p, q1d), p, q1d, ne, basis, pa,
+                               pa_kind, gmap, mask, x, out, accumulate, s);
+}
+
+void diffusion_apply_lp_variant(int v, int p, int q1d, int64_t ne, const float *basis,
+                                const void *pa, int pa_kind, const int *gmap,
+                                const uint8_t *mask, const float *x, const EOutT<float> &out,
+                                bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
-    const int v = get_diffusion_variant(p, q1d);
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
         if (pa_kind == 1)                                                                  \

@@ -15,6 +15,18 @@ constexpr int nb_for(int Q, int buf_doubles) {
     return nb;
 }
 
+// Work-groups are dispatched to the 8 XCDs round-robin (block b runs on XCD
+// b % 8), and each XCD has its own L2.  xcd_block(b, G) renumbers the blocks
+// so that every XCD processes one contiguous range of the grid: elements that
+// are neighbours in the (space-filling-curve) element order, and so share
+// DOFs, run on the same XCD and find each other's x lines in its L2.  The map
+// is a bijection of [0, G).
+__host__ __device__ inline int64_t xcd_block(int64_t b, int64_t G) {
+    constexpr int64_t X = 8;
+    const int64_t q = G / X, r = G % X, k = b % X, j = b / X;
+    return k * q + (k < r ? k : r) + j;
+}
+
 // PA-data element -> compute type (bf16 is stored as its upper 16 bits).
 template <typename T, typename TP>
 __device__ __forceinline__ T to_t(TP v) { return static_cast<T>(v); }

@@ -61,6 +61,8 @@ struct SlotDesc {
                                        // mode the solution x, mixed the update xs
     double *r64, *dinv64, *ye64;       // mixed: FP64 residual, Jacobi, E-vector
     int64_t ne, n, nperm;
+    int64_t bb;                        // first row written directly by the element kernel
+                                       // (interior DOFs; n when none)
     int part0, nchunks;
     double rel_tol, abs_tol;           // CGSolver settings (mixed: rel_tol on the FP64 measure)
     double delta;                      // mixed: refinement trigger
@@ -253,6 +255,10 @@ kb_diffusion(const ElemTask *__restrict__ tasks, const SlotDesc *__restrict__ de
     __shared__ T smem[NB * diffusion_buf<D, Q>()];
     EOutT<T> out;
     out.l2n = sd.l2n;
+    if (sd.bb < sd.n) {  // interior DOFs straight to the L-vector (z, or r64 for RU)
+        out.yL = RU ? reinterpret_cast<T *>(sd.r64) : vp<T>(sd.z);
+        out.bb = sd.bb;
+    }
     if constexpr (RU) {
         out.ye = reinterpret_cast<T *>(sd.ye64);
         diffusion_elements<T, TP, D, Q, NB, U, K>(tk.e0, sd.ne, basis,
@@ -280,16 +286,20 @@ kb_scatter(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
     H1PA_CHUNK_LOOP(t) {
         const int64_t i = t < sd.nperm ? sd.perm[t] : t;
         T v;
-        if (sd.mask && sd.mask[i]) {
-            v = d[i];
+        if (t >= sd.bb) {  // interior DOF: written by the element kernel
+            v = z[i];
         } else {
-            v = T(0);
-            for (int j = sd.offsets[i]; j < sd.offsets[i + 1]; ++j) {
-                const int ix = sd.indices[j];
-                v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
+            if (sd.mask && sd.mask[i]) {
+                v = d[i];
+            } else {
+                v = T(0);
+                for (int j = sd.offsets[i]; j < sd.offsets[i + 1]; ++j) {
+                    const int ix = sd.indices[j];
+                    v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
+                }
             }
+            z[i] = v;
         }
-        z[i] = v;
         acc[0] += (double)d[i] * (double)v;
     }
     store_chunk<1>(acc, sd, tk.chunk, partials, sh);
@@ -443,7 +453,9 @@ kbm_ru_res(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
     H1PA_CHUNK_LOOP(t) {
         const int64_t i = t < sd.nperm ? sd.perm[t] : t;
         double y;
-        if (sd.mask && sd.mask[i]) {
+        if (t >= sd.bb) {  // interior DOF: (A x)_i written into r64 by the element kernel
+            y = sd.r64[i];
+        } else if (sd.mask && sd.mask[i]) {
             y = sd.x[i];
         } else {
             y = 0.0;
@@ -591,8 +603,13 @@ struct BatchSlot {
     DevBuf<double> v64[5];  // FP64 mode: r z d dinv ye;  mixed: r64 dinv64 ye64
     DevBuf<float> v32[5];   // mixed: r z d dinv xs
     SlotDesc desc{};
+    SlotState init{};       // state the slot starts from
+    hipEvent_t ready = nullptr;  // pending slot: its setup work on the caller's stream
     size_t bytes = 0;
     bool live = false;      // added and not yet polled
+    ~BatchSlot() {
+        if (ready) (void)hipEventDestroy(ready);
+    }
 };
 
 struct BatchGroup {
@@ -627,12 +644,19 @@ struct h1pa_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t replays = 0, captures = 0;
+    // solves added since the last launch; merged into `slots` by the next one
+    std::vector<std::unique_ptr<h1pa::BatchSlot>> pending;
+    bool in_flight = false;
+    h1pa::SlotState *h_pinned = nullptr;  // replay's state copy (pinned host memory)
+    size_t h_pinned_n = 0;
 
     ~h1pa_batch() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        pending.clear();
         slots.clear();
+        if (h_pinned) (void)hipHostFree(h_pinned);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -651,7 +675,9 @@ int group_for(Batch &B, Form &f) {
     BatchGroup G;
     G.p = p;
     G.q1d = f.q1d;
-    G.variant = kern::get_diffusion_variant(p, f.q1d);
+    G.variant = B.o.precision == H1PA_PREC_FP64
+                    ? kern::get_diffusion_variant(p, f.q1d)
+                    : kern::get_diffusion_variant_lp(B.o.precision == H1PA_PREC_BF16_PA, p, f.q1d);
     if (!variant_ok(B.o.precision, G.variant)) G.variant = 0;
     G.nb = nb_of_dq(p + 1, f.q1d, G.variant);
     B.groups.push_back(std::move(G));
@@ -671,13 +697,19 @@ void drop_graph(Batch &B) {
     }
 }
 
+// Work list of group g: (slot, first element) per block, ordered so that
+// block b (dispatched to XCD b % 8) covers a contiguous range of the list per
+// XCD (xcd_block), keeping each slot's neighbouring elements on one L2.
 void element_tasks(const Batch &B, int g, int nb, std::vector<ElemTask> &out) {
-    out.clear();
+    std::vector<ElemTask> lin;
     for (size_t k = 0; k < B.slots.size(); ++k) {
         const BatchSlot &sl = *B.slots[k];
         if (sl.group != g || !sl.live || !running_phase(B.h_state[k].phase)) continue;
-        for (int64_t e = 0; e < sl.ne; e += nb) out.push_back({(int)k, (int)e});
+        for (int64_t e = 0; e < sl.ne; e += nb) lin.push_back({(int)k, (int)e});
     }
+    const int64_t G = (int64_t)lin.size();
+    out.resize(lin.size());
+    for (int64_t b = 0; b < G; ++b) out[b] = lin[kern::xcd_block(b, G)];
 }
 
 // Rebuild descriptors, work lists and partial slices for the running slots;
@@ -845,17 +877,20 @@ Batch *batch_create(const h1pa_batch_opts &o) {
     return B.release();
 }
 
+// Set a solve up on the caller's stream `cs` (shadow PA data, constraint
+// mask, Jacobi diagonal) and queue it; the next launch merges it.  Nothing
+// here touches data that an in-flight replay uses, and nothing frees device
+// memory (a free would wait for the replay).
 int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *b, double *x,
-              const h1pa_pcg_opts &po, int64_t tag) {
+              const h1pa_pcg_opts &po, int64_t tag, hipStream_t cs) {
     Form &f = form->f;
     Space &sp = *f.space;
     H1PA_CHECK(f.desc.integrators == H1PA_INTEG_DIFFUSION, H1PA_EUNSUP,
                "batch: diffusion-only forms are supported");
     H1PA_CHECK(f.q1d == sp.p + 2, H1PA_EUNSUP, "batch: q1d must be p + 2");
     H1PA_CHECK(po.max_iter >= 1, H1PA_EINVAL, "batch: max_iter must be >= 1");
-    hipStream_t s = B.stream;
     const bool mixed = mixed_mode(B);
-    if (mixed) ensure_low_precision(f, B.o.precision, s);
+    if (mixed) ensure_low_precision(f, B.o.precision, cs);
     std::unique_ptr<BatchSlot> sl(new BatchSlot);
     sl->form = form;
     sl->tag = tag;
@@ -863,26 +898,24 @@ int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *
     sl->ne = sp.ne;
     sl->group = group_for(B, f);
     BatchGroup &G = B.groups[sl->group];
-    if (!G.basis64_own.p) {
+    if (!G.basis64_own.p) {  // a new group: not referenced by any captured replay yet
         G.basis64_own.alloc(f.basis.n);
         H1PA_HIP(hipMemcpyAsync(G.basis64_own.p, f.basis.p, f.basis.n * sizeof(double),
-                                hipMemcpyDeviceToDevice, s));
+                                hipMemcpyDeviceToDevice, cs));
         G.basis64 = G.basis64_own.p;
         G.basis = G.basis64_own.p;
         if (mixed) {
             G.basis32_own.alloc(f.basis32.n);
             H1PA_HIP(hipMemcpyAsync(G.basis32_own.p, f.basis32.p, f.basis32.n * sizeof(float),
-                                    hipMemcpyDeviceToDevice, s));
+                                    hipMemcpyDeviceToDevice, cs));
             G.basis = G.basis32_own.p;
         }
     }
     const int64_t n = sp.ndofs, nE = sp.ne * sp.dof;
     if (mask_src) {
         sl->mask.alloc(n);
-        H1PA_HIP(hipMemcpyAsync(sl->mask.p, mask_src, n, hipMemcpyDeviceToDevice, s));
+        H1PA_HIP(hipMemcpyAsync(sl->mask.p, mask_src, n, hipMemcpyDeviceToDevice, cs));
     }
-    DevBuf<double> diag(n);
-    operator_diagonal(f, diag.p, s);
     SlotDesc &d = sl->desc;
     d.gmap = sp.restr.gather_map.p;
     d.offsets = sp.restr.offsets.p;
@@ -895,6 +928,8 @@ int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *
     d.x = x;
     d.ne = sp.ne;
     d.n = n;
+    // interior DOFs bypass the E-vector unless a constrained DOF may be interior
+    d.bb = (!mask_src || mask_src == sp.ess_mask.p) ? sp.restr.bubble_begin : n;
     d.pa64 = f.pa_diff.p;
     d.rel_tol = po.rel_tol;
     d.abs_tol = po.abs_tol;
@@ -909,7 +944,9 @@ int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *
         d.ye = sl->v64[4].p;
         d.xi = x;
         d.pa = f.pa_diff.p;
-        kern::jacobi_dinv(n, diag.p, sl->mask.p, sl->v64[3].p, s);
+        // the diagonal passes through r, which the first replay overwrites
+        operator_diagonal(f, sl->v64[0].p, cs);
+        kern::jacobi_dinv(n, sl->v64[0].p, sl->mask.p, sl->v64[3].p, cs);
         bytes += (4 * n + nE) * 8;
     } else {
         for (int k = 0; k < 5; ++k) sl->v32[k].alloc(n);
@@ -929,100 +966,149 @@ int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *
         d.pa = f.pa_diff_lp.p;
         d.delta = B.o.delta;
         d.max_outer = B.o.max_outer;
-        kern::jacobi_dinv(n, diag.p, sl->mask.p, sl->v64[1].p, s);
-        kern::jacobi_dinv_f32(n, diag.p, sl->mask.p, sl->v32[3].p, s);
+        // the diagonal passes through r64, which the first replay overwrites
+        operator_diagonal(f, sl->v64[0].p, cs);
+        kern::jacobi_dinv(n, sl->v64[0].p, sl->mask.p, sl->v64[1].p, cs);
+        kern::jacobi_dinv_f32(n, sl->v64[0].p, sl->mask.p, sl->v32[3].p, cs);
         bytes += 5 * n * 4 + 2 * n * 8 + nE * 8;
     }
     sl->bytes = bytes;
     sl->live = true;
-    SlotState st{};
-    st.phase = PH_INIT;
-    st.max_iter = mixed ? B.o.max_inner : po.max_iter;
-    // reuse a retired position if there is one
-    int k = -1;
-    for (size_t i = 0; i < B.slots.size(); ++i)
-        if (!B.slots[i]->live && B.h_state[i].phase == PH_EMPTY) {
-            k = (int)i;
-            break;
+    sl->init.phase = PH_INIT;
+    sl->init.max_iter = mixed ? B.o.max_inner : po.max_iter;
+    H1PA_HIP(hipEventCreateWithFlags(&sl->ready, hipEventDisableTiming));
+    H1PA_HIP(hipEventRecord(sl->ready, cs));
+    B.pending.push_back(std::move(sl));
+    return (int)B.pending.size() - 1;
+}
+
+namespace {
+
+// Merge the pending solves (nothing in flight): the batch stream waits for
+// their setup, each takes a retired position or a new one.
+void merge_pending(Batch &B) {
+    if (B.pending.empty()) return;
+    hipStream_t s = B.stream;
+    for (auto &up : B.pending) {
+        H1PA_HIP(hipStreamWaitEvent(s, up->ready, 0));
+        int k = -1;
+        for (size_t i = 0; i < B.slots.size(); ++i)
+            if (!B.slots[i]->live && B.h_state[i].phase == PH_EMPTY) {
+                k = (int)i;
+                break;
+            }
+        if (k < 0) {
+            k = (int)B.slots.size();
+            if (B.d_state.n < (size_t)k + 1) {
+                DevBuf<SlotState> nst(std::max<size_t>(64, 2 * ((size_t)k + 1)));
+                if (k > 0)
+                    H1PA_HIP(hipMemcpyAsync(nst.p, B.d_state.p, sizeof(SlotState) * k,
+                                            hipMemcpyDeviceToDevice, s));
+                H1PA_HIP(hipStreamSynchronize(s));
+                B.d_state = std::move(nst);
+            }
+            B.slots.push_back(nullptr);
+            B.h_state.push_back(SlotState{});
         }
-    if (k < 0) {
-        k = (int)B.slots.size();
-        if (B.d_state.n < (size_t)k + 1) {
-            DevBuf<SlotState> nst(std::max<size_t>(64, 2 * ((size_t)k + 1)));
-            if (k > 0)
-                H1PA_HIP(hipMemcpyAsync(nst.p, B.d_state.p, sizeof(SlotState) * k,
-                                        hipMemcpyDeviceToDevice, s));
-            H1PA_HIP(hipStreamSynchronize(s));
-            B.d_state = std::move(nst);
-        }
-        B.slots.push_back(nullptr);
-        B.h_state.push_back(SlotState{});
+        B.h_state[k] = up->init;
+        B.slots[k] = std::move(up);
+        H1PA_HIP(hipMemcpyAsync(B.d_state.p + k, &B.h_state[k], sizeof(SlotState),
+                                hipMemcpyHostToDevice, s));
     }
-    B.slots[k] = std::move(sl);
-    B.h_state[k] = st;
-    H1PA_HIP(hipMemcpyAsync(B.d_state.p + k, &B.h_state[k], sizeof(SlotState),
-                            hipMemcpyHostToDevice, s));
+    B.pending.clear();
     H1PA_HIP(hipStreamSynchronize(s));
     B.dirty = true;
-    return k;
+}
+
+int running_slots(const Batch &B) {
+    int live = (int)B.pending.size();
+    for (size_t k = 0; k < B.slots.size(); ++k)
+        if (B.slots[k]->live && running_phase(B.h_state[k].phase)) ++live;
+    return live;
+}
+
+}  // namespace
+
+// Launch one replay (asynchronously) for the running solves; returns false
+// when nothing runs.
+bool batch_launch(Batch &B) {
+    H1PA_CHECK(!B.in_flight, H1PA_EINVAL, "batch: a replay is in flight (call wait first)");
+    hipStream_t s = B.stream;
+    merge_pending(B);
+    if (running_slots(B) == 0) return false;
+    // compact once a third of the work list belongs to finished slots
+    if (!B.dirty) {
+        int64_t dead = 0, all = 0;
+        for (size_t k = 0; k < B.slots.size(); ++k) {
+            const int nch = B.slots[k]->desc.nchunks;
+            all += nch;
+            if (!running_phase(B.h_state[k].phase)) dead += nch;
+        }
+        if (all > 0 && 3 * dead > all) B.dirty = true;
+    }
+    if (B.dirty) {
+        rebuild(B);
+        autotune(B);
+        if (B.dirty) rebuild(B);  // a variant with another block size was chosen
+    }
+    if (B.o.use_graph) {
+        if (!B.exec) {
+            H1PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue_replay(B, s);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            H1PA_HIP(hipStreamEndCapture(s, &B.graph));
+            H1PA_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+            ++B.captures;
+        }
+        H1PA_HIP(hipGraphLaunch(B.exec, s));
+    } else {
+        enqueue_replay(B, s);
+    }
+    if (B.h_pinned_n < B.slots.size()) {
+        if (B.h_pinned) H1PA_HIP(hipHostFree(B.h_pinned));
+        B.h_pinned = nullptr;
+        B.h_pinned_n = 0;
+        const size_t cap = std::max<size_t>(64, 2 * B.slots.size());
+        H1PA_HIP(hipHostMalloc((void **)&B.h_pinned, cap * sizeof(SlotState), hipHostMallocDefault));
+        B.h_pinned_n = cap;
+    }
+    H1PA_HIP(hipMemcpyAsync(B.h_pinned, B.d_state.p, sizeof(SlotState) * B.slots.size(),
+                            hipMemcpyDeviceToHost, s));
+    B.in_flight = true;
+    return true;
+}
+
+// Wait for the replay in flight (if any); returns the number of running solves.
+int batch_wait(Batch &B) {
+    if (B.in_flight) {
+        H1PA_HIP(hipStreamSynchronize(B.stream));
+        std::copy(B.h_pinned, B.h_pinned + B.slots.size(), B.h_state.begin());
+        B.in_flight = false;
+        ++B.replays;
+    }
+    return running_slots(B);
 }
 
 // Runs up to `max_replays` replays (stops early once no slot is running).
 // Returns the number of slots still running.
 int batch_run(Batch &B, int max_replays) {
-    hipStream_t s = B.stream;
-    auto running = [&]() {
-        int live = 0;
-        for (size_t k = 0; k < B.slots.size(); ++k)
-            if (B.slots[k]->live && running_phase(B.h_state[k].phase)) ++live;
-        return live;
-    };
+    batch_wait(B);
     for (int rp = 0; rp < max_replays; ++rp) {
-        if (running() == 0) break;
-        // compact once a third of the work list belongs to finished slots
-        if (!B.dirty) {
-            int64_t dead = 0, all = 0;
-            for (size_t k = 0; k < B.slots.size(); ++k) {
-                const int nch = B.slots[k]->desc.nchunks;
-                all += nch;
-                if (!running_phase(B.h_state[k].phase)) dead += nch;
-            }
-            if (all > 0 && 3 * dead > all) B.dirty = true;
-        }
-        if (B.dirty) {
-            rebuild(B);
-            autotune(B);
-            if (B.dirty) rebuild(B);  // a variant with another block size was chosen
-        }
-        if (B.o.use_graph) {
-            if (!B.exec) {
-                H1PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                try {
-                    enqueue_replay(B, s);
-                } catch (...) {
-                    hipGraph_t g = nullptr;
-                    (void)hipStreamEndCapture(s, &g);
-                    if (g) (void)hipGraphDestroy(g);
-                    throw;
-                }
-                H1PA_HIP(hipStreamEndCapture(s, &B.graph));
-                H1PA_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
-                ++B.captures;
-            }
-            H1PA_HIP(hipGraphLaunch(B.exec, s));
-        } else {
-            enqueue_replay(B, s);
-        }
-        ++B.replays;
-        H1PA_HIP(hipMemcpyAsync(B.h_state.data(), B.d_state.p, sizeof(SlotState) * B.slots.size(),
-                                hipMemcpyDeviceToHost, s));
-        H1PA_HIP(hipStreamSynchronize(s));
+        if (!batch_launch(B)) break;
+        batch_wait(B);
     }
-    return running();
+    return running_slots(B);
 }
 
 // Finished slots not yet reported: fills up to `cap` results and retires them.
 int batch_poll(Batch &B, int cap, int64_t *tags, h1pa_batch_result *res) {
+    batch_wait(B);
     int n = 0;
     const bool mixed = mixed_mode(B);
     for (size_t k = 0; k < B.slots.size() && n < cap; ++k) {
@@ -1055,7 +1141,8 @@ int batch_poll(Batch &B, int cap, int64_t *tags, h1pa_batch_result *res) {
 }
 
 void batch_info(const Batch &B, int64_t *info) {
-    int64_t live = 0, running = 0, bytes = 0;
+    int64_t live = (int64_t)B.pending.size(), running = live, bytes = 0;
+    for (const auto &p : B.pending) bytes += (int64_t)p->bytes;
     for (size_t k = 0; k < B.slots.size(); ++k) {
         const BatchSlot &sl = *B.slots[k];
         if (!sl.live) continue;

@@ -800,18 +800,26 @@ int h1pa_batch_add(h1pa_batch *batch, h1pa_form *form, const int *d_ess, int64_t
     REQUIRE(batch && form && b && x && opts && (n_ess == 0 || d_ess), "batch_add: null argument");
     Form &f = form->f;
     require_assembled(f);
-    hipStream_t s = batch_stream(*batch);
-    {   // order the batch stream after the caller's work on `stream`
-        hipEvent_t ev;
-        H1PA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-        hipError_t e1 = hipEventRecord(ev, as_stream(stream));
-        hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(s, ev, 0) : e1;
-        (void)hipEventDestroy(ev);
-        H1PA_HIP(e2);
-    }
-    const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
-    const int k = batch_add(*batch, form, mask, b, x, *opts, tag);
+    hipStream_t cs = as_stream(stream);
+    const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, cs) : nullptr;
+    const int k = batch_add(*batch, form, mask, b, x, *opts, tag, cs);
     if (slot) *slot = k;
+    API_END
+}
+
+int h1pa_batch_launch(h1pa_batch *batch, int *launched) {
+    API_BEGIN
+    REQUIRE(batch, "batch_launch: null argument");
+    const bool l = batch_launch(*batch);
+    if (launched) *launched = l ? 1 : 0;
+    API_END
+}
+
+int h1pa_batch_wait(h1pa_batch *batch, int *n_running) {
+    API_BEGIN
+    REQUIRE(batch, "batch_wait: null argument");
+    const int r = batch_wait(*batch);
+    if (n_running) *n_running = r;
     API_END
 }
 

@@ -207,6 +207,14 @@ using EOut = EOutT<double>;
 void diffusion_apply_lp(int p, int q1d, int64_t ne, const float *basis, const void *pa,
                         int pa_kind, const int *gmap, const uint8_t *mask, const float *x,
                         const EOutT<float> &out, bool accumulate, hipStream_t s);
+// Variants of the reduced-precision diffusion apply: 0, 1 (unroll 2), 4 (PA
+// prefetch 2 planes); bit-identical results.
+void diffusion_apply_lp_variant(int variant, int p, int q1d, int64_t ne, const float *basis,
+                                const void *pa, int pa_kind, const int *gmap,
+                                const uint8_t *mask, const float *x, const EOutT<float> &out,
+                                bool accumulate, hipStream_t s);
+void set_diffusion_variant_lp(int pa_kind, int p, int q1d, int variant);
+int get_diffusion_variant_lp(int pa_kind, int p, int q1d);
 void mass_apply_lp(int p, int q1d, int64_t ne, const float *basis, const void *pa, int pa_kind,
                    const int *gmap, const uint8_t *mask, const float *x, const EOutT<float> &out,
                    bool accumulate, hipStream_t s);
@@ -252,9 +260,13 @@ void operator_diagonal(Form &f, double *diag, hipStream_t s);
 h1pa_batch *batch_create(const h1pa_batch_opts &o);
 void batch_destroy(h1pa_batch *B);
 hipStream_t batch_stream(const h1pa_batch &B);
-// mask: the solve's constraint mask (copied) or null; returns the slot index
+// mask: the solve's constraint mask (copied) or null; the setup work runs on
+// `cs`; the solve joins at the next launch.  Returns its index among the
+// pending solves.
 int batch_add(h1pa_batch &B, h1pa_form *form, const uint8_t *mask, const double *b, double *x,
-              const h1pa_pcg_opts &po, int64_t tag);
+              const h1pa_pcg_opts &po, int64_t tag, hipStream_t cs);
+bool batch_launch(h1pa_batch &B);
+int batch_wait(h1pa_batch &B);
 int batch_run(h1pa_batch &B, int max_replays);
 int batch_poll(h1pa_batch &B, int cap, int64_t *tags, h1pa_batch_result *res);
 void batch_info(const h1pa_batch &B, int64_t *info);

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "internal.hpp"
 
@@ -143,17 +144,20 @@ k_pcg_fin_init(int nb, const double *__restrict__ partials, PcgState *st, double
 // z = A d from the element output, DIAG_ONE rows copy d; partials of (d, z).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
-              const T *__restrict__ ye, const uint8_t *__restrict__ mask, const T *__restrict__ d,
-              T *__restrict__ z, double *__restrict__ partials, const PcgState *st,
-              const int *__restrict__ perm, int64_t nperm) {
+k_pcg_scatter(int64_t n, int64_t bb, const int *__restrict__ offsets,
+              const int *__restrict__ indices, const T *__restrict__ ye,
+              const uint8_t *__restrict__ mask, const T *__restrict__ d, T *__restrict__ z,
+              double *__restrict__ partials, const PcgState *st, const int *__restrict__ perm,
+              int64_t nperm) {
     __shared__ double sh[kThreads];
     if (st->done) return;
     double acc[1] = {0.0};
     H1PA_CHUNK(t, n) {
         const int64_t i = t < nperm ? perm[t] : t;
         T v;
-        if (mask && mask[i]) {
+        if (t >= bb) {  // interior DOF: written by the element kernel
+            v = z[i];
+        } else if (mask && mask[i]) {
             v = d[i];
         } else {
             v = T(0);
@@ -162,7 +166,7 @@ k_pcg_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict_
                 v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
             }
         }
-        z[i] = v;
+        if (t < bb) z[i] = v;
         acc[0] += (double)d[i] * (double)v;
     }
     store_partials<1>(acc, partials, sh);
@@ -341,17 +345,20 @@ k_ir_fin_init(int nb, const double *__restrict__ partials, IrState *st) {
 
 // q = A d (FP32; DIAG_ONE rows copy d), partials of (d, q).
 __global__ void __launch_bounds__(kThreads)
-k_ir_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
-             const float *__restrict__ ye, const uint8_t *__restrict__ mask,
-             const float *__restrict__ d, float *__restrict__ q, double *__restrict__ partials,
-             const IrState *st, const int *__restrict__ perm, int64_t nperm) {
+k_ir_scatter(int64_t n, int64_t bb, const int *__restrict__ offsets,
+             const int *__restrict__ indices, const float *__restrict__ ye,
+             const uint8_t *__restrict__ mask, const float *__restrict__ d,
+             float *__restrict__ q, double *__restrict__ partials, const IrState *st,
+             const int *__restrict__ perm, int64_t nperm) {
     __shared__ double sh[kThreads];
     if (st->done) return;
     double acc[1] = {0.0};
     H1PA_CHUNK(t, n) {
         const int64_t i = t < nperm ? perm[t] : t;
         float v;
-        if (mask && mask[i]) {
+        if (t >= bb) {  // interior DOF: written by the element kernel
+            v = q[i];
+        } else if (mask && mask[i]) {
             v = d[i];
         } else {
             v = 0.f;
@@ -360,7 +367,7 @@ k_ir_scatter(int64_t n, const int *__restrict__ offsets, const int *__restrict__
                 v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
             }
         }
-        q[i] = v;
+        if (t < bb) q[i] = v;
         acc[0] += (double)d[i] * (double)v;
     }
     store_partials<1>(acc, partials, sh);
@@ -430,8 +437,9 @@ k_ir_ru_x(int64_t n, float *__restrict__ xs, double *__restrict__ x, const IrSta
 // r64 = b - A x; partials of (B r64, r64), ||r64||^2 and, with
 // r = (float) r64, z = dinv r, of (r, z).
 __global__ void __launch_bounds__(kThreads)
-k_ir_ru_res(int64_t n, const int *__restrict__ offsets, const int *__restrict__ indices,
-            const double *__restrict__ ye64, const uint8_t *__restrict__ mask,
+k_ir_ru_res(int64_t n, int64_t bb, const int *__restrict__ offsets,
+            const int *__restrict__ indices, const double *__restrict__ ye64,
+            const uint8_t *__restrict__ mask,
             const double *__restrict__ b, const double *__restrict__ x,
             const double *__restrict__ dinv64, const float *__restrict__ dinv,
             double *__restrict__ r64, float *__restrict__ r, float *__restrict__ z,
@@ -443,7 +451,9 @@ k_ir_ru_res(int64_t n, const int *__restrict__ offsets, const int *__restrict__ 
     H1PA_CHUNK(t, n) {
         const int64_t i = t < nperm ? perm[t] : t;
         double y;
-        if (mask && mask[i]) {
+        if (t >= bb) {  // interior DOF: (A x)_i written into r64 by the element kernel
+            y = r64[i];
+        } else if (mask && mask[i]) {
             y = x[i];
         } else {
             y = 0.0;
@@ -541,10 +551,17 @@ void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, co
 namespace {
 
 // Element part of the constrained apply, d -> A_E d, E-vector in native
-// order (read back through indices_nat).  Stops early once `stop` is set.
+// order (read back through indices_nat); with yL, the interior DOFs go
+// straight to yL.  Stops early once `stop` is set.
+
+// Interior DOFs may bypass the E-vector when no constrained DOF is interior:
+// the mask is absent or is the space's own boundary mask.
+int64_t direct_rows(const Space &sp, const uint8_t *mask) {
+    return (!mask || mask == sp.ess_mask.p) ? sp.restr.bubble_begin : sp.ndofs;
+}
 template <typename T>
-void pcg_element_apply(Form &f, int prec, const uint8_t *mask, const T *d, T *ye, const int *stop,
-                       hipStream_t s) {
+void pcg_element_apply(Form &f, int prec, const uint8_t *mask, const T *d, T *ye, T *yL,
+                       const int *stop, hipStream_t s) {
     Space &sp = *f.space;
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
@@ -552,6 +569,10 @@ void pcg_element_apply(Form &f, int prec, const uint8_t *mask, const T *d, T *ye
     out.ye = ye;
     out.l2n = sp.restr.lex2nat.p;
     out.stop = stop;
+    if (yL) {  // interior DOFs straight to the L-vector
+        out.yL = yL;
+        out.bb = sp.restr.bubble_begin;
+    }
     if constexpr (sizeof(T) == 8) {
         (void)prec;
         if (diff)
@@ -593,9 +614,10 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
                        partials.p);
     hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p,
                        o.rel_tol, o.abs_tol);
+    const int64_t bb = direct_rows(sp, mask);
     auto apply = [&]() {
-        pcg_element_apply<T>(f, prec, mask, d.p, ye, stop, s);
-        hipLaunchKernelGGL(kern::k_pcg_scatter<T>, grid, blk, 0, s, n, sp.restr.offsets.p,
+        pcg_element_apply<T>(f, prec, mask, d.p, ye, bb < n ? z.p : nullptr, stop, s);
+        hipLaunchKernelGGL(kern::k_pcg_scatter<T>, grid, blk, 0, s, n, bb, sp.restr.offsets.p,
                            sp.restr.indices_nat.p, ye, mask, d.p, z.p, partials.p, st.p,
                            sp.restr.perm.p, sp.restr.nperm);
         hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
@@ -625,6 +647,50 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
     stats.seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
+}  // namespace
+
+namespace {
+// Pick the reduced-precision diffusion variant for (kind, p, Q) once per
+// process on the first large enough form: 2 warm-up + 5 timed launches each.
+bool g_lp_tuned[2][10][11] = {};
+void autotune_lp(Form &f, int kind, hipStream_t s) {
+    Space &sp = *f.space;
+    const int p = sp.p, Q = f.q1d;
+    const char *at = std::getenv("H1PA_AUTOTUNE");
+    if ((at && at[0] == '0') || sp.ne < 20000 || p < 1 || p > 8 || Q > 10 || g_lp_tuned[kind][p][Q])
+        return;
+    g_lp_tuned[kind][p][Q] = true;
+    DevBuf<float> x(sp.ndofs);
+    H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * sp.ndofs, s));
+    hipEvent_t e0, e1;
+    H1PA_HIP(hipEventCreate(&e0));
+    H1PA_HIP(hipEventCreate(&e1));
+    kern::EOutT<float> out;
+    out.ye = f.ework32.p;
+    out.l2n = sp.restr.lex2nat.p;
+    int best = 0;
+    float best_ms = 1e30f;
+    for (int v : {0, 1, 4}) {
+        for (int w = 0; w < 2; ++w)
+            kern::diffusion_apply_lp_variant(v, p, Q, sp.ne, f.basis32.p, f.pa_diff_lp.p, kind,
+                                             sp.restr.gather_map.p, nullptr, x.p, out, false, s);
+        H1PA_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < 5; ++r)
+            kern::diffusion_apply_lp_variant(v, p, Q, sp.ne, f.basis32.p, f.pa_diff_lp.p, kind,
+                                             sp.restr.gather_map.p, nullptr, x.p, out, false, s);
+        H1PA_HIP(hipEventRecord(e1, s));
+        H1PA_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        H1PA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    kern::set_diffusion_variant_lp(kind, p, Q, best);
+}
 }  // namespace
 
 void ensure_low_precision(Form &f, int precision, hipStream_t s) {
@@ -661,12 +727,13 @@ void ensure_low_precision(Form &f, int precision, hipStream_t s) {
     convert(f.pa_mass, f.pa_mass_lp);
     if (f.ework32.n < (size_t)(sp.ne * sp.dof)) f.ework32.alloc((size_t)(sp.ne * sp.dof));
     f.lp_kind = kind;
+    if (f.pa_diff_lp.p) autotune_lp(f, kind, s);
 }
 
 void form_mult_lp(Form &f, int precision, const float *x, float *y, hipStream_t s) {
     ensure_low_precision(f, precision, s);
     Space &sp = *f.space;
-    pcg_element_apply<float>(f, precision, nullptr, x, f.ework32.p, nullptr, s);
+    pcg_element_apply<float>(f, precision, nullptr, x, f.ework32.p, nullptr, nullptr, s);
     hipLaunchKernelGGL(kern::k_scatter_set<float>, dim3((unsigned)((sp.ndofs + 255) / 256)),
                        dim3(256), 0, s, sp.ndofs, sp.restr.offsets.p, sp.restr.indices_nat.p,
                        f.ework32.p, y, sp.restr.perm.p, sp.restr.nperm);
@@ -705,6 +772,7 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
     const int *done = &st.p->done;
     const int *nru = &st.p->nru;
+    const int64_t bb = direct_rows(sp, mask);
     const dim3 grid(nb), blk(kern::kThreads), one(1);
     auto t0 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(kern::k_ir_init, grid, blk, 0, s, n, b, dinv64.p, dinv.p, x, r64.p, r.p,
@@ -714,8 +782,9 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     const int limit = o.max_inner + o.max_outer + 1;
     for (int it = 1; it <= limit; ++it) {
         // q = A d with the reduced-precision operator
-        pcg_element_apply<float>(f, o.inner_precision, mask, d.p, f.ework32.p, done, s);
-        hipLaunchKernelGGL(kern::k_ir_scatter, grid, blk, 0, s, n, sp.restr.offsets.p,
+        pcg_element_apply<float>(f, o.inner_precision, mask, d.p, f.ework32.p,
+                                 bb < n ? q.p : nullptr, done, s);
+        hipLaunchKernelGGL(kern::k_ir_scatter, grid, blk, 0, s, n, bb, sp.restr.offsets.p,
                            sp.restr.indices_nat.p, f.ework32.p, mask, d.p, q.p, partials.p, st.p,
                            sp.restr.perm.p, sp.restr.nperm);
         hipLaunchKernelGGL(kern::k_ir_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
@@ -723,8 +792,9 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
                            d.p, partials.p, st.p);
         hipLaunchKernelGGL(kern::k_ir_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
         hipLaunchKernelGGL(kern::k_ir_ru_x, dim3(nbx), blk, 0, s, n, xs.p, x, st.p);
-        pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, x, f.ework0.p, nru, s);
-        hipLaunchKernelGGL(kern::k_ir_ru_res, grid, blk, 0, s, n, sp.restr.offsets.p,
+        pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, x, f.ework0.p,
+                                  bb < n ? r64.p : nullptr, nru, s);
+        hipLaunchKernelGGL(kern::k_ir_ru_res, grid, blk, 0, s, n, bb, sp.restr.offsets.p,
                            sp.restr.indices_nat.p, f.ework0.p, mask, b, x, dinv64.p, dinv.p,
                            r64.p, r.p, z.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
         hipLaunchKernelGGL(kern::k_ir_fin_ru, one, blk, 0, s, (int)nb, partials.p, st.p);

@@ -9,10 +9,10 @@
 // relative measure at rel_tol.
 //
 //   --mode batch   continuous batching: arrived requests are set up and
-//                  admitted while the HBM budget allows; all admitted solves
-//                  advance together (h1pa_batch_run, one graph replay per
-//                  loop turn); finished solves are retired and their memory
-//                  released.
+//                  admitted while the HBM budget allows, overlapping the replay
+//                  in flight; all admitted solves advance together (one graph
+//                  replay per loop turn, h1pa_batch_launch / _wait); finished
+//                  solves are retired and their memory released.
 //   --mode serial  plain FP64 GPU PCG: requests are set up and solved one at
 //                  a time, in arrival order (h1pa_pcg_solve).
 //
@@ -276,6 +276,8 @@ int main(int argc, char **argv) {
     std::deque<int> queue;
     int64_t used = 0, peak_used = 0;
     int next = 0, done = 0, peak_live = 0;
+    double setup_s = 0.0;  // host time spent setting requests up (GPU idle in serial mode)
+    bool inflight = false;  // batch mode: a replay is running on the GPU
     const auto t0 = std::chrono::steady_clock::now();
     std::vector<int64_t> tags(4096);
     std::vector<h1pa_batch_result> res(4096);
@@ -288,7 +290,9 @@ int main(int argc, char **argv) {
                 const Request &q = reqs[queue.front()];
                 const int64_t est = estimate_bytes(q.n, q.p, prec, true);
                 if (!live.empty() && used + est > budget) break;
+                const double ts = now_s(t0);
                 Live L = setup(q, s);
+                setup_s += now_s(t0) - ts;
                 const int *ess;
                 int64_t nEss;
                 check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
@@ -304,6 +308,26 @@ int main(int argc, char **argv) {
             }
             peak_used = std::max(peak_used, used);
             peak_live = std::max(peak_live, (int)live.size());
+            if (inflight) {  // the admissions above overlapped this replay
+                int running = 0;
+                check(h1pa_batch_wait(B, &running), "batch_wait");
+                inflight = false;
+                int nd = 0;
+                check(h1pa_batch_poll(B, (int)tags.size(), tags.data(), res.data(), &nd),
+                      "poll");
+                const double tf = now_s(t0);
+                for (int i = 0; i < nd; ++i) {
+                    Request &q = reqs[tags[i]];
+                    q.t_done = tf;
+                    q.iterations = res[i].iterations;
+                    q.converged = res[i].converged;
+                    release(live[tags[i]]);
+                    live.erase(tags[i]);
+                    used -= live_est[tags[i]];
+                    live_est.erase(tags[i]);
+                    ++done;
+                }
+            }
             if (live.empty()) {
                 if (next < R) {
                     const double wait = reqs[next].t_arrive - now_s(t0);
@@ -311,22 +335,9 @@ int main(int argc, char **argv) {
                 }
                 continue;
             }
-            int running = 0;
-            check(h1pa_batch_run(B, 1, &running), "batch_run");
-            int nd = 0;
-            check(h1pa_batch_poll(B, (int)tags.size(), tags.data(), res.data(), &nd), "poll");
-            const double tf = now_s(t0);
-            for (int i = 0; i < nd; ++i) {
-                Request &q = reqs[tags[i]];
-                q.t_done = tf;
-                q.iterations = res[i].iterations;
-                q.converged = res[i].converged;
-                release(live[tags[i]]);
-                live.erase(tags[i]);
-                used -= live_est[tags[i]];
-                live_est.erase(tags[i]);
-                ++done;
-            }
+            int launched = 0;
+            check(h1pa_batch_launch(B, &launched), "batch_launch");
+            inflight = launched != 0;
         } else {
             if (queue.empty()) {
                 if (next < R) {
@@ -337,7 +348,10 @@ int main(int argc, char **argv) {
             }
             Request &q = reqs[queue.front()];
             queue.pop_front();
+            const double ts = now_s(t0);
             Live L = setup(q, s);
+            hcheck(hipStreamSynchronize(s), "sync");
+            setup_s += now_s(t0) - ts;
             peak_used = std::max(peak_used, L.bytes);
             peak_live = 1;
             const int *ess;
@@ -373,10 +387,11 @@ int main(int argc, char **argv) {
         "\"requests\": %d, \"converged\": %lld, \"makespan_s\": %.3f, \"solves_per_s\": %.3f, "
         "\"latency_p50_s\": %.4f, \"latency_p90_s\": %.4f, \"latency_p99_s\": %.4f, "
         "\"latency_max_s\": %.4f, \"mean_dofs\": %.0f, \"mean_iterations\": %.1f, "
-        "\"peak_live\": %d, \"peak_bytes\": %lld, \"budget_bytes\": %lld}\n",
+        "\"peak_live\": %d, \"peak_bytes\": %lld, \"budget_bytes\": %lld, "
+        "\"setup_s\": %.3f}\n",
         o.mode.c_str(), prec == H1PA_PREC_FP64 ? "fp64" : prec == H1PA_PREC_FP32 ? "fp32" : "bf16",
         o.rate, o.seconds, R, (long long)conv, makespan, R / makespan, pct(lat, 0.5),
         pct(lat, 0.9), pct(lat, 0.99), pct(lat, 1.0), R ? (double)dofs / R : 0.0,
-        R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget);
+        R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget, setup_s);
     return 0;
 }

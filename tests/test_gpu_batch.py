@@ -179,3 +179,30 @@ def test_mixed_batch_vs_oracle(orc, h1pa, torch_cuda):
             assert res[seed]["converged"]
             assert meas <= 1.05e-10
             assert err <= 1e-6
+
+
+def test_add_while_replay_in_flight(h1pa, torch_cuda, probs):
+    """Solves added while a replay runs (Launch ... Add ... Wait) join at the
+    next launch and give the same bits as alone."""
+    P = h1pa.PREC_BF16_PA
+    first, later = probs[:3], probs[3:6]
+    bs = h1pa.BatchSolver(P, iters_per_replay=4)
+    xs = {}
+    for Q in first:
+        xs[Q.seed] = Q.fresh_x()
+        bs.Add(Q.A, Q.B, xs[Q.seed], rel_tol=1e-10, tag=Q.seed)
+    assert bs.Launch()
+    for Q in later:  # replay in flight
+        xs[Q.seed] = Q.fresh_x()
+        bs.Add(Q.A, Q.B, xs[Q.seed], rel_tol=1e-10, tag=Q.seed)
+    res = {}
+    while True:
+        bs.Wait()
+        res.update(dict(bs.Poll()))
+        if not bs.Launch():
+            break
+    assert sorted(res) == sorted(xs)
+    for Q in later:
+        alone = _run_batch(h1pa, [Q], P, rel_tol=1e-10, k=4)
+        assert alone[Q.seed][1] == res[Q.seed]
+        assert np.array_equal(alone[Q.seed][0], xs[Q.seed].cpu().numpy())
