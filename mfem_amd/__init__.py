@@ -25,7 +25,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libh1pa.so")
+# The in-tree build; H1PA_LIB names another build of the same library (for
+# side-by-side measurements of two versions).
+LIB_PATH = os.environ.get("H1PA_LIB") or os.path.join(_HERE, "libh1pa.so")
 
 _lib = None
 

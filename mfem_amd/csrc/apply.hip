@@ -14,15 +14,19 @@
 //  * z-lines: thread (qx,qy) keeps the z-line in registers, so the z
 //    contraction, the pointwise 3x3 product and the backward z contraction
 //    never touch LDS;
+//  * every 1D contraction uses the even-odd split of the centro-symmetric B
+//    and centro-antisymmetric G (apply_body.hpp): half the multiply-adds, and
+//    the z-stage walks plane pairs (q, Q-1-q);
 //  * B and G enter the FMAs as scalar (SGPR) operands.  The contracted index
 //    is unrolled, the output-line loop is not: each iteration needs one row
-//    (or column) of B and G -- at most 2*10 doubles, loaded with wide scalar
-//    loads from a contiguous copy -- so the SGPR file never spills;
-//  * one LDS buffer of max(2 D^2 Q, 3 D Q^2) doubles per element, reused
+//    of the even-odd tables, loaded with wide scalar loads, so the SGPR file
+//    never spills;
+//  * one LDS buffer of max(2 D^2 Q, 3 D Q^2) values per element, reused
 //    across the stages with a barrier between the reads and the writes.
 //
-// basis layout (4*Q*D doubles): [ B(q,d) q-fastest | G q-fastest |
-//                                 B d-fastest      | G d-fastest ].
+// basis layout: [ B(q,d) q-fastest | G q-fastest | B d-fastest | G d-fastest |
+//                 even-odd tables (EO<D, Q> in apply_body.hpp) ]; the diagonal
+// kernels read the first four blocks.
 #include "apply_body.hpp"
 
 namespace h1pa {
@@ -227,7 +231,7 @@ void launch_diffusion(int variant, int64_t ne, const double *basis, const double
                       bool acc, hipStream_t s) {
     using T = double;
     using TP = double;
-    constexpr int NBmax = nb_for(Q, 3 * D * Q * Q);
+    constexpr int NBmax = nb_for(Q, 3 * D * Q * Q, sizeof(T));
     constexpr int NBhalf = NBmax / 2 > 0 ? NBmax / 2 : 1;
     switch (variant) {
         case 1: launch_diffusion_v<T, TP, D, Q, NBmax, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
@@ -241,7 +245,7 @@ void launch_diffusion(int variant, int64_t ne, const double *basis, const double
 template <typename T, typename TP, int D, int Q>
 void launch_mass(int64_t ne, const T *basis, const TP *pa, const int *gmap, const uint8_t *mask,
                  const T *x, const EOutT<T> &out, bool acc, hipStream_t s) {
-    constexpr int NB = nb_for(Q, D * Q * Q);
+    constexpr int NB = nb_for(Q, D * Q * Q, sizeof(T));
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
     hipLaunchKernelGGL((k_mass_apply<T, TP, D, Q, NB>), dim3(grid), dim3(NB * Q * Q), 0, s, ne,
                        basis, pa, gmap, mask, x, out, acc ? 1 : 0);
@@ -253,7 +257,7 @@ template <typename TP, int D, int Q>
 void launch_diffusion_lp(int variant, int64_t ne, const float *basis, const TP *pa,
                          const int *gmap, const uint8_t *mask, const float *x,
                          const EOutT<float> &out, bool acc, hipStream_t s) {
-    constexpr int NBmax = nb_for(Q, 3 * D * Q * Q);
+    constexpr int NBmax = nb_for(Q, 3 * D * Q * Q, sizeof(float));
     if (variant == 1)
         launch_diffusion_v<float, TP, D, Q, NBmax, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s);
     else if (variant == 4)

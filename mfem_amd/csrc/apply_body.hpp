@@ -8,10 +8,36 @@
 namespace h1pa {
 namespace kern {
 
-constexpr int nb_for(int Q, int buf_doubles) {
-    int nb = 256 / (Q * Q);
+// Fraction of the lanes of its 64-wide waves that nb elements of Q^2 threads use.
+constexpr double lane_use(int nb, int QQ) {
+    const int t = nb * QQ;
+    return (double)t / (64.0 * ((t + 63) / 64));
+}
+
+// Elements per work-group for an element body of `buf` LDS values of `bytes`
+// each: about 256 threads within 48 KiB of LDS; where that leaves more than
+// a tenth of the lanes idle (Q = 9, 10: 81 or 100 threads per element) and
+// the values are 4-byte, the count with the best lane use within 1024 threads
+// and 64 KiB of LDS.  FP64 bodies keep the 256-thread count: their larger LDS
+// footprint per element makes the wider blocks cost more resident waves than
+// the idle lanes do (p = 8 FP64 apply measured 23.2k -> 17.9k MDOF/s with 3
+// instead of 2 elements per block).
+constexpr int nb_for(int Q, int buf, int bytes) {
+    const int QQ = Q * Q;
+    int nb = 256 / QQ;
     if (nb < 1) nb = 1;
-    while (nb > 1 && nb * buf_doubles * 8 > 48 * 1024) --nb;
+    while (nb > 1 && nb * buf * bytes > 48 * 1024) --nb;
+    const double u0 = lane_use(nb, QQ);
+    if (bytes == 4 && u0 < 0.9) {
+        int best = nb;
+        double bu = u0;
+        for (int n = nb + 1; n * QQ <= 1024 && n * buf * bytes <= 64 * 1024; ++n)
+            if (lane_use(n, QQ) > bu + 1e-9) {
+                best = n;
+                bu = lane_use(n, QQ);
+            }
+        if (bu > u0 + 0.05) nb = best;
+    }
     return nb;
 }
 
@@ -68,11 +94,38 @@ __device__ __forceinline__ void store_out(const EOutT<T> &out, int64_t e, int a,
     else *dst = v;
 }
 
+// Even-odd form of the 1D operators.  B (GLL Lagrange basis at the GL
+// points) is centro-symmetric, B(Q-1-q, D-1-d) = B(q, d), and G = B' is
+// centro-antisymmetric, G(Q-1-q, D-1-d) = -G(q, d).  Splitting a line u into
+// e_d = u_d + u_{D-1-d}, o_d = u_d - u_{D-1-d} (d < D/2) and the middle value
+// m (D odd), and a row q of M into Me = (M(q,d) + M(q,D-1-d))/2,
+// Mo = (M(q,d) - M(q,D-1-d))/2, Mm = M(q, D/2):
+//   A = sum Me e + Mm m,  C = sum Mo o,
+//   (M u)_q = A + C,  (M u)_{Q-1-q} = s (A - C)           (s = +1 B, -1 G)
+// and for the transpose, with P_q = v_q + s v_{Q-1-q}, N_q = v_q - s v_{Q-1-q}
+// (q < Q/2; both equal v_{Q/2} at the middle q of an odd Q):
+//   E_d = sum_q Me(q,d) P_q,  F_d = sum_q Mo(q,d) N_q,
+//   (M^T v)_d = E_d + F_d,  (M^T v)_{D-1-d} = E_d - F_d,  (M^T v)_{D/2} = sum_q Mm(q) P_q.
+// Each 1D contraction then costs about half the multiply-adds.  The tables
+// follow the four plain blocks of the basis array (eo_tables below).
+template <int D, int Q>
+struct EO {
+    static constexpr int HD = D / 2, OD = D % 2, HQ = Q / 2, OQ = Q % 2;
+    static constexpr int R = HQ + OQ;        // row pairs (+ middle row)
+    static constexpr int W = 2 * HD + OD;    // forward row: Me | Mo | Mm
+    static constexpr int RB = 2 * R;         // backward row: Me column | Mo column
+    static constexpr int FB = 4 * Q * D;     // offsets in the basis array
+    static constexpr int FG = FB + R * W;
+    static constexpr int BB = FG + R * W;
+    static constexpr int BG = BB + (HD + OD) * RB;
+    static constexpr int SIZE = BG + (HD + OD) * RB;  // total basis length
+};
+
 // U: unroll factor of the output-line loops (1 or 2).
-// K: PA-data pipeline depth.  0: each z-plane's 6 values are loaded in its
-//    own iteration.  K > 0: the first K planes are loaded at kernel entry
-//    (their latency hides behind the x/y stages) and plane qz + K is issued
-//    while plane qz is computed, from a K x 6 register ring.
+// K: PA-data pipeline depth in z-planes.  0: the 12 values of a plane pair
+//    are loaded in its own iteration.  K > 0: (K+1)/2 plane pairs are loaded
+//    at kernel entry (their latency hides behind the x/y stages) and pair
+//    j + (K+1)/2 is issued while pair j is computed.
 // LDS doubles (or floats) per element of the diffusion body.
 template <int D, int Q>
 constexpr int diffusion_buf() {
@@ -82,7 +135,565 @@ constexpr int diffusion_buf() {
 // The elements e0 + (threadIdx.x / Q^2), NB per work-group, of one problem.
 template <typename T, typename TP, int D, int Q, int NB, int U, int K>
 __device__ __forceinline__ void
-diffusion_elements(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
+diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
+                   const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
+                   const T *__restrict__ x, const EOutT<T> &out, int accumulate,
+                   T *__restrict__ smem) {
+    using E = EO<D, Q>;
+    constexpr int HD = E::HD, OD = E::OD, HQ = E::HQ, R = E::R, W = E::W, RB = E::RB;
+    constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
+    constexpr int DDQ = D * D * Q, DQQ = D * Q * Q;
+    constexpr int BUF = diffusion_buf<D, Q>();
+    const T *__restrict__ FBt = basis + E::FB;  // forward rows of B
+    const T *__restrict__ FGt = basis + E::FG;  // forward rows of G
+    const T *__restrict__ BBt = basis + E::BB;  // backward rows of B
+    const T *__restrict__ BGt = basis + E::BG;  // backward rows of G
+    const int le = threadIdx.x / QQ;
+    const int t = threadIdx.x - le * QQ;
+    const int a = t % Q, b = t / Q;
+    const int64_t e = e0 + le;
+    const bool valid = e < ne;
+    T *s = smem + le * BUF;
+    constexpr int KP = K > 0 ? ((K + 1) / 2 < R ? (K + 1) / 2 : R) : 0;  // pairs ahead
+    const TP *pcol = pa + (int64_t)QQQ * 6 * (valid ? e : 0) + a + Q * b;
+    // the 12 PA values of plane pair j: planes j and Q-1-j (the same plane
+    // twice at the middle of an odd Q)
+    auto load_pair = [&](int j, T (&r)[12]) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            r[c] = to_t<T>(pcol[QQQ * c + QQ * j]);
+            r[6 + c] = to_t<T>(pcol[QQQ * c + QQ * (Q - 1 - j)]);
+        }
+    };
+    T ring[KP > 0 ? KP : 1][12];
+    if constexpr (KP > 0) {
+        if (valid) {
+#pragma unroll
+            for (int j = 0; j < KP; ++j) load_pair(j, ring[j]);
+        }
+    }
+
+    // ---- forward x: thread (dy, dz) = (a, b); u -> B u, G u along x
+    if (valid && a < D && b < D) {
+        T ue[HD > 0 ? HD : 1], uo[HD > 0 ? HD : 1], um = T(0);
+        {
+            T u[D];
+            const int64_t base = (int64_t)DDD * e + D * (a + D * b);
+#pragma unroll
+            for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                ue[d] = u[d] + u[D - 1 - d];
+                uo[d] = u[d] - u[D - 1 - d];
+            }
+            if constexpr (OD) um = u[HD];
+        }
+        T *o = s + Q * (a + D * b);
+#pragma unroll U
+        for (int q = 0; q < R; ++q) {
+            const T *fb = FBt + W * q, *fg = FGt + W * q;
+            T Ab = OD ? fb[2 * HD] * um : T(0), Ag = OD ? fg[2 * HD] * um : T(0);
+            T Cb = T(0), Cg = T(0);
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                Ab = fma(fb[d], ue[d], Ab);
+                Cb = fma(fb[HD + d], uo[d], Cb);
+                Ag = fma(fg[d], ue[d], Ag);
+                Cg = fma(fg[HD + d], uo[d], Cg);
+            }
+            o[q] = Ab + Cb;
+            o[DDQ + q] = Ag + Cg;
+            if (q < HQ) {
+                o[Q - 1 - q] = Ab - Cb;
+                o[DDQ + Q - 1 - q] = Cg - Ag;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- forward y: thread (qx, dz) = (a, b); lines r0 = B_x u, r1 = G_x u
+    {
+        T r0e[HD > 0 ? HD : 1], r0o[HD > 0 ? HD : 1], r1e[HD > 0 ? HD : 1], r1o[HD > 0 ? HD : 1];
+        T r0m = T(0), r1m = T(0);
+        const bool act = valid && b < D;
+        if (act) {
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                const T p0 = s[a + Q * (d + D * b)], q0 = s[a + Q * (D - 1 - d + D * b)];
+                const T p1 = s[DDQ + a + Q * (d + D * b)], q1 = s[DDQ + a + Q * (D - 1 - d + D * b)];
+                r0e[d] = p0 + q0;
+                r0o[d] = p0 - q0;
+                r1e[d] = p1 + q1;
+                r1o[d] = p1 - q1;
+            }
+            if constexpr (OD) {
+                r0m = s[a + Q * (HD + D * b)];
+                r1m = s[DDQ + a + Q * (HD + D * b)];
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll U
+            for (int q = 0; q < R; ++q) {
+                const T *fb = FBt + W * q, *fg = FGt + W * q;
+                T A0 = OD ? fb[2 * HD] * r0m : T(0), G0 = OD ? fg[2 * HD] * r0m : T(0);
+                T A1 = OD ? fb[2 * HD] * r1m : T(0);
+                T C0 = T(0), H0 = T(0), C1 = T(0);
+#pragma unroll
+                for (int d = 0; d < HD; ++d) {
+                    A0 = fma(fb[d], r0e[d], A0);
+                    C0 = fma(fb[HD + d], r0o[d], C0);
+                    G0 = fma(fg[d], r0e[d], G0);
+                    H0 = fma(fg[HD + d], r0o[d], H0);
+                    A1 = fma(fb[d], r1e[d], A1);
+                    C1 = fma(fb[HD + d], r1o[d], C1);
+                }
+                // bb = B_y B_x u (-> d/dz), gb = G_y B_x u (-> d/dy), bg = B_y G_x u (-> d/dx)
+                T *o = s + a + Q * (q + Q * b);
+                o[0 * DQQ] = A1 + C1;
+                o[1 * DQQ] = G0 + H0;
+                o[2 * DQQ] = A0 + C0;
+                if (q < HQ) {
+                    T *o2 = s + a + Q * (Q - 1 - q + Q * b);
+                    o2[0 * DQQ] = A1 - C1;
+                    o2[1 * DQQ] = H0 - G0;
+                    o2[2 * DQQ] = A0 - C0;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- z in registers: thread (qx, qy) = (a, b); plane pairs (j, Q-1-j)
+    {
+        T z0e[HD > 0 ? HD : 1], z0o[HD > 0 ? HD : 1], z1e[HD > 0 ? HD : 1], z1o[HD > 0 ? HD : 1];
+        T z2e[HD > 0 ? HD : 1], z2o[HD > 0 ? HD : 1];
+        T z0m = T(0), z1m = T(0), z2m = T(0);
+        if (valid) {
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                const T *i = s + a + Q * (b + Q * d);
+                const T *k = s + a + Q * (b + Q * (D - 1 - d));
+                z0e[d] = i[0 * DQQ] + k[0 * DQQ];
+                z0o[d] = i[0 * DQQ] - k[0 * DQQ];
+                z1e[d] = i[1 * DQQ] + k[1 * DQQ];
+                z1o[d] = i[1 * DQQ] - k[1 * DQQ];
+                z2e[d] = i[2 * DQQ] + k[2 * DQQ];
+                z2o[d] = i[2 * DQQ] - k[2 * DQQ];
+            }
+            if constexpr (OD) {
+                const T *i = s + a + Q * (b + Q * HD);
+                z0m = i[0 * DQQ];
+                z1m = i[1 * DQQ];
+                z2m = i[2 * DQQ];
+            }
+        }
+        __syncthreads();
+        if (valid) {
+            // backward accumulators in even-odd form: E (Me), F (Mo), M (Mm)
+            T E0[HD > 0 ? HD : 1], F0[HD > 0 ? HD : 1], E1[HD > 0 ? HD : 1], F1[HD > 0 ? HD : 1];
+            T E2[HD > 0 ? HD : 1], F2[HD > 0 ? HD : 1];
+            T M0 = T(0), M1 = T(0), M2 = T(0);
+#pragma unroll
+            for (int d = 0; d < HD; ++d) E0[d] = F0[d] = E1[d] = F1[d] = E2[d] = F2[d] = T(0);
+#pragma unroll 1
+            for (int jb = 0; jb < R; jb += (KP > 0 ? KP : 1)) {
+#pragma unroll
+                for (int jj = 0; jj < (KP > 0 ? KP : 1); ++jj) {
+                    const int j = jb + jj;
+                    if (j < R) {
+                        T pd[12];
+                        if constexpr (KP > 0) {
+#pragma unroll
+                            for (int c = 0; c < 12; ++c) pd[c] = ring[jj][c];
+                            if (j + KP < R) load_pair(j + KP, ring[jj]);
+                        } else {
+                            load_pair(j, pd);
+                        }
+                        const T *fb = FBt + W * j, *fg = FGt + W * j;
+                        T A0 = OD ? fb[2 * HD] * z0m : T(0), A1 = OD ? fb[2 * HD] * z1m : T(0);
+                        T A2 = OD ? fg[2 * HD] * z2m : T(0);
+                        T C0 = T(0), C1 = T(0), C2 = T(0);
+#pragma unroll
+                        for (int d = 0; d < HD; ++d) {
+                            A0 = fma(fb[d], z0e[d], A0);
+                            C0 = fma(fb[HD + d], z0o[d], C0);
+                            A1 = fma(fb[d], z1e[d], A1);
+                            C1 = fma(fb[HD + d], z1o[d], C1);
+                            A2 = fma(fg[d], z2e[d], A2);
+                            C2 = fma(fg[HD + d], z2o[d], C2);
+                        }
+                        // gradient at plane j and at plane Q-1-j
+                        const T g0 = A0 + C0, g1 = A1 + C1, g2 = A2 + C2;
+                        const T h0 = A0 - C0, h1 = A1 - C1, h2 = C2 - A2;
+                        const T O0 = fma(pd[0], g0, fma(pd[1], g1, pd[2] * g2));
+                        const T O1 = fma(pd[1], g0, fma(pd[3], g1, pd[4] * g2));
+                        const T O2 = fma(pd[2], g0, fma(pd[4], g1, pd[5] * g2));
+                        T P0, N0, P1, N1, P2, N2;
+                        if (j < HQ) {
+                            const T Q0 = fma(pd[6], h0, fma(pd[7], h1, pd[8] * h2));
+                            const T Q1 = fma(pd[7], h0, fma(pd[9], h1, pd[10] * h2));
+                            const T Q2 = fma(pd[8], h0, fma(pd[10], h1, pd[11] * h2));
+                            P0 = O0 + Q0;  // B: s = +1
+                            N0 = O0 - Q0;
+                            P1 = O1 + Q1;
+                            N1 = O1 - Q1;
+                            P2 = O2 - Q2;  // G: s = -1
+                            N2 = O2 + Q2;
+                        } else {  // middle plane of an odd Q
+                            P0 = N0 = O0;
+                            P1 = N1 = O1;
+                            P2 = N2 = O2;
+                        }
+#pragma unroll
+                        for (int d = 0; d < HD; ++d) {
+                            E0[d] = fma(fb[d], P0, E0[d]);
+                            F0[d] = fma(fb[HD + d], N0, F0[d]);
+                            E1[d] = fma(fb[d], P1, E1[d]);
+                            F1[d] = fma(fb[HD + d], N1, F1[d]);
+                            E2[d] = fma(fg[d], P2, E2[d]);
+                            F2[d] = fma(fg[HD + d], N2, F2[d]);
+                        }
+                        if constexpr (OD) {
+                            M0 = fma(fb[2 * HD], P0, M0);
+                            M1 = fma(fb[2 * HD], P1, M1);
+                            M2 = fma(fg[2 * HD], P2, M2);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                T *o = s + a + Q * (b + Q * d);
+                T *k = s + a + Q * (b + Q * (D - 1 - d));
+                o[0 * DQQ] = E0[d] + F0[d];
+                k[0 * DQQ] = E0[d] - F0[d];
+                o[1 * DQQ] = E1[d] + F1[d];
+                k[1 * DQQ] = E1[d] - F1[d];
+                o[2 * DQQ] = E2[d] + F2[d];
+                k[2 * DQQ] = E2[d] - F2[d];
+            }
+            if constexpr (OD) {
+                T *o = s + a + Q * (b + Q * HD);
+                o[0 * DQQ] = M0;
+                o[1 * DQQ] = M1;
+                o[2 * DQQ] = M2;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- backward y: thread (qx, dz) = (a, b)
+    //      sa = B_y^T t0,  sb = G_y^T t1 + B_y^T t2
+    {
+        T P0[R], N0[R], P1[R], N1[R], P2[R], N2[R];
+        const bool act = valid && b < D;
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const T *i = s + a + Q * (j + Q * b);
+                if (j < HQ) {
+                    const T *k = s + a + Q * (Q - 1 - j + Q * b);
+                    P0[j] = i[0 * DQQ] + k[0 * DQQ];
+                    N0[j] = i[0 * DQQ] - k[0 * DQQ];
+                    P1[j] = i[1 * DQQ] - k[1 * DQQ];  // G
+                    N1[j] = i[1 * DQQ] + k[1 * DQQ];
+                    P2[j] = i[2 * DQQ] + k[2 * DQQ];
+                    N2[j] = i[2 * DQQ] - k[2 * DQQ];
+                } else {
+                    P0[j] = N0[j] = i[0 * DQQ];
+                    P1[j] = N1[j] = i[1 * DQQ];
+                    P2[j] = N2[j] = i[2 * DQQ];
+                }
+            }
+        }
+        __syncthreads();
+        if (act) {
+#pragma unroll U
+            for (int d = 0; d < HD + OD; ++d) {
+                const T *bb = BBt + RB * d, *bg = BGt + RB * d;
+                T Ea = T(0), Fa = T(0), Eb = T(0), Fb = T(0);
+                if (d < HD) {
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        Ea = fma(bb[j], P0[j], Ea);
+                        Fa = fma(bb[R + j], N0[j], Fa);
+                        Eb = fma(bg[j], P1[j], Eb);
+                        Fb = fma(bg[R + j], N1[j], Fb);
+                        Eb = fma(bb[j], P2[j], Eb);
+                        Fb = fma(bb[R + j], N2[j], Fb);
+                    }
+                    s[a + Q * (d + D * b)] = Ea + Fa;
+                    s[a + Q * (D - 1 - d + D * b)] = Ea - Fa;
+                    s[DDQ + a + Q * (d + D * b)] = Eb + Fb;
+                    s[DDQ + a + Q * (D - 1 - d + D * b)] = Eb - Fb;
+                } else {  // middle d of an odd D
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        Ea = fma(bb[j], P0[j], Ea);
+                        Eb = fma(bg[j], P1[j], Eb);
+                        Eb = fma(bb[j], P2[j], Eb);
+                    }
+                    s[a + Q * (d + D * b)] = Ea;
+                    s[DDQ + a + Q * (d + D * b)] = Eb;
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- backward x: thread (dy, dz) = (a, b); v = G_x^T sa + B_x^T sb
+    if (valid && a < D && b < D) {
+        T Pa[R], Na[R], Pb[R], Nb[R];
+        const T *in = s + Q * (a + D * b);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (j < HQ) {
+                const T sa0 = in[j], sa1 = in[Q - 1 - j];
+                const T sb0 = in[DDQ + j], sb1 = in[DDQ + Q - 1 - j];
+                Pa[j] = sa0 - sa1;  // G
+                Na[j] = sa0 + sa1;
+                Pb[j] = sb0 + sb1;  // B
+                Nb[j] = sb0 - sb1;
+            } else {
+                Pa[j] = Na[j] = in[j];
+                Pb[j] = Nb[j] = in[DDQ + j];
+            }
+        }
+#pragma unroll U
+        for (int d = 0; d < HD + OD; ++d) {
+            const T *bb = BBt + RB * d, *bg = BGt + RB * d;
+            T Ev = T(0), Fv = T(0);
+            if (d < HD) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    Ev = fma(bg[j], Pa[j], Ev);
+                    Fv = fma(bg[R + j], Na[j], Fv);
+                    Ev = fma(bb[j], Pb[j], Ev);
+                    Fv = fma(bb[R + j], Nb[j], Fv);
+                }
+                store_out<D, T>(out, e, a, b, d, Ev + Fv, accumulate);
+                store_out<D, T>(out, e, a, b, D - 1 - d, Ev - Fv, accumulate);
+            } else {
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    Ev = fma(bg[j], Pa[j], Ev);
+                    Ev = fma(bb[j], Pb[j], Ev);
+                }
+                store_out<D, T>(out, e, a, b, d, Ev, accumulate);
+            }
+        }
+    }
+}
+
+template <typename T, typename TP, int D, int Q, int NB>
+__device__ __forceinline__ void
+mass_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
+              const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
+              const T *__restrict__ x, const EOutT<T> &out, int accumulate,
+              T *__restrict__ smem) {
+    using E = EO<D, Q>;
+    constexpr int HD = E::HD, OD = E::OD, HQ = E::HQ, R = E::R, W = E::W, RB = E::RB;
+    constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
+    constexpr int DQQ = D * Q * Q;
+    constexpr int BUF = DQQ;
+    const T *__restrict__ FBt = basis + E::FB;
+    const T *__restrict__ BBt = basis + E::BB;
+    const int le = threadIdx.x / QQ;
+    const int t = threadIdx.x - le * QQ;
+    const int a = t % Q, b = t / Q;
+    const int64_t e = e0 + le;
+    const bool valid = e < ne;
+    T *s = smem + le * BUF;
+    // forward 1D contraction of a line held in even-odd form, B only
+    auto fwd = [&](const T (&le_)[HD > 0 ? HD : 1], const T (&lo_)[HD > 0 ? HD : 1], T lm, int q,
+                   T &lo, T &hi) {
+        const T *fb = FBt + W * q;
+        T A = OD ? fb[2 * HD] * lm : T(0), C = T(0);
+#pragma unroll
+        for (int d = 0; d < HD; ++d) {
+            A = fma(fb[d], le_[d], A);
+            C = fma(fb[HD + d], lo_[d], C);
+        }
+        lo = A + C;
+        hi = A - C;
+    };
+    // x: thread (dy, dz)
+    if (valid && a < D && b < D) {
+        T ue[HD > 0 ? HD : 1], uo[HD > 0 ? HD : 1], um = T(0);
+        {
+            T u[D];
+            const int64_t base = (int64_t)DDD * e + D * (a + D * b);
+#pragma unroll
+            for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                ue[d] = u[d] + u[D - 1 - d];
+                uo[d] = u[d] - u[D - 1 - d];
+            }
+            if constexpr (OD) um = u[HD];
+        }
+#pragma unroll 1
+        for (int q = 0; q < R; ++q) {
+            T lo, hi;
+            fwd(ue, uo, um, q, lo, hi);
+            s[q + Q * (a + D * b)] = lo;
+            if (q < HQ) s[Q - 1 - q + Q * (a + D * b)] = hi;
+        }
+    }
+    __syncthreads();
+    // y: thread (qx, dz)
+    {
+        T re[HD > 0 ? HD : 1], ro[HD > 0 ? HD : 1], rm = T(0);
+        const bool act = valid && b < D;
+        if (act) {
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                const T p = s[a + Q * (d + D * b)], k = s[a + Q * (D - 1 - d + D * b)];
+                re[d] = p + k;
+                ro[d] = p - k;
+            }
+            if constexpr (OD) rm = s[a + Q * (HD + D * b)];
+        }
+        __syncthreads();
+        if (act)
+#pragma unroll 1
+            for (int q = 0; q < R; ++q) {
+                T lo, hi;
+                fwd(re, ro, rm, q, lo, hi);
+                s[a + Q * (q + Q * b)] = lo;
+                if (q < HQ) s[a + Q * (Q - 1 - q + Q * b)] = hi;
+            }
+    }
+    __syncthreads();
+    // z: thread (qx, qy); pairs of planes
+    {
+        T ze[HD > 0 ? HD : 1], zo[HD > 0 ? HD : 1], zm = T(0);
+        if (valid) {
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                const T p = s[a + Q * (b + Q * d)], k = s[a + Q * (b + Q * (D - 1 - d))];
+                ze[d] = p + k;
+                zo[d] = p - k;
+            }
+            if constexpr (OD) zm = s[a + Q * (b + Q * HD)];
+        }
+        __syncthreads();
+        if (valid) {
+            const TP *pcol = pa + (int64_t)QQQ * e + a + Q * b;
+            T Ez[HD > 0 ? HD : 1], Fz[HD > 0 ? HD : 1], Mz = T(0);
+#pragma unroll
+            for (int d = 0; d < HD; ++d) Ez[d] = Fz[d] = T(0);
+#pragma unroll 1
+            for (int j = 0; j < R; ++j) {
+                T lo, hi;
+                fwd(ze, zo, zm, j, lo, hi);
+                const T v0 = lo * to_t<T>(pcol[QQ * j]);
+                T P, N;
+                if (j < HQ) {
+                    const T v1 = hi * to_t<T>(pcol[QQ * (Q - 1 - j)]);
+                    P = v0 + v1;
+                    N = v0 - v1;
+                } else {
+                    P = N = v0;
+                }
+                const T *fb = FBt + W * j;
+#pragma unroll
+                for (int d = 0; d < HD; ++d) {
+                    Ez[d] = fma(fb[d], P, Ez[d]);
+                    Fz[d] = fma(fb[HD + d], N, Fz[d]);
+                }
+                if constexpr (OD) Mz = fma(fb[2 * HD], P, Mz);
+            }
+#pragma unroll
+            for (int d = 0; d < HD; ++d) {
+                s[a + Q * (b + Q * d)] = Ez[d] + Fz[d];
+                s[a + Q * (b + Q * (D - 1 - d))] = Ez[d] - Fz[d];
+            }
+            if constexpr (OD) s[a + Q * (b + Q * HD)] = Mz;
+        }
+    }
+    __syncthreads();
+    // backward: line of Q values -> D values with B^T
+    auto bwd = [&](const T (&P)[R], const T (&N)[R], int d, T &lo, T &hi) {
+        const T *bb = BBt + RB * d;
+        T Ev = T(0), Fv = T(0);
+        if (d < HD) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                Ev = fma(bb[j], P[j], Ev);
+                Fv = fma(bb[R + j], N[j], Fv);
+            }
+            lo = Ev + Fv;
+            hi = Ev - Fv;
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; ++j) Ev = fma(bb[j], P[j], Ev);
+            lo = hi = Ev;
+        }
+    };
+    // y^T: thread (qx, dz)
+    {
+        T P[R], N[R];
+        const bool act = valid && b < D;
+        if (act) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const T v0 = s[a + Q * (j + Q * b)];
+                if (j < HQ) {
+                    const T v1 = s[a + Q * (Q - 1 - j + Q * b)];
+                    P[j] = v0 + v1;
+                    N[j] = v0 - v1;
+                } else {
+                    P[j] = N[j] = v0;
+                }
+            }
+        }
+        __syncthreads();
+        if (act)
+#pragma unroll 1
+            for (int d = 0; d < HD + OD; ++d) {
+                T lo, hi;
+                bwd(P, N, d, lo, hi);
+                s[a + Q * (d + D * b)] = lo;
+                if (d < HD) s[a + Q * (D - 1 - d + D * b)] = hi;
+            }
+    }
+    __syncthreads();
+    // x^T: thread (dy, dz)
+    if (valid && a < D && b < D) {
+        T P[R], N[R];
+        const T *in = s + Q * (a + D * b);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            if (j < HQ) {
+                P[j] = in[j] + in[Q - 1 - j];
+                N[j] = in[j] - in[Q - 1 - j];
+            } else {
+                P[j] = N[j] = in[j];
+            }
+        }
+#pragma unroll 1
+        for (int d = 0; d < HD + OD; ++d) {
+            T lo, hi;
+            bwd(P, N, d, lo, hi);
+            store_out<D, T>(out, e, a, b, d, lo, accumulate);
+            if (d < HD) store_out<D, T>(out, e, a, b, D - 1 - d, hi, accumulate);
+        }
+    }
+}
+
+// Direct form (no even-odd split), used where occupancy matters more than
+// VALU issue: FP64 at p <= 5.
+// U: unroll factor of the output-line loops (1 or 2).
+// K: PA-data pipeline depth.  0: each z-plane's 6 values are loaded in its
+//    own iteration.  K > 0: the first K planes are loaded at kernel entry
+//    (their latency hides behind the x/y stages) and plane qz + K is issued
+//    while plane qz is computed, from a K x 6 register ring.
+// The elements e0 + (threadIdx.x / Q^2), NB per work-group, of one problem.
+template <typename T, typename TP, int D, int Q, int NB, int U, int K>
+__device__ __forceinline__ void
+diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
                    const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
                    const T *__restrict__ x, const EOutT<T> &out, int accumulate,
                    T *__restrict__ smem) {
@@ -317,7 +928,7 @@ diffusion_elements(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP
 
 template <typename T, typename TP, int D, int Q, int NB>
 __device__ __forceinline__ void
-mass_elements(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
+mass_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
               const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
               const T *__restrict__ x, const EOutT<T> &out, int accumulate,
               T *__restrict__ smem) {
@@ -422,6 +1033,40 @@ mass_elements(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__r
             store_out<D, T>(out, e, a, b, dx, v, accumulate);
         }
     }
+}
+
+
+// Even-odd where the kernels are VALU-bound (every FP32/BF16 instantiation,
+// FP64 from p = 6); the direct form keeps more waves resident where HBM
+// bounds the FP64 kernel (p <= 5).  Both give the same operator to rounding.
+template <typename T, int D>
+constexpr bool use_even_odd() { return sizeof(T) == 4 || D >= 7; }
+
+template <typename T, typename TP, int D, int Q, int NB, int U, int K>
+__device__ __forceinline__ void
+diffusion_elements(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
+                   const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
+                   const T *__restrict__ x, const EOutT<T> &out, int accumulate,
+                   T *__restrict__ smem) {
+    if constexpr (use_even_odd<T, D>())
+        diffusion_elements_eo<T, TP, D, Q, NB, U, K>(e0, ne, basis, pa, gmap, mask, x, out,
+                                                     accumulate, smem);
+    else
+        diffusion_elements_direct<T, TP, D, Q, NB, U, K>(e0, ne, basis, pa, gmap, mask, x, out,
+                                                         accumulate, smem);
+}
+
+template <typename T, typename TP, int D, int Q, int NB>
+__device__ __forceinline__ void
+mass_elements(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
+              const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
+              const T *__restrict__ x, const EOutT<T> &out, int accumulate,
+              T *__restrict__ smem) {
+    if constexpr (use_even_odd<T, D>())
+        mass_elements_eo<T, TP, D, Q, NB>(e0, ne, basis, pa, gmap, mask, x, out, accumulate, smem);
+    else
+        mass_elements_direct<T, TP, D, Q, NB>(e0, ne, basis, pa, gmap, mask, x, out, accumulate,
+                                              smem);
 }
 
 }  // namespace kern

@@ -135,4 +135,50 @@ void h1_dof_map(int p, int *dm) {
             for (int i = 1; i < p; ++i) dm[L(i, j, k)] = o++;
 }
 
+// ---- even-odd tables ---------------------------------------------------------
+// For M in (B, G), M(q, d) = M[q + Q d]:
+//   forward, R rows q < R = Q/2 + Q%2:  Me(q, 0..HD) | Mo(q, 0..HD) | Mm(q)
+//   backward, rows d < HD:  Me(0..R, d) | Mo(0..R, d);  row HD (D odd): Mm(0..R) | 0
+// with Me = (M(q,d) + M(q,D-1-d)) / 2, Mo = (M(q,d) - M(q,D-1-d)) / 2,
+// Mm(q) = M(q, D/2); HD = D/2.
+int even_odd_size(int D, int Q) {
+    const int HD = D / 2, OD = D % 2, R = Q / 2 + Q % 2, W = 2 * HD + OD;
+    return 2 * R * W + 2 * (HD + OD) * 2 * R;
+}
+
+int even_odd_tables(int D, int Q, const double *B, const double *G, double *out) {
+    const int HD = D / 2, OD = D % 2, R = Q / 2 + Q % 2, W = 2 * HD + OD, RB = 2 * R;
+    double *o = out;
+    const double *Ms[2] = {B, G};
+    for (const double *M : Ms) {  // forward rows
+        for (int q = 0; q < R; ++q) {
+            double *row = o + W * q;
+            for (int d = 0; d < HD; ++d) {
+                const double m0 = M[q + Q * d], m1 = M[q + Q * (D - 1 - d)];
+                row[d] = 0.5 * (m0 + m1);
+                row[HD + d] = 0.5 * (m0 - m1);
+            }
+            if (OD) row[2 * HD] = M[q + Q * HD];
+        }
+        o += R * W;
+    }
+    for (const double *M : Ms) {  // backward rows
+        for (int d = 0; d < HD + OD; ++d) {
+            double *row = o + RB * d;
+            for (int q = 0; q < R; ++q) {
+                if (d < HD) {
+                    const double m0 = M[q + Q * d], m1 = M[q + Q * (D - 1 - d)];
+                    row[q] = 0.5 * (m0 + m1);
+                    row[R + q] = 0.5 * (m0 - m1);
+                } else {
+                    row[q] = M[q + Q * HD];
+                    row[R + q] = 0.0;
+                }
+            }
+        }
+        o += (HD + OD) * RB;
+    }
+    return (int)(o - out);
+}
+
 }  // namespace h1pa

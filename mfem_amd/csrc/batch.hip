@@ -10,8 +10,10 @@
 //               elements (a work list of (slot, first element) blocks)
 //             scatter   z = A d (DIAG_ONE rows copy d), den = (d, z)
 //             update    x += a d, r -= a z, z = dinv r, (r, z), stop test
-//             [mixed: refinement step of the slots whose inner residual
-//              fell by delta: x += xs, r64 = b - A x in FP64, r = (float) r64]
+//             [mixed, last iteration of the replay: refinement step of the
+//              slots whose inner residual fell by delta since their last one
+//              (they wait in PH_RU until then): x += xs, r64 = b - A x in
+//              FP64, r = (float) r64]
 //             dir       d = z + b d                                       }
 //
 // FP64 mode is MFEM's CGSolver on each slot.  Mixed mode is an FP32 inner CG
@@ -114,10 +116,22 @@ template <int NV>
 __device__ inline void slot_totals(const SlotDesc &sd, const double *partials, double *sh,
                                    double (&tot)[NV]) {
     const double *p = partials + (int64_t)NV * sd.part0;
+    constexpr int U8 = 8;  // independent loads in flight per thread
     for (int k = 0; k < NV; ++k) {
-        double v = 0.0;
-        for (int i = threadIdx.x; i < sd.nchunks; i += kT) v += p[(int64_t)NV * i + k];
-        tot[k] = bsum(v, sh);
+        double v[U8];
+#pragma unroll
+        for (int u = 0; u < U8; ++u) v[u] = 0.0;
+        for (int i0 = 0; i0 < sd.nchunks; i0 += U8 * kT) {
+#pragma unroll
+            for (int u = 0; u < U8; ++u) {
+                const int i = i0 + u * kT + threadIdx.x;
+                if (i < sd.nchunks) v[u] += p[(int64_t)NV * i + k];
+            }
+        }
+        double w = 0.0;
+#pragma unroll
+        for (int u = 0; u < U8; ++u) w += v[u];
+        tot[k] = bsum(w, sh);
     }
 }
 
@@ -509,19 +523,20 @@ void launch_kb(const ElemTask *tasks, int ntasks, const SlotDesc *desc, const Sl
                        dim3(NB * Q * Q), 0, s, tasks, desc, st, basis, force);
 }
 
-template <int D, int Q>
-constexpr int nb_max() { return nb_for(Q, 3 * D * Q * Q); }
-template <int D, int Q>
-constexpr int nb_half() { return nb_max<D, Q>() / 2 > 0 ? nb_max<D, Q>() / 2 : 1; }
+template <typename T, int D, int Q>
+constexpr int nb_max() { return nb_for(Q, 3 * D * Q * Q, sizeof(T)); }
+template <typename T, int D, int Q>
+constexpr int nb_half() { return nb_max<T, D, Q>() / 2 > 0 ? nb_max<T, D, Q>() / 2 : 1; }
 
-// Elements per block of variant v (same table as the single-problem kernels).
-template <int D, int Q>
-int nb_of(int v) { return (v == 2 || v == 3) ? nb_half<D, Q>() : nb_max<D, Q>(); }
+// Elements per block of variant v for value type T (same table as the
+// single-problem kernels).
+template <typename T, int D, int Q>
+int nb_of(int v) { return (v == 2 || v == 3) ? nb_half<T, D, Q>() : nb_max<T, D, Q>(); }
 
 template <typename T, typename TP, int D, int Q, bool RU>
 void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc *desc,
                        const SlotState *st, const T *basis, int force, hipStream_t s) {
-    constexpr int NM = nb_max<D, Q>(), NH = nb_half<D, Q>();
+    constexpr int NM = nb_max<T, D, Q>(), NH = nb_half<T, D, Q>();
     if constexpr (sizeof(T) == 8) {
         switch (v) {
             case 1: launch_kb<T, TP, D, Q, NM, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
@@ -556,9 +571,11 @@ bool variant_ok(int prec, int v) {
 
 #define H1PA_BATCH_DQ(X) X(2, 3) X(3, 4) X(4, 5) X(5, 6) X(6, 7) X(7, 8) X(8, 9) X(9, 10)
 
-int nb_of_dq(int D, int Q, int v) {
+// Elements per block of variant v: FP64 kernels (fp64 = true) or FP32 ones.
+int nb_of_dq(int D, int Q, int v, bool fp64) {
 #define X(DD, QQ) \
-    if (D == DD && Q == QQ) return kern::nb_of<DD, QQ>(v);
+    if (D == DD && Q == QQ) \
+        return fp64 ? kern::nb_of<double, DD, QQ>(v) : kern::nb_of<float, DD, QQ>(v);
     H1PA_BATCH_DQ(X)
 #undef X
     throw Error{H1PA_EUNSUP, "batch: unsupported (p, q1d)"};
@@ -613,7 +630,9 @@ struct BatchSlot {
 };
 
 struct BatchGroup {
-    int p = 0, q1d = 0, variant = 0, nb = 1;
+    int p = 0, q1d = 0, variant = 0;
+    int nb = 1;    // elements per block of the inner kernels (the batch's type)
+    int nb64 = 1;  // of the FP64 refinement kernels of a mixed batch
     // the group's own copies of the basis (forms may be destroyed before the
     // batch is): FP64, and FP32 for a mixed batch
     DevBuf<double> basis64_own;
@@ -622,6 +641,8 @@ struct BatchGroup {
     const double *basis64 = nullptr;  // FP64 basis (refinement residual)
     DevBuf<ElemTask> tasks;
     int ntasks = 0;
+    DevBuf<ElemTask> tasks64;  // mixed: work list of the FP64 refinement kernels
+    int ntasks64 = 0;
     bool tuned = false;
 };
 
@@ -679,7 +700,8 @@ int group_for(Batch &B, Form &f) {
                     ? kern::get_diffusion_variant(p, f.q1d)
                     : kern::get_diffusion_variant_lp(B.o.precision == H1PA_PREC_BF16_PA, p, f.q1d);
     if (!variant_ok(B.o.precision, G.variant)) G.variant = 0;
-    G.nb = nb_of_dq(p + 1, f.q1d, G.variant);
+    G.nb = nb_of_dq(p + 1, f.q1d, G.variant, B.o.precision == H1PA_PREC_FP64);
+    G.nb64 = nb_of_dq(p + 1, f.q1d, G.variant, true);
     B.groups.push_back(std::move(G));
     return (int)B.groups.size() - 1;
 }
@@ -751,6 +773,12 @@ void rebuild(Batch &B) {
         G.ntasks = (int)et.size();
         if (G.tasks.n < et.size()) G.tasks.alloc(et.size());
         G.tasks.upload(et.data(), et.size(), B.stream);
+        if (mixed_mode(B)) {
+            element_tasks(B, (int)g, G.nb64, et);
+            G.ntasks64 = (int)et.size();
+            if (G.tasks64.n < et.size()) G.tasks64.alloc(et.size());
+            G.tasks64.upload(et.data(), et.size(), B.stream);
+        }
     }
     H1PA_HIP(hipStreamSynchronize(B.stream));
     B.dirty = false;
@@ -775,7 +803,7 @@ void autotune(Batch &B) {
             int bestv = G.variant;
             for (int v = 0; v < 6; ++v) {
                 if (!variant_ok(B.o.precision, v)) continue;
-                const int nb = nb_of_dq(G.p + 1, G.q1d, v);
+                const int nb = nb_of_dq(G.p + 1, G.q1d, v, B.o.precision == H1PA_PREC_FP64);
                 element_tasks(B, (int)g, nb, et);
                 tmp.upload(et.data(), et.size(), B.stream);
                 const int nt = (int)et.size();
@@ -797,7 +825,8 @@ void autotune(Batch &B) {
             }
             if (bestv != G.variant) {
                 G.variant = bestv;
-                G.nb = nb_of_dq(G.p + 1, G.q1d, bestv);
+                G.nb = nb_of_dq(G.p + 1, G.q1d, bestv, B.o.precision == H1PA_PREC_FP64);
+                G.nb64 = nb_of_dq(G.p + 1, G.q1d, bestv, true);
                 B.dirty = true;
             }
         }
@@ -821,9 +850,14 @@ void enqueue_replay(Batch &B, hipStream_t s) {
     SlotState *st = B.d_state.p;
     double *pt = B.partials.p;
     auto elements = [&](bool ru) {
-        for (BatchGroup &G : B.groups)
-            launch_group(prec, ru, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks, ds, st,
-                         ru ? (const void *)G.basis64 : G.basis, 0, s);
+        for (BatchGroup &G : B.groups) {
+            if (ru)
+                launch_group(prec, true, G.p + 1, G.q1d, G.variant, G.tasks64.p, G.ntasks64, ds,
+                             st, G.basis64, 0, s);
+            else
+                launch_group(prec, false, G.p + 1, G.q1d, G.variant, G.tasks.p, G.ntasks, ds, st,
+                             G.basis, 0, s);
+        }
     };
     if (!mixed_mode(B)) {
         hipLaunchKernelGGL(kern::kb_init, grid, blk, 0, s, vt, ds, st, pt);
@@ -845,10 +879,16 @@ void enqueue_replay(Batch &B, hipStream_t s) {
             hipLaunchKernelGGL(kern::kb_fin_den<true>, fin, blk, 0, s, rl, ds, st, pt);
             hipLaunchKernelGGL(kern::kbm_update, grid, blk, 0, s, vt, ds, st, pt);
             hipLaunchKernelGGL(kern::kbm_fin_update, fin, blk, 0, s, rl, ds, st, pt);
-            hipLaunchKernelGGL(kern::kbm_ru_x, grid, blk, 0, s, vt, ds, st);
-            elements(true);
-            hipLaunchKernelGGL(kern::kbm_ru_res, grid, blk, 0, s, vt, ds, st, pt);
-            hipLaunchKernelGGL(kern::kbm_fin_ru, fin, blk, 0, s, rl, ds, st, pt);
+            if (it == B.o.iters_per_replay - 1) {
+                // refinement steps once per replay: a slot whose inner residual
+                // fell by delta waits in PH_RU (skipped by the inner kernels)
+                // until here, instead of every iteration launching the
+                // refinement kernels over all slots
+                hipLaunchKernelGGL(kern::kbm_ru_x, grid, blk, 0, s, vt, ds, st);
+                elements(true);
+                hipLaunchKernelGGL(kern::kbm_ru_res, grid, blk, 0, s, vt, ds, st, pt);
+                hipLaunchKernelGGL(kern::kbm_fin_ru, fin, blk, 0, s, rl, ds, st, pt);
+            }
             hipLaunchKernelGGL(kern::kb_dir<float>, grid, blk, 0, s, vt, ds, st);
         }
     }

@@ -452,8 +452,9 @@ int h1pa_form_assemble(h1pa_form *form, void *stream) {
     std::vector<double> xq(Q), wq(Q), B(Q * D), G(Q * D);
     gauss_legendre(Q, xq.data(), wq.data());
     lagrange_basis(sp.p, Q, xq.data(), B.data(), G.data());
-    // packed basis: B, G with q fastest, then their transposes (d fastest)
-    std::vector<double> pack(4 * Q * D);
+    // packed basis: B, G with q fastest, then their transposes (d fastest),
+    // then the even-odd tables of the apply kernels
+    std::vector<double> pack(4 * Q * D + even_odd_size(D, Q));
     for (int q = 0; q < Q; ++q)
         for (int d = 0; d < D; ++d) {
             pack[q + Q * d] = B[q + Q * d];
@@ -461,6 +462,7 @@ int h1pa_form_assemble(h1pa_form *form, void *stream) {
             pack[2 * Q * D + d + D * q] = B[q + Q * d];
             pack[3 * Q * D + d + D * q] = G[q + Q * d];
         }
+    even_odd_tables(D, Q, B.data(), G.data(), pack.data() + 4 * Q * D);
     f.basis.upload(pack.data(), pack.size(), s);
     const int64_t nq = (int64_t)Q * Q * Q;
     if (f.desc.integrators & H1PA_INTEG_DIFFUSION) f.pa_diff.alloc(nq * 6 * sp.ne);

@@ -77,6 +77,11 @@ void gauss_lobatto(int n, double *x, double *w);
 // B[q + n*d] = l_d(x_q), G likewise; Lagrange basis on p+1 GLL nodes.
 void lagrange_basis(int p, int n, const double *xq, double *B, double *G);
 void h1_dof_map(int p, int *dof_map);  // lex -> native (fem/fe/fe_base.cpp:2557-2674)
+// Even-odd tables of B and G (Q x D, q fastest) appended to the packed basis
+// after its 4 Q D plain entries; returns the number of doubles written to
+// out (the layout the apply kernels read, EO<D, Q> in apply_body.hpp).
+int even_odd_tables(int D, int Q, const double *B, const double *G, double *out);
+int even_odd_size(int D, int Q);
 
 // ---- mesh (host topology, device coordinates) -------------------------------
 struct Mesh {

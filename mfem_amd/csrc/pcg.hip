@@ -41,18 +41,26 @@ namespace kern {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPer = 4;                      // entries per thread per chunk
-constexpr int kChunk = kThreads * kPer;      // entries per block
 
-inline unsigned chunks(int64_t n) {
-    const int64_t b = (n + kChunk - 1) / kChunk;
-    return (unsigned)std::max<int64_t>(b, 1);
+// Chunking of a vector of n entries: `per` entries per thread, kThreads * per
+// per block.  per grows from 1 to 8 with n so that large vectors get ~2048
+// blocks (few partials to sum) and small ones still spread over the CUs.
+struct Chunks {
+    unsigned blocks;
+    int per;
+};
+inline Chunks chunks(int64_t n) {
+    int64_t per = (n + (int64_t)kThreads * 2048 - 1) / ((int64_t)kThreads * 2048);
+    per = std::min<int64_t>(std::max<int64_t>(per, 1), 8);
+    const int64_t c = kThreads * per;
+    return {(unsigned)std::max<int64_t>((n + c - 1) / c, 1), (int)per};
 }
 
-// The entries of this block's chunk owned by this thread (kPer, kThreads apart).
+// The entries of this block's chunk owned by this thread (per of them,
+// kThreads apart); `per` is a kernel argument.
 #define H1PA_CHUNK(i, n)                                                                       \
-    for (int64_t i = (int64_t)blockIdx.x * kChunk + threadIdx.x,                               \
-                 i##_end = min((int64_t)(n), (int64_t)(blockIdx.x + 1) * kChunk);              \
+    for (int64_t i = (int64_t)blockIdx.x * kThreads * per + threadIdx.x,                       \
+                 i##_end = min((int64_t)(n), (int64_t)(blockIdx.x + 1) * kThreads * per);      \
          i < i##_end; i += kThreads)
 
 // Block sum in a fixed tree order, returned to every thread.
@@ -78,13 +86,28 @@ __device__ inline void store_partials(const double (&acc)[NV], double *partials,
 }
 
 // One block: tot[k] = sum over b < nb of partials[k][b], in a fixed order.
+// Eight independent loads in flight per thread (the partials are read once,
+// latency rather than bandwidth bounds this).
 template <int NV>
 __device__ inline void sum_partials(const double *partials, int nb, double *sh,
                                     double (&tot)[NV]) {
+    constexpr int U8 = 8;
     for (int k = 0; k < NV; ++k) {
-        double v = 0.0;
-        for (int b = threadIdx.x; b < nb; b += kThreads) v += partials[(int64_t)k * nb + b];
-        tot[k] = block_sum(v, sh);
+        const double *p = partials + (int64_t)k * nb;
+        double v[U8];
+#pragma unroll
+        for (int u = 0; u < U8; ++u) v[u] = 0.0;
+        for (int b0 = 0; b0 < nb; b0 += U8 * kThreads) {
+#pragma unroll
+            for (int u = 0; u < U8; ++u) {
+                const int b = b0 + u * kThreads + threadIdx.x;
+                if (b < nb) v[u] += p[b];
+            }
+        }
+        double w = 0.0;
+#pragma unroll
+        for (int u = 0; u < U8; ++u) w += v[u];
+        tot[k] = block_sum(w, sh);
     }
 }
 
@@ -99,7 +122,7 @@ __global__ void k_jacobi_dinv(int64_t n, const double *__restrict__ diag,
 // ---- FP64 / FP32 Jacobi-PCG -------------------------------------------------
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_init(int64_t n, const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ x,
+k_pcg_init(int64_t n, int per, const T *__restrict__ b, const T *__restrict__ dinv, T *__restrict__ x,
            T *__restrict__ r, T *__restrict__ z, T *__restrict__ d, double *__restrict__ partials) {
     __shared__ double sh[kThreads];
     double acc[1] = {0.0};
@@ -144,7 +167,7 @@ k_pcg_fin_init(int nb, const double *__restrict__ partials, PcgState *st, double
 // z = A d from the element output, DIAG_ONE rows copy d; partials of (d, z).
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_scatter(int64_t n, int64_t bb, const int *__restrict__ offsets,
+k_pcg_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
               const int *__restrict__ indices, const T *__restrict__ ye,
               const uint8_t *__restrict__ mask, const T *__restrict__ d, T *__restrict__ z,
               double *__restrict__ partials, const PcgState *st, const int *__restrict__ perm,
@@ -188,7 +211,7 @@ k_pcg_fin_den(int nb, const double *__restrict__ partials, PcgState *st) {
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_update(int64_t n, const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
+k_pcg_update(int64_t n, int per, const T *__restrict__ dinv, T *__restrict__ x, T *__restrict__ r,
              T *__restrict__ z, const T *__restrict__ d, double *__restrict__ partials,
              const PcgState *st) {
     __shared__ double sh[kThreads];
@@ -237,7 +260,7 @@ k_pcg_fin_update(int nb, const double *__restrict__ partials, PcgState *st) {
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-k_pcg_dir(int64_t n, const T *__restrict__ z, T *__restrict__ d, const PcgState *st) {
+k_pcg_dir(int64_t n, int per, const T *__restrict__ z, T *__restrict__ d, const PcgState *st) {
     if (st->done) return;
     const T beta = (T)st->beta;
     H1PA_CHUNK(i, n) d[i] = z[i] + beta * d[i];
@@ -299,7 +322,7 @@ struct IrState {
 };
 
 __global__ void __launch_bounds__(kThreads)
-k_ir_init(int64_t n, const double *__restrict__ b, const double *__restrict__ dinv64,
+k_ir_init(int64_t n, int per, const double *__restrict__ b, const double *__restrict__ dinv64,
           const float *__restrict__ dinv, double *__restrict__ x, double *__restrict__ r64,
           float *__restrict__ r, float *__restrict__ xs, float *__restrict__ z,
           float *__restrict__ d, double *__restrict__ partials) {
@@ -345,7 +368,7 @@ k_ir_fin_init(int nb, const double *__restrict__ partials, IrState *st) {
 
 // q = A d (FP32; DIAG_ONE rows copy d), partials of (d, q).
 __global__ void __launch_bounds__(kThreads)
-k_ir_scatter(int64_t n, int64_t bb, const int *__restrict__ offsets,
+k_ir_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
              const int *__restrict__ indices, const float *__restrict__ ye,
              const uint8_t *__restrict__ mask, const float *__restrict__ d,
              float *__restrict__ q, double *__restrict__ partials, const IrState *st,
@@ -383,7 +406,7 @@ k_ir_fin_den(int nb, const double *__restrict__ partials, IrState *st) {
 }
 
 __global__ void __launch_bounds__(kThreads)
-k_ir_update(int64_t n, const float *__restrict__ dinv, float *__restrict__ xs,
+k_ir_update(int64_t n, int per, const float *__restrict__ dinv, float *__restrict__ xs,
             float *__restrict__ r, float *__restrict__ z, const float *__restrict__ q,
             const float *__restrict__ d, double *__restrict__ partials, const IrState *st) {
     __shared__ double sh[kThreads];
@@ -437,7 +460,7 @@ k_ir_ru_x(int64_t n, float *__restrict__ xs, double *__restrict__ x, const IrSta
 // r64 = b - A x; partials of (B r64, r64), ||r64||^2 and, with
 // r = (float) r64, z = dinv r, of (r, z).
 __global__ void __launch_bounds__(kThreads)
-k_ir_ru_res(int64_t n, int64_t bb, const int *__restrict__ offsets,
+k_ir_ru_res(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
             const int *__restrict__ indices, const double *__restrict__ ye64,
             const uint8_t *__restrict__ mask,
             const double *__restrict__ b, const double *__restrict__ x,
@@ -500,7 +523,7 @@ k_ir_fin_ru(int nb, const double *__restrict__ partials, IrState *st) {
 }
 
 __global__ void __launch_bounds__(kThreads)
-k_ir_dir(int64_t n, const float *__restrict__ z, float *__restrict__ d, const IrState *st) {
+k_ir_dir(int64_t n, int per, const float *__restrict__ z, float *__restrict__ d, const IrState *st) {
     if (st->done) return;
     const float beta = (float)st->beta;
     H1PA_CHUNK(i, n) d[i] = z[i] + beta * d[i];
@@ -599,7 +622,9 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
     DevBuf<T> r(n), z(n), d(n);
-    const unsigned nb = kern::chunks(n);
+    const kern::Chunks ch = kern::chunks(n);
+    const unsigned nb = ch.blocks;
+    const int per = ch.per;
     DevBuf<double> partials(nb);
     DevBuf<PcgState> st(1);
     PcgState h{};
@@ -610,14 +635,14 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
     const dim3 grid(nb), blk(kern::kThreads), one(1);
 
     auto t0 = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(kern::k_pcg_init<T>, grid, blk, 0, s, n, b, dinv, x, r.p, z.p, d.p,
+    hipLaunchKernelGGL(kern::k_pcg_init<T>, grid, blk, 0, s, n, per, b, dinv, x, r.p, z.p, d.p,
                        partials.p);
     hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p,
                        o.rel_tol, o.abs_tol);
     const int64_t bb = direct_rows(sp, mask);
     auto apply = [&]() {
         pcg_element_apply<T>(f, prec, mask, d.p, ye, bb < n ? z.p : nullptr, stop, s);
-        hipLaunchKernelGGL(kern::k_pcg_scatter<T>, grid, blk, 0, s, n, bb, sp.restr.offsets.p,
+        hipLaunchKernelGGL(kern::k_pcg_scatter<T>, grid, blk, 0, s, n, per, bb, sp.restr.offsets.p,
                            sp.restr.indices_nat.p, ye, mask, d.p, z.p, partials.p, st.p,
                            sp.restr.perm.p, sp.restr.nperm);
         hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
@@ -625,10 +650,10 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
     apply();
     H1PA_HIP(hipGetLastError());
     for (int it = 1; it <= o.max_iter; ++it) {
-        hipLaunchKernelGGL(kern::k_pcg_update<T>, grid, blk, 0, s, n, dinv, x, r.p, z.p, d.p,
+        hipLaunchKernelGGL(kern::k_pcg_update<T>, grid, blk, 0, s, n, per, dinv, x, r.p, z.p, d.p,
                            partials.p, st.p);
         hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_pcg_dir<T>, grid, blk, 0, s, n, z.p, d.p, st.p);
+        hipLaunchKernelGGL(kern::k_pcg_dir<T>, grid, blk, 0, s, n, per, z.p, d.p, st.p);
         apply();
         H1PA_HIP(hipGetLastError());
         if (it % check == 0 || it == o.max_iter) {
@@ -753,7 +778,9 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
     const unsigned nb256 = (unsigned)((n + 255) / 256);
-    const unsigned nb = kern::chunks(n);
+    const kern::Chunks ch = kern::chunks(n);
+    const unsigned nb = ch.blocks;
+    const int per = ch.per;
     const unsigned nbx = std::min<unsigned>(nb, 1024);
     ensure_low_precision(f, o.inner_precision, s);
     DevBuf<float> dinv(n), r(n), xs(n), z(n), d(n), q(n);
@@ -775,7 +802,7 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     const int64_t bb = direct_rows(sp, mask);
     const dim3 grid(nb), blk(kern::kThreads), one(1);
     auto t0 = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(kern::k_ir_init, grid, blk, 0, s, n, b, dinv64.p, dinv.p, x, r64.p, r.p,
+    hipLaunchKernelGGL(kern::k_ir_init, grid, blk, 0, s, n, per, b, dinv64.p, dinv.p, x, r64.p, r.p,
                        xs.p, z.p, d.p, partials.p);
     hipLaunchKernelGGL(kern::k_ir_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p);
     H1PA_HIP(hipGetLastError());
@@ -784,21 +811,21 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
         // q = A d with the reduced-precision operator
         pcg_element_apply<float>(f, o.inner_precision, mask, d.p, f.ework32.p,
                                  bb < n ? q.p : nullptr, done, s);
-        hipLaunchKernelGGL(kern::k_ir_scatter, grid, blk, 0, s, n, bb, sp.restr.offsets.p,
+        hipLaunchKernelGGL(kern::k_ir_scatter, grid, blk, 0, s, n, per, bb, sp.restr.offsets.p,
                            sp.restr.indices_nat.p, f.ework32.p, mask, d.p, q.p, partials.p, st.p,
                            sp.restr.perm.p, sp.restr.nperm);
         hipLaunchKernelGGL(kern::k_ir_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_ir_update, grid, blk, 0, s, n, dinv.p, xs.p, r.p, z.p, q.p,
+        hipLaunchKernelGGL(kern::k_ir_update, grid, blk, 0, s, n, per, dinv.p, xs.p, r.p, z.p, q.p,
                            d.p, partials.p, st.p);
         hipLaunchKernelGGL(kern::k_ir_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
         hipLaunchKernelGGL(kern::k_ir_ru_x, dim3(nbx), blk, 0, s, n, xs.p, x, st.p);
         pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, x, f.ework0.p,
                                   bb < n ? r64.p : nullptr, nru, s);
-        hipLaunchKernelGGL(kern::k_ir_ru_res, grid, blk, 0, s, n, bb, sp.restr.offsets.p,
+        hipLaunchKernelGGL(kern::k_ir_ru_res, grid, blk, 0, s, n, per, bb, sp.restr.offsets.p,
                            sp.restr.indices_nat.p, f.ework0.p, mask, b, x, dinv64.p, dinv.p,
                            r64.p, r.p, z.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
         hipLaunchKernelGGL(kern::k_ir_fin_ru, one, blk, 0, s, (int)nb, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_ir_dir, grid, blk, 0, s, n, z.p, d.p, st.p);
+        hipLaunchKernelGGL(kern::k_ir_dir, grid, blk, 0, s, n, per, z.p, d.p, st.p);
         H1PA_HIP(hipGetLastError());
         if (it % 8 == 0 || it == limit) {
             H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));

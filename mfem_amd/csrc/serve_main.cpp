@@ -46,6 +46,7 @@ struct Opts {
     int nmin = 6, nmax = 32, pmin = 1, pmax = 8;
     double rel_tol = 1e-10;
     int iters_per_replay = 8;
+    int use_graph = 1;        // --no-graph: plain launches (per-kernel profiling)
     uint64_t seed = 1;
     int max_requests = 1 << 30;
 };
@@ -212,6 +213,7 @@ Opts parse(int argc, char **argv) {
         else if (a == "--rel-tol") o.rel_tol = std::atof(val().c_str());
         else if (a == "--iters-per-replay") o.iters_per_replay = std::atoi(val().c_str());
         else if (a == "--seed") o.seed = std::strtoull(val().c_str(), nullptr, 10);
+        else if (a == "--no-graph") o.use_graph = 0;
         else if (a == "--max-requests") o.max_requests = std::atoi(val().c_str());
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
@@ -263,7 +265,7 @@ int main(int argc, char **argv) {
         h1pa_batch_opts bo{};
         bo.precision = prec;
         bo.iters_per_replay = o.iters_per_replay;
-        bo.use_graph = 1;
+        bo.use_graph = o.use_graph;
         bo.autotune = 1;
         bo.delta = 0.1;
         bo.max_outer = 1000;
