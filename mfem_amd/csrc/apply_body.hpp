@@ -61,17 +61,35 @@ __device__ __forceinline__ float to_t<float, bf16_t>(bf16_t v) {
     return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
 }
 
-// Read x at E-DOF `i` through the gather map (L-vector input) or directly
-// (E-vector input); constrained L-DOFs read as zero.
-template <typename T>
-__device__ __forceinline__ T load_x(const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
-                                    const T *__restrict__ x, int64_t i) {
-    if (!gmap) return x[i];
-    const int g = gmap[i];
-    const int j = g >= 0 ? g : -1 - g;
-    if (mask && mask[j]) return T(0);
-    const T v = x[j];
-    return g >= 0 ? v : -v;
+// Read the D consecutive E-DOFs base .. base+D-1 (one x-line of an element)
+// through the gather map (L-vector input) or directly (E-vector input);
+// constrained L-DOFs read as zero.  All index loads are issued first, then all
+// value and mask loads, with no branch between them: a line costs two memory
+// round trips, not D dependent (index, value) pairs.
+template <int D, typename T>
+__device__ __forceinline__ void load_line(const int *__restrict__ gmap,
+                                          const uint8_t *__restrict__ mask,
+                                          const T *__restrict__ x, int64_t base, T (&u)[D]) {
+    if (!gmap) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] = x[base + d];
+        return;
+    }
+    int g[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) g[d] = gmap[base + d];
+#pragma unroll
+    for (int d = 0; d < D; ++d) u[d] = x[g[d] >= 0 ? g[d] : -1 - g[d]];
+    if (mask) {
+        uint8_t m[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) m[d] = mask[g[d] >= 0 ? g[d] : -1 - g[d]];
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] = m[d] ? T(0) : (g[d] >= 0 ? u[d] : -u[d]);
+    } else {
+#pragma unroll
+        for (int d = 0; d < D; ++d) u[d] = g[d] >= 0 ? u[d] : -u[d];
+    }
 }
 
 // Store the E-DOF (dx, a, b) of element e.  With out.yL != nullptr the
@@ -179,8 +197,7 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
         {
             T u[D];
             const int64_t base = (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
-            for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
+            load_line<D>(gmap, mask, x, base, u);
 #pragma unroll
             for (int d = 0; d < HD; ++d) {
                 ue[d] = u[d] + u[D - 1 - d];
@@ -524,8 +541,7 @@ mass_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *
         {
             T u[D];
             const int64_t base = (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
-            for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
+            load_line<D>(gmap, mask, x, base, u);
 #pragma unroll
             for (int d = 0; d < HD; ++d) {
                 ue[d] = u[d] + u[D - 1 - d];
@@ -726,8 +742,7 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
     if (valid && a < D && b < D) {
         T u[D];
         const int64_t base = (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
-        for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
+        load_line<D>(gmap, mask, x, base, u);
         T *out = s + Q * (a + D * b);
 #pragma unroll U
         for (int qx = 0; qx < Q; ++qx) {
@@ -946,8 +961,7 @@ mass_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const 
     if (valid && a < D && b < D) {
         T u[D];
         const int64_t base = (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll
-        for (int dx = 0; dx < D; ++dx) u[dx] = load_x(gmap, mask, x, base + dx);
+        load_line<D>(gmap, mask, x, base, u);
 #pragma unroll 1
         for (int qx = 0; qx < Q; ++qx) {
             const T *br = Bd + D * qx;

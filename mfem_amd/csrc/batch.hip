@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "apply_body.hpp"
+#include "row_sum.hpp"
 
 namespace h1pa {
 
@@ -306,11 +307,7 @@ kb_scatter(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
             if (sd.mask && sd.mask[i]) {
                 v = d[i];
             } else {
-                v = T(0);
-                for (int j = sd.offsets[i]; j < sd.offsets[i + 1]; ++j) {
-                    const int ix = sd.indices[j];
-                    v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
-                }
+                v = row_sum(sd.indices, sd.offsets[i], sd.offsets[i + 1], ye);
             }
             z[i] = v;
         }
@@ -472,11 +469,7 @@ kbm_ru_res(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
         } else if (sd.mask && sd.mask[i]) {
             y = sd.x[i];
         } else {
-            y = 0.0;
-            for (int j = sd.offsets[i]; j < sd.offsets[i + 1]; ++j) {
-                const int ix = sd.indices[j];
-                y += ix >= 0 ? sd.ye64[ix] : -sd.ye64[-1 - ix];
-            }
+            y = row_sum(sd.indices, sd.offsets[i], sd.offsets[i + 1], sd.ye64);
         }
         const double ri = sd.b[i] - y;
         sd.r64[i] = ri;

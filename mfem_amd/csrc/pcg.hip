@@ -24,6 +24,7 @@
 #include <cstdlib>
 
 #include "internal.hpp"
+#include "row_sum.hpp"
 
 namespace h1pa {
 
@@ -183,11 +184,7 @@ k_pcg_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
         } else if (mask && mask[i]) {
             v = d[i];
         } else {
-            v = T(0);
-            for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-                const int ix = indices[j];
-                v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
-            }
+            v = row_sum(indices, offsets[i], offsets[i + 1], ye);
         }
         if (t < bb) z[i] = v;
         acc[0] += (double)d[i] * (double)v;
@@ -294,12 +291,7 @@ __global__ void k_scatter_set(int64_t n, const int *__restrict__ offsets,
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t i = t < nperm ? perm[t] : t;
-    T v = T(0);
-    for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-        const int ix = indices[j];
-        v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
-    }
-    y[i] = v;
+    y[i] = row_sum(indices, offsets[i], offsets[i + 1], ye);
 }
 
 // ---- mixed precision: CG with reliable updates -----------------------------
@@ -384,11 +376,7 @@ k_ir_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
         } else if (mask && mask[i]) {
             v = d[i];
         } else {
-            v = 0.f;
-            for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-                const int ix = indices[j];
-                v += ix >= 0 ? ye[ix] : -ye[-1 - ix];
-            }
+            v = row_sum(indices, offsets[i], offsets[i + 1], ye);
         }
         if (t < bb) q[i] = v;
         acc[0] += (double)d[i] * (double)v;
@@ -479,11 +467,7 @@ k_ir_ru_res(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
         } else if (mask && mask[i]) {
             y = x[i];
         } else {
-            y = 0.0;
-            for (int j = offsets[i]; j < offsets[i + 1]; ++j) {
-                const int ix = indices[j];
-                y += ix >= 0 ? ye64[ix] : -ye64[-1 - ix];
-            }
+            y = row_sum(indices, offsets[i], offsets[i + 1], ye64);
         }
         const double ri = b[i] - y;
         r64[i] = ri;

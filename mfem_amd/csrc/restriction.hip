@@ -13,6 +13,7 @@
 //  * E->L: one thread per L-DOF sums its E-entries in table order -- no
 //    floating-point atomics, deterministic (restriction.cpp:159-228).
 #include "internal.hpp"
+#include "row_sum.hpp"
 
 namespace h1pa {
 namespace kern {
@@ -253,11 +254,7 @@ __global__ void k_restrict_mult_t(int64_t nrows, const int *__restrict__ offsets
             v = ix >= 0 ? xe[ix] : -xe[-1 - ix];
         }
     } else {
-        for (int j = b; j < e; ++j) {
-            const int ix = indices[j];
-            if (ix >= 0) v += xe[ix];
-            else v += (mode == 2) ? xe[-1 - ix] : -xe[-1 - ix];
-        }
+        v = row_sum(indices, b, e, xe, mode == 2);
     }
     if (mode == 1) y[i] += v;
     else y[i] = v;
@@ -277,13 +274,7 @@ __global__ void k_scatter_constrained(int64_t nrows, const int *__restrict__ off
         return;
     }
     const int b = offsets[i], e = offsets[i + 1];
-    double v = 0.0;
-    for (int j = b; j < e; ++j) {
-        const int ix = indices[j];
-        if (ix >= 0) v += xe[ix];
-        else v += -xe[-1 - ix];
-    }
-    y[i] = v;
+    y[i] = row_sum(indices, b, e, xe);
 }
 
 __global__ void k_native_indices(int64_t nE, int dof, const int *__restrict__ indices,
