@@ -92,24 +92,44 @@ __device__ __forceinline__ void load_line(const int *__restrict__ gmap,
     }
 }
 
-// Store the E-DOF (dx, a, b) of element e.  With out.yL != nullptr the
-// interior ("bubble") DOFs, which no other element touches, go straight into
-// the L-vector at yL[bb + e (D-2)^3 + lex interior index] (MFEM's bubble
-// block, fem/fespace.cpp:3518-3527); everything else goes to the E-vector, at
-// its lexicographic position or, with out.l2n, at its native position.
+// Store the D E-DOFs (0..D-1, a, b) of element e held in y.  With
+// out.yL != nullptr the interior ("bubble") DOFs, which no other element
+// touches, go straight into the L-vector at yL[bb + e (D-2)^3 + lex interior
+// index] (MFEM's bubble block, fem/fespace.cpp:3518-3527); everything else
+// goes to the E-vector, at its lexicographic position or, with out.l2n, at
+// its native position.  The native positions and, when accumulating, the old
+// values are loaded as one batch before the first store, so the stores of a
+// line wait on no memory round trip of their own.
 template <int D, typename T>
-__device__ __forceinline__ void store_out(const EOutT<T> &out, int64_t e, int a, int b, int dx,
-                                          T v, int accumulate) {
+__device__ __forceinline__ void store_line(const EOutT<T> &out, int64_t e, int a, int b,
+                                           const T (&y)[D], int accumulate) {
     constexpr int P = D - 1, NI = D - 2;
-    T *dst;
-    if (out.yL && a >= 1 && a < P && b >= 1 && b < P && dx >= 1 && dx < P) {
-        dst = out.yL + out.bb + e * (int64_t)(NI * NI * NI) + (dx - 1) + NI * ((a - 1) + NI * (b - 1));
+    const int lex0 = D * (a + D * b);
+    int pos[D];
+    if (out.l2n) {
+#pragma unroll
+        for (int dx = 0; dx < D; ++dx) pos[dx] = out.l2n[lex0 + dx];
     } else {
-        const int lex = dx + D * (a + D * b);
-        dst = out.ye + (int64_t)(D * D * D) * e + (out.l2n ? out.l2n[lex] : lex);
+#pragma unroll
+        for (int dx = 0; dx < D; ++dx) pos[dx] = lex0 + dx;
     }
-    if (accumulate) *dst += v;
-    else *dst = v;
+    const bool inner = out.yL && a >= 1 && a < P && b >= 1 && b < P;
+    T *const ye = out.ye + (int64_t)(D * D * D) * e;
+    T *const yb = inner ? out.yL + out.bb + e * (int64_t)(NI * NI * NI) + NI * ((a - 1) + NI * (b - 1)) - 1
+                        : nullptr;
+    T *dst[D];
+#pragma unroll
+    for (int dx = 0; dx < D; ++dx) dst[dx] = (inner && dx >= 1 && dx < P) ? yb + dx : ye + pos[dx];
+    if (accumulate) {
+        T old[D];
+#pragma unroll
+        for (int dx = 0; dx < D; ++dx) old[dx] = *dst[dx];
+#pragma unroll
+        for (int dx = 0; dx < D; ++dx) *dst[dx] = old[dx] + y[dx];
+    } else {
+#pragma unroll
+        for (int dx = 0; dx < D; ++dx) *dst[dx] = y[dx];
+    }
 }
 
 // Even-odd form of the 1D operators.  B (GLL Lagrange basis at the GL
@@ -313,7 +333,11 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
             T M0 = T(0), M1 = T(0), M2 = T(0);
 #pragma unroll
             for (int d = 0; d < HD; ++d) E0[d] = F0[d] = E1[d] = F1[d] = E2[d] = F2[d] = T(0);
-#pragma unroll 1
+            // with a prefetch ring, fully unrolled: the ring slots become
+            // plain values, so no loop-carried copy waits for a prefetch that
+            // is still in flight
+            constexpr int ZU = KP > 0 ? R : 1;
+#pragma unroll ZU
             for (int jb = 0; jb < R; jb += (KP > 0 ? KP : 1)) {
 #pragma unroll
                 for (int jj = 0; jj < (KP > 0 ? KP : 1); ++jj) {
@@ -477,7 +501,8 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
                 Pb[j] = Nb[j] = in[DDQ + j];
             }
         }
-#pragma unroll U
+        T yv[D];
+#pragma unroll
         for (int d = 0; d < HD + OD; ++d) {
             const T *bb = BBt + RB * d, *bg = BGt + RB * d;
             T Ev = T(0), Fv = T(0);
@@ -489,17 +514,18 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
                     Ev = fma(bb[j], Pb[j], Ev);
                     Fv = fma(bb[R + j], Nb[j], Fv);
                 }
-                store_out<D, T>(out, e, a, b, d, Ev + Fv, accumulate);
-                store_out<D, T>(out, e, a, b, D - 1 - d, Ev - Fv, accumulate);
+                yv[d] = Ev + Fv;
+                yv[D - 1 - d] = Ev - Fv;
             } else {
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
                     Ev = fma(bg[j], Pa[j], Ev);
                     Ev = fma(bb[j], Pb[j], Ev);
                 }
-                store_out<D, T>(out, e, a, b, d, Ev, accumulate);
+                yv[d] = Ev;
             }
         }
+        store_line<D, T>(out, e, a, b, yv, accumulate);
     }
 }
 
@@ -689,13 +715,15 @@ mass_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *
                 P[j] = N[j] = in[j];
             }
         }
-#pragma unroll 1
+        T yv[D];
+#pragma unroll
         for (int d = 0; d < HD + OD; ++d) {
             T lo, hi;
             bwd(P, N, d, lo, hi);
-            store_out<D, T>(out, e, a, b, d, lo, accumulate);
-            if (d < HD) store_out<D, T>(out, e, a, b, D - 1 - d, hi, accumulate);
+            yv[d] = lo;
+            if (d < HD) yv[D - 1 - d] = hi;
         }
+        store_line<D, T>(out, e, a, b, yv, accumulate);
     }
 }
 
@@ -808,7 +836,9 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
             T T0[D], T1[D], T2[D];
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) T0[dz] = T1[dz] = T2[dz] = 0.0;
-#pragma unroll 1
+            // fully unrolled: the ring slots become plain values, so no
+            // loop-carried copy waits for a prefetch that is still in flight
+#pragma unroll Q
             for (int qb = 0; qb < Q; qb += (KK > 0 ? KK : 1)) {
 #pragma unroll
                 for (int j = 0; j < (KK > 0 ? KK : 1); ++j) {
@@ -927,7 +957,8 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
             sa[qx] = in[qx];
             sb[qx] = in[DDQ + qx];
         }
-#pragma unroll U
+        T yv[D];
+#pragma unroll
         for (int dx = 0; dx < D; ++dx) {
             const T *bc = Bq + Q * dx, *gc = Gq + Q * dx;
             T v = 0.0;
@@ -936,8 +967,9 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
                 v = fma(gc[qx], sa[qx], v);
                 v = fma(bc[qx], sb[qx], v);
             }
-            store_out<D, T>(out, e, a, b, dx, v, accumulate);
+            yv[dx] = v;
         }
+        store_line<D, T>(out, e, a, b, yv, accumulate);
     }
 }
 
@@ -1038,14 +1070,16 @@ mass_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const 
         T sx[Q];
 #pragma unroll
         for (int qx = 0; qx < Q; ++qx) sx[qx] = s[qx + Q * (a + D * b)];
-#pragma unroll 1
+        T yv[D];
+#pragma unroll
         for (int dx = 0; dx < D; ++dx) {
             const T *bc = Bq + Q * dx;
             T v = 0.0;
 #pragma unroll
             for (int qx = 0; qx < Q; ++qx) v = fma(bc[qx], sx[qx], v);
-            store_out<D, T>(out, e, a, b, dx, v, accumulate);
+            yv[dx] = v;
         }
+        store_line<D, T>(out, e, a, b, yv, accumulate);
     }
 }
 
