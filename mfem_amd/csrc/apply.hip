@@ -221,7 +221,7 @@ void launch_diffusion_v(int64_t ne, const T *basis, const TP *pa, const int *gma
 }
 
 // Variants of the diffusion apply: (elements per block, line-loop unroll,
-// PA-data pipeline depth).  Every variant performs the same arithmetic per element
+// PA-data pipeline depth: 0, 2 planes or all of them).  Every variant performs the same arithmetic per element
 // in the same order, so they give bit-identical results and differ only in
 // speed.
 constexpr int kDiffVariants = 6;
@@ -238,7 +238,7 @@ void launch_diffusion(int variant, int64_t ne, const double *basis, const double
         case 2: launch_diffusion_v<T, TP, D, Q, NBhalf, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         case 3: launch_diffusion_v<T, TP, D, Q, NBhalf, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         case 4: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 2>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
-        case 5: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 3>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 5: launch_diffusion_v<T, TP, D, Q, NBmax, 1, kAllPlanes>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         default: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
     }
 }
@@ -262,6 +262,9 @@ void launch_diffusion_lp(int variant, int64_t ne, const float *basis, const TP *
         launch_diffusion_v<float, TP, D, Q, NBmax, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s);
     else if (variant == 4)
         launch_diffusion_v<float, TP, D, Q, NBmax, 1, 2>(ne, basis, pa, gmap, mask, x, out, acc, s);
+    else if (variant == 5)
+        launch_diffusion_v<float, TP, D, Q, NBmax, 1, kAllPlanes>(ne, basis, pa, gmap, mask, x,
+                                                                  out, acc, s);
     else
         launch_diffusion_v<float, TP, D, Q, NBmax, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s);
 }

@@ -8,6 +8,10 @@
 namespace h1pa {
 namespace kern {
 
+// PA-data pipeline depth (template parameter K of the element bodies) that
+// loads every plane of the element before the forward contractions start.
+constexpr int kAllPlanes = 16;
+
 // Fraction of the lanes of its 64-wide waves that nb elements of Q^2 threads use.
 constexpr double lane_use(int nb, int QQ) {
     const int t = nb * QQ;
@@ -53,13 +57,32 @@ __host__ __device__ inline int64_t xcd_block(int64_t b, int64_t G) {
     return k * q + (k < r ? k : r) + j;
 }
 
-// PA-data element -> compute type (bf16 is stored as its upper 16 bits).
-template <typename T, typename TP>
-__device__ __forceinline__ T to_t(TP v) { return static_cast<T>(v); }
-template <>
-__device__ __forceinline__ float to_t<float, bf16_t>(bf16_t v) {
-    return __uint_as_float(static_cast<uint32_t>(v.bits) << 16);
+// Load one PA value as the compute type (bf16 is stored as the upper 16 bits
+// of an FP32).  PA data is streamed once per apply, so with NT it is loaded
+// non-temporally and does not push the gathered x-values (re-read by up to 8
+// neighbouring elements) out of the L2.
+template <typename T, bool NT, typename TP>
+__device__ __forceinline__ T load_pa(const TP *p) {
+    if constexpr (NT) return static_cast<T>(__builtin_nontemporal_load(p));
+    else return static_cast<T>(*p);
 }
+template <typename T, bool NT>
+__device__ __forceinline__ float load_pa(const bf16_t *p) {
+    const uint16_t *u = reinterpret_cast<const uint16_t *>(p);
+    uint16_t b;
+    if constexpr (NT) b = __builtin_nontemporal_load(u);
+    else b = *u;
+    return __uint_as_float(static_cast<uint32_t>(b) << 16);
+}
+
+// Whether the PA planes of a Q^3 element are streamed non-temporally.  A
+// plane of one component is Q^2 values read by one load per thread; when it
+// does not fill whole 128-byte cache lines, neighbouring planes share lines,
+// and a non-temporal load makes the second plane fetch the shared line
+// again.  Measured (MI355X): non-temporal is 10-15% faster from Q = 8 on and
+// 10-15% slower at Q = 6 (288-byte FP64 planes).
+template <int Q, typename TP>
+constexpr bool stream_pa() { return Q >= 8 || (Q * Q * sizeof(TP)) % 128 == 0; }
 
 // Read the D consecutive E-DOFs base .. base+D-1 (one x-line of an element)
 // through the gather map (L-vector input) or directly (E-vector input);
@@ -199,8 +222,8 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
     auto load_pair = [&](int j, T (&r)[12]) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
-            r[c] = to_t<T>(pcol[QQQ * c + QQ * j]);
-            r[6 + c] = to_t<T>(pcol[QQQ * c + QQ * (Q - 1 - j)]);
+            r[c] = load_pa<T, stream_pa<Q, TP>()>(pcol + QQQ * c + QQ * j);
+            r[6 + c] = load_pa<T, stream_pa<Q, TP>()>(pcol + QQQ * c + QQ * (Q - 1 - j));
         }
     };
     T ring[KP > 0 ? KP : 1][12];
@@ -630,10 +653,10 @@ mass_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *
             for (int j = 0; j < R; ++j) {
                 T lo, hi;
                 fwd(ze, zo, zm, j, lo, hi);
-                const T v0 = lo * to_t<T>(pcol[QQ * j]);
+                const T v0 = lo * load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * j);
                 T P, N;
                 if (j < HQ) {
-                    const T v1 = hi * to_t<T>(pcol[QQ * (Q - 1 - j)]);
+                    const T v1 = hi * load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * (Q - 1 - j));
                     P = v0 + v1;
                     N = v0 - v1;
                 } else {
@@ -762,7 +785,7 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
 #pragma unroll
             for (int j = 0; j < KK; ++j)
 #pragma unroll
-                for (int c = 0; c < 6; ++c) ring[j][c] = to_t<T>(pcol[QQQ * c + QQ * j]);
+                for (int c = 0; c < 6; ++c) ring[j][c] = load_pa<T, stream_pa<Q, TP>()>(pcol + QQQ * c + QQ * j);
         }
     }
 
@@ -850,7 +873,7 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
                         if (qz + KK < Q) {
 #pragma unroll
                             for (int c = 0; c < 6; ++c)
-                                ring[j][c] = to_t<T>(pcol[QQQ * c + QQ * (qz + KK)]);
+                                ring[j][c] = load_pa<T, stream_pa<Q, TP>()>(pcol + QQQ * c + QQ * (qz + KK));
                         }
                         T g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
@@ -886,8 +909,8 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
             for (int qz = 0; qz < Q; ++qz) {
                 const T *br = Bd + D * qz, *gr = Gd + D * qz;
                 const TP *pq = pcol + QQ * qz;
-                const T d11 = to_t<T>(pq[0 * QQQ]), d21 = to_t<T>(pq[1 * QQQ]), d31 = to_t<T>(pq[2 * QQQ]);
-                const T d22 = to_t<T>(pq[3 * QQQ]), d32 = to_t<T>(pq[4 * QQQ]), d33 = to_t<T>(pq[5 * QQQ]);
+                const T d11 = load_pa<T, stream_pa<Q, TP>()>(pq + 0 * QQQ), d21 = load_pa<T, stream_pa<Q, TP>()>(pq + 1 * QQQ), d31 = load_pa<T, stream_pa<Q, TP>()>(pq + 2 * QQQ);
+                const T d22 = load_pa<T, stream_pa<Q, TP>()>(pq + 3 * QQQ), d32 = load_pa<T, stream_pa<Q, TP>()>(pq + 4 * QQQ), d33 = load_pa<T, stream_pa<Q, TP>()>(pq + 5 * QQQ);
                 T g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) {
@@ -1039,7 +1062,7 @@ mass_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const 
                 T v = 0.0;
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) v = fma(br[dz], z[dz], v);
-                v *= to_t<T>(pcol[QQ * qz]);
+                v *= load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * qz);
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) tz[dz] = fma(br[dz], v, tz[dz]);
             }

@@ -536,7 +536,7 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
             case 2: launch_kb<T, TP, D, Q, NH, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 3: launch_kb<T, TP, D, Q, NH, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 4: launch_kb<T, TP, D, Q, NM, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
-            case 5: launch_kb<T, TP, D, Q, NM, 1, 3, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 5: launch_kb<T, TP, D, Q, NM, 1, kAllPlanes, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
         }
     } else {
@@ -544,6 +544,7 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
         switch (v) {
             case 1: launch_kb<T, TP, D, Q, NM, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 4: launch_kb<T, TP, D, Q, NM, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 5: launch_kb<T, TP, D, Q, NM, 1, kAllPlanes, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
         }
     }
@@ -557,9 +558,9 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
 // ---------------------------------------------------------------------------
 namespace {
 
-// Variants valid for the inner type (FP32 applies have 0, 1, 4).
+// Variants valid for the inner type (FP32 applies have 0, 1, 4, 5).
 bool variant_ok(int prec, int v) {
-    return prec == H1PA_PREC_FP64 ? (v >= 0 && v < 6) : (v == 0 || v == 1 || v == 4);
+    return prec == H1PA_PREC_FP64 ? (v >= 0 && v < 6) : (v == 0 || v == 1 || v == 4 || v == 5);
 }
 
 #define H1PA_BATCH_DQ(X) X(2, 3) X(3, 4) X(4, 5) X(5, 6) X(6, 7) X(7, 8) X(8, 9) X(9, 10)

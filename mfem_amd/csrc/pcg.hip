@@ -679,7 +679,7 @@ void autotune_lp(Form &f, int kind, hipStream_t s) {
     out.l2n = sp.restr.lex2nat.p;
     int best = 0;
     float best_ms = 1e30f;
-    for (int v : {0, 1, 4}) {
+    for (int v : {0, 1, 4, 5}) {
         for (int w = 0; w < 2; ++w)
             kern::diffusion_apply_lp_variant(v, p, Q, sp.ne, f.basis32.p, f.pa_diff_lp.p, kind,
                                              sp.restr.gather_map.p, nullptr, x.p, out, false, s);
