@@ -8,6 +8,19 @@
 namespace h1pa {
 namespace kern {
 
+// A pointer read from a device-side descriptor, marked as pointing to global
+// memory.  The compiler cannot tell where such a pointer points and emits
+// flat loads and stores for it, which also count against the LDS wait
+// counter: every LDS wait of an element body would then wait for the PA
+// prefetch too.  The empty asm hides the integer from the optimizer, so the
+// cast to the global address space is not folded away.
+template <typename T>
+__device__ __forceinline__ T *global_ptr(T *p) {
+    uint64_t u = reinterpret_cast<uint64_t>(p);
+    asm("" : "+s"(u));
+    return (T *)(__attribute__((address_space(1))) T *)u;
+}
+
 // PA-data pipeline depth (template parameter K of the element bodies) that
 // loads every plane of the element before the forward contractions start.
 constexpr int kAllPlanes = 16;

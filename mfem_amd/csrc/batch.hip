@@ -88,14 +88,16 @@ constexpr int kT = 256;          // threads per vector block
 constexpr int kPer = 4;          // DOFs per thread per chunk
 constexpr int kChunk = kT * kPer;
 
+// Block sum of v over the kT threads, the same value in every thread: a
+// butterfly within each 64-lane wave (lane 0's total is kept), then the four
+// wave totals in a fixed order.
+static_assert(kT == 256, "bsum assumes four 64-lane waves");
 __device__ inline double bsum(double v, double *sh) {
-    sh[threadIdx.x] = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int w = kT / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    const double r = sh[0];
+    const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
     __syncthreads();
     return r;
 }
@@ -139,11 +141,41 @@ __device__ inline void slot_totals(const SlotDesc &sd, const double *partials, d
 template <typename T>
 __device__ inline T *vp(void *p) { return reinterpret_cast<T *>(p); }
 
-// The DOFs of chunk tk.chunk that this thread owns (kPer of them, kT apart).
+// The kPer DOFs t[k] = chunk kChunk + k kT + tid of this thread's share of a
+// chunk (ok[k]: t[k] < n).  Kernels load every operand of all kPer DOFs at
+// the clamped index lt[k] (in bounds for every k) before the first store, so
+// the loads of a share go out together instead of one DOF's round trip after
+// another; stores and sums then run over the ok entries in k order.
+struct Share {
+    int64_t t[kPer], lt[kPer];
+    bool ok[kPer];
+};
+__device__ __forceinline__ Share share_of(int chunk, int64_t n) {
+    Share h;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t t = (int64_t)chunk * kChunk + k * kT + threadIdx.x;
+        h.t[k] = t;
+        h.ok[k] = t < n;
+        h.lt[k] = t < n ? t : n - 1;
+    }
+    return h;
+}
+
+// The DOFs of chunk tk.chunk that this thread owns (kPer of them, kT apart),
+// one after another (kernels whose DOFs branch on their kind).
 #define H1PA_CHUNK_LOOP(i)                                                                 \
     for (int64_t i = (int64_t)tk.chunk * kChunk + threadIdx.x,                             \
                  i##_end = min(sd.n, (int64_t)(tk.chunk + 1) * kChunk);                    \
          i < i##_end; i += kT)
+
+// Descriptor pointer of type T as a global-memory pointer.
+template <typename T>
+__device__ __forceinline__ T *gv(void *p) { return global_ptr(reinterpret_cast<T *>(p)); }
+template <typename T>
+__device__ __forceinline__ const T *gv(const void *p) {
+    return global_ptr(reinterpret_cast<const T *>(p));
+}
 
 // Producer kernels run one block per (slot, chunk) work item and skip the
 // slots not in the phase they serve; each finalize kernel runs one block per
@@ -166,13 +198,23 @@ kb_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
         const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
     H1PA_TASK_PROLOGUE(ph_ == PH_INIT)
-    double *r = vp<double>(sd.r), *z = vp<double>(sd.z), *d = vp<double>(sd.d);
-    double *x = sd.x;
-    const double *dinv = vp<double>(sd.dinv);
+    double *r = gv<double>(sd.r), *z = gv<double>(sd.z), *d = gv<double>(sd.d);
+    double *x = global_ptr(sd.x);
+    const double *dinv = gv<double>(sd.dinv), *b = global_ptr(sd.b);
+    const Share h = share_of(tk.chunk, sd.n);
+    double vb[kPer], vi[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        vb[k] = b[h.lt[k]];
+        vi[k] = dinv[h.lt[k]];
+    }
     double acc[1] = {0.0};
-    H1PA_CHUNK_LOOP(i) {
-        const double ri = sd.b[i];
-        const double zi = dinv[i] * ri;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (!h.ok[k]) continue;
+        const int64_t i = h.t[k];
+        const double ri = vb[k];
+        const double zi = vi[k] * ri;
         r[i] = ri;
         x[i] = 0.0;
         z[i] = zi;
@@ -214,22 +256,36 @@ kbm_init(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
          const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
     H1PA_TASK_PROLOGUE(ph_ == PH_INIT)
-    float *r = vp<float>(sd.r), *z = vp<float>(sd.z), *d = vp<float>(sd.d);
-    float *xs = vp<float>(sd.xi);
-    const float *dinv = vp<float>(sd.dinv);
+    float *r = gv<float>(sd.r), *z = gv<float>(sd.z), *d = gv<float>(sd.d);
+    float *xs = gv<float>(sd.xi);
+    const float *dinv = gv<float>(sd.dinv);
+    double *x = global_ptr(sd.x), *r64 = global_ptr(sd.r64);
+    const double *b = global_ptr(sd.b), *dinv64 = global_ptr(sd.dinv64);
+    const Share h = share_of(tk.chunk, sd.n);
+    double vb[kPer], vi64[kPer];
+    float vi[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        vb[k] = b[h.lt[k]];
+        vi[k] = dinv[h.lt[k]];
+        vi64[k] = dinv64[h.lt[k]];
+    }
     double acc[3] = {0.0, 0.0, 0.0};
-    H1PA_CHUNK_LOOP(i) {
-        const double bi = sd.b[i];
-        sd.x[i] = 0.0;
-        sd.r64[i] = bi;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (!h.ok[k]) continue;
+        const int64_t i = h.t[k];
+        const double bi = vb[k];
+        x[i] = 0.0;
+        r64[i] = bi;
         const float ri = (float)bi;
-        const float zi = dinv[i] * ri;
+        const float zi = vi[k] * ri;
         r[i] = ri;
         xs[i] = 0.f;
         z[i] = zi;
         d[i] = zi;
         acc[0] += (double)ri * (double)zi;
-        acc[1] += bi * sd.dinv64[i] * bi;
+        acc[1] += bi * vi64[k] * bi;
         acc[2] += bi * bi;
     }
     store_chunk<3>(acc, sd, tk.chunk, partials, sh);
@@ -269,22 +325,21 @@ kb_diffusion(const ElemTask *__restrict__ tasks, const SlotDesc *__restrict__ de
     if (!force && states[tk.slot].phase != (RU ? PH_RU : PH_INNER)) return;
     __shared__ T smem[NB * diffusion_buf<D, Q>()];
     EOutT<T> out;
-    out.l2n = sd.l2n;
+    out.l2n = global_ptr(sd.l2n);
     if (sd.bb < sd.n) {  // interior DOFs straight to the L-vector (z, or r64 for RU)
-        out.yL = RU ? reinterpret_cast<T *>(sd.r64) : vp<T>(sd.z);
+        out.yL = RU ? gv<T>(sd.r64) : gv<T>(sd.z);
         out.bb = sd.bb;
     }
+    const int *gmap = global_ptr(sd.gmap);
+    const uint8_t *mask = sd.mask ? global_ptr(sd.mask) : nullptr;
     if constexpr (RU) {
-        out.ye = reinterpret_cast<T *>(sd.ye64);
-        diffusion_elements<T, TP, D, Q, NB, U, K>(tk.e0, sd.ne, basis,
-                                                  reinterpret_cast<const TP *>(sd.pa64), sd.gmap,
-                                                  sd.mask, reinterpret_cast<const T *>(sd.x), out,
-                                                  0, smem);
+        out.ye = gv<T>(sd.ye64);
+        diffusion_elements<T, TP, D, Q, NB, U, K>(tk.e0, sd.ne, basis, gv<TP>(sd.pa64), gmap, mask,
+                                                  gv<T>(sd.x), out, 0, smem);
     } else {
-        out.ye = vp<T>(sd.ye);
-        diffusion_elements<T, TP, D, Q, NB, U, K>(tk.e0, sd.ne, basis,
-                                                  reinterpret_cast<const TP *>(sd.pa), sd.gmap,
-                                                  sd.mask, vp<T>(sd.d), out, 0, smem);
+        out.ye = gv<T>(sd.ye);
+        diffusion_elements<T, TP, D, Q, NB, U, K>(tk.e0, sd.ne, basis, gv<TP>(sd.pa), gmap, mask,
+                                                  gv<T>(sd.d), out, 0, smem);
     }
 }
 
@@ -343,14 +398,28 @@ kb_update(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
     H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
     const SlotState *st = states + tk.slot;
     const double alpha = st->nom / st->den;
-    double *x = sd.x, *r = vp<double>(sd.r), *z = vp<double>(sd.z);
-    const double *d = vp<double>(sd.d), *dinv = vp<double>(sd.dinv);
+    double *x = global_ptr(sd.x), *r = gv<double>(sd.r), *z = gv<double>(sd.z);
+    const double *d = gv<double>(sd.d), *dinv = gv<double>(sd.dinv);
+    const Share h = share_of(tk.chunk, sd.n);
+    double vx[kPer], vd[kPer], vr[kPer], vz[kPer], vi[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t j = h.lt[k];
+        vx[k] = x[j];
+        vd[k] = d[j];
+        vr[k] = r[j];
+        vz[k] = z[j];
+        vi[k] = dinv[j];
+    }
     double acc[1] = {0.0};
-    H1PA_CHUNK_LOOP(i) {
-        x[i] += alpha * d[i];
-        const double ri = r[i] - alpha * z[i];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (!h.ok[k]) continue;
+        const int64_t i = h.t[k];
+        x[i] = vx[k] + alpha * vd[k];
+        const double ri = vr[k] - alpha * vz[k];
         r[i] = ri;
-        const double zi = dinv[i] * ri;
+        const double zi = vi[k] * ri;
         z[i] = zi;
         acc[0] += ri * zi;
     }
@@ -393,14 +462,28 @@ kbm_update(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
     H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
     const SlotState *st = states + tk.slot;
     const float alpha = (float)(st->nom / st->den);
-    float *xs = vp<float>(sd.xi), *r = vp<float>(sd.r), *z = vp<float>(sd.z);
-    const float *d = vp<float>(sd.d), *dinv = vp<float>(sd.dinv);
+    float *xs = gv<float>(sd.xi), *r = gv<float>(sd.r), *z = gv<float>(sd.z);
+    const float *d = gv<float>(sd.d), *dinv = gv<float>(sd.dinv);
+    const Share h = share_of(tk.chunk, sd.n);
+    float vx[kPer], vd[kPer], vr[kPer], vz[kPer], vi[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t j = h.lt[k];
+        vx[k] = xs[j];
+        vd[k] = d[j];
+        vr[k] = r[j];
+        vz[k] = z[j];
+        vi[k] = dinv[j];
+    }
     double acc[1] = {0.0};
-    H1PA_CHUNK_LOOP(i) {
-        xs[i] += alpha * d[i];
-        const float ri = r[i] - alpha * z[i];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (!h.ok[k]) continue;
+        const int64_t i = h.t[k];
+        xs[i] = vx[k] + alpha * vd[k];
+        const float ri = vr[k] - alpha * vz[k];
         r[i] = ri;
-        const float zi = dinv[i] * ri;
+        const float zi = vi[k] * ri;
         z[i] = zi;
         acc[0] += (double)ri * (double)zi;
     }
@@ -433,9 +516,18 @@ kb_dir(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
        const SlotState *__restrict__ states) {
     H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
     const T beta = (T)states[tk.slot].beta;
-    const T *z = vp<T>(sd.z);
-    T *d = vp<T>(sd.d);
-    H1PA_CHUNK_LOOP(i) d[i] = z[i] + beta * d[i];
+    const T *z = gv<T>(sd.z);
+    T *d = gv<T>(sd.d);
+    const Share h = share_of(tk.chunk, sd.n);
+    T vz[kPer], vd[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        vz[k] = z[h.lt[k]];
+        vd[k] = d[h.lt[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (h.ok[k]) d[h.t[k]] = vz[k] + beta * vd[k];
 }
 
 // Refinement step, part 1 (PH_RU): x += xs, xs = 0.
@@ -443,10 +535,21 @@ __global__ void __launch_bounds__(kT)
 kbm_ru_x(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
          const SlotState *__restrict__ states) {
     H1PA_TASK_PROLOGUE(ph_ == PH_RU)
-    float *xs = vp<float>(sd.xi);
-    H1PA_CHUNK_LOOP(i) {
-        sd.x[i] += (double)xs[i];
-        xs[i] = 0.f;
+    float *xs = gv<float>(sd.xi);
+    double *x = global_ptr(sd.x);
+    const Share h = share_of(tk.chunk, sd.n);
+    double vx[kPer];
+    float vs[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        vx[k] = x[h.lt[k]];
+        vs[k] = xs[h.lt[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (!h.ok[k]) continue;
+        x[h.t[k]] = vx[k] + (double)vs[k];
+        xs[h.t[k]] = 0.f;
     }
 }
 
