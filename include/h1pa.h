@@ -214,6 +214,35 @@ int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const doubl
                    double *d_x, const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats,
                    void *stream);
 
+/* ---- Chebyshev smoother ---------------------------------------------------
+ * OperatorChebyshevSmoother (linalg/solvers.hpp:502-585, solvers.cpp:455-656)
+ * on the DIAG_ONE constrained operator of a form, with the Jacobi diagonal
+ * of AssembleDiagonal (ess rows 1), and PowerMethod::EstimateLargestEigenvalue
+ * (linalg/operator.cpp:886-935) for its eigenvalue bound. */
+typedef struct h1pa_chebyshev h1pa_chebyshev;
+/* d_diag: the L-vector diagonal (MFEM's `d`; NULL -> AssembleDiagonal of the
+ * form), read during the call.  order 1..5.  max_eig_estimate > 0 is used as
+ * given (the constructor taking max_eig_estimate); otherwise the largest
+ * eigenvalue of diag^-1 A is estimated by power_iterations power steps
+ * (relative tolerance power_tolerance) from Vector::Randomize(power_seed) --
+ * the C library's srand/rand sequence, as MFEM draws it; MFEM's defaults are
+ * 10, 1e-8, 12345.  The essential list is copied; the form must outlive the
+ * smoother. */
+int h1pa_chebyshev_create(h1pa_form *form, const double *d_diag, const int *d_ess,
+                          int64_t n_ess, int order, double max_eig_estimate,
+                          int power_iterations, double power_tolerance, int power_seed,
+                          h1pa_chebyshev **cheb, void *stream);
+int h1pa_chebyshev_destroy(h1pa_chebyshev *cheb);
+/* The eigenvalue estimate and the order polynomial coefficients (coeffs has
+ * room for order values; either pointer may be NULL). */
+int h1pa_chebyshev_info(const h1pa_chebyshev *cheb, double *max_eig, double *coeffs);
+/* OperatorChebyshevSmoother::Mult: y = sum_k c_k (diag^-1 A)^k diag^-1 x. */
+int h1pa_chebyshev_mult(h1pa_chebyshev *cheb, const double *d_x, double *d_y, void *stream);
+/* CGSolver with SetPreconditioner(smoother) on the smoother's constrained
+ * operator, iterative_mode = false; as h1pa_pcg_solve otherwise. */
+int h1pa_pcg_solve_chebyshev(h1pa_chebyshev *cheb, const double *d_b, double *d_x,
+                             const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats, void *stream);
+
 /* ---- mixed precision ------------------------------------------------------
  * No MFEM counterpart: MFEM's precision is one build-wide real_t
  * (config/config.hpp:41-48) and it has no mixed-precision PA or PCG.  The

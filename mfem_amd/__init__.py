@@ -130,6 +130,13 @@ SIGNATURES = {
     "h1pa_pcg_solve": (_i, [_vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                             _vp]),
     "h1pa_form_mult_f32": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "h1pa_chebyshev_create": (_i, [_vp, _vp, _vp, _i64, _i, _d, _i, _d, _i, C.POINTER(_vp),
+                                   _vp]),
+    "h1pa_chebyshev_destroy": (_i, [_vp]),
+    "h1pa_chebyshev_info": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
+    "h1pa_chebyshev_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_pcg_solve_chebyshev": (_i, [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
+                                      _vp]),
     "h1pa_batch_create": (_i, [C.POINTER(BatchOpts), C.POINTER(_vp)]),
     "h1pa_batch_destroy": (_i, [_vp]),
     "h1pa_batch_add": (_i, [_vp, _vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), _i64,
@@ -721,9 +728,59 @@ class OperatorJacobiSmoother:
         self.a, self.ess = a, ess_tdof_list
 
 
+class OperatorChebyshevSmoother:
+    """OperatorChebyshevSmoother (linalg/solvers.hpp:502-585,
+    solvers.cpp:455-656) on the DIAG_ONE operator from FormLinearSystem.
+
+    OperatorChebyshevSmoother(A, diag, ess, order, power_iterations=10,
+    power_tolerance=1e-8, power_seed=12345) estimates the largest eigenvalue
+    of diag^-1 A with MFEM's power method from Vector::Randomize(power_seed);
+    max_eig_estimate=... uses the given bound instead (MFEM's other
+    constructor).  diag=None takes the form's AssembleDiagonal."""
+
+    def __init__(self, A, diag, ess_tdof_list, order, power_iterations=10,
+                 power_tolerance=1e-8, power_seed=12345, max_eig_estimate=None):
+        if not isinstance(A, ConstrainedOperator):
+            raise H1paError(H1PA_EUNSUP, "the operator must come from FormLinearSystem")
+        f = A.form._single()
+        n = A.form.fes.ndofs
+        if diag is not None:
+            _check_vec(diag, n)
+        self.A, self.ess, self.order = A, ess_tdof_list, int(order)
+        self._form = f  # keeps the form alive
+        h = _vp()
+        _check(lib().h1pa_chebyshev_create(
+            f._h, _ptr(diag) if diag is not None else None, _ptr(ess_tdof_list),
+            ess_tdof_list.numel(), self.order,
+            float(max_eig_estimate) if max_eig_estimate else 0.0, int(power_iterations),
+            float(power_tolerance), int(power_seed), C.byref(h), _stream()))
+        self._h = h
+
+    def __del__(self, _d=_destroy):
+        _d("h1pa_chebyshev_destroy", getattr(self, "_h", None))
+        self._h = None
+
+    def GetMaxEigEstimate(self):
+        e = _d()
+        _check(lib().h1pa_chebyshev_info(self._h, C.byref(e), None))
+        return e.value
+
+    def GetCoefficients(self):
+        c = (_d * self.order)()
+        _check(lib().h1pa_chebyshev_info(self._h, None, c))
+        return list(c)
+
+    def Mult(self, x, y):
+        n = self.A.form.fes.ndofs
+        _check_vec(x, n)
+        _check_vec(y, n)
+        _check(lib().h1pa_chebyshev_mult(self._h, _ptr(x), _ptr(y), _stream()))
+
+
 class CGSolver:
     """CGSolver (linalg/solvers.hpp:626, solvers.cpp:869-1050) with the PA
-    operator and OperatorJacobiSmoother, run fully on the device."""
+    operator and OperatorJacobiSmoother or OperatorChebyshevSmoother, run
+    fully on the device."""
 
     def __init__(self):
         self.rel_tol, self.abs_tol, self.max_iter = 0.0, 0.0, 10
@@ -750,8 +807,9 @@ class CGSolver:
         self.op = op
 
     def SetPreconditioner(self, prec):
-        if not isinstance(prec, OperatorJacobiSmoother):
-            raise H1paError(H1PA_EUNSUP, "only OperatorJacobiSmoother is provided")
+        if not isinstance(prec, (OperatorJacobiSmoother, OperatorChebyshevSmoother)):
+            raise H1paError(H1PA_EUNSUP,
+                            "OperatorJacobiSmoother and OperatorChebyshevSmoother are provided")
         self.prec = prec
 
     def Mult(self, b, x):
@@ -760,8 +818,14 @@ class CGSolver:
         f = self.op.form._single()
         o = PcgOpts(self.rel_tol, self.abs_tol, self.max_iter, self.check_every)
         st = self.stats
-        rc = lib().h1pa_pcg_solve(f._h, _ptr(self.op.ess), self.op.ess.numel(), _ptr(b), _ptr(x),
-                                  C.byref(o), C.byref(st), _stream())
+        if isinstance(self.prec, OperatorChebyshevSmoother):
+            if self.prec.A.form is not self.op.form:
+                raise H1paError(H1PA_EUNSUP, "the smoother must be built on this operator")
+            rc = lib().h1pa_pcg_solve_chebyshev(self.prec._h, _ptr(b), _ptr(x), C.byref(o),
+                                                C.byref(st), _stream())
+        else:
+            rc = lib().h1pa_pcg_solve(f._h, _ptr(self.op.ess), self.op.ess.numel(), _ptr(b),
+                                      _ptr(x), C.byref(o), C.byref(st), _stream())
         if rc not in (H1PA_OK, H1PA_ENOTCONV):
             _check(rc)
 

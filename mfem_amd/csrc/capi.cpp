@@ -4,6 +4,7 @@
 // a status code plus a thread-local message; nothing throws across the ABI.
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <exception>
 #include <new>
 
@@ -735,6 +736,72 @@ int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const doubl
     DevBuf<double> diag(sp.ndofs);
     operator_diagonal(f, diag.p, s);
     pcg_solve(f, mask, diag.p, b, x, *opts, *stats, s);
+    if (!stats->converged) {
+        set_error("PCG: No convergence!");
+        return H1PA_ENOTCONV;
+    }
+    API_END
+}
+
+int h1pa_chebyshev_create(h1pa_form *form, const double *d_diag, const int *d_ess,
+                          int64_t n_ess, int order, double max_eig_estimate, int power_iterations, double power_tolerance,
+                          int power_seed, h1pa_chebyshev **cheb, void *stream) {
+    API_BEGIN
+    REQUIRE(form && cheb && (n_ess == 0 || d_ess), "chebyshev_create: null argument");
+    REQUIRE(order >= 1 && order <= 5, "chebyshev_create: order must be 1..5");
+    Form &f = form->f;
+    require_assembled(f);
+    REQUIRE(f.desc.integrators != 0, "chebyshev_create: form has no integrator");
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    auto h = std::make_unique<h1pa_chebyshev>();
+    Chebyshev &c = h->c;
+    c.f = &f;
+    c.order = order;
+    if (n_ess == 0) {
+        c.mask = nullptr;
+    } else if (d_ess == sp.ess.p && n_ess == sp.n_ess) {
+        c.mask = sp.ess_mask.p;
+    } else {  // own copy: the form's scratch mask is rebuilt by later calls
+        c.own.alloc(sp.ndofs);
+        kern::list_to_mask(d_ess, n_ess, c.own.p, sp.ndofs, s);
+        c.mask = c.own.p;
+    }
+    chebyshev_setup(c, d_diag, max_eig_estimate, power_iterations, power_tolerance, power_seed,
+                    s);
+    H1PA_HIP(hipStreamSynchronize(s));
+    *cheb = h.release();
+    API_END
+}
+
+int h1pa_chebyshev_destroy(h1pa_chebyshev *cheb) {
+    API_BEGIN
+    delete cheb;
+    API_END
+}
+
+int h1pa_chebyshev_info(const h1pa_chebyshev *cheb, double *max_eig, double *coeffs) {
+    API_BEGIN
+    REQUIRE(cheb, "chebyshev_info: null argument");
+    if (max_eig) *max_eig = cheb->c.max_eig;
+    if (coeffs)
+        for (int k = 0; k < cheb->c.order; ++k) coeffs[k] = cheb->c.coeffs[k];
+    API_END
+}
+
+int h1pa_chebyshev_mult(h1pa_chebyshev *cheb, const double *x, double *y, void *stream) {
+    API_BEGIN
+    REQUIRE(cheb && x && y, "chebyshev_mult: null argument");
+    chebyshev_mult(cheb->c, x, y, as_stream(stream));
+    API_END
+}
+
+int h1pa_pcg_solve_chebyshev(h1pa_chebyshev *cheb, const double *b, double *x,
+                             const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats, void *stream) {
+    API_BEGIN
+    REQUIRE(cheb && b && x && opts && stats, "pcg_solve_chebyshev: null argument");
+    REQUIRE(opts->max_iter >= 1, "pcg_solve_chebyshev: max_iter must be >= 1");
+    pcg_chebyshev(cheb->c, b, x, *opts, *stats, as_stream(stream));
     if (!stats->converged) {
         set_error("PCG: No convergence!");
         return H1PA_ENOTCONV;

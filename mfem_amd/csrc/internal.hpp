@@ -249,6 +249,24 @@ void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, co
             double *B, hipStream_t s);
 }  // namespace kern
 
+// OperatorChebyshevSmoother on a form's DIAG_ONE constrained operator.
+struct Chebyshev {
+    Form *f = nullptr;
+    const uint8_t *mask = nullptr;  // constraint mask (the space's or own.p)
+    DevBuf<uint8_t> own;
+    int order = 0;
+    double max_eig = 0.0;
+    double coeffs[5] = {0, 0, 0, 0, 0};
+    DevBuf<double> dinv, t, u;
+};
+// dinv from d_diag (null: the form's assembled diagonal), the eigenvalue
+// estimate (max_eig > 0: given; else the power method), the coefficients.
+void chebyshev_setup(Chebyshev &c, const double *d_diag, double max_eig, int power_iterations,
+                     double power_tolerance, int power_seed, hipStream_t s);
+void chebyshev_mult(Chebyshev &c, const double *x, double *y, hipStream_t s);
+void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts &o,
+                   h1pa_pcg_stats &stats, hipStream_t s);
+
 void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
                const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s);
 // Builds f's reduced-precision shadows for `precision` (H1PA_PREC_FP32 or
@@ -289,3 +307,4 @@ struct h1pa_restriction {
     const h1pa::Restriction &R() const { return view ? *view : own; }
 };
 struct h1pa_form { h1pa::Form f; };
+struct h1pa_chebyshev { h1pa::Chebyshev c; };

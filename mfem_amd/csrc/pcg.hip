@@ -22,6 +22,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
+#include <stdlib.h>
 
 #include "internal.hpp"
 #include "row_sum.hpp"
@@ -513,6 +515,111 @@ k_ir_dir(int64_t n, int per, const float *__restrict__ z, float *__restrict__ d,
     H1PA_CHUNK(i, n) d[i] = z[i] + beta * d[i];
 }
 
+// ---- Chebyshev smoother (OperatorChebyshevSmoother::Mult, solvers.cpp:630-656)
+// and the CG steps around it.  `stop` (nullable) ends the work once a solve
+// is done.
+
+// t = x * dinv; y = 0 + c0 t  (k = 0 of the smoother)
+__global__ void __launch_bounds__(kThreads)
+k_cheb_first(int64_t n, int per, const double *__restrict__ dinv, const double *__restrict__ x,
+             double *__restrict__ t, double *__restrict__ y, double c0, const int *stop) {
+    if (stop && *stop) return;
+    H1PA_CHUNK(i, n) {
+        const double ti = x[i] * dinv[i];
+        t[i] = ti;
+        y[i] = 0.0 + c0 * ti;
+    }
+}
+
+// k >= 1: u = A t (constrained: interior rows written by the element kernel
+// into u, DIAG_ONE rows copy t); t = u * dinv; y += ck t
+__global__ void __launch_bounds__(kThreads)
+k_cheb_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
+               const int *__restrict__ indices, const double *__restrict__ ye,
+               const uint8_t *__restrict__ mask, const double *__restrict__ dinv,
+               const double *__restrict__ u, double *__restrict__ t, double *__restrict__ y,
+               double ck, const int *stop, const int *__restrict__ perm, int64_t nperm) {
+    if (stop && *stop) return;
+    H1PA_CHUNK(r, n) {
+        const int64_t i = r < nperm ? perm[r] : r;
+        double v;
+        if (r >= bb) v = u[i];
+        else if (mask && mask[i]) v = t[i];
+        else v = row_sum(indices, offsets[i], offsets[i + 1], ye);
+        const double ti = v * dinv[i];
+        t[i] = ti;
+        y[i] = y[i] + ck * ti;
+    }
+}
+
+// CGSolver::Mult start without a fused preconditioner: r = b, x = 0.
+__global__ void __launch_bounds__(kThreads)
+k_cg_start(int64_t n, int per, const double *__restrict__ b, double *__restrict__ x,
+           double *__restrict__ r) {
+    H1PA_CHUNK(i, n) {
+        r[i] = b[i];
+        x[i] = 0.0;
+    }
+}
+
+// d = z (the preconditioned r); partials of (d, r)
+__global__ void __launch_bounds__(kThreads)
+k_cg_first_dir(int64_t n, int per, const double *__restrict__ z, const double *__restrict__ r,
+               double *__restrict__ d, double *__restrict__ partials) {
+    __shared__ double sh[kThreads];
+    double acc[1] = {0.0};
+    H1PA_CHUNK(i, n) {
+        const double di = z[i];
+        d[i] = di;
+        acc[0] += di * r[i];
+    }
+    store_partials<1>(acc, partials, sh);
+}
+
+// x += alpha d, r -= alpha q
+__global__ void __launch_bounds__(kThreads)
+k_cg_update_xr(int64_t n, int per, double *__restrict__ x, double *__restrict__ r,
+               const double *__restrict__ q, const double *__restrict__ d, const PcgState *st) {
+    if (st->done) return;
+    const double alpha = st->nom / st->den;
+    H1PA_CHUNK(i, n) {
+        x[i] += alpha * d[i];
+        r[i] = r[i] - alpha * q[i];
+    }
+}
+
+// partials of (a, b) for a solve still running (stop null or 0)
+__global__ void __launch_bounds__(kThreads)
+k_dot(int64_t n, int per, const double *__restrict__ a, const double *__restrict__ b,
+      double *__restrict__ partials, const int *stop) {
+    __shared__ double sh[kThreads];
+    if (stop && *stop) return;
+    double acc[1] = {0.0};
+    H1PA_CHUNK(i, n) acc[0] += a[i] * b[i];
+    store_partials<1>(acc, partials, sh);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_dot_fin(int nb, const double *__restrict__ partials, double *__restrict__ out) {
+    __shared__ double sh[kThreads];
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x == 0) out[0] = tot[0];
+}
+
+// v *= m
+__global__ void __launch_bounds__(kThreads)
+k_scale(int64_t n, int per, double *__restrict__ v, double m) {
+    H1PA_CHUNK(i, n) v[i] *= m;
+}
+
+// y = 0 + dinv u (OperatorJacobiSmoother::Mult with iterative_mode off)
+__global__ void __launch_bounds__(kThreads)
+k_jacobi_mult(int64_t n, int per, const double *__restrict__ dinv, const double *__restrict__ u,
+              double *__restrict__ y) {
+    H1PA_CHUNK(i, n) y[i] = 0.0 + dinv[i] * u[i];
+}
+
 __global__ void k_form_x(int64_t n, const uint8_t *__restrict__ mask, const double *__restrict__ x,
                          double *__restrict__ X) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -755,6 +862,238 @@ void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b
     DevBuf<double> dinv(sp.ndofs);
     kern::jacobi_dinv(sp.ndofs, diag, mask, dinv.p, s);
     pcg_run<double>(f, H1PA_PREC_FP64, mask, dinv.p, b, x, f.ework0.p, o, stats, s);
+}
+
+// ---- OperatorChebyshevSmoother ----------------------------------------------
+namespace {
+
+// Vector::Randomize(seed) (linalg/vector.cpp:955-966): srand(seed), then
+// rand() / (RAND_MAX + 1) per entry.  Drawn from the C library's own
+// generator through its reentrant interface with a private 128-byte state
+// (glibc's rand() is that generator on a 128-byte state), so the caller's
+// rand() sequence is left alone.
+void randomize_like_mfem(int seed, std::vector<double> &v) {
+    struct random_data rd;
+    std::memset(&rd, 0, sizeof(rd));
+    char state[128];
+    H1PA_CHECK(initstate_r((unsigned)seed, state, sizeof(state), &rd) == 0, H1PA_EINVAL,
+               "chebyshev: initstate_r failed");
+    for (double &x : v) {
+        int32_t r = 0;
+        random_r(&rd, &r);
+        x = (double)r / ((double)RAND_MAX + 1.0);
+    }
+}
+
+// Coefficients of OperatorChebyshevSmoother::Setup (solvers.cpp:552-628):
+// 1 - lambda p(lambda) is the Chebyshev polynomial of the first kind of the
+// given order on [0.3, 1.2] max_eig, scaled to 1 at lambda = 0.
+void chebyshev_coeffs(int order, double max_eig, double *c) {
+    const double upper = 1.2 * max_eig, lower = 0.3 * max_eig;
+    const double th = 0.5 * (upper + lower), de = 0.5 * (upper - lower);
+    switch (order) {
+        case 1: c[0] = 1.0 / th; break;
+        case 2: {
+            const double a = 1.0 / (std::pow(de, 2) - 2 * std::pow(th, 2));
+            c[0] = -4 * th * a;
+            c[1] = 2 * a;
+            break;
+        }
+        case 3: {
+            const double a = 3 * std::pow(de, 2), b2 = std::pow(th, 2);
+            const double g = 1.0 / (-4 * std::pow(th, 3) + th * a);
+            c[0] = g * (a - 12 * b2);
+            c[1] = 12 / (a - 4 * b2);
+            c[2] = -4 * g;
+            break;
+        }
+        case 4: {
+            const double d2 = std::pow(de, 2), t2 = std::pow(th, 2), e8 = 8 * d2;
+            const double g = 1.0 / (std::pow(de, 4) + 8 * std::pow(th, 4) - t2 * e8);
+            c[0] = g * (32 * std::pow(th, 3) - 16 * th * d2);
+            c[1] = g * (-48 * t2 + e8);
+            c[2] = 32 * th * g;
+            c[3] = -8 * g;
+            break;
+        }
+        case 5: {
+            const double d4x5 = 5 * std::pow(de, 4), t4 = std::pow(th, 4), t2 = std::pow(th, 2);
+            const double d2 = std::pow(de, 2), d2x60 = 60 * d2, d2x20 = 20 * d2;
+            const double g = 1.0 / (16 * std::pow(th, 5) - std::pow(th, 3) * d2x20 + th * d4x5);
+            const double t2x160 = 160 * t2;
+            const double h = 1.0 / (d4x5 + 16 * t4 - t2 * d2x20);
+            c[0] = g * (d4x5 + 80 * t4 - t2 * d2x60);
+            c[1] = h * (d2x60 - t2x160);
+            c[2] = g * (-d2x20 + t2x160);
+            c[3] = -80 * h;
+            c[4] = 16 * g;
+            break;
+        }
+        default: throw Error{H1PA_EUNSUP, "chebyshev: order must be 1..5"};
+    }
+}
+
+// (a, b) on the host (used by the power method's set-up iterations).
+double device_dot(int64_t n, const double *a, const double *b, DevBuf<double> &partials,
+                  DevBuf<double> &out, hipStream_t s) {
+    const kern::Chunks ch = kern::chunks(n);
+    hipLaunchKernelGGL(kern::k_dot, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n, ch.per, a, b,
+                       partials.p, (const int *)nullptr);
+    hipLaunchKernelGGL(kern::k_dot_fin, dim3(1), dim3(kern::kThreads), 0, s, (int)ch.blocks,
+                       partials.p, out.p);
+    H1PA_HIP(hipGetLastError());
+    double h = 0.0;
+    H1PA_HIP(hipMemcpyAsync(&h, out.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    H1PA_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+// u = A v on the DIAG_ONE constrained operator; interior rows go straight
+// into u when the mask allows it.
+void constrained_apply(Chebyshev &c, const double *v, double *u, const int *stop, hipStream_t s) {
+    Form &f = *c.f;
+    Space &sp = *f.space;
+    const int64_t bb = direct_rows(sp, c.mask);
+    pcg_element_apply<double>(f, H1PA_PREC_FP64, c.mask, v, f.ework0.p,
+                              bb < sp.ndofs ? u : nullptr, stop, s);
+    kern::scatter_constrained(bb, sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, c.mask,
+                              v, u, sp.restr.perm.p, sp.restr.nperm, s);
+}
+
+// y = sum_k c_k (D^-1 A)^k D^-1 x
+void chebyshev_apply(Chebyshev &c, const double *x, double *y, const int *stop, hipStream_t s) {
+    Form &f = *c.f;
+    Space &sp = *f.space;
+    const int64_t n = sp.ndofs;
+    const kern::Chunks ch = kern::chunks(n);
+    const int64_t bb = direct_rows(sp, c.mask);
+    hipLaunchKernelGGL(kern::k_cheb_first, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n, ch.per,
+                       c.dinv.p, x, c.t.p, y, c.coeffs[0], stop);
+    for (int k = 1; k < c.order; ++k) {
+        pcg_element_apply<double>(f, H1PA_PREC_FP64, c.mask, c.t.p, f.ework0.p,
+                                  bb < n ? c.u.p : nullptr, stop, s);
+        hipLaunchKernelGGL(kern::k_cheb_scatter, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n,
+                           ch.per, bb, sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p,
+                           c.mask, c.dinv.p, c.u.p, c.t.p, y, c.coeffs[k], stop, sp.restr.perm.p,
+                           sp.restr.nperm);
+    }
+    H1PA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void chebyshev_setup(Chebyshev &c, const double *d_diag, double max_eig, int power_iterations,
+                     double power_tolerance, int power_seed, hipStream_t s) {
+    H1PA_CHECK(c.order >= 1 && c.order <= 5, H1PA_EUNSUP, "chebyshev: order must be 1..5");
+    Form &f = *c.f;
+    Space &sp = *f.space;
+    const int64_t n = sp.ndofs;
+    DevBuf<double> diag;
+    if (!d_diag) {
+        diag.alloc(n);
+        operator_diagonal(f, diag.p, s);
+        d_diag = diag.p;
+    }
+    c.dinv.alloc(n);
+    kern::jacobi_dinv(n, d_diag, c.mask, c.dinv.p, s);
+    c.t.alloc(n);
+    c.u.alloc(n);
+    if (max_eig > 0.0) {
+        c.max_eig = max_eig;
+    } else {
+        // PowerMethod::EstimateLargestEigenvalue (linalg/operator.cpp:886-935)
+        // on diag^-1 A from Vector::Randomize(power_seed)
+        H1PA_CHECK(power_seed != 0, H1PA_EINVAL, "chebyshev: invalid seed");
+        H1PA_CHECK(power_iterations >= 1, H1PA_EINVAL, "chebyshev: power_iterations must be >= 1");
+        std::vector<double> h(n);
+        randomize_like_mfem(power_seed, h);
+        DevBuf<double> v0(n), v1(n), partials(kern::chunks(n).blocks), out(1);
+        v0.upload(h.data(), n, s);
+        const kern::Chunks ch = kern::chunks(n);
+        double *a = v0.p, *b = v1.p;
+        double eig = 1.0;
+        for (int it = 0; it < power_iterations; ++it) {
+            const double nrm = device_dot(n, a, a, partials, out, s);
+            hipLaunchKernelGGL(kern::k_scale, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n,
+                               ch.per, a, 1.0 / std::sqrt(nrm));
+            constrained_apply(c, a, c.u.p, nullptr, s);
+            hipLaunchKernelGGL(kern::k_jacobi_mult, dim3(ch.blocks), dim3(kern::kThreads), 0, s,
+                               n, ch.per, c.dinv.p, c.u.p, b);
+            const double enew = device_dot(n, a, b, partials, out, s);
+            const double diff = std::abs((enew - eig) / eig);
+            eig = enew;
+            std::swap(a, b);
+            if (diff < power_tolerance) break;
+        }
+        c.max_eig = eig;
+    }
+    chebyshev_coeffs(c.order, c.max_eig, c.coeffs);
+}
+
+void chebyshev_mult(Chebyshev &c, const double *x, double *y, hipStream_t s) {
+    chebyshev_apply(c, x, y, nullptr, s);
+}
+
+// CGSolver::Mult (solvers.cpp:869-1050, iterative_mode = false) with the
+// Chebyshev smoother as preconditioner: the Jacobi-PCG loop with z = M r
+// computed by the smoother instead of fused into the update.
+void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts &o,
+                   h1pa_pcg_stats &stats, hipStream_t s) {
+    Form &f = *c.f;
+    Space &sp = *f.space;
+    const int64_t n = sp.ndofs;
+    DevBuf<double> r(n), q(n), d(n), z(n);
+    const kern::Chunks ch = kern::chunks(n);
+    const unsigned nb = ch.blocks;
+    const int per = ch.per;
+    DevBuf<double> partials(nb);
+    DevBuf<PcgState> st(1);
+    PcgState h{};
+    h.max_iter = o.max_iter;
+    H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    const int check = o.check_every > 0 ? o.check_every : 8;
+    const int *stop = &st.p->done;
+    const dim3 grid(nb), blk(kern::kThreads), one(1);
+    const int64_t bb = direct_rows(sp, c.mask);
+
+    auto t0 = std::chrono::steady_clock::now();
+    hipLaunchKernelGGL(kern::k_cg_start, grid, blk, 0, s, n, per, b, x, r.p);
+    chebyshev_apply(c, r.p, z.p, stop, s);
+    hipLaunchKernelGGL(kern::k_cg_first_dir, grid, blk, 0, s, n, per, z.p, r.p, d.p, partials.p);
+    hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p, o.rel_tol,
+                       o.abs_tol);
+    auto apply = [&]() {  // q = A d, den = (d, q)
+        pcg_element_apply<double>(f, H1PA_PREC_FP64, c.mask, d.p, f.ework0.p,
+                                  bb < n ? q.p : nullptr, stop, s);
+        hipLaunchKernelGGL(kern::k_pcg_scatter<double>, grid, blk, 0, s, n, per, bb,
+                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, c.mask, d.p,
+                           q.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
+    };
+    apply();
+    H1PA_HIP(hipGetLastError());
+    for (int it = 1; it <= o.max_iter; ++it) {
+        hipLaunchKernelGGL(kern::k_cg_update_xr, grid, blk, 0, s, n, per, x, r.p, q.p, d.p, st.p);
+        chebyshev_apply(c, r.p, z.p, stop, s);
+        hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, per, r.p, z.p, partials.p, stop);
+        hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_pcg_dir<double>, grid, blk, 0, s, n, per, z.p, d.p, st.p);
+        apply();
+        H1PA_HIP(hipGetLastError());
+        if (it % check == 0 || it == o.max_iter) {
+            H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            H1PA_HIP(hipStreamSynchronize(s));
+            if (h.done) break;
+        }
+    }
+    H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    H1PA_HIP(hipStreamSynchronize(s));
+    auto t1 = std::chrono::steady_clock::now();
+    stats.converged = h.converged;
+    stats.iterations = h.done ? h.final_iter : o.max_iter;
+    stats.initial_norm = h.nom0 >= 0.0 ? std::sqrt(h.nom0) : h.nom0;
+    stats.final_norm = h.betanom >= 0.0 ? std::sqrt(h.betanom) : h.betanom;
+    stats.seconds = std::chrono::duration<double>(t1 - t0).count();
 }
 
 void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,

@@ -90,6 +90,16 @@ def lib():
         L.orc_form_linear_system.argtypes = [C.POINTER(_PaOp), _pd, _pd, _pd, _pd]
         L.orc_pcg.argtypes = [C.POINTER(_PaOp), _pd, _pd, _pd, _d, _d, _i, _pi, _pd, _pd]
         L.orc_pcg.restype = _i
+        L.orc_glibc_rand.argtypes = [C.c_uint, _i, _pi]
+        L.orc_randomize.argtypes = [C.c_uint, _i, _pd]
+        L.orc_chebyshev_coeffs.argtypes = [_i, _d, _pd]
+        L.orc_chebyshev_coeffs.restype = _i
+        L.orc_power_method.argtypes = [C.POINTER(_PaOp), _pd, _pd, _i, _d]
+        L.orc_power_method.restype = _d
+        L.orc_chebyshev_mult.argtypes = [C.POINTER(_PaOp), _pd, _i, _pd, _pd, _pd]
+        L.orc_pcg_chebyshev.argtypes = [C.POINTER(_PaOp), _pd, _i, _pd, _pd, _pd, _d, _d, _i, _pi,
+                                        _pd, _pd]
+        L.orc_pcg_chebyshev.restype = _i
         L.orc_domain_lf.argtypes = [_i, _i, _i, _pd, _d, _pd]
         L.orc_num_threads.restype = _i
     return _lib
@@ -312,6 +322,28 @@ def domain_lf(p, ne, pg, X, f=1.0):
     return bE
 
 
+def glibc_rand(seed, n):
+    """srand(seed); n x rand() of glibc's default generator (restated)."""
+    o = _ia(n)
+    lib().orc_glibc_rand(seed, n, _ip(o))
+    return o
+
+
+def randomize(seed, n):
+    """Vector::Randomize(seed): rand() / (RAND_MAX + 1)."""
+    v = _da(n)
+    lib().orc_randomize(seed, n, _dp(v))
+    return v
+
+
+def chebyshev_coeffs(order, max_eig):
+    """OperatorChebyshevSmoother::Setup coefficients (orders 1..5)."""
+    c = _da(5)
+    if not lib().orc_chebyshev_coeffs(order, max_eig, _dp(c)):
+        raise ValueError(f"Chebyshev order {order} not supported")
+    return c[:order].copy()
+
+
 def num_threads():
     return lib().orc_num_threads()
 
@@ -431,6 +463,42 @@ class Problem:
                                      _dp(np.ascontiguousarray(b, dtype=np.float64)), _dp(X),
                                      _dp(Bv))
         return X, Bv
+
+    def power_method(self, v0, steps=10, tol=1e-8, diag=None):
+        """PowerMethod::EstimateLargestEigenvalue on diag^-1 A (constrained);
+        returns (estimate, final vector)."""
+        diag = self.diagonal() if diag is None else diag
+        v = np.ascontiguousarray(v0, dtype=np.float64).copy()
+        e = lib().orc_power_method(C.byref(self._op), _dp(diag), _dp(v), steps, tol)
+        return e, v
+
+    def chebyshev(self, order, power_iterations=10, power_tolerance=1e-8, power_seed=12345,
+                  max_eig=None):
+        """OperatorChebyshevSmoother set-up: (max_eig, coeffs)."""
+        if max_eig is None:
+            max_eig, _ = self.power_method(randomize(power_seed, self.ndofs), power_iterations,
+                                           power_tolerance)
+        return max_eig, chebyshev_coeffs(order, max_eig)
+
+    def chebyshev_mult(self, coeffs, x, diag=None):
+        diag = self.diagonal() if diag is None else diag
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        y = _da(self.ndofs)
+        lib().orc_chebyshev_mult(C.byref(self._op), _dp(diag), c.size, _dp(c),
+                                 _dp(np.ascontiguousarray(x, dtype=np.float64)), _dp(y))
+        return y
+
+    def pcg_chebyshev(self, coeffs, b, rel_tol=1e-12, abs_tol=0.0, max_iter=2000, diag=None):
+        diag = self.diagonal() if diag is None else diag
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        x = _da(self.ndofs)
+        it, fn, n0 = _i(0), _d(0), _d(0)
+        conv = lib().orc_pcg_chebyshev(C.byref(self._op), _dp(diag), c.size, _dp(c),
+                                       _dp(np.ascontiguousarray(b, dtype=np.float64)), _dp(x),
+                                       rel_tol, abs_tol, max_iter, C.byref(it), C.byref(fn),
+                                       C.byref(n0))
+        return x, dict(converged=bool(conv), iterations=it.value, final_norm=fn.value,
+                       initial_norm=n0.value)
 
     def pcg(self, b, rel_tol=1e-12, abs_tol=0.0, max_iter=2000, diag=None):
         diag = self.diagonal() if diag is None else diag

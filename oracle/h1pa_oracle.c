@@ -13,6 +13,7 @@
 #include "h1pa_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -1270,4 +1271,217 @@ void orc_domain_lf(int p, int ne, int pg, const double *X, double f, double *bE)
         }
     }
     free(J);
+}
+
+/* ======================================================================
+ * Chebyshev smoother and power method.
+ * ====================================================================== */
+
+/* srand(seed) then n calls of rand() for glibc's default generator (the
+ * additive feedback generator of its random_r.c, TYPE_3: degree 31,
+ * separation 3), which Vector::Randomize (linalg/vector.cpp:955-966) draws
+ * from.  Published form: r[0] = seed (0 -> 1); r[i] = 16807 r[i-1] mod
+ * (2^31 - 1) for i = 1..30 (Schrage's method, negative results + 2^31 - 1);
+ * r[i] = r[i-31] for i = 31..33; r[i] = r[i-31] + r[i-3] mod 2^32 from
+ * i = 34 on; the k-th output is r[k + 344] >> 1. */
+void orc_glibc_rand(unsigned seed, int n, int *out)
+{
+    uint32_t r[34];
+    int32_t w = (int32_t)(seed == 0 ? 1u : seed);
+    r[0] = (uint32_t)w;
+    for (int i = 1; i < 31; i++) {
+        const long hi = w / 127773, lo = w % 127773;
+        long word = 16807 * lo - 2836 * hi;
+        if (word < 0) word += 2147483647;
+        w = (int32_t)word;
+        r[i] = (uint32_t)w;
+    }
+    for (int i = 31; i < 34; i++) r[i] = r[i - 31];
+    /* ring of the last 34 values: r[i] lives at i % 34 */
+    for (long i = 34; i < 344 + (long)n; i++) {
+        const uint32_t v = r[(i - 31) % 34] + r[(i - 3) % 34];
+        r[i % 34] = v;
+        if (i >= 344) out[i - 344] = (int)(v >> 1);
+    }
+}
+
+/* Vector::Randomize(seed): data[i] = rand() / (RAND_MAX + 1.0), RAND_MAX =
+ * 2^31 - 1 (linalg/vector.hpp:61-77). */
+void orc_randomize(unsigned seed, int n, double *v)
+{
+    int *o = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+    orc_glibc_rand(seed, n, o);
+    for (int i = 0; i < n; i++) v[i] = (double)o[i] / 2147483648.0;
+    free(o);
+}
+
+/* OperatorChebyshevSmoother::Setup coefficients (linalg/solvers.cpp:552-628):
+ * bounds 0.3 and 1.2 times max_eig, theta and delta their mid-point and
+ * half-width; order 1..5.  Returns 0 on an unsupported order. */
+int orc_chebyshev_coeffs(int order, double max_eig, double *c)
+{
+    const double upper = 1.2 * max_eig, lower = 0.3 * max_eig;
+    const double th = 0.5 * (upper + lower), de = 0.5 * (upper - lower);
+    switch (order - 1) {
+    case 0:
+        c[0] = 1.0 / th;
+        return 1;
+    case 1: {
+        const double t0 = 1.0 / (pow(de, 2) - 2 * pow(th, 2));
+        c[0] = -4 * th * t0;
+        c[1] = 2 * t0;
+        return 1;
+    }
+    case 2: {
+        const double t0 = 3 * pow(de, 2), t1 = pow(th, 2);
+        const double t2 = 1.0 / (-4 * pow(th, 3) + th * t0);
+        c[0] = t2 * (t0 - 12 * t1);
+        c[1] = 12 / (t0 - 4 * t1);
+        c[2] = -4 * t2;
+        return 1;
+    }
+    case 3: {
+        const double t0 = pow(de, 2), t1 = pow(th, 2), t2 = 8 * t0;
+        const double t3 = 1.0 / (pow(de, 4) + 8 * pow(th, 4) - t1 * t2);
+        c[0] = t3 * (32 * pow(th, 3) - 16 * th * t0);
+        c[1] = t3 * (-48 * t1 + t2);
+        c[2] = 32 * th * t3;
+        c[3] = -8 * t3;
+        return 1;
+    }
+    case 4: {
+        const double t0 = 5 * pow(de, 4), t1 = pow(th, 4), t2 = pow(th, 2), t3 = pow(de, 2);
+        const double t4 = 60 * t3, t5 = 20 * t3;
+        const double t6 = 1.0 / (16 * pow(th, 5) - pow(th, 3) * t5 + th * t0);
+        const double t7 = 160 * t2;
+        const double t8 = 1.0 / (t0 + 16 * t1 - t2 * t5);
+        c[0] = t6 * (t0 + 80 * t1 - t2 * t4);
+        c[1] = t8 * (t4 - t7);
+        c[2] = t6 * (-t5 + t7);
+        c[3] = -80 * t8;
+        c[4] = 16 * t6;
+        return 1;
+    }
+    default:
+        return 0;
+    }
+}
+
+/* Jacobi inverse with ess rows 1 (OperatorJacobiSmoother::Setup, damping 1,
+ * solvers.cpp:401-425; the same in OperatorChebyshevSmoother::Setup). */
+static void jacobi_inverse(const orc_pa_op *op, const double *diag, double *dinv)
+{
+    for (int i = 0; i < op->ndofs; i++) dinv[i] = 1.0 / diag[i];
+    for (int i = 0; i < op->n_ess; i++) dinv[op->ess[i]] = 1.0;
+}
+
+/* PowerMethod::EstimateLargestEigenvalue (linalg/operator.cpp:886-935) on
+ * diag^-1 A (ProductOperator of OperatorJacobiSmoother and the constrained
+ * operator, solvers.cpp:496-511); v0 holds the start vector on entry. */
+double orc_power_method(const orc_pa_op *op, const double *diag, double *v0, int steps,
+                        double tol)
+{
+    const int n = op->ndofs;
+    double *dinv = (double *)malloc(sizeof(double) * n);
+    double *v1 = (double *)malloc(sizeof(double) * n);
+    double *u = (double *)malloc(sizeof(double) * n);
+    double *a = v0, *b = v1;
+    jacobi_inverse(op, diag, dinv);
+    double eig = 1.0;
+    for (int it = 0; it < steps; it++) {
+        const double nrm = dot(n, a, a);
+        const double m = 1.0 / sqrt(nrm);
+        for (int i = 0; i < n; i++) a[i] *= m;
+        orc_constrained_mult(op, a, u);
+        for (int i = 0; i < n; i++) b[i] = 0.0 + dinv[i] * u[i];
+        const double enew = dot(n, a, b);
+        const double diff = fabs((enew - eig) / eig);
+        eig = enew;
+        double *t = a; a = b; b = t;
+        if (diff < tol) break;
+    }
+    if (a != v0) memcpy(v0, a, sizeof(double) * n);
+    free(dinv); free(v1); free(u);
+    return eig;
+}
+
+/* OperatorChebyshevSmoother::Mult (solvers.cpp:630-656):
+ * y = sum_k c_k (diag^-1 A)^k diag^-1 x on the constrained operator. */
+void orc_chebyshev_mult(const orc_pa_op *op, const double *diag, int order, const double *c,
+                        const double *x, double *y)
+{
+    const int n = op->ndofs;
+    double *dinv = (double *)malloc(sizeof(double) * n);
+    double *res = (double *)malloc(sizeof(double) * n);
+    double *h = (double *)malloc(sizeof(double) * n);
+    jacobi_inverse(op, diag, dinv);
+    memcpy(res, x, sizeof(double) * n);
+    for (int i = 0; i < n; i++) y[i] = 0.0;
+    for (int k = 0; k < order; k++) {
+        if (k > 0) {
+            orc_constrained_mult(op, res, h);
+            memcpy(res, h, sizeof(double) * n);
+        }
+        for (int i = 0; i < n; i++) res[i] *= dinv[i];
+        for (int i = 0; i < n; i++) y[i] += c[k] * res[i];
+    }
+    free(dinv); free(res); free(h);
+}
+
+/* CGSolver::Mult (solvers.cpp:869-1050, iterative_mode = false) with the
+ * Chebyshev smoother as preconditioner, on the constrained operator. */
+int orc_pcg_chebyshev(const orc_pa_op *op, const double *diag, int order, const double *c,
+                      const double *b, double *x, double rel_tol, double abs_tol, int max_iter,
+                      int *iters, double *final_norm, double *initial_norm)
+{
+    const int n = op->ndofs;
+    double *r = (double *)malloc(sizeof(double) * n);
+    double *d = (double *)malloc(sizeof(double) * n);
+    double *z = (double *)malloc(sizeof(double) * n);
+    int converged = 0, final_iter = 0;
+    double nom, nom0, den, betanom = 0.0, alpha, beta, r0;
+    memcpy(r, b, sizeof(double) * n);
+    for (int i = 0; i < n; i++) x[i] = 0.0;
+    orc_chebyshev_mult(op, diag, order, c, r, z);
+    memcpy(d, z, sizeof(double) * n);
+    nom0 = nom = dot(n, d, r);
+    *initial_norm = nom0 >= 0.0 ? sqrt(nom0) : nom0;
+    if (nom < 0.0) {
+        *iters = 0; *final_norm = nom; converged = 0;
+        goto done;
+    }
+    r0 = nom * rel_tol * rel_tol;
+    if (abs_tol * abs_tol > r0) r0 = abs_tol * abs_tol;
+    if (nom <= r0) {
+        *iters = 0; *final_norm = sqrt(nom); converged = 1;
+        goto done;
+    }
+    orc_constrained_mult(op, d, z);
+    den = dot(n, z, d);
+    if (den <= 0.0 && den == 0.0) {
+        *iters = 0; *final_norm = sqrt(nom); converged = 0;
+        goto done;
+    }
+    final_iter = max_iter;
+    for (int i = 1;;) {
+        alpha = nom / den;
+        for (int k = 0; k < n; k++) x[k] += alpha * d[k];
+        for (int k = 0; k < n; k++) r[k] -= alpha * z[k];
+        orc_chebyshev_mult(op, diag, order, c, r, z);
+        betanom = dot(n, r, z);
+        if (betanom < 0.0) { converged = 0; final_iter = i; break; }
+        if (betanom <= r0) { converged = 1; final_iter = i; break; }
+        if (++i > max_iter) break;
+        beta = betanom / nom;
+        for (int k = 0; k < n; k++) d[k] = z[k] + beta * d[k];
+        orc_constrained_mult(op, d, z);
+        den = dot(n, d, z);
+        if (den <= 0.0 && den == 0.0) { final_iter = i; break; }
+        nom = betanom;
+    }
+    *iters = final_iter;
+    *final_norm = sqrt(betanom);
+done:
+    free(r); free(d); free(z);
+    return converged;
 }

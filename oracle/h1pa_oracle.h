@@ -162,6 +162,25 @@ int  orc_pcg(const orc_pa_op *op, const double *diag, const double *b, double *x
              double rel_tol, double abs_tol, int max_iter, int *iters,
              double *final_norm, double *initial_norm);
 
+/* ---- Chebyshev smoother (linalg/solvers.cpp:455-656) and power method
+ *      (linalg/operator.cpp:886-935) ------------------------------------- */
+/* srand(seed); n x rand() of glibc's default generator. */
+void orc_glibc_rand(unsigned seed, int n, int *out);
+/* Vector::Randomize(seed) (linalg/vector.cpp:955-966). */
+void orc_randomize(unsigned seed, int n, double *v);
+/* Coefficients of OperatorChebyshevSmoother::Setup; returns 0 for order > 5. */
+int  orc_chebyshev_coeffs(int order, double max_eig, double *c);
+/* Largest eigenvalue estimate of diag^-1 A (constrained), v0 = start vector. */
+double orc_power_method(const orc_pa_op *op, const double *diag, double *v0, int steps,
+                        double tol);
+/* OperatorChebyshevSmoother::Mult. */
+void orc_chebyshev_mult(const orc_pa_op *op, const double *diag, int order, const double *c,
+                        const double *x, double *y);
+/* CGSolver preconditioned by the Chebyshev smoother; as orc_pcg. */
+int  orc_pcg_chebyshev(const orc_pa_op *op, const double *diag, int order, const double *c,
+                       const double *b, double *x, double rel_tol, double abs_tol, int max_iter,
+                       int *iters, double *final_norm, double *initial_norm);
+
 /* ---- RHS: DomainLFIntegrator (fem/lininteg.cpp:51-79) ------------------ */
 /* b_E[D^3][ne] += sum_q w_q detJ f(x_q) phi(x_q), Q = p+1 GL points,
  * constant f. */
