@@ -126,11 +126,32 @@ int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *
 /* ---- quadrature interpolation of the mesh nodes (geometric factors)
  *      GeometricFactors::Compute (mesh/mesh.cpp:15445-15497) via
  *      QuadratureInterpolator::Mult (fem/quadinterpolator.cpp:226-341) ------- */
-enum { H1PA_QI_VALUES = 1, H1PA_QI_DERIVATIVES = 2, H1PA_QI_DETERMINANTS = 4 };
+enum {
+    H1PA_QI_VALUES = 1,               /* QuadratureInterpolator::VALUES               */
+    H1PA_QI_DERIVATIVES = 2,          /* ::DERIVATIVES (reference coordinates)        */
+    H1PA_QI_DETERMINANTS = 4,         /* ::DETERMINANTS                               */
+    H1PA_QI_PHYSICAL_DERIVATIVES = 8  /* ::PHYSICAL_DERIVATIVES (J^-T applied)        */
+};
+enum { H1PA_QI_BY_NODES = 0, H1PA_QI_BY_VDIM = 1 };  /* QVectorLayout */
 /* Evaluate the mesh nodes at the q1d^3 Gauss-Legendre points: X[nq][3][ne],
  * J[nq][3][3][ne] (J(q,r,c,e) = dx_r/dxi_c), detJ[nq][ne]; NULL skips. */
 int h1pa_mesh_geometric_factors(const h1pa_mesh *mesh, int q1d, unsigned flags,
                                 double *d_X, double *d_J, double *d_detJ, void *stream);
+
+/* QuadratureInterpolator::Mult (fem/quadinterpolator.cpp:226-341, tensor
+ * kernels fem/qinterp/eval.hpp, grad.hpp:233-375, det.hpp) for an E-vector of
+ * the space: d_e is lexicographic [D^3][vdim][ne] (the ElementRestriction
+ * output of a vdim-component field), evaluated at the q1d^3 Gauss-Legendre
+ * points, q1d in [p+1, min(p+3, 10)].  Outputs (NULL where not requested), nq = q1d^3:
+ *   VALUES       byNODES val[nq][vdim][ne],       byVDIM val[vdim][nq][ne]
+ *   DERIVATIVES or PHYSICAL_DERIVATIVES (not both)
+ *                byNODES der[nq][vdim][3][ne],    byVDIM der[vdim][3][nq][ne]
+ *   DETERMINANTS (vdim = 3) det[nq][ne] of the reference derivative matrix.
+ * Physical derivatives are J^-T times the reference ones, J from the mesh's
+ * geometric factors at the same points. */
+int h1pa_space_qinterp(const h1pa_space *space, int q1d, int vdim, unsigned flags, int layout,
+                       const double *d_e, double *d_val, double *d_der, double *d_det,
+                       void *stream);
 
 /* ---- PA bilinear form ---------------------------------------------------- */
 enum { H1PA_INTEG_MASS = 1, H1PA_INTEG_DIFFUSION = 2 };

@@ -37,7 +37,7 @@ _pi64 = C.POINTER(C.c_int64)
 H1PA_OK, H1PA_EINVAL, H1PA_ENOMEM, H1PA_EHIP, H1PA_ENOTCONV, H1PA_EUNSUP = range(6)
 INTEG_MASS, INTEG_DIFFUSION = 1, 2
 COEFF_CONSTANT, COEFF_SINCOS, COEFF_PER_ELEMENT, COEFF_QUADRATURE, COEFF_FOURIER = range(5)
-QI_VALUES, QI_DERIVATIVES, QI_DETERMINANTS = 1, 2, 4
+QI_VALUES, QI_DERIVATIVES, QI_DETERMINANTS, QI_PHYSICAL_DERIVATIVES = 1, 2, 4, 8
 
 
 class H1paError(RuntimeError):
@@ -130,6 +130,7 @@ SIGNATURES = {
     "h1pa_pcg_solve": (_i, [_vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                             _vp]),
     "h1pa_form_mult_f32": (_i, [_vp, _i, _vp, _vp, _vp]),
+    "h1pa_space_qinterp": (_i, [_vp, _i, _i, _u, _i, _vp, _vp, _vp, _vp, _vp]),
     "h1pa_chebyshev_create": (_i, [_vp, _vp, _vp, _i64, _i, _d, _i, _d, _i, C.POINTER(_vp),
                                    _vp]),
     "h1pa_chebyshev_destroy": (_i, [_vp]),
@@ -482,6 +483,60 @@ class FiniteElementSpace:
         d, n = C.c_void_p(), C.c_int64()
         _check(lib().h1pa_space_boundary_dofs(self._h, C.byref(d), C.byref(n)))
         return d, n.value
+
+    def GetQuadratureInterpolator(self, q1d, vdim=1):
+        """QuadratureInterpolator at the q1d^3 Gauss-Legendre points
+        (FiniteElementSpace::GetQuadratureInterpolator(ir) with the tensor
+        rule of q1d points per direction); vdim: components of the E-vectors
+        it will interpolate."""
+        return QuadratureInterpolator(self, q1d, vdim)
+
+
+class QVectorLayout:
+    byNODES, byVDIM = 0, 1
+
+
+class QuadratureInterpolator:
+    """QuadratureInterpolator (fem/quadinterpolator.hpp:21-160) for
+    lexicographic E-vectors [D^3][vdim][ne] of an H1 space."""
+
+    VALUES, DERIVATIVES, DETERMINANTS, PHYSICAL_DERIVATIVES = 1, 2, 4, 8
+
+    def __init__(self, fes, q1d, vdim=1):
+        self.fes, self.q1d, self.vdim = fes, int(q1d), int(vdim)
+        self.layout = QVectorLayout.byNODES
+
+    def SetOutputLayout(self, layout):
+        self.layout = int(layout)
+
+    def GetOutputLayout(self):
+        return self.layout
+
+    def Mult(self, e_vec, eval_flags, q_val, q_der, q_det):
+        fes, nq = self.fes, self.q1d ** 3
+        _check_vec(e_vec, fes.dof * self.vdim * fes.ne)
+        if eval_flags & self.VALUES:
+            _check_vec(q_val, nq * self.vdim * fes.ne)
+        if eval_flags & (self.DERIVATIVES | self.PHYSICAL_DERIVATIVES):
+            _check_vec(q_der, nq * self.vdim * 3 * fes.ne)
+        if eval_flags & self.DETERMINANTS:
+            _check_vec(q_det, nq * fes.ne)
+        opt = lambda t: _ptr(t) if t is not None else None  # noqa: E731
+        _check(lib().h1pa_space_qinterp(fes._h, self.q1d, self.vdim, int(eval_flags),
+                                        self.layout, _ptr(e_vec), opt(q_val), opt(q_der),
+                                        opt(q_det), _stream()))
+
+    def Values(self, e_vec, q_val):
+        self.Mult(e_vec, self.VALUES, q_val, None, None)
+
+    def Derivatives(self, e_vec, q_der):
+        self.Mult(e_vec, self.DERIVATIVES, None, q_der, None)
+
+    def PhysDerivatives(self, e_vec, q_der):
+        self.Mult(e_vec, self.PHYSICAL_DERIVATIVES, None, q_der, None)
+
+    def Determinants(self, e_vec, q_det):
+        self.Mult(e_vec, self.DETERMINANTS, None, None, q_det)
 
 
 # ---------------------------------------------------------------------------

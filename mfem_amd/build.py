@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libh1pa.so")
 SERVE = os.path.join(HERE, "h1pa_serve")
 
 SOURCES = ["basis.cpp", "mesh.cpp", "capi.cpp", "restriction.hip", "geometry.hip",
-           "apply.hip", "pcg.hip", "batch.hip"]
+           "apply.hip", "pcg.hip", "batch.hip", "qinterp.hip"]
 
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",

@@ -142,6 +142,29 @@ int h1pa_mesh_geometric_factors(const h1pa_mesh *mesh, int q1d, unsigned flags, 
     API_END
 }
 
+int h1pa_space_qinterp(const h1pa_space *space, int q1d, int vdim, unsigned flags, int layout,
+                       const double *e, double *val, double *der, double *det, void *stream) {
+    API_BEGIN
+    REQUIRE(space && e, "space_qinterp: null argument");
+    Space &sp = const_cast<h1pa_space *>(space)->s;
+    REQUIRE(kern::qinterp_supported(sp.p, q1d), "space_qinterp: q1d must be in [p+1, min(p+3, 10)]");
+    REQUIRE(vdim >= 1 && vdim <= 3, "space_qinterp: vdim must be 1, 2 or 3");
+    REQUIRE(layout == H1PA_QI_BY_NODES || layout == H1PA_QI_BY_VDIM,
+            "space_qinterp: layout must be H1PA_QI_BY_NODES or H1PA_QI_BY_VDIM");
+    REQUIRE((flags & ~15u) == 0, "space_qinterp: unknown flag");
+    REQUIRE(!((flags & H1PA_QI_DERIVATIVES) && (flags & H1PA_QI_PHYSICAL_DERIVATIVES)),
+            "space_qinterp: DERIVATIVES and PHYSICAL_DERIVATIVES are exclusive");
+    REQUIRE(!(flags & H1PA_QI_VALUES) || val, "space_qinterp: VALUES needs d_val");
+    REQUIRE(!(flags & (H1PA_QI_DERIVATIVES | H1PA_QI_PHYSICAL_DERIVATIVES)) || der,
+            "space_qinterp: derivatives need d_der");
+    REQUIRE(!(flags & H1PA_QI_DETERMINANTS) || (det && vdim == 3),
+            "space_qinterp: DETERMINANTS needs d_det and vdim = 3");
+    hipStream_t s = as_stream(stream);
+    kern::qinterp(sp, q1d, vdim, flags, layout == H1PA_QI_BY_VDIM, e, val, der, det, s);
+    H1PA_HIP(hipStreamSynchronize(s));
+    API_END
+}
+
 // ---- space ----------------------------------------------------------------
 int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space) {
     API_BEGIN

@@ -186,6 +186,12 @@ void scatter_constrained(int64_t nrows, const int *offsets, const int *indices,
 void perturb_vertices(Mesh &m, double amp, uint64_t seed);  // host
 void nodes_trilinear(const Mesh &m, double *d_nodes, hipStream_t s);
 void nodes_curved(const Mesh &m, int pg, double amp, double *d_nodes, hipStream_t s);
+// QuadratureInterpolator::Mult on an E-vector [D^3][vdim][ne] of the space
+// at the q1d^3 Gauss-Legendre points (H1PA_QI_* flags; byVDIM layout when
+// by_vdim).
+bool qinterp_supported(int p, int q1d);
+void qinterp(Space &sp, int q1d, int vdim, unsigned flags, int by_vdim, const double *x,
+             double *val, double *der, double *det, hipStream_t s);
 void geometric_factors(const Mesh &m, int q1d, double *X, double *J, double *detJ,
                        hipStream_t s);
 void pa_setup(const Mesh &m, const Form &f, hipStream_t s);

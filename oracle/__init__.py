@@ -261,10 +261,52 @@ def tensor_weights(Q):
 
 
 def geom(ne, pg, Q, X):
+    if Q > 11 or pg > 10:
+        raise ValueError("oracle geometry supports Q <= 11 and node order <= 10")
     nq = Q ** 3
     J, P = _da(nq * 9 * ne), _da(nq * 3 * ne)
     lib().orc_geom(ne, pg, Q, _dp(X), _dp(J), _dp(P))
     return J, P
+
+
+QI_VALUES, QI_DERIVATIVES, QI_DETERMINANTS, QI_PHYSICAL_DERIVATIVES = 1, 2, 4, 8
+
+
+def qinterp(p, Q, vdim, ne, xE, flags, by_vdim=False, J=None):
+    """QuadratureInterpolator::Mult (fem/quadinterpolator.cpp:226-341) with
+    the tensor kernels of fem/qinterp/eval.hpp and grad.hpp:233-375 and the
+    determinant of fem/qinterp/det.hpp, restated with numpy contractions.
+
+    xE: lexicographic E-vector [D^3][vdim][ne] (dx fastest).  J: geometric
+    factors [nq][3][3][ne] (J(q, r, c, e) = dx_r/dxi_c) for physical
+    derivatives, which are J^-T times the reference ones.  Returns (val, der,
+    det) flat in MFEM's byNODES or byVDIM Q-vector layout (None where not
+    requested)."""
+    D, nq = p + 1, Q ** 3
+    B, G = basis(p, Q)  # (Q, D)
+    X = np.asarray(xE, dtype=np.float64).reshape(ne, vdim, D, D, D)  # [e][c][dz][dy][dx]
+    val = der = det = None
+    want_der = flags & (QI_DERIVATIVES | QI_PHYSICAL_DERIVATIVES)
+    if flags & QI_VALUES:
+        v = np.einsum("ecijk,zi,yj,xk->eczyx", X, B, B, B).reshape(ne, vdim, nq)
+        val = (v.transpose(0, 2, 1) if by_vdim else v).ravel()
+    if want_der or flags & QI_DETERMINANTS:
+        g = np.stack([np.einsum("ecijk,zi,yj,xk->eczyx", X, B, B, G),
+                      np.einsum("ecijk,zi,yj,xk->eczyx", X, B, G, B),
+                      np.einsum("ecijk,zi,yj,xk->eczyx", X, G, B, B)],
+                     axis=1).reshape(ne, 3, vdim, nq)  # [e][d][c][q]
+        if flags & QI_DETERMINANTS:
+            M = g.transpose(0, 3, 2, 1)  # [e][q][c][d]
+            det = np.linalg.det(M).ravel()  # [e][q] -> q + nq e
+        if want_der:
+            if flags & QI_PHYSICAL_DERIVATIVES:
+                Jm = np.asarray(J).reshape(ne, 3, 3, nq)  # [e][col][r][q]
+                Mt = Jm.transpose(0, 3, 1, 2)  # [e][q][col][r] = J^T
+                gq = g.transpose(0, 3, 1, 2)  # [e][q][d][c]
+                gp = np.linalg.solve(Mt, gq)  # J^T grad_x = grad_xi
+                g = gp.transpose(0, 2, 3, 1)  # [e][d][c][q]
+            der = (g.transpose(0, 3, 1, 2) if by_vdim else g).ravel()
+    return val, der, det
 
 
 def coeff_sincos(Q, ne, P):

@@ -732,11 +732,13 @@ void orc_geom(int ne, int pg, int Q, const double *X, double *J, double *P)
     /* QuadratureInterpolator derivatives/values by nodes
      * (fem/qinterp/grad.hpp:233-375, eval.hpp): three 1D contractions. */
     const int Dg = pg + 1, nd = Dg * Dg * Dg, nq = Q * Q * Q;
+    /* stage buffers hold Q Dg^2 and Q^2 Dg values: sized for Q, Dg <= 11 */
     double B[256], G[256];
+    if (Q > 11 || Dg > 11) abort();
     orc_basis(pg, Q, B, G);
 #pragma omp parallel for schedule(static)
     for (long e = 0; e < ne; e++) {
-        double t1[2][1000], t2[3][1000];
+        double t1[2][1331], t2[3][1331];
         for (int c = 0; c < 3; c++) {
             const double *x = X + nd * (c + 3 * e);
             /* contract x: t1[0]=B, t1[1]=G over dx -> (qx,dy,dz) */
