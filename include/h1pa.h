@@ -60,6 +60,15 @@ int h1pa_stream_synchronize(void *stream);
 /* ---- mesh: Mesh::MakeCartesian3D(nx, ny, nz, HEXAHEDRON, 1, 1, 1, sfc)
  *      (mesh/mesh.cpp:3841-4121, :4786-4794) -------------------------------- */
 int h1pa_mesh_create_cartesian(int nx, int ny, int nz, int sfc_ordering, h1pa_mesh **mesh);
+/* Mesh(istream) / Mesh::ReadMFEMMesh (mesh/mesh_readers.cpp:40-130) on the text
+ * of an "MFEM mesh v1.0" file (len bytes, not NUL-terminated): a conforming
+ * mesh of hexahedra (geometry 5) with a quadrilateral boundary list (geometry
+ * 3) and straight vertex coordinates.  Elements keep the file order; edges and
+ * faces are numbered as FinalizeTopology numbers them (first encounter); the
+ * boundary DOFs of spaces on it are those of the file's boundary faces (all
+ * attributes).  A 'nodes' section, other element types, inverted elements and
+ * other dimensions give H1PA_EUNSUP. */
+int h1pa_mesh_load(const char *text, int64_t len, h1pa_mesh **mesh);
 int h1pa_mesh_destroy(h1pa_mesh *mesh);
 /* Perturb interior vertices by U(-amp*h, amp*h) per coordinate (counter-based
  * splitmix64 with `seed`); boundary vertices stay on the cube faces. */

@@ -92,6 +92,7 @@ SIGNATURES = {
     "h1pa_set_device": (_i, [_i]),
     "h1pa_stream_synchronize": (_i, [_vp]),
     "h1pa_mesh_create_cartesian": (_i, [_i, _i, _i, _i, C.POINTER(_vp)]),
+    "h1pa_mesh_load": (_i, [C.c_char_p, _i64, C.POINTER(_vp)]),
     "h1pa_mesh_destroy": (_i, [_vp]),
     "h1pa_mesh_perturb": (_i, [_vp, _d, C.c_uint64]),
     "h1pa_mesh_set_curvature": (_i, [_vp, _i, _d]),
@@ -269,11 +270,22 @@ class Element:
 # mesh
 # ---------------------------------------------------------------------------
 class Mesh:
-    """Cartesian hexahedral mesh (Mesh::MakeCartesian3D, mesh/mesh.cpp:4786-4794)."""
+    """Hexahedral mesh: Cartesian (Mesh::MakeCartesian3D, mesh/mesh.cpp:4786-4794)
+    or read from an MFEM v1.0 mesh file (Mesh(path), mesh/mesh_readers.cpp:40-130)."""
 
-    def __init__(self, handle, nx, ny, nz):
-        self._h = handle
+    def __init__(self, handle_or_path, nx=0, ny=0, nz=0):
+        if isinstance(handle_or_path, (str, os.PathLike)):
+            with open(handle_or_path, "rb") as f:
+                text = f.read()
+            h = C.c_void_p()
+            _check(lib().h1pa_mesh_load(text, len(text), C.byref(h)))
+            handle_or_path = h
+        self._h = handle_or_path
         self.nx, self.ny, self.nz = nx, ny, nz
+
+    @staticmethod
+    def LoadFromFile(path):
+        return Mesh(path)
 
     @staticmethod
     def MakeCartesian3D(nx, ny, nz, type=Element.HEXAHEDRON, sx=1.0, sy=1.0, sz=1.0,

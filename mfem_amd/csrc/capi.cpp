@@ -76,6 +76,15 @@ int h1pa_mesh_create_cartesian(int nx, int ny, int nz, int sfc, h1pa_mesh **mesh
     API_END
 }
 
+int h1pa_mesh_load(const char *text, int64_t len, h1pa_mesh **mesh) {
+    API_BEGIN
+    REQUIRE(text && mesh && len > 0, "mesh_load: null argument");
+    auto m = std::make_unique<h1pa_mesh>();
+    m->m.load_mfem_v10(text, (size_t)len);
+    *mesh = m.release();
+    API_END
+}
+
 int h1pa_mesh_destroy(h1pa_mesh *mesh) {
     API_BEGIN
     delete mesh;

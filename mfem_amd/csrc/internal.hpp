@@ -95,9 +95,14 @@ struct Mesh {
     double curve_amp = 0.0;
     DevBuf<double> d_nodes;
     bool nodes_valid = false;
-    DevBuf<int> d_topo;   // [ne][29] vertex, edge, face ids + grid coordinates
+    std::vector<uint8_t> bface;              // [ne] bit j: local face j is on the boundary
+    bool from_file = false;                  // read from an MFEM mesh file (no grid)
+    DevBuf<int> d_topo;   // [ne][29] vertex, edge, face ids, boundary-face bits, 2 unused
     DevBuf<int8_t> d_ori; // [ne][12+6] edge (+-1) and face (0..7) orientations
     void build_topology(bool sfc);
+    // ReadMFEMMesh for "MFEM mesh v1.0" hexahedral text; sets ev, xyz and the
+    // topology (first-encounter edges/faces, boundary from the file's list).
+    void load_mfem_v10(const char *text, size_t len);
     void ensure_device(hipStream_t s);
     void ensure_nodes(hipStream_t s);
 };

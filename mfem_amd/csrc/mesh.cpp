@@ -1,5 +1,6 @@
-// mesh.cpp -- Cartesian hexahedral mesh with MFEM's element, edge and face
-// numbering, built in O(NE) with direct-indexed structured arrays.
+// mesh.cpp -- hexahedral meshes with MFEM's element, edge and face numbering:
+// Cartesian meshes built in O(NE) with direct-indexed structured arrays, and
+// meshes read from MFEM's v1.0 mesh files with hash tables.
 //
 // MFEM numbers edges and faces by first encounter while it walks the elements
 // in order and each element's local edges/faces in Geometry::Constants<CUBE>
@@ -8,8 +9,12 @@
 // numbering falls out of one pass with an "already numbered?" array instead of
 // hash tables.
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
+#include <sstream>
+#include <string>
+#include <unordered_map>
 
 #include "internal.hpp"
 
@@ -199,6 +204,14 @@ void Mesh::build_topology(bool sfc) {
         }
     }
     nfaces = next;
+    bface.assign(ne, 0);
+    const int n_axis[3] = {nx, ny, nz};
+    for (int64_t e = 0; e < ne; ++e)
+        for (int j = 0; j < 6; ++j) {
+            const int a = kFaceAxis[j], c = ecoord[3 * e + a];
+            if ((kFaceOff[j] == 0 && c == 0) || (kFaceOff[j] == 1 && c == n_axis[a] - 1))
+                bface[e] |= (uint8_t)(1u << j);
+        }
     xyz.resize(3 * nv);
     int64_t v = 0;
     for (int z = 0; z <= nz; ++z)
@@ -208,6 +221,190 @@ void Mesh::build_topology(bool sfc) {
                 xyz[3 * v + 1] = (double)y / ny;
                 xyz[3 * v + 2] = (double)z / nz;
             }
+}
+
+// ---- MFEM mesh files ----------------------------------------------------------
+namespace {
+
+// Whitespace tokens of an MFEM mesh text; '#' starts a comment to the end of
+// the line (skip_comment_lines in mesh/mesh_readers.cpp).
+struct Tokens {
+    std::istringstream in;
+    explicit Tokens(const std::string &t) : in(t) {}
+    std::string next() {
+        std::string w;
+        while (in >> w) {
+            if (w[0] != '#') return w;
+            std::string rest;
+            std::getline(in, rest);
+        }
+        throw Error{H1PA_EINVAL, "mesh file: unexpected end of input"};
+    }
+    long long integer(const char *what) {
+        const std::string w = next();
+        char *end = nullptr;
+        const long long v = std::strtoll(w.c_str(), &end, 10);
+        H1PA_CHECK(end && *end == '\0', H1PA_EINVAL, std::string("mesh file: bad ") + what);
+        return v;
+    }
+    double real() {
+        const std::string w = next();
+        char *end = nullptr;
+        const double v = std::strtod(w.c_str(), &end);
+        H1PA_CHECK(end && *end == '\0', H1PA_EINVAL, "mesh file: bad vertex coordinate");
+        return v;
+    }
+    void expect(const char *word) {
+        H1PA_CHECK(next() == word, H1PA_EINVAL, std::string("mesh file: expected '") + word + "'");
+    }
+};
+
+struct QuadKey {
+    std::array<int, 4> v;
+    bool operator==(const QuadKey &o) const { return v == o.v; }
+};
+struct QuadHash {
+    size_t operator()(const QuadKey &k) const {
+        uint64_t h = 1469598103934665603ULL;
+        for (int x : k.v) h = (h ^ (uint64_t)(uint32_t)x) * 1099511628211ULL;
+        return (size_t)h;
+    }
+};
+QuadKey quad_key(int a, int b, int c, int d) {
+    QuadKey k{{a, b, c, d}};
+    std::sort(k.v.begin(), k.v.end());
+    return k;
+}
+
+}  // namespace
+
+// Mesh::ReadMFEMMesh (mesh/mesh_readers.cpp:40-130), version 1.0, for
+// conforming meshes of hexahedra (geometry 5) with a boundary of quadrilaterals
+// (geometry 3) and straight vertex coordinates; a high-order "nodes" section,
+// other element types and other dimensions are rejected.  The elements keep
+// the file order.  Topology as FinalizeTopology makes it: faces and edges
+// numbered by first encounter over the elements in order and their local
+// faces/edges in Geometry::Constants<CUBE> order (STable3D / DSTable pushes,
+// mesh/mesh.cpp:8526-8549, :8996-9080), face orientation from the first
+// element's vertex cycle (GetQuadOrientation), edge orientation from the
+// vertex order.  The essential boundary is every face of the file's boundary
+// list (all attributes marked, as GetBoundaryTrueDofs does).
+void Mesh::load_mfem_v10(const char *text, size_t len) {
+    const std::string t(text, len);
+    {
+        std::istringstream first(t);
+        std::string line;
+        std::getline(first, line);
+        while (!line.empty() && (line.back() == '\r' || line.back() == ' ')) line.pop_back();
+        H1PA_CHECK(line == "MFEM mesh v1.0", H1PA_EUNSUP,
+                   "mesh file: only 'MFEM mesh v1.0' is supported");
+    }
+    Tokens tk(t.substr(t.find('\n') + 1));
+    tk.expect("dimension");
+    H1PA_CHECK(tk.integer("dimension") == 3, H1PA_EUNSUP, "mesh file: only 3D meshes are supported");
+    tk.expect("elements");
+    const long long n_el = tk.integer("element count");
+    H1PA_CHECK(n_el >= 1 && n_el < (1LL << 28), H1PA_EINVAL, "mesh file: bad element count");
+    ev.resize(8 * (size_t)n_el);
+    for (long long e = 0; e < n_el; ++e) {
+        (void)tk.integer("attribute");
+        H1PA_CHECK(tk.integer("geometry") == 5, H1PA_EUNSUP,
+                   "mesh file: only hexahedral elements (geometry 5) are supported");
+        for (int k = 0; k < 8; ++k) ev[8 * e + k] = (int)tk.integer("vertex index");
+    }
+    tk.expect("boundary");
+    const long long n_bdr = tk.integer("boundary count");
+    H1PA_CHECK(n_bdr >= 0 && n_bdr <= 6 * n_el, H1PA_EINVAL, "mesh file: bad boundary count");
+    std::vector<int> bq(4 * (size_t)n_bdr);
+    for (long long b = 0; b < n_bdr; ++b) {
+        (void)tk.integer("attribute");
+        H1PA_CHECK(tk.integer("geometry") == 3, H1PA_EUNSUP,
+                   "mesh file: only quadrilateral boundary elements (geometry 3) are supported");
+        for (int k = 0; k < 4; ++k) bq[4 * b + k] = (int)tk.integer("vertex index");
+    }
+    tk.expect("vertices");
+    const long long n_v = tk.integer("vertex count");
+    H1PA_CHECK(n_v >= 8 && n_v < (1LL << 31), H1PA_EINVAL, "mesh file: bad vertex count");
+    const std::string sd = tk.next();
+    H1PA_CHECK(sd != "nodes", H1PA_EUNSUP,
+               "mesh file: curved meshes (a 'nodes' section) are not supported");
+    H1PA_CHECK(sd == "3", H1PA_EUNSUP, "mesh file: vertices must have 3 coordinates");
+    xyz.resize(3 * (size_t)n_v);
+    for (auto &c : xyz) c = tk.real();
+    for (int v : ev) H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
+    for (int v : bq) H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
+    nx = ny = nz = 0;
+    nv = n_v;
+    ne = n_el;
+    from_file = true;
+    ecoord.assign(3 * ne, 0);
+    // an inverted element would flip the sign of every derivative: refuse it
+    // (trilinear Jacobian at the element centre)
+    for (int64_t e = 0; e < ne; ++e) {
+        double J[3][3] = {};
+        static const int sgn[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                      {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+        for (int k = 0; k < 8; ++k)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c) J[r][c] += 0.25 * sgn[k][c] * xyz[3 * ev[8 * e + k] + r];
+        const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                           J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        H1PA_CHECK(det > 0.0, H1PA_EUNSUP, "mesh file: inverted or degenerate element " +
+                                               std::to_string(e));
+    }
+    // faces
+    ef.resize(6 * ne);
+    efo.resize(6 * ne);
+    std::unordered_map<QuadKey, int, QuadHash> fmap;
+    fmap.reserve(4 * ne);
+    std::vector<std::array<int, 4>> fverts;  // first element's vertex cycle of each face
+    std::vector<int> fcount;
+    for (int64_t e = 0; e < ne; ++e)
+        for (int j = 0; j < 6; ++j) {
+            const int *cyc = kFaceVerts[j];
+            std::array<int, 4> vv;
+            for (int k = 0; k < 4; ++k) vv[k] = ev[8 * e + cyc[k]];
+            auto it = fmap.emplace(quad_key(vv[0], vv[1], vv[2], vv[3]), (int)fverts.size());
+            const int id = it.first->second;
+            if (it.second) {
+                fverts.push_back(vv);
+                fcount.push_back(1);
+                efo[6 * e + j] = 0;
+            } else {
+                H1PA_CHECK(++fcount[id] <= 2, H1PA_EINVAL, "mesh file: face shared by 3 elements");
+                efo[6 * e + j] = quad_orientation(fverts[id].data(), vv.data());
+            }
+            ef[6 * e + j] = id;
+        }
+    nfaces = (int64_t)fverts.size();
+    // edges
+    ee.resize(12 * ne);
+    eeo.resize(12 * ne);
+    std::unordered_map<uint64_t, int> emap;
+    emap.reserve(4 * ne);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int j = 0; j < 12; ++j) {
+            const int va = ev[8 * e + kEdgeVerts[j][0]], vb = ev[8 * e + kEdgeVerts[j][1]];
+            const uint64_t key = (uint64_t)std::min(va, vb) * (uint64_t)nv + (uint64_t)std::max(va, vb);
+            auto it = emap.emplace(key, (int)emap.size());
+            ee[12 * e + j] = it.first->second;
+            eeo[12 * e + j] = va < vb ? 1 : -1;
+        }
+    nedges = (int64_t)emap.size();
+    // boundary faces: the file's list
+    std::vector<uint8_t> on(nfaces, 0);
+    for (long long b = 0; b < n_bdr; ++b) {
+        auto it = fmap.find(quad_key(bq[4 * b], bq[4 * b + 1], bq[4 * b + 2], bq[4 * b + 3]));
+        H1PA_CHECK(it != fmap.end(), H1PA_EINVAL,
+                   "mesh file: boundary element " + std::to_string(b) + " is not an element face");
+        on[it->second] = 1;
+    }
+    bface.assign(ne, 0);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int j = 0; j < 6; ++j)
+            if (on[ef[6 * e + j]]) bface[e] |= (uint8_t)(1u << j);
+    nodes_valid = false;
 }
 
 void Mesh::ensure_device(hipStream_t s) {
@@ -226,7 +423,8 @@ void Mesh::ensure_device(hipStream_t s) {
             t[20 + k] = ef[6 * e + k];
             ori[18 * e + 12 + k] = (int8_t)efo[6 * e + k];
         }
-        for (int k = 0; k < 3; ++k) t[26 + k] = ecoord[3 * e + k];
+        t[26] = bface[e];
+        t[27] = t[28] = 0;
     }
     d_topo.upload(topo.data(), topo.size(), s);
     d_ori.upload(ori.data(), ori.size(), s);
@@ -255,6 +453,7 @@ inline uint64_t splitmix64(uint64_t x) {
 }  // namespace
 
 void perturb_vertices(Mesh &m, double amp, uint64_t seed) {
+    H1PA_CHECK(!m.from_file, H1PA_EUNSUP, "perturb: only Cartesian meshes can be perturbed");
     // Interior vertices move by U(-amp h, amp h) per coordinate; the uniform
     // draw for coordinate c of vertex v is splitmix64(seed ^ splitmix64(3v+c)).
     const double h[3] = {1.0 / m.nx, 1.0 / m.ny, 1.0 / m.nz};

@@ -190,19 +190,20 @@ void inclusive_scan(int *a, int64_t n, hipStream_t s) {
     H1PA_HIP(hipStreamSynchronize(s));  // `sums` is freed on return
 }
 
-__global__ void k_boundary_mark(int64_t ne, int p, int nx, int ny, int nz,
-                                const int *__restrict__ topo, const int *__restrict__ gmap,
-                                uint8_t *__restrict__ mask) {
+// DOFs on the element's boundary faces (topo[26]: bit j set when local face j
+// -- z = 0, y = 0, x = p, y = p, x = 0, z = p in Geometry::Constants<CUBE>
+// order -- is a boundary face).
+__global__ void k_boundary_mark(int64_t ne, int p, const int *__restrict__ topo,
+                                const int *__restrict__ gmap, uint8_t *__restrict__ mask) {
     const int D = p + 1, dof = D * D * D;
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= ne * dof) return;
     const int64_t e = i / dof;
     const int lex = (int)(i - e * dof);
     const int a = lex % D, b = (lex / D) % D, c = lex / (D * D);
-    const int *t = topo + 29 * e;
-    const int x = t[26], y = t[27], z = t[28];
-    const bool on = (x == 0 && a == 0) || (x == nx - 1 && a == p) || (y == 0 && b == 0) ||
-                    (y == ny - 1 && b == p) || (z == 0 && c == 0) || (z == nz - 1 && c == p);
+    const int f = topo[29 * e + 26];
+    const bool on = ((f & 1) && c == 0) || ((f & 2) && b == 0) || ((f & 4) && a == p) ||
+                    ((f & 8) && b == p) || ((f & 16) && a == 0) || ((f & 32) && c == p);
     if (on) {
         const int g = gmap[i];
         mask[g >= 0 ? g : -1 - g] = 1;
@@ -359,8 +360,8 @@ void boundary_mask(const Mesh &m, int p, const int *gmap, uint8_t *mask, int64_t
     const int D = p + 1;
     const int64_t n = m.ne * D * D * D;
     H1PA_HIP(hipMemsetAsync(mask, 0, ndofs, s));
-    hipLaunchKernelGGL(k_boundary_mark, dim3(nblocks(n)), dim3(kBlock), 0, s, m.ne, p, m.nx, m.ny,
-                       m.nz, m.d_topo.p, gmap, mask);
+    hipLaunchKernelGGL(k_boundary_mark, dim3(nblocks(n)), dim3(kBlock), 0, s, m.ne, p,
+                       m.d_topo.p, gmap, mask);
     H1PA_HIP(hipGetLastError());
 }
 

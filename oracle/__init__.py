@@ -31,7 +31,8 @@ _pd = C.POINTER(C.c_double)
 class _Mesh(C.Structure):
     _fields_ = [("nx", _i), ("ny", _i), ("nz", _i), ("nv", _i), ("ne", _i),
                 ("nedges", _i), ("nfaces", _i), ("ev", _pi), ("ee", _pi),
-                ("eeo", _pi), ("ef", _pi), ("efo", _pi), ("f_elem1", _pi)]
+                ("eeo", _pi), ("ef", _pi), ("efo", _pi), ("f_elem1", _pi),
+                ("f_bdr", C.c_void_p)]
 
 
 class _PaOp(C.Structure):
@@ -58,6 +59,8 @@ def lib():
         L = _lib
         L.orc_mesh_cartesian.argtypes = [_i, _i, _i, _i, C.POINTER(_Mesh)]
         L.orc_mesh_free.argtypes = [C.POINTER(_Mesh)]
+        L.orc_mesh_from_elements.argtypes = [_i, _i, _pi, _i, _pi, C.POINTER(_Mesh)]
+        L.orc_mesh_from_elements.restype = _i
         L.orc_grid_sfc3d.argtypes = [_i, _i, _i, _pi]
         L.orc_mesh_vertices.argtypes = [_i, _i, _i, _pd]
         L.orc_perturb_vertices.argtypes = [_i, _i, _i, _d, C.c_ulonglong, _pd]
@@ -130,12 +133,68 @@ def grid_sfc3d(nx, ny, nz):
     return c.reshape(-1, 3)
 
 
-class CartesianMesh:
-    """Mesh::MakeCartesian3D(nx, ny, nz, HEXAHEDRON, 1, 1, 1, sfc) topology."""
+def read_mfem_mesh(text):
+    """Mesh::ReadMFEMMesh (mesh/mesh_readers.cpp:40-130) restated for
+    'MFEM mesh v1.0' text of hexahedra with a quadrilateral boundary and
+    straight vertices: returns (ev [ne, 8], bquads [nb, 4], xyz [nv, 3])."""
+    lines = text.splitlines()
+    if not lines or lines[0].strip() != "MFEM mesh v1.0":
+        raise ValueError("not an MFEM mesh v1.0 file")
+    toks = []
+    for ln in lines[1:]:
+        ln = ln.split("#", 1)[0]
+        toks.extend(ln.split())
+    it = iter(toks)
 
-    def __init__(self, nx, ny, nz, sfc=True):
+    def word(expect=None):
+        w = next(it)
+        if expect is not None and w != expect:
+            raise ValueError(f"expected {expect!r}, got {w!r}")
+        return w
+
+    word("dimension")
+    if int(word()) != 3:
+        raise ValueError("only 3D meshes")
+    word("elements")
+    ne = int(word())
+    ev = np.zeros((ne, 8), np.int32)
+    for e in range(ne):
+        word()  # attribute
+        if int(word()) != 5:
+            raise ValueError("only hexahedra")
+        ev[e] = [int(word()) for _ in range(8)]
+    word("boundary")
+    nb = int(word())
+    bq = np.zeros((nb, 4), np.int32)
+    for b in range(nb):
+        word()
+        if int(word()) != 3:
+            raise ValueError("only quadrilateral boundary elements")
+        bq[b] = [int(word()) for _ in range(4)]
+    word("vertices")
+    nv = int(word())
+    sd = word()
+    if sd != "3":
+        raise ValueError("only straight 3D vertices")
+    xyz = np.array([float(word()) for _ in range(3 * nv)]).reshape(nv, 3)
+    return ev, bq, xyz
+
+
+class CartesianMesh:
+    """Mesh::MakeCartesian3D(nx, ny, nz, HEXAHEDRON, 1, 1, 1, sfc) topology, or
+    (elements=(ev, bquads, nv)) a mesh read from a file."""
+
+    def __init__(self, nx, ny, nz, sfc=True, elements=None):
         m = _Mesh()
-        if lib().orc_mesh_cartesian(nx, ny, nz, 1 if sfc else 0, C.byref(m)) != 0:
+        if elements is not None:
+            ev, bq, nv = elements
+            ev = np.ascontiguousarray(ev, dtype=np.int32).ravel()
+            bq = np.ascontiguousarray(bq, dtype=np.int32).ravel()
+            rc = lib().orc_mesh_from_elements(int(nv), ev.size // 8, _ip(ev), bq.size // 4,
+                                              _ip(bq), C.byref(m))
+            if rc != 0:
+                raise ValueError(f"orc_mesh_from_elements: {rc}")
+        elif lib().orc_mesh_cartesian(nx, ny, nz, 1 if sfc else 0, C.byref(m)) != 0:
             raise MemoryError("orc_mesh_cartesian")
         self.nx, self.ny, self.nz = nx, ny, nz
         self.nv, self.ne = m.nv, m.ne
@@ -416,14 +475,21 @@ class Problem:
     """
 
     def __init__(self, n, p, Q=None, perturb=0.0, seed=0, curve=0.0, coeff="one",
-                 integrators=("diffusion",), sfc=True, dims=None):
-        nx, ny, nz = dims if dims is not None else (n, n, n)
+                 integrators=("diffusion",), sfc=True, dims=None, mesh_text=None):
         self.p, self.D = p, p + 1
         self.Q = Q if Q is not None else p + 2
-        self.mesh = CartesianMesh(nx, ny, nz, sfc=sfc)
-        m = self.mesh
-        self.ne = m.ne
-        self.xyz = m.vertices(perturb, seed)
+        if mesh_text is not None:  # a mesh file's text
+            ev, bq, xyz = read_mfem_mesh(mesh_text)
+            self.mesh = CartesianMesh(0, 0, 0, elements=(ev, bq, xyz.shape[0]))
+            m = self.mesh
+            self.ne = m.ne
+            self.xyz = xyz
+        else:
+            nx, ny, nz = dims if dims is not None else (n, n, n)
+            self.mesh = CartesianMesh(nx, ny, nz, sfc=sfc)
+            m = self.mesh
+            self.ne = m.ne
+            self.xyz = m.vertices(perturb, seed)
         self.elem_dof, self.ndofs = m.elem_dofs(p)
         self.dof_map = dof_map(p)
         self.gather_map, self.offsets, self.indices = restriction(self.elem_dof, self.dof_map,

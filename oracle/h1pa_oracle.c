@@ -247,13 +247,15 @@ static int quad_orientation(const int *base, const int *test)
     return 2 * i + 1;
 }
 
+static int orc_mesh_topology(orc_mesh *m);
+
 int orc_mesh_cartesian(int nx, int ny, int nz, int sfc, orc_mesh *m)
 {
     memset(m, 0, sizeof(*m));
     m->nx = nx; m->ny = ny; m->nz = nz;
     m->nv = (nx + 1) * (ny + 1) * (nz + 1);
     m->ne = nx * ny * nz;
-    const int ne = m->ne, nv = m->nv;
+    const int ne = m->ne;
     m->ev = (int *)malloc(sizeof(int) * 8 * (size_t)ne);
     m->ee = (int *)malloc(sizeof(int) * 12 * (size_t)ne);
     m->eeo = (int *)malloc(sizeof(int) * 12 * (size_t)ne);
@@ -289,7 +291,13 @@ int orc_mesh_cartesian(int nx, int ny, int nz, int sfc, orc_mesh *m)
                 }
     }
 #undef VTX
+    return orc_mesh_topology(m);
+}
 
+/* Faces and edges of the elements in m->ev (m->nv, m->ne set). */
+static int orc_mesh_topology(orc_mesh *m)
+{
+    const int ne = m->ne, nv = m->nv;
     /* Faces first (FinalizeTopology, mesh/mesh.cpp:3700-3703):
      * GetElementToFaceTable (:8996-9080) then GenerateFaces (:8768+). */
     {
@@ -364,9 +372,72 @@ int orc_mesh_cartesian(int nx, int ny, int nz, int sfc, orc_mesh *m)
     return 0;
 }
 
+static int cmp_int4(const void *a, const void *b)
+{
+    const int *x = (const int *)a, *y = (const int *)b;
+    for (int k = 0; k < 4; k++)
+        if (x[k] != y[k]) return x[k] < y[k] ? -1 : 1;
+    return 0;
+}
+
+static void sort4(int *v)
+{
+    for (int i = 1; i < 4; i++)
+        for (int j = i; j > 0 && v[j - 1] > v[j]; j--) {
+            const int t = v[j]; v[j] = v[j - 1]; v[j - 1] = t;
+        }
+}
+
+int orc_mesh_from_elements(int nv, int ne, const int *ev, int nb, const int *bquads,
+                           orc_mesh *m)
+{
+    memset(m, 0, sizeof(*m));
+    m->nv = nv;
+    m->ne = ne;
+    m->ev = (int *)malloc(sizeof(int) * 8 * (size_t)ne);
+    m->ee = (int *)malloc(sizeof(int) * 12 * (size_t)ne);
+    m->eeo = (int *)malloc(sizeof(int) * 12 * (size_t)ne);
+    m->ef = (int *)malloc(sizeof(int) * 6 * (size_t)ne);
+    m->efo = (int *)malloc(sizeof(int) * 6 * (size_t)ne);
+    if (!m->ev || !m->ee || !m->eeo || !m->ef || !m->efo) return -1;
+    memcpy(m->ev, ev, sizeof(int) * 8 * (size_t)ne);
+    if (orc_mesh_topology(m) != 0) return -1;
+    /* boundary: the faces listed by the file (GetBoundaryTrueDofs marks every
+     * boundary attribute); each face's sorted vertex set, looked up by bisection */
+    const int nf = m->nfaces;
+    int *key = (int *)malloc(sizeof(int) * 5 * (size_t)nf);
+    m->f_bdr = (char *)calloc((size_t)nf, 1);
+    if (!key || !m->f_bdr) return -1;
+    for (int i = 0; i < ne; i++)
+        for (int j = 0; j < 6; j++) {
+            const int f = m->ef[6 * (size_t)i + j];
+            int *k = key + 5 * (size_t)f;
+            for (int c = 0; c < 4; c++) k[c] = ev[8 * (size_t)i + HEX_FACES[j][c]];
+            sort4(k);
+            k[4] = f;
+        }
+    qsort(key, (size_t)nf, sizeof(int) * 5, cmp_int4);
+    int rc = 0;
+    for (int b = 0; b < nb && rc == 0; b++) {
+        int q[4] = {bquads[4 * b], bquads[4 * b + 1], bquads[4 * b + 2], bquads[4 * b + 3]};
+        sort4(q);
+        int lo = 0, hi = nf - 1, found = -1;
+        while (lo <= hi) {
+            const int mid = (lo + hi) / 2, c = cmp_int4(q, key + 5 * (size_t)mid);
+            if (c == 0) { found = key[5 * (size_t)mid + 4]; break; }
+            if (c < 0) hi = mid - 1; else lo = mid + 1;
+        }
+        if (found < 0) rc = -2;
+        else m->f_bdr[found] = 1;
+    }
+    free(key);
+    return rc;
+}
+
 void orc_mesh_free(orc_mesh *m)
 {
     free(m->ev); free(m->ee); free(m->eeo); free(m->ef); free(m->efo); free(m->f_elem1);
+    free(m->f_bdr);
     memset(m, 0, sizeof(*m));
 }
 
@@ -477,7 +548,21 @@ int orc_h1_boundary_dofs(const orc_mesh *m, int p, const int *elem_dof, int *ess
         if (elem_dof[e] + 1 > ndofs) ndofs = elem_dof[e] + 1;
     char *mark = (char *)calloc((size_t)ndofs, 1);
     const int nx = m->nx, ny = m->ny, nz = m->nz;
-    for (long e = 0; e < m->ne; e++) {
+    /* file meshes: DOFs of the listed boundary faces; local face j of the
+     * hex is z = 0, y = 0, x = p, y = p, x = 0, z = p (fem/geom.cpp:1030-1035) */
+    for (long e = 0; m->f_bdr && e < m->ne; e++)
+        for (int j = 0; j < 6; j++) {
+            if (!m->f_bdr[m->ef[6 * e + j]]) continue;
+            for (int k = 0; k < p1; k++)
+                for (int jj = 0; jj < p1; jj++)
+                    for (int i = 0; i < p1; i++) {
+                        const int on = (j == 0 && k == 0) || (j == 1 && jj == 0) ||
+                                       (j == 2 && i == p) || (j == 3 && jj == p) ||
+                                       (j == 4 && i == 0) || (j == 5 && k == p);
+                        if (on) mark[elem_dof[e * nd + dof_map[i + p1 * (jj + p1 * k)]]] = 1;
+                    }
+        }
+    for (long e = 0; !m->f_bdr && e < m->ne; e++) {
         const int v0 = m->ev[8 * e];
         const int x = v0 % (nx + 1), y = (v0 / (nx + 1)) % (ny + 1), z = v0 / ((nx + 1) * (ny + 1));
         for (int k = 0; k < p1; k++)

@@ -50,9 +50,17 @@ typedef struct {
     int *ef;    /* [ne][6]   element faces                             */
     int *efo;   /* [ne][6]   face orientation 0..7                     */
     int *f_elem1; /* [nfaces] first element that met the face            */
+    char *f_bdr;  /* [nfaces] 1 on boundary faces (file meshes; NULL for
+                     Cartesian meshes, whose boundary is the cube's)     */
 } orc_mesh;
 
 int  orc_mesh_cartesian(int nx, int ny, int nz, int sfc, orc_mesh *m);
+/* A mesh given by its hexahedra (MFEM vertex order, file order kept) and its
+ * boundary quadrilaterals, as Mesh::ReadMFEMMesh + FinalizeTopology build it
+ * (mesh/mesh_readers.cpp:40-130); nx = ny = nz = 0.  Returns -1 when out of
+ * memory, -2 when a boundary quadrilateral is not an element face. */
+int  orc_mesh_from_elements(int nv, int ne, const int *ev, int nb, const int *bquads,
+                            orc_mesh *m);
 void orc_mesh_free(orc_mesh *m);
 /* Unit-cube vertex coordinates, x fastest, [nv][3] (mesh/mesh.cpp:3870-3882). */
 void orc_mesh_vertices(int nx, int ny, int nz, double *xyz);
