@@ -34,8 +34,8 @@ namespace kern {
 
 namespace {
 
-template <typename T, typename TP, int D, int Q, int NB, int U, int K>
-__global__ void __launch_bounds__(NB * Q * Q)
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, int W>
+__global__ void __launch_bounds__(NB * Q * Q, W)
 k_diffusion_apply(int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
                   const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
                   const T *__restrict__ x, const EOutT<T> out, int accumulate) {
@@ -211,20 +211,23 @@ k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restri
 }
 
 // ---- dispatch over (D, Q) and variants -------------------------------------
-template <typename T, typename TP, int D, int Q, int NB, int U, int K>
+// W: minimum waves per SIMD the compiler must allow (1: no bound; the
+// occupancy-capped variants pass lds_waves()).
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, int W = 1>
 void launch_diffusion_v(int64_t ne, const T *basis, const TP *pa, const int *gmap,
                         const uint8_t *mask, const T *x, const EOutT<T> &out, bool acc,
                         hipStream_t s) {
     const unsigned grid = (unsigned)((ne + NB - 1) / NB);
-    hipLaunchKernelGGL((k_diffusion_apply<T, TP, D, Q, NB, U, K>), dim3(grid), dim3(NB * Q * Q), 0,
-                       s, ne, basis, pa, gmap, mask, x, out, acc ? 1 : 0);
+    hipLaunchKernelGGL((k_diffusion_apply<T, TP, D, Q, NB, U, K, W>), dim3(grid), dim3(NB * Q * Q),
+                       0, s, ne, basis, pa, gmap, mask, x, out, acc ? 1 : 0);
 }
 
 // Variants of the diffusion apply: (elements per block, line-loop unroll,
-// PA-data pipeline depth: 0, 2 planes or all of them).  Every variant performs the same arithmetic per element
-// in the same order, so they give bit-identical results and differ only in
-// speed.
-constexpr int kDiffVariants = 6;
+// PA-data pipeline depth: 0, 2 planes or all of them; 6 and 7 are 0 and 4
+// with the registers capped so that as many blocks stay resident as the LDS
+// allows).  Every variant performs the same arithmetic per element in the
+// same order, so they give bit-identical results and differ only in speed.
+constexpr int kDiffVariants = 8;
 template <int D, int Q>
 void launch_diffusion(int variant, int64_t ne, const double *basis, const double *pa,
                       const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
@@ -239,6 +242,8 @@ void launch_diffusion(int variant, int64_t ne, const double *basis, const double
         case 3: launch_diffusion_v<T, TP, D, Q, NBhalf, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         case 4: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 2>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         case 5: launch_diffusion_v<T, TP, D, Q, NBmax, 1, kAllPlanes>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 6: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 0, lds_waves<T, D, Q, NBmax>()>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 7: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 2, lds_waves<T, D, Q, NBmax>()>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         default: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
     }
 }
@@ -265,6 +270,12 @@ void launch_diffusion_lp(int variant, int64_t ne, const float *basis, const TP *
     else if (variant == 5)
         launch_diffusion_v<float, TP, D, Q, NBmax, 1, kAllPlanes>(ne, basis, pa, gmap, mask, x,
                                                                   out, acc, s);
+    else if (variant == 6)
+        launch_diffusion_v<float, TP, D, Q, NBmax, 1, 0, lds_waves<float, D, Q, NBmax>()>(
+            ne, basis, pa, gmap, mask, x, out, acc, s);
+    else if (variant == 7)
+        launch_diffusion_v<float, TP, D, Q, NBmax, 1, 2, lds_waves<float, D, Q, NBmax>()>(
+            ne, basis, pa, gmap, mask, x, out, acc, s);
     else
         launch_diffusion_v<float, TP, D, Q, NBmax, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s);
 }

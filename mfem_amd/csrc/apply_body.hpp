@@ -206,6 +206,20 @@ constexpr int diffusion_buf() {
     return (2 * D * D * Q > 3 * D * Q * Q) ? 2 * D * D * Q : 3 * D * Q * Q;
 }
 
+// Waves per SIMD that the LDS of an NB-element diffusion block allows: a CU's
+// 160 KiB holds floor(160 KiB / block LDS) blocks, each putting ceil(waves/4)
+// waves on every SIMD.  Used as the minimum-waves launch bound of the
+// occupancy-capped variants, so that registers, not LDS, stop limiting the
+// resident blocks (the compiler then keeps the VGPR count within 512 / W).
+template <typename T, int D, int Q, int NB>
+constexpr int lds_waves() {
+    const int lds = NB * diffusion_buf<D, Q>() * (int)sizeof(T);
+    const int blocks = (160 * 1024) / lds;
+    const int per_simd = ((NB * Q * Q + 63) / 64 + 3) / 4;
+    const int w = blocks * per_simd;
+    return w > 8 ? 8 : (w < 1 ? 1 : w);
+}
+
 // The elements e0 + (threadIdx.x / Q^2), NB per work-group, of one problem.
 template <typename T, typename TP, int D, int Q, int NB, int U, int K>
 __device__ __forceinline__ void

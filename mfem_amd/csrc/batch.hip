@@ -316,8 +316,8 @@ kbm_fin_init(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
 // Element kernel over a group's work list.  RU = true: the FP64 residual
 // apply (x -> ye64) of mixed slots in PH_RU; otherwise d -> ye for slots in
 // PH_INNER.  force = 1 ignores the phase (timing only).
-template <typename T, typename TP, int D, int Q, int NB, int U, int K, bool RU>
-__global__ void __launch_bounds__(NB * Q * Q)
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, bool RU, int W>
+__global__ void __launch_bounds__(NB * Q * Q, W)
 kb_diffusion(const ElemTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
              const SlotState *__restrict__ states, const T *__restrict__ basis, int force) {
     const ElemTask tk = tasks[blockIdx.x];
@@ -612,10 +612,10 @@ kbm_fin_ru(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
 }
 
 // ---- element-kernel dispatch over (D, Q, variant) --------------------------
-template <typename T, typename TP, int D, int Q, int NB, int U, int K, bool RU>
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, bool RU, int W = 1>
 void launch_kb(const ElemTask *tasks, int ntasks, const SlotDesc *desc, const SlotState *st,
                const T *basis, int force, hipStream_t s) {
-    hipLaunchKernelGGL((kb_diffusion<T, TP, D, Q, NB, U, K, RU>), dim3((unsigned)ntasks),
+    hipLaunchKernelGGL((kb_diffusion<T, TP, D, Q, NB, U, K, RU, W>), dim3((unsigned)ntasks),
                        dim3(NB * Q * Q), 0, s, tasks, desc, st, basis, force);
 }
 
@@ -633,6 +633,7 @@ template <typename T, typename TP, int D, int Q, bool RU>
 void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc *desc,
                        const SlotState *st, const T *basis, int force, hipStream_t s) {
     constexpr int NM = nb_max<T, D, Q>(), NH = nb_half<T, D, Q>();
+    constexpr int OW = lds_waves<T, D, Q, NM>();
     if constexpr (sizeof(T) == 8) {
         switch (v) {
             case 1: launch_kb<T, TP, D, Q, NM, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
@@ -640,6 +641,8 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
             case 3: launch_kb<T, TP, D, Q, NH, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 4: launch_kb<T, TP, D, Q, NM, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 5: launch_kb<T, TP, D, Q, NM, 1, kAllPlanes, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 6: launch_kb<T, TP, D, Q, NM, 1, 0, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 7: launch_kb<T, TP, D, Q, NM, 1, 2, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
             default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
         }
     } else {
@@ -648,6 +651,8 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
             case 1: launch_kb<T, TP, D, Q, NM, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 4: launch_kb<T, TP, D, Q, NM, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 5: launch_kb<T, TP, D, Q, NM, 1, kAllPlanes, RU>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 6: launch_kb<T, TP, D, Q, NM, 1, 0, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 7: launch_kb<T, TP, D, Q, NM, 1, 2, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
             default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
         }
     }
@@ -661,9 +666,9 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
 // ---------------------------------------------------------------------------
 namespace {
 
-// Variants valid for the inner type (FP32 applies have 0, 1, 4, 5).
+// Variants valid for the inner type (FP32 applies have 0, 1, 4, 5, 6, 7).
 bool variant_ok(int prec, int v) {
-    return prec == H1PA_PREC_FP64 ? (v >= 0 && v < 6) : (v == 0 || v == 1 || v == 4 || v == 5);
+    return prec == H1PA_PREC_FP64 ? (v >= 0 && v < 8) : (v == 0 || v == 1 || (v >= 4 && v < 8));
 }
 
 #define H1PA_BATCH_DQ(X) X(2, 3) X(3, 4) X(4, 5) X(5, 6) X(6, 7) X(7, 8) X(8, 9) X(9, 10)
@@ -898,7 +903,7 @@ void autotune(Batch &B) {
             G.tuned = true;
             float best = 1e30f;
             int bestv = G.variant;
-            for (int v = 0; v < 6; ++v) {
+            for (int v = 0; v < 8; ++v) {
                 if (!variant_ok(B.o.precision, v)) continue;
                 const int nb = nb_of_dq(G.p + 1, G.q1d, v, B.o.precision == H1PA_PREC_FP64);
                 element_tasks(B, (int)g, nb, et);

@@ -437,7 +437,10 @@ void autotune_diffusion(Form &f, hipStream_t s) {
     Space &sp = *f.space;
     const int p = sp.p, Q = f.q1d;
     if (const char *v = std::getenv("H1PA_DIFF_VARIANT")) {
-        kern::set_diffusion_variant(p, Q, std::atoi(v));
+        const int fv = std::atoi(v);
+        H1PA_CHECK(fv >= 0 && fv < kern::diffusion_variant_count(), H1PA_EINVAL,
+                   "H1PA_DIFF_VARIANT: no such diffusion-apply variant");
+        kern::set_diffusion_variant(p, Q, fv);
         return;
     }
     const char *at = std::getenv("H1PA_AUTOTUNE");
