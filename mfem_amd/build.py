@@ -18,6 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libh1pa.so")
 SERVE = os.path.join(HERE, "h1pa_serve")
+CEILINGS = os.path.join(HERE, "h1pa_ceilings")
 
 SOURCES = ["basis.cpp", "mesh.cpp", "capi.cpp", "restriction.hip", "geometry.hip",
            "apply.hip", "pcg.hip", "batch.hip", "qinterp.hip"]
@@ -64,6 +65,14 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"h1pa_serve build failed:\n{r.stderr}")
     os.replace(tmp, SERVE)
+    # the roofline probes the kernels are measured against (standalone HIP)
+    tmp = CEILINGS + ".tmp"
+    cmd = [_hipcc(), *[f for f in FLAGS if f != "-fPIC"], os.path.join(CSRC, "ceilings.hip"),
+           "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"h1pa_ceilings build failed:\n{r.stderr}")
+    os.replace(tmp, CEILINGS)
     if verbose:
         print(f"built {LIB} and {SERVE}")
     return LIB

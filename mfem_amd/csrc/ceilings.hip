@@ -1,0 +1,465 @@
+// ceilings.hip -- h1pa_ceilings: the MI355X limits the element kernels are
+// measured against, on the box itself.
+//
+//   1. Bandwidth of the FP64 diffusion apply's access pattern at p = 4
+//      (Q = 6, 7 elements per block, 79^3 elements): a contiguous read of the
+//      same bytes as its PA data, the PA read pattern alone, the PA read plus
+//      the E-vector write of 125 doubles per element, and that write alone.
+//   2. Instruction throughput: FP64 and FP32 (non-packed) vector FMA against
+//      the 16x16x4 FP64 and FP32 matrix-core instructions.
+//   3. The z-stage of the diffusion apply at p = 6 (D = 7, Q = 8; the three
+//      1D contractions along z, the pointwise 3x3 PA product and the three
+//      transposed contractions, per element) written twice: one thread per
+//      (qx, qy) column with vector FMAs, as the element kernels do, and one
+//      wave per element on 16x16x4 FP64 matrix-core tiles.  Both read their
+//      input from LDS and write their output to LDS on every repetition; the
+//      program checks that they agree before timing them.
+//
+// Output: one line per measurement (ms per launch, GB/s or TFLOP/s).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#define CK(x)                                                                        \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess) {                                                      \
+            std::fprintf(stderr, "HIP error %s at line %d\n", hipGetErrorString(e_), \
+                         __LINE__);                                                  \
+            std::exit(1);                                                            \
+        }                                                                            \
+    } while (0)
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr double kMagic = 1.2345e300;  // never produced: keeps results live
+
+// ---- 1. bandwidth -----------------------------------------------------------
+
+// 8 doubles per thread, grid-stride apart.
+template <bool NT>
+__global__ void __launch_bounds__(256) k_read(const double *__restrict__ x, int64_t n,
+                                              double *__restrict__ out) {
+    const int64_t T = (int64_t)gridDim.x * blockDim.x;
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int64_t j = i + k * T;
+        v[k] = j < n ? (NT ? __builtin_nontemporal_load(x + j) : x[j]) : 0.0;
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+    if (s == kMagic) out[0] = s;
+}
+
+// The PA read of the p = 4 apply: 7 elements of 36 threads per block, each
+// thread the 6 components of its (qx, qy) column on all 6 planes; with WRITE,
+// threads t < 25 then store 5 consecutive doubles of the element's 125.
+template <bool WRITE>
+__global__ void __launch_bounds__(256) k_pa_pattern(const double *__restrict__ pa, int64_t ne,
+                                                    double *__restrict__ ye) {
+    constexpr int Q = 6, QQ = 36, QQQ = 216, NB = 7;
+    const int le = threadIdx.x / QQ, t = threadIdx.x - le * QQ;
+    const int64_t e = (int64_t)blockIdx.x * NB + le;
+    if (le >= NB || e >= ne) return;
+    const double *p = pa + (int64_t)QQQ * 6 * e + t;
+    double r[6 * Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j)
+#pragma unroll
+        for (int c = 0; c < 6; ++c) r[6 * j + c] = p[QQQ * c + QQ * j];
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6 * Q; ++k) s += r[k];
+    if (WRITE) {
+        if (t < 25) {
+            double *o = ye + 125 * e + 5 * t;
+#pragma unroll
+            for (int d = 0; d < 5; ++d) o[d] = s + d;
+        }
+    } else if (s == kMagic) {
+        ye[e] = s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_write_pattern(int64_t ne, double *__restrict__ ye) {
+    const int le = threadIdx.x / 36, t = threadIdx.x - le * 36;
+    const int64_t e = (int64_t)blockIdx.x * 7 + le;
+    if (le >= 7 || e >= ne || t >= 25) return;
+    double *o = ye + 125 * e + 5 * t;
+#pragma unroll
+    for (int d = 0; d < 5; ++d) o[d] = 1.0 + d;
+}
+
+// ---- 2. instruction throughput ----------------------------------------------
+
+// 8 independent FMA chains per thread.
+template <typename T>
+__global__ void __launch_bounds__(256) k_valu(const T *__restrict__ in, T *__restrict__ out,
+                                              int iters) {
+    T a[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = in[(threadIdx.x + k) & 63];
+    const T m = in[64], c = in[65];
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = fma(a[k], m, c);
+    T s = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += a[k];
+    if (s == in[66]) out[0] = s;  // in[66] is negative, s is not
+}
+
+// 8 independent 16x16x4 accumulations per wave.
+__global__ void __launch_bounds__(256) k_mfma_f64(const double *__restrict__ in,
+                                                  double *__restrict__ out, int iters) {
+    const int lane = threadIdx.x & 63;
+    const double a = in[lane], b = in[67 + (lane & 31)];
+    v4d acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[k], 0, 0, 0);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    if (s == in[66]) out[0] = s;
+}
+
+__global__ void __launch_bounds__(256) k_mfma_f32(const float *__restrict__ in,
+                                                  float *__restrict__ out, int iters) {
+    const int lane = threadIdx.x & 63;
+    const float a = in[lane], b = in[67 + (lane & 31)];
+    v4f acc[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int it = 0; it < iters; ++it)
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[k], 0, 0, 0);
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += acc[k][0] + acc[k][1] + acc[k][2] + acc[k][3];
+    if (s == in[66]) out[0] = s;
+}
+
+// ---- 3. the z-stage at D = 7, Q = 8 -------------------------------------------
+//
+// Per element, with Z_c[dz][col] (c = 0, 1, 2; col = qx + 8 qy) in LDS:
+//   G_c[qz][col] = sum_dz M_c(qz, dz) Z_c[dz][col]      (M_0 = M_1 = B, M_2 = G)
+//   O = PA(qz, col) G                                     (symmetric 3x3, 6 values)
+//   T_c[dz][col] = sum_qz M_c(qz, dz) O_c[qz][col]
+//   Z_c[dz][col] += 1e-6 T_c[dz][col]                      (feeds the next repetition)
+// Basis matrices bm[0] = B, bm[1] = G as [qz][dz]; PA as pa[k][qz * 64 + col].
+constexpr int ZD = 7, ZQ = 8, ZC = 64;  // D, Q, columns (Q^2)
+constexpr int ZE = 4;                    // elements (waves) per block
+constexpr int ZS = 3 * 8 * ZC;           // LDS doubles per element: [c][dz (8)][col]
+
+__global__ void __launch_bounds__(64 * ZE) k_zstage_valu(const double *__restrict__ zin,
+                                                         const double *__restrict__ bm,
+                                                         const double *__restrict__ pa,
+                                                         double *__restrict__ tout, int64_t ne,
+                                                         int reps) {
+    __shared__ double zs[ZE * ZS];
+    const int w = threadIdx.x >> 6, col = threadIdx.x & 63;
+    const int64_t e = (int64_t)blockIdx.x * ZE + w;
+    double *z = zs + w * ZS;
+    const double *src = zin + (e % 4096) * ZS;
+    for (int i = col; i < ZS; i += 64) z[i] = src[i];
+    __syncthreads();
+    double t0[ZD], t1[ZD], t2[ZD];
+    for (int r = 0; r < reps; ++r) {
+        double z0[ZD], z1[ZD], z2[ZD];
+#pragma unroll
+        for (int d = 0; d < ZD; ++d) {
+            z0[d] = z[(0 * 8 + d) * ZC + col];
+            z1[d] = z[(1 * 8 + d) * ZC + col];
+            z2[d] = z[(2 * 8 + d) * ZC + col];
+            t0[d] = t1[d] = t2[d] = 0.0;
+        }
+#pragma unroll
+        for (int q = 0; q < ZQ; ++q) {
+            double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < ZD; ++d) {
+                g0 = fma(bm[q * ZD + d], z0[d], g0);
+                g1 = fma(bm[q * ZD + d], z1[d], g1);
+                g2 = fma(bm[ZQ * ZD + q * ZD + d], z2[d], g2);
+            }
+            const int pq = q * ZC + col;
+            const double d11 = pa[0 * 512 + pq], d21 = pa[1 * 512 + pq], d31 = pa[2 * 512 + pq];
+            const double d22 = pa[3 * 512 + pq], d32 = pa[4 * 512 + pq], d33 = pa[5 * 512 + pq];
+            const double o0 = fma(d11, g0, fma(d21, g1, d31 * g2));
+            const double o1 = fma(d21, g0, fma(d22, g1, d32 * g2));
+            const double o2 = fma(d31, g0, fma(d32, g1, d33 * g2));
+#pragma unroll
+            for (int d = 0; d < ZD; ++d) {
+                t0[d] = fma(bm[q * ZD + d], o0, t0[d]);
+                t1[d] = fma(bm[q * ZD + d], o1, t1[d]);
+                t2[d] = fma(bm[ZQ * ZD + q * ZD + d], o2, t2[d]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int d = 0; d < ZD; ++d) {
+            z[(0 * 8 + d) * ZC + col] += 1e-6 * t0[d];
+            z[(1 * 8 + d) * ZC + col] += 1e-6 * t1[d];
+            z[(2 * 8 + d) * ZC + col] += 1e-6 * t2[d];
+        }
+        __syncthreads();
+    }
+    if (e < ne) {
+        double *o = tout + e * (3 * ZD * ZC);
+#pragma unroll
+        for (int d = 0; d < ZD; ++d) {
+            o[(0 * ZD + d) * ZC + col] = t0[d];
+            o[(1 * ZD + d) * ZC + col] = t1[d];
+            o[(2 * ZD + d) * ZC + col] = t2[d];
+        }
+    }
+}
+
+// One wave per element on v_mfma_f64_16x16x4 tiles (A: lane l holds
+// A[l % 16][l / 16]; B: B[l / 16][l % 16]; D: lane l register r holds
+// D[l / 16 + 4 r][l % 16], as measured on gfx950).  The Q = 8 quadrature rows
+// and D = 7 (+1 zero) DOF rows are tile rows 0..7, i.e. registers 0 and 1 of
+// the four lane groups, so the forward result's registers 0 and 1 feed the
+// transposed contraction's B operand directly as its two K-steps (k-slot g of
+// step s is row g + 4 s), with no exchange between lanes; tile rows 8..15 are
+// zero.  Each contraction is 4 column tiles x 2 K-steps per component.
+__global__ void __launch_bounds__(64 * ZE) k_zstage_mfma(const double *__restrict__ zin,
+                                                         const double *__restrict__ bm,
+                                                         const double *__restrict__ pa,
+                                                         double *__restrict__ tout, int64_t ne,
+                                                         int reps) {
+    __shared__ double zs[ZE * ZS];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int64_t e = (int64_t)blockIdx.x * ZE + w;
+    double *z = zs + w * ZS;
+    const double *src = zin + (e % 4096) * ZS;
+    for (int i = lane; i < ZS; i += 64) z[i] = src[i];
+    // A operands.  Forward, step s: A[m][g'] = M(qz = m, dz = g' + 4 s).
+    // Transposed, step s: A[m][g'] = M(qz = g' + 4 s, dz = m).
+    double af[2][2], at[2][2];  // [B or G][step]
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int k = g + 4 * s;  // dz (forward) or qz (transposed) of k-slot g
+            af[b][s] = (c16 < ZQ && k < ZD) ? bm[b * ZQ * ZD + c16 * ZD + k] : 0.0;
+            at[b][s] = (c16 < ZD) ? bm[b * ZQ * ZD + k * ZD + c16] : 0.0;
+        }
+    __syncthreads();
+    v4d tacc[3][4];
+    for (int r = 0; r < reps; ++r) {
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+            const int col = 16 * nt + c16;
+            v4d gc[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gc[c] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const double zb = z[(c * 8 + g + 4 * s) * ZC + col];  // dz 7 row is zero
+                    gc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[c == 2][s], zb, gc[c], 0, 0, 0);
+                }
+            }
+            // pointwise at registers s = 0, 1: tile rows qz = g + 4 s
+            double o[3][2];
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int pq = (g + 4 * s) * ZC + col;
+                const double d11 = pa[0 * 512 + pq], d21 = pa[1 * 512 + pq], d31 = pa[2 * 512 + pq];
+                const double d22 = pa[3 * 512 + pq], d32 = pa[4 * 512 + pq], d33 = pa[5 * 512 + pq];
+                const double g0 = gc[0][s], g1 = gc[1][s], g2 = gc[2][s];
+                o[0][s] = fma(d11, g0, fma(d21, g1, d31 * g2));
+                o[1][s] = fma(d21, g0, fma(d22, g1, d32 * g2));
+                o[2][s] = fma(d31, g0, fma(d32, g1, d33 * g2));
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                tacc[c][nt] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int s = 0; s < 2; ++s)
+                    tacc[c][nt] = __builtin_amdgcn_mfma_f64_16x16x4f64(at[c == 2][s], o[c][s],
+                                                                       tacc[c][nt], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        // register s holds tile row dz = g + 4 s
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int dz = g + 4 * s;
+                    if (dz < ZD) z[(c * 8 + dz) * ZC + 16 * nt + c16] += 1e-6 * tacc[c][nt][s];
+                }
+        __syncthreads();
+    }
+    if (e < ne) {
+        double *o = tout + e * (3 * ZD * ZC);
+#pragma unroll
+        for (int nt = 0; nt < 4; ++nt)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int dz = g + 4 * s;
+                    if (dz < ZD) o[(c * ZD + dz) * ZC + 16 * nt + c16] = tacc[c][nt][s];
+                }
+    }
+}
+
+// ---- timing ------------------------------------------------------------------
+
+template <typename F>
+float time_ms(F launch, int reps) {
+    hipEvent_t a, b;
+    CK(hipEventCreate(&a));
+    CK(hipEventCreate(&b));
+    for (int i = 0; i < 3; ++i) launch();
+    CK(hipGetLastError());
+    CK(hipEventRecord(a));
+    for (int i = 0; i < reps; ++i) launch();
+    CK(hipEventRecord(b));
+    CK(hipEventSynchronize(b));
+    float ms = 0.f;
+    CK(hipEventElapsedTime(&ms, a, b));
+    CK(hipEventDestroy(a));
+    CK(hipEventDestroy(b));
+    return ms / reps;
+}
+
+void line(const char *name, float ms, double amount, const char *unit) {
+    std::printf("%-44s %9.4f ms  %9.1f %s\n", name, ms, amount / (ms * 1e-3) / 1e9, unit);
+    std::fflush(stdout);
+}
+
+}  // namespace
+
+int main() {
+    // 1. bandwidth, p = 4 apply at 79^3 elements
+    const int64_t ne = 493039, npa = ne * 216 * 6, nye = ne * 125;
+    double *pa = nullptr, *ye = nullptr, *scratch = nullptr;
+    CK(hipMalloc(&pa, npa * sizeof(double)));
+    CK(hipMalloc(&ye, nye * sizeof(double)));
+    CK(hipMalloc(&scratch, 1024 * sizeof(double)));
+    CK(hipMemset(pa, 0, npa * sizeof(double)));
+    {
+        const unsigned g8 = (unsigned)((npa + 8 * 256 - 1) / (8 * 256));
+        const unsigned g7 = (unsigned)((ne + 6) / 7);
+        const double bpa = npa * 8.0, bye = nye * 8.0;
+        line("read, contiguous", time_ms([&] { hipLaunchKernelGGL(k_read<false>, dim3(g8), dim3(256), 0, 0, pa, npa, scratch); }, 20), bpa, "GB/s");
+        line("read, contiguous, non-temporal", time_ms([&] { hipLaunchKernelGGL(k_read<true>, dim3(g8), dim3(256), 0, 0, pa, npa, scratch); }, 20), bpa, "GB/s");
+        line("PA read pattern (p=4, 7 elem/block)", time_ms([&] { hipLaunchKernelGGL(k_pa_pattern<false>, dim3(g7), dim3(256), 0, 0, pa, ne, ye); }, 20), bpa, "GB/s");
+        line("PA read + E-vector write (125 doubles/elem)", time_ms([&] { hipLaunchKernelGGL(k_pa_pattern<true>, dim3(g7), dim3(256), 0, 0, pa, ne, ye); }, 20), bpa + bye, "GB/s");
+        line("E-vector write alone", time_ms([&] { hipLaunchKernelGGL(k_write_pattern, dim3(g7), dim3(256), 0, 0, ne, ye); }, 20), bye, "GB/s");
+    }
+    CK(hipFree(pa));
+    CK(hipFree(ye));
+
+    // 2. instruction throughput: 256 CUs x 8 blocks of 256 threads
+    {
+        std::vector<double> h(128);
+        for (int i = 0; i < 128; ++i) h[i] = 1.0 + 1e-3 * i;
+        h[64] = 0.999999;
+        h[65] = 1e-7;
+        h[66] = -1e30;  // no FMA chain or tile sum reaches it
+        double *in64;
+        float *in32, *out32;
+        CK(hipMalloc(&in64, 128 * sizeof(double)));
+        CK(hipMalloc(&in32, 128 * sizeof(float)));
+        CK(hipMalloc(&out32, 16 * sizeof(float)));
+        CK(hipMemcpy(in64, h.data(), 128 * sizeof(double), hipMemcpyHostToDevice));
+        std::vector<float> hf(h.begin(), h.end());
+        CK(hipMemcpy(in32, hf.data(), 128 * sizeof(float), hipMemcpyHostToDevice));
+        const int blocks = 256 * 8, iters = 4096;
+        const double threads = blocks * 256.0, waves = threads / 64;
+        const double valu = 2.0 * 8 * iters * threads, mf = 2.0 * 16 * 16 * 4 * 8 * iters * waves;
+        line("FP64 vector FMA", time_ms([&] { hipLaunchKernelGGL(k_valu<double>, dim3(blocks), dim3(256), 0, 0, in64, scratch, iters); }, 5), valu * 1e-3, "TFLOP/s");
+        line("FP32 vector FMA (non-packed)", time_ms([&] { hipLaunchKernelGGL(k_valu<float>, dim3(blocks), dim3(256), 0, 0, in32, out32, iters); }, 5), valu * 1e-3, "TFLOP/s");
+        line("FP64 MFMA 16x16x4", time_ms([&] { hipLaunchKernelGGL(k_mfma_f64, dim3(blocks), dim3(256), 0, 0, in64, scratch, iters); }, 5), mf * 1e-3, "TFLOP/s");
+        line("FP32 MFMA 16x16x4", time_ms([&] { hipLaunchKernelGGL(k_mfma_f32, dim3(blocks), dim3(256), 0, 0, in32, out32, iters); }, 5), mf * 1e-3, "TFLOP/s");
+        CK(hipFree(in64));
+        CK(hipFree(in32));
+        CK(hipFree(out32));
+    }
+
+    // 3. z-stage, D = 7, Q = 8
+    {
+        const int64_t nz = 65536;  // elements (a multiple of ZE)
+        std::vector<double> hz(4096 * (int64_t)ZS, 0.0), hb(2 * ZQ * ZD), hp(6 * 512);
+        unsigned s = 12345u;
+        auto rnd = [&] { s = s * 1664525u + 1013904223u; return (s >> 8) * (1.0 / 16777216.0) - 0.5; };
+        for (int64_t el = 0; el < 4096; ++el)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < ZD; ++d)  // row dz = 7 stays zero
+                    for (int k = 0; k < ZC; ++k) hz[el * ZS + (c * 8 + d) * ZC + k] = rnd();
+        for (auto &v : hb) v = rnd();
+        for (int q = 0; q < 512; ++q) {
+            hp[0 * 512 + q] = 1.0 + 0.2 * rnd();
+            hp[1 * 512 + q] = 0.1 * rnd();
+            hp[2 * 512 + q] = 0.1 * rnd();
+            hp[3 * 512 + q] = 1.0 + 0.2 * rnd();
+            hp[4 * 512 + q] = 0.1 * rnd();
+            hp[5 * 512 + q] = 1.0 + 0.2 * rnd();
+        }
+        double *zin, *bm, *pz, *t1, *t2;
+        CK(hipMalloc(&zin, hz.size() * sizeof(double)));
+        CK(hipMalloc(&bm, hb.size() * sizeof(double)));
+        CK(hipMalloc(&pz, hp.size() * sizeof(double)));
+        const int64_t nt = nz * 3 * ZD * ZC;
+        CK(hipMalloc(&t1, nt * sizeof(double)));
+        CK(hipMalloc(&t2, nt * sizeof(double)));
+        CK(hipMemcpy(zin, hz.data(), hz.size() * sizeof(double), hipMemcpyHostToDevice));
+        CK(hipMemcpy(bm, hb.data(), hb.size() * sizeof(double), hipMemcpyHostToDevice));
+        CK(hipMemcpy(pz, hp.data(), hp.size() * sizeof(double), hipMemcpyHostToDevice));
+        const unsigned gz = (unsigned)(nz / ZE);
+        for (int reps : {1, 3}) {  // agreement after one and after three repetitions
+            hipLaunchKernelGGL(k_zstage_valu, dim3(gz), dim3(64 * ZE), 0, 0, zin, bm, pz, t1, nz, reps);
+            hipLaunchKernelGGL(k_zstage_mfma, dim3(gz), dim3(64 * ZE), 0, 0, zin, bm, pz, t2, nz, reps);
+            CK(hipGetLastError());
+            CK(hipDeviceSynchronize());
+            std::vector<double> a(nt), b(nt);
+            CK(hipMemcpy(a.data(), t1, nt * sizeof(double), hipMemcpyDeviceToHost));
+            CK(hipMemcpy(b.data(), t2, nt * sizeof(double), hipMemcpyDeviceToHost));
+            double num = 0.0, den = 0.0;
+            for (int64_t i = 0; i < nt; ++i) {
+                num += (a[i] - b[i]) * (a[i] - b[i]);
+                den += a[i] * a[i];
+            }
+            const double rel = std::sqrt(num / den);
+            std::printf("z-stage check, %d repetition(s): matrix-core vs vector rel. diff %.3e\n",
+                        reps, rel);
+            if (!(rel < 1e-13)) {
+                std::printf("z-stage check FAILED\n");
+                return 2;
+            }
+        }
+        const int reps = 32;
+        // useful flops: 2 x (2 contractions of 3 D Q per column + 9 Q pointwise) x Q^2 columns
+        const double flops = 2.0 * (2.0 * 3 * ZD * ZQ + 9.0 * ZQ) * ZC * nz * reps;
+        line("z-stage p=6 FP64, vector (thread per column)", time_ms([&] { hipLaunchKernelGGL(k_zstage_valu, dim3(gz), dim3(64 * ZE), 0, 0, zin, bm, pz, t1, nz, reps); }, 5), flops * 1e-3, "TFLOP/s");
+        line("z-stage p=6 FP64, matrix cores (wave per element)", time_ms([&] { hipLaunchKernelGGL(k_zstage_mfma, dim3(gz), dim3(64 * ZE), 0, 0, zin, bm, pz, t2, nz, reps); }, 5), flops * 1e-3, "TFLOP/s");
+        CK(hipFree(zin));
+        CK(hipFree(bm));
+        CK(hipFree(pz));
+        CK(hipFree(t1));
+        CK(hipFree(t2));
+    }
+    CK(hipFree(scratch));
+    std::printf("done\n");
+    return 0;
+}
