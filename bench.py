@@ -113,6 +113,76 @@ def run_apply(M, torch, p, n, steps, warmup):
                 bytes_element=b_elem, variant=variant)
 
 
+def _event_ms(torch, fn, reps):
+    """Average ms of fn() over reps launches, torch events on the current
+    stream (the library launches on it)."""
+    fn()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+# vector streams of one PCG iteration beyond the fused apply: the scatter's
+# (d, q) dot reads d; the update reads x, d, r, q, dinv and writes x, r, z;
+# the direction update reads z, d and writes d
+PCG_VECTOR_STREAMS = 12
+
+
+def run_path_kernels(M, torch, p, n, reps=20, pcg_iters=50):
+    """The other kernels of the path on the same workload, each against its
+    own algorithmic bytes: ElementRestriction gather (NE D^3 (8+4) + N 8),
+    scatter (NE D^3 (8+4) + (N+1) 4 + N 8), PA diagonal (NE (6 Q^3 8 + D^3 4)
+    + N 8) and one Jacobi-PCG iteration (the fused apply's bytes + 12 N 8)."""
+    mesh, fes, a, x, y = build_problem(M, torch, p, n)
+    ne, N = fes.GetNE(), fes.GetNDofs()
+    D, Q = p + 1, p + 2
+    D3, Q3 = D ** 3, Q ** 3
+    R = fes.GetElementRestriction()
+    xe = torch.empty(ne * D3, dtype=torch.float64, device="cuda")
+    out = {}
+
+    def rec(name, ms, nbytes, **kw):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = dict(ms=round(ms, 4), bytes=int(nbytes), gbs=round(gbs, 1),
+                         frac=round(gbs / HBM_PEAK_GBS, 4), **kw)
+
+    rec("gather", _event_ms(torch, lambda: R.Mult(x, xe), reps), ne * D3 * 12 + N * 8)
+    rec("scatter", _event_ms(torch, lambda: R.MultTranspose(xe, y), reps),
+        ne * D3 * 12 + (N + 1) * 4 + N * 8)
+    diag = torch.empty_like(x)
+    rec("diagonal", _event_ms(torch, lambda: a.AssembleDiagonal(diag), reps),
+        ne * (6 * Q3 * 8 + D3 * 4) + N * 8)
+    ess = fes.GetBoundaryTrueDofs()
+    A, X, B = a.FormLinearSystem(ess, torch.zeros_like(x), x)
+    cg = M.CGSolver()
+    cg.SetRelTol(0.0)
+    cg.SetAbsTol(0.0)
+    cg.SetMaxIter(pcg_iters)
+    cg.SetOperator(A)
+    cg.SetPreconditioner(M.OperatorJacobiSmoother(a, ess))
+    X0 = X.clone()
+    X.copy_(X0)
+    cg.Mult(B, X)  # warm-up
+    iters = []
+
+    def solve():
+        X.copy_(X0)
+        cg.Mult(B, X)
+        iters.append(cg.GetNumIterations())
+
+    ms = _event_ms(torch, solve, 3)
+    it = max(iters[-1], 1)
+    b_apply = ne * (6 * Q3 * 8 + D3 * 4) + 2 * N * 8
+    rec("pcg_iteration", ms / it, b_apply + PCG_VECTOR_STREAMS * N * 8, iterations=it,
+        vector_streams=PCG_VECTOR_STREAMS)
+    del mesh, fes, a, x, y, xe, diag, A, X, B, X0, cg
+    return out
+
+
 def cpu_baseline(p, target_seconds=12.0):
     """The CPU oracle (C + OpenMP restatement of MFEM's PA path) on a bounded
     sample of the same workload: same p, perturbation and coefficient, on a
@@ -384,6 +454,10 @@ def main():
     r = run_apply(M, torch, args.p, n, args.steps, args.warmup)
     log(str(r))
     torch.cuda.empty_cache()
+    log("path kernels: gather, scatter, diagonal, PCG iteration")
+    path_kernels = run_path_kernels(M, torch, args.p, n)
+    log(str(path_kernels))
+    torch.cuda.empty_cache()
     sweep = []
     if args.sweep:
         for p in range(1, 9):
@@ -435,6 +509,7 @@ def main():
             "scatter_ms": round(r["ms_scatter"], 4),
             "element_variant": r["variant"],
         },
+        "kernels": path_kernels,
         "roofline": {
             "bound": "hbm",
             "achieved": round(r["element_gbs"], 1),

@@ -59,27 +59,32 @@ k_mass_apply(int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
 // Diagonal of the element diffusion matrices (PADiffusionDiagonal3D,
 // bilininteg_diffusion_kernels.hpp:268-367), all nine (i,j) pairs grouped by
 // the 1D factor in x: GG (O11), BB (O22, O33, 2 O23), GB (2 O12, 2 O13).
-template <int D, int Q>
-__global__ void __launch_bounds__(Q * Q)
+// NB elements per block (Q^2 threads each); the z loop is unrolled so that
+// the PA planes' loads can be issued ahead of their use.
+template <int D, int Q, int NB>
+__global__ void __launch_bounds__(NB * Q * Q)
 k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
                  double *__restrict__ ye, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DQQ = D * Q * Q, DDQ = D * D * Q;
-    __shared__ double s[6 * DQQ];
+    __shared__ double smem[NB * 6 * DQQ];
     const double *__restrict__ Bq = basis;
     const double *__restrict__ Gq = basis + Q * D;
     const double *__restrict__ Bd = basis + 2 * Q * D;
     const double *__restrict__ Gd = basis + 3 * Q * D;
-    const int a = threadIdx.x % Q, b = threadIdx.x / Q;
-    const int64_t e = blockIdx.x;
-    {   // z: thread (qx, qy)
+    const int le = threadIdx.x / QQ, t = threadIdx.x - le * QQ;
+    const int a = t % Q, b = t / Q;
+    const int64_t e = (int64_t)blockIdx.x * NB + le;
+    const bool valid = e < ne;
+    double *s = smem + le * 6 * DQQ;
+    if (valid) {   // z: thread (qx, qy)
         const double *p = pa + (int64_t)QQQ * 6 * e + a + Q * b;
         double acc[6][D];
 #pragma unroll
         for (int k = 0; k < 6; ++k)
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) acc[k][dz] = 0.0;
-#pragma unroll 1
+#pragma unroll
         for (int qz = 0; qz < Q; ++qz) {
             const double *br = Bd + D * qz, *gr = Gd + D * qz;
             const double O11 = p[QQQ * 0 + QQ * qz], O12 = p[QQQ * 1 + QQ * qz],
@@ -103,7 +108,7 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__r
     }
     __syncthreads();
     double ygg[D], ybb[D], ygb[D];
-    const bool acty = b < D;
+    const bool acty = valid && b < D;
     if (acty) {   // y: thread (qx, dz)
 #pragma unroll
         for (int dy = 0; dy < D; ++dy) ygg[dy] = ybb[dy] = ygb[dy] = 0.0;
@@ -132,7 +137,7 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__r
         }
     }
     __syncthreads();
-    if (a < D && b < D) {   // x: thread (dy, dz)
+    if (valid && a < D && b < D) {   // x: thread (dy, dz)
         double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
 #pragma unroll 1
         for (int dx = 0; dx < D; ++dx) {
@@ -151,23 +156,26 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__r
     }
 }
 
-template <int D, int Q>
-__global__ void __launch_bounds__(Q * Q)
+template <int D, int Q, int NB>
+__global__ void __launch_bounds__(NB * Q * Q)
 k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
             double *__restrict__ ye, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DQQ = D * Q * Q;
-    __shared__ double s[DQQ];
+    __shared__ double smem[NB * DQQ];
     const double *__restrict__ Bq = basis;
     const double *__restrict__ Bd = basis + 2 * Q * D;
-    const int a = threadIdx.x % Q, b = threadIdx.x / Q;
-    const int64_t e = blockIdx.x;
-    {
+    const int le = threadIdx.x / QQ, t = threadIdx.x - le * QQ;
+    const int a = t % Q, b = t / Q;
+    const int64_t e = (int64_t)blockIdx.x * NB + le;
+    const bool valid = e < ne;
+    double *s = smem + le * DQQ;
+    if (valid) {
         const double *p = pa + (int64_t)QQQ * e + a + Q * b;
         double acc[D];
 #pragma unroll
         for (int dz = 0; dz < D; ++dz) acc[dz] = 0.0;
-#pragma unroll 1
+#pragma unroll
         for (int qz = 0; qz < Q; ++qz) {
             const double *br = Bd + D * qz;
             const double v = p[QQ * qz];
@@ -179,7 +187,7 @@ k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restri
     }
     __syncthreads();
     double y2[D];
-    const bool acty = b < D;
+    const bool acty = valid && b < D;
     if (acty) {
 #pragma unroll
         for (int dy = 0; dy < D; ++dy) y2[dy] = 0.0;
@@ -196,7 +204,7 @@ k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restri
 #pragma unroll
         for (int dy = 0; dy < D; ++dy) s[a + Q * (dy + D * b)] = y2[dy];
     __syncthreads();
-    if (a < D && b < D) {
+    if (valid && a < D && b < D) {
         double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
 #pragma unroll 1
         for (int dx = 0; dx < D; ++dx) {
@@ -282,14 +290,16 @@ void launch_diffusion_lp(int variant, int64_t ne, const float *basis, const TP *
 template <int D, int Q>
 void launch_diff_diag(int64_t ne, const double *basis, const double *pa, double *ye, bool acc,
                       hipStream_t s) {
-    hipLaunchKernelGGL((k_diffusion_diag<D, Q>), dim3((unsigned)ne), dim3(Q * Q), 0, s, ne, basis,
-                       pa, ye, acc ? 1 : 0);
+    constexpr int NB = nb_for(Q, 6 * D * Q * Q, sizeof(double));
+    hipLaunchKernelGGL((k_diffusion_diag<D, Q, NB>), dim3((unsigned)((ne + NB - 1) / NB)),
+                       dim3(NB * Q * Q), 0, s, ne, basis, pa, ye, acc ? 1 : 0);
 }
 template <int D, int Q>
 void launch_mass_diag(int64_t ne, const double *basis, const double *pa, double *ye, bool acc,
                       hipStream_t s) {
-    hipLaunchKernelGGL((k_mass_diag<D, Q>), dim3((unsigned)ne), dim3(Q * Q), 0, s, ne, basis, pa,
-                       ye, acc ? 1 : 0);
+    constexpr int NB = nb_for(Q, D * Q * Q, sizeof(double));
+    hipLaunchKernelGGL((k_mass_diag<D, Q, NB>), dim3((unsigned)((ne + NB - 1) / NB)),
+                       dim3(NB * Q * Q), 0, s, ne, basis, pa, ye, acc ? 1 : 0);
 }
 
 // (D, Q) pairs compiled: Q = D (q = p+1) and Q = D + 1 (q = p+2, MFEM default).

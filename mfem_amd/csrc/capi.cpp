@@ -215,6 +215,9 @@ int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space) {
                                        s.nvdofs, r.perm, st);
         r.nperm = s.nvdofs;
         r.bubble_begin = s.nvdofs + s.nedofs + s.nfdofs;
+        kern::first_contribution_order(s.ne * s.dof, r.gather_map.p, r.offsets.p, r.indices.p,
+                                       r.bubble_begin, r.perm_lex, st);
+        r.nperm_lex = r.bubble_begin;
         {
             std::vector<int> dm(s.dof);
             h1_dof_map(order, dm.data());  // lex -> native
@@ -360,7 +363,7 @@ int h1pa_restriction_mult_transpose(const h1pa_restriction *r, const double *x, 
     API_BEGIN
     REQUIRE(r && x && y, "restriction_mult_transpose: null argument");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 0,
-                                  r->R().perm.p, r->R().nperm,
+                                  r->R().perm_lex.p, r->R().nperm_lex,
                                   as_stream(stream));
     API_END
 }
@@ -371,7 +374,7 @@ int h1pa_restriction_add_mult_transpose(const h1pa_restriction *r, const double 
     REQUIRE(r && x && y, "restriction_add_mult_transpose: null argument");
     REQUIRE(a == 1.0, "General coefficient case is not yet supported!");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 1,
-                                  r->R().perm.p, r->R().nperm,
+                                  r->R().perm_lex.p, r->R().nperm_lex,
                                   as_stream(stream));
     API_END
 }
@@ -381,7 +384,7 @@ int h1pa_restriction_abs_mult_transpose(const h1pa_restriction *r, const double 
     API_BEGIN
     REQUIRE(r && x && y, "restriction_abs_mult_transpose: null argument");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 2,
-                                  r->R().perm.p, r->R().nperm,
+                                  r->R().perm_lex.p, r->R().nperm_lex,
                                   as_stream(stream));
     API_END
 }
@@ -391,7 +394,7 @@ int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *
     API_BEGIN
     REQUIRE(r && x && y, "restriction_mult_left_inverse: null argument");
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 3,
-                                  r->R().perm.p, r->R().nperm,
+                                  r->R().perm_lex.p, r->R().nperm_lex,
                                   as_stream(stream));
     API_END
 }
@@ -564,7 +567,7 @@ void h1pa::operator_diagonal(Form &f, double *diag, hipStream_t s) {
         kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
     if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  diag, 2, sp.restr.perm.p, sp.restr.nperm, s);
+                                  diag, 2, sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
 }
 
 int64_t h1pa::form_device_bytes(const Form &f) {
@@ -575,7 +578,8 @@ int64_t h1pa::form_device_bytes(const Form &f) {
     b += f.lwork0.n * 8 + f.lwork1.n * 8 + f.cmask.n + f.basis32.n * 4;
     b += f.pa_diff_lp.n * 4 + f.pa_mass_lp.n * 4 + f.ework32.n * 4;
     const Restriction &r = sp.restr;
-    b += (r.gather_map.n + r.offsets.n + r.indices.n + r.perm.n + r.lex2nat.n + r.indices_nat.n) * 4;
+    b += (r.gather_map.n + r.offsets.n + r.indices.n + r.perm.n + r.perm_lex.n + r.lex2nat.n +
+          r.indices_nat.n) * 4;
     b += (sp.elem_dof.n + sp.ess.n) * 4 + sp.ess_mask.n;
     b += m.d_nodes.n * 8 + m.d_topo.n * 4 + m.d_ori.n;
     return b;
@@ -665,7 +669,7 @@ int h1pa_form_assemble_diagonal(h1pa_form *form, double *diag, void *stream) {
     if (mass)
         kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  diag, 2, sp.restr.perm.p, sp.restr.nperm, s);
+                                  diag, 2, sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
     API_END
 }
 
@@ -703,8 +707,8 @@ int h1pa_form_constrained_mult(h1pa_form *form, const int *d_ess, int64_t n_ess,
     element_apply(f, sp.restr.gather_map.p, mask, x, f.ework0.p, direct ? y : nullptr, false, s);
     kern::scatter_constrained(direct ? sp.restr.bubble_begin : sp.ndofs, sp.restr.offsets.p,
                               direct ? sp.restr.indices_nat.p : sp.restr.indices.p, f.ework0.p,
-                              mask, x, y, sp.restr.perm.p,
-                              sp.restr.nperm, s);
+                              mask, x, y, direct ? sp.restr.perm.p : sp.restr.perm_lex.p,
+                              direct ? sp.restr.nperm : sp.restr.nperm_lex, s);
     API_END
 }
 
@@ -753,7 +757,7 @@ int h1pa_space_domain_lf(h1pa_space *space, double fval, double *b, void *stream
     DevBuf<double> bE(sp.ne * sp.dof);
     kern::domain_lf(*sp.mesh, sp.p, fval, bE.p, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, bE.p, b, 0,
-                                  sp.restr.perm.p, sp.restr.nperm, s);
+                                  sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
     H1PA_HIP(hipStreamSynchronize(s));
     API_END
 }

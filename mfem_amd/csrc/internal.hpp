@@ -116,6 +116,12 @@ struct Restriction {
     // E-entries of neighbouring elements.  Empty: natural order.
     DevBuf<int> perm;
     int64_t nperm = 0;
+    // Scatter order of the first `nperm_lex` L-DOFs (every row below the
+    // bubble block) for E-vectors in the lexicographic layout, by first
+    // contributing E-index: rows that read the same element's cache lines run
+    // together, so each line is fetched about once.  Empty: natural order.
+    DevBuf<int> perm_lex;
+    int64_t nperm_lex = 0;
     // First L-DOF of the bubble block (interior DOFs, one E-entry each) or
     // ndofs when the numbering has no such block.
     int64_t bubble_begin = 0;

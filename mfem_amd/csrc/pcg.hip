@@ -66,6 +66,33 @@ inline Chunks chunks(int64_t n) {
                  i##_end = min((int64_t)(n), (int64_t)(blockIdx.x + 1) * kThreads * per);      \
          i < i##_end; i += kThreads)
 
+// The same entries taken kGrp at a time: a kernel loads every operand of a
+// group at the clamped indices g.lt[k] before the group's first store, so the
+// loads of a group go out together instead of one entry's round trip after
+// another; stores and sums then run over the g.ok entries in k order, which
+// is the H1PA_CHUNK order (dot products come out bit-identical).
+constexpr int kGrp = 4;
+struct Grp {
+    int64_t t[kGrp], lt[kGrp];
+    bool ok[kGrp];
+};
+__device__ __forceinline__ Grp grp_of(int64_t b, int64_t end) {
+    Grp g;
+#pragma unroll
+    for (int k = 0; k < kGrp; ++k) {
+        const int64_t t = b + (int64_t)k * kThreads;
+        g.t[k] = t;
+        g.ok[k] = t < end;
+        g.lt[k] = t < end ? t : end - 1;
+    }
+    return g;
+}
+#define H1PA_GROUPS(g, n)                                                                      \
+    for (int64_t g##_b = (int64_t)blockIdx.x * kThreads * per + threadIdx.x,                   \
+                 g##_e = min((int64_t)(n), (int64_t)(blockIdx.x + 1) * kThreads * per);        \
+         g##_b < g##_e; g##_b += (int64_t)kGrp * kThreads)                                     \
+        for (const Grp g = grp_of(g##_b, g##_e), *g##_1 = &g; g##_1; g##_1 = nullptr)
+
 // Block sum in a fixed tree order, returned to every thread.
 __device__ inline double block_sum(double v, double *sh) {
     sh[threadIdx.x] = v;
@@ -129,14 +156,25 @@ k_pcg_init(int64_t n, int per, const T *__restrict__ b, const T *__restrict__ di
            T *__restrict__ r, T *__restrict__ z, T *__restrict__ d, double *__restrict__ partials) {
     __shared__ double sh[kThreads];
     double acc[1] = {0.0};
-    H1PA_CHUNK(i, n) {
-        const T ri = b[i];
-        const T zi = dinv[i] * ri;
-        r[i] = ri;
-        x[i] = T(0);
-        z[i] = zi;
-        d[i] = zi;
-        acc[0] += (double)zi * (double)ri;
+    H1PA_GROUPS(g, n) {
+        T vb[kGrp], vi[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            vb[k] = b[g.lt[k]];
+            vi[k] = dinv[g.lt[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;
+            const int64_t i = g.t[k];
+            const T ri = vb[k];
+            const T zi = vi[k] * ri;
+            r[i] = ri;
+            x[i] = T(0);
+            z[i] = zi;
+            d[i] = zi;
+            acc[0] += (double)zi * (double)ri;
+        }
     }
     store_partials<1>(acc, partials, sh);
 }
@@ -178,18 +216,33 @@ k_pcg_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
     __shared__ double sh[kThreads];
     if (st->done) return;
     double acc[1] = {0.0};
-    H1PA_CHUNK(t, n) {
-        const int64_t i = t < nperm ? perm[t] : t;
-        T v;
-        if (t >= bb) {  // interior DOF: written by the element kernel
-            v = z[i];
-        } else if (mask && mask[i]) {
-            v = d[i];
-        } else {
-            v = row_sum(indices, offsets[i], offsets[i + 1], ye);
+    H1PA_GROUPS(g, n) {
+        // the group's rows, then their offsets, masks and d values, as batches
+        int64_t ii[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        T vd[kGrp];
+        int ob[kGrp], oe[kGrp];
+        bool m[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            vd[k] = d[ii[k]];
+            ob[k] = offsets[ii[k]];
+            oe[k] = offsets[ii[k] + 1];
+            m[k] = mask && mask[ii[k]];
         }
-        if (t < bb) z[i] = v;
-        acc[0] += (double)d[i] * (double)v;
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            T v;
+            if (g.t[k] >= bb) {  // interior DOF: written by the element kernel
+                v = z[ii[k]];
+            } else {
+                v = m[k] ? vd[k] : row_sum(indices, ob[k], oe[k], ye);
+            }
+            if (!g.ok[k]) continue;
+            if (g.t[k] < bb) z[ii[k]] = v;
+            acc[0] += (double)vd[k] * (double)v;
+        }
     }
     store_partials<1>(acc, partials, sh);
 }
@@ -217,14 +270,28 @@ k_pcg_update(int64_t n, int per, const T *__restrict__ dinv, T *__restrict__ x, 
     if (st->done) return;
     const T alpha = (T)(st->nom / st->den);
     double acc[1] = {0.0};
-    H1PA_CHUNK(i, n) {
-        const T di = d[i];
-        x[i] += alpha * di;
-        const T ri = r[i] - alpha * z[i];
-        r[i] = ri;
-        const T zi = dinv[i] * ri;
-        z[i] = zi;
-        acc[0] += (double)ri * (double)zi;
+    H1PA_GROUPS(g, n) {
+        T vd[kGrp], vx[kGrp], vr[kGrp], vz[kGrp], vi[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            const int64_t j = g.lt[k];
+            vd[k] = d[j];
+            vx[k] = x[j];
+            vr[k] = r[j];
+            vz[k] = z[j];
+            vi[k] = dinv[j];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;
+            const int64_t i = g.t[k];
+            x[i] = vx[k] + alpha * vd[k];
+            const T ri = vr[k] - alpha * vz[k];
+            r[i] = ri;
+            const T zi = vi[k] * ri;
+            z[i] = zi;
+            acc[0] += (double)ri * (double)zi;
+        }
     }
     store_partials<1>(acc, partials, sh);
 }
@@ -262,7 +329,17 @@ __global__ void __launch_bounds__(kThreads)
 k_pcg_dir(int64_t n, int per, const T *__restrict__ z, T *__restrict__ d, const PcgState *st) {
     if (st->done) return;
     const T beta = (T)st->beta;
-    H1PA_CHUNK(i, n) d[i] = z[i] + beta * d[i];
+    H1PA_GROUPS(g, n) {
+        T vz[kGrp], vd[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            vz[k] = z[g.lt[k]];
+            vd[k] = d[g.lt[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k)
+            if (g.ok[k]) d[g.t[k]] = vz[k] + beta * vd[k];
+    }
 }
 
 __global__ void k_to_float(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
@@ -370,18 +447,32 @@ k_ir_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
     __shared__ double sh[kThreads];
     if (st->done) return;
     double acc[1] = {0.0};
-    H1PA_CHUNK(t, n) {
-        const int64_t i = t < nperm ? perm[t] : t;
-        float v;
-        if (t >= bb) {  // interior DOF: written by the element kernel
-            v = q[i];
-        } else if (mask && mask[i]) {
-            v = d[i];
-        } else {
-            v = row_sum(indices, offsets[i], offsets[i + 1], ye);
+    H1PA_GROUPS(g, n) {
+        int64_t ii[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        float vd[kGrp];
+        int ob[kGrp], oe[kGrp];
+        bool m[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            vd[k] = d[ii[k]];
+            ob[k] = offsets[ii[k]];
+            oe[k] = offsets[ii[k] + 1];
+            m[k] = mask && mask[ii[k]];
         }
-        if (t < bb) q[i] = v;
-        acc[0] += (double)d[i] * (double)v;
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            float v;
+            if (g.t[k] >= bb) {  // interior DOF: written by the element kernel
+                v = q[ii[k]];
+            } else {
+                v = m[k] ? vd[k] : row_sum(indices, ob[k], oe[k], ye);
+            }
+            if (!g.ok[k]) continue;
+            if (g.t[k] < bb) q[ii[k]] = v;
+            acc[0] += (double)vd[k] * (double)v;
+        }
     }
     store_partials<1>(acc, partials, sh);
 }
@@ -404,13 +495,28 @@ k_ir_update(int64_t n, int per, const float *__restrict__ dinv, float *__restric
     const double den = st->den;
     const float alpha = den != 0.0 ? (float)(st->rho / den) : 0.f;
     double acc[1] = {0.0};
-    H1PA_CHUNK(i, n) {
-        xs[i] += alpha * d[i];
-        const float ri = r[i] - alpha * q[i];
-        r[i] = ri;
-        const float zi = dinv[i] * ri;
-        z[i] = zi;
-        acc[0] += (double)ri * (double)zi;
+    H1PA_GROUPS(g, n) {
+        float vs[kGrp], vd[kGrp], vr[kGrp], vq[kGrp], vi[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            const int64_t j = g.lt[k];
+            vs[k] = xs[j];
+            vd[k] = d[j];
+            vr[k] = r[j];
+            vq[k] = q[j];
+            vi[k] = dinv[j];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;
+            const int64_t i = g.t[k];
+            xs[i] = vs[k] + alpha * vd[k];
+            const float ri = vr[k] - alpha * vq[k];
+            r[i] = ri;
+            const float zi = vi[k] * ri;
+            z[i] = zi;
+            acc[0] += (double)ri * (double)zi;
+        }
     }
     store_partials<1>(acc, partials, sh);
 }
@@ -512,7 +618,17 @@ __global__ void __launch_bounds__(kThreads)
 k_ir_dir(int64_t n, int per, const float *__restrict__ z, float *__restrict__ d, const IrState *st) {
     if (st->done) return;
     const float beta = (float)st->beta;
-    H1PA_CHUNK(i, n) d[i] = z[i] + beta * d[i];
+    H1PA_GROUPS(g, n) {
+        float vz[kGrp], vd[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            vz[k] = z[g.lt[k]];
+            vd[k] = d[g.lt[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k)
+            if (g.ok[k]) d[g.t[k]] = vz[k] + beta * vd[k];
+    }
 }
 
 // ---- Chebyshev smoother (OperatorChebyshevSmoother::Mult, solvers.cpp:630-656)
@@ -582,9 +698,22 @@ k_cg_update_xr(int64_t n, int per, double *__restrict__ x, double *__restrict__ 
                const double *__restrict__ q, const double *__restrict__ d, const PcgState *st) {
     if (st->done) return;
     const double alpha = st->nom / st->den;
-    H1PA_CHUNK(i, n) {
-        x[i] += alpha * d[i];
-        r[i] = r[i] - alpha * q[i];
+    H1PA_GROUPS(g, n) {
+        double vx[kGrp], vd[kGrp], vr[kGrp], vq[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            const int64_t j = g.lt[k];
+            vx[k] = x[j];
+            vd[k] = d[j];
+            vr[k] = r[j];
+            vq[k] = q[j];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;
+            x[g.t[k]] = vx[k] + alpha * vd[k];
+            r[g.t[k]] = vr[k] - alpha * vq[k];
+        }
     }
 }
 
@@ -595,7 +724,17 @@ k_dot(int64_t n, int per, const double *__restrict__ a, const double *__restrict
     __shared__ double sh[kThreads];
     if (stop && *stop) return;
     double acc[1] = {0.0};
-    H1PA_CHUNK(i, n) acc[0] += a[i] * b[i];
+    H1PA_GROUPS(g, n) {
+        double va[kGrp], vb[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            va[k] = a[g.lt[k]];
+            vb[k] = b[g.lt[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k)
+            if (g.ok[k]) acc[0] += va[k] * vb[k];
+    }
     store_partials<1>(acc, partials, sh);
 }
 

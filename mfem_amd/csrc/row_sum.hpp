@@ -36,11 +36,12 @@ __device__ __forceinline__ T row_sum_batch(const int *__restrict__ indices, int 
 //
 // A row's first entries are read as one batch of 2, 4 or 8 -- the smallest
 // that holds every row of the wave (face, edge and vertex rows of a Cartesian
-// hexahedral mesh; the vertex rows come first in the row permutation, so
-// waves are mostly of one kind): every index load is issued before any value
-// load, with no branch between them, so a row costs two dependent memory
-// round trips instead of two per entry.  Entries past the 8th continue in a
-// loop.
+// hexahedral mesh; in the fused operator's row order the vertex rows come
+// first, so its waves are mostly of one kind, while the element-major order of
+// the lexicographic scatters mixes them and takes the batch of 8): every index
+// load is issued before any value load, with no branch between them, so a row
+// costs two dependent memory round trips instead of two per entry.  Entries
+// past the 8th continue in a loop.
 template <typename T>
 __device__ __forceinline__ T row_sum(const int *__restrict__ indices, int b, int e,
                                      const T *__restrict__ xe, bool unsigned_ = false) {
