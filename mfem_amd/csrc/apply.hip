@@ -64,7 +64,7 @@ k_mass_apply(int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
 template <int D, int Q, int NB>
 __global__ void __launch_bounds__(NB * Q * Q)
 k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
-                 double *__restrict__ ye, int accumulate) {
+                 const EOut out, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DQQ = D * Q * Q, DDQ = D * D * Q;
     __shared__ double smem[NB * 6 * DQQ];
@@ -138,8 +138,8 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__r
     }
     __syncthreads();
     if (valid && a < D && b < D) {   // x: thread (dy, dz)
-        double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll 1
+        double yv[D];
+#pragma unroll
         for (int dx = 0; dx < D; ++dx) {
             const double *bc = Bq + Q * dx, *gc = Gq + Q * dx;
             double v = 0.0;
@@ -150,16 +150,16 @@ k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__r
                 v = fma(bc[qx] * bc[qx], s[1 * DDQ + id], v);
                 v = fma(gc[qx] * bc[qx], s[2 * DDQ + id], v);
             }
-            if (accumulate) yp[dx] += v;
-            else yp[dx] = v;
+            yv[dx] = v;
         }
+        store_line<D, double>(out, e, a, b, yv, accumulate);
     }
 }
 
 template <int D, int Q, int NB>
 __global__ void __launch_bounds__(NB * Q * Q)
 k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
-            double *__restrict__ ye, int accumulate) {
+            const EOut out, int accumulate) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DQQ = D * Q * Q;
     __shared__ double smem[NB * DQQ];
@@ -205,16 +205,16 @@ k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restri
         for (int dy = 0; dy < D; ++dy) s[a + Q * (dy + D * b)] = y2[dy];
     __syncthreads();
     if (valid && a < D && b < D) {
-        double *yp = ye + (int64_t)DDD * e + D * (a + D * b);
-#pragma unroll 1
+        double yv[D];
+#pragma unroll
         for (int dx = 0; dx < D; ++dx) {
             const double *bc = Bq + Q * dx;
             double v = 0.0;
 #pragma unroll
             for (int qx = 0; qx < Q; ++qx) v = fma(bc[qx] * bc[qx], s[qx + Q * (a + D * b)], v);
-            if (accumulate) yp[dx] += v;
-            else yp[dx] = v;
+            yv[dx] = v;
         }
+        store_line<D, double>(out, e, a, b, yv, accumulate);
     }
 }
 
@@ -288,18 +288,18 @@ void launch_diffusion_lp(int variant, int64_t ne, const float *basis, const TP *
         launch_diffusion_v<float, TP, D, Q, NBmax, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s);
 }
 template <int D, int Q>
-void launch_diff_diag(int64_t ne, const double *basis, const double *pa, double *ye, bool acc,
-                      hipStream_t s) {
+void launch_diff_diag(int64_t ne, const double *basis, const double *pa, const EOut &out,
+                      bool acc, hipStream_t s) {
     constexpr int NB = nb_for(Q, 6 * D * Q * Q, sizeof(double));
     hipLaunchKernelGGL((k_diffusion_diag<D, Q, NB>), dim3((unsigned)((ne + NB - 1) / NB)),
-                       dim3(NB * Q * Q), 0, s, ne, basis, pa, ye, acc ? 1 : 0);
+                       dim3(NB * Q * Q), 0, s, ne, basis, pa, out, acc ? 1 : 0);
 }
 template <int D, int Q>
-void launch_mass_diag(int64_t ne, const double *basis, const double *pa, double *ye, bool acc,
-                      hipStream_t s) {
+void launch_mass_diag(int64_t ne, const double *basis, const double *pa, const EOut &out,
+                      bool acc, hipStream_t s) {
     constexpr int NB = nb_for(Q, D * Q * Q, sizeof(double));
     hipLaunchKernelGGL((k_mass_diag<D, Q, NB>), dim3((unsigned)((ne + NB - 1) / NB)),
-                       dim3(NB * Q * Q), 0, s, ne, basis, pa, ye, acc ? 1 : 0);
+                       dim3(NB * Q * Q), 0, s, ne, basis, pa, out, acc ? 1 : 0);
 }
 
 // (D, Q) pairs compiled: Q = D (q = p+1) and Q = D + 1 (q = p+2, MFEM default).
@@ -364,12 +364,12 @@ void mass_apply(int p, int q1d, int64_t ne, const double *basis, const double *p
 }
 
 void diffusion_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
-                        double *ye, bool accumulate, hipStream_t s) {
+                        const EOut &out, bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
-        launch_diff_diag<DD, QQ>(ne, basis, pa, ye, accumulate, s);                        \
+        launch_diff_diag<DD, QQ>(ne, basis, pa, out, accumulate, s);                       \
         H1PA_HIP(hipGetLastError());                                                       \
         return;                                                                            \
     }
@@ -378,13 +378,13 @@ void diffusion_diagonal(int p, int q1d, int64_t ne, const double *basis, const d
     throw Error{H1PA_EUNSUP, "diffusion diagonal: unsupported (p, q1d)"};
 }
 
-void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa, double *ye,
-                   bool accumulate, hipStream_t s) {
+void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
+                   const EOut &out, bool accumulate, hipStream_t s) {
     if (ne <= 0) return;
     const int D = p + 1;
 #define X(DD, QQ)                                                                          \
     if (D == DD && q1d == QQ) {                                                            \
-        launch_mass_diag<DD, QQ>(ne, basis, pa, ye, accumulate, s);                        \
+        launch_mass_diag<DD, QQ>(ne, basis, pa, out, accumulate, s);                       \
         H1PA_HIP(hipGetLastError());                                                       \
         return;                                                                            \
     }

@@ -558,16 +558,23 @@ const uint8_t *constraint_mask(Form &f, const int *d_ess, int64_t n_ess, hipStre
 
 }  // namespace
 
-// Jacobi diagonal of the unconstrained operator into diag (L-vector).
+// Jacobi diagonal of the unconstrained operator into diag (L-vector): the
+// element diagonals in the fused layout (bubble DOFs straight into diag, the
+// rest in native order), then AbsMultTranspose of the other rows.
 void h1pa::operator_diagonal(Form &f, double *diag, hipStream_t s) {
     Space &sp = *f.space;
     const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
     const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
-    if (diff)
-        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
-    if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
-    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  diag, 2, sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
+    kern::EOut out;
+    out.ye = f.ework0.p;
+    out.yL = diag;
+    out.bb = sp.restr.bubble_begin;
+    out.l2n = sp.restr.lex2nat.p;
+    if (diff) kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, out, false, s);
+    if (mass) kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, out, diff, s);
+    kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                  sp.restr.indices_nat.p, f.ework0.p, diag, 2, sp.restr.perm.p,
+                                  sp.restr.nperm, s);
 }
 
 int64_t h1pa::form_device_bytes(const Form &f) {
@@ -662,14 +669,7 @@ int h1pa_form_assemble_diagonal(h1pa_form *form, double *diag, void *stream) {
     require_assembled(f);
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
-    const bool diff = f.desc.integrators & H1PA_INTEG_DIFFUSION;
-    const bool mass = f.desc.integrators & H1PA_INTEG_MASS;
-    if (diff)
-        kern::diffusion_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p, f.ework0.p, false, s);
-    if (mass)
-        kern::mass_diagonal(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, f.ework0.p, diff, s);
-    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, f.ework0.p,
-                                  diag, 2, sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
+    operator_diagonal(f, diag, s);
     API_END
 }
 

@@ -246,10 +246,11 @@ void diffusion_apply(int p, int q1d, int64_t ne, const double *basis, const doub
 void mass_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
                 const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
                 bool accumulate, hipStream_t s);
+// Element diagonals into an element output (same layouts as the applies).
 void diffusion_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
-                        double *ye, bool accumulate, hipStream_t s);
-void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa, double *ye,
-                   bool accumulate, hipStream_t s);
+                        const EOut &out, bool accumulate, hipStream_t s);
+void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
+                   const EOut &out, bool accumulate, hipStream_t s);
 bool apply_supported(int p, int q1d);
 // Speed variants of the diffusion apply (bit-identical results).
 int diffusion_variant_count();
