@@ -640,10 +640,20 @@ __global__ void __launch_bounds__(kThreads)
 k_cheb_first(int64_t n, int per, const double *__restrict__ dinv, const double *__restrict__ x,
              double *__restrict__ t, double *__restrict__ y, double c0, const int *stop) {
     if (stop && *stop) return;
-    H1PA_CHUNK(i, n) {
-        const double ti = x[i] * dinv[i];
-        t[i] = ti;
-        y[i] = 0.0 + c0 * ti;
+    H1PA_GROUPS(g, n) {
+        double vx[kGrp], vi[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            vx[k] = x[g.lt[k]];
+            vi[k] = dinv[g.lt[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;
+            const double ti = vx[k] * vi[k];
+            t[g.t[k]] = ti;
+            y[g.t[k]] = 0.0 + c0 * ti;
+        }
     }
 }
 
@@ -656,15 +666,32 @@ k_cheb_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
                const double *__restrict__ u, double *__restrict__ t, double *__restrict__ y,
                double ck, const int *stop, const int *__restrict__ perm, int64_t nperm) {
     if (stop && *stop) return;
-    H1PA_CHUNK(r, n) {
-        const int64_t i = r < nperm ? perm[r] : r;
-        double v;
-        if (r >= bb) v = u[i];
-        else if (mask && mask[i]) v = t[i];
-        else v = row_sum(indices, offsets[i], offsets[i + 1], ye);
-        const double ti = v * dinv[i];
-        t[i] = ti;
-        y[i] = y[i] + ck * ti;
+    H1PA_GROUPS(g, n) {
+        int64_t ii[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        double vi[kGrp], vy[kGrp], vt[kGrp];
+        int ob[kGrp], oe[kGrp];
+        bool m[kGrp];
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            vi[k] = dinv[ii[k]];
+            vy[k] = y[ii[k]];
+            vt[k] = t[ii[k]];
+            ob[k] = offsets[ii[k]];
+            oe[k] = offsets[ii[k] + 1];
+            m[k] = mask && mask[ii[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < kGrp; ++k) {
+            double v;
+            if (g.t[k] >= bb) v = u[ii[k]];
+            else v = m[k] ? vt[k] : row_sum(indices, ob[k], oe[k], ye);
+            if (!g.ok[k]) continue;
+            const double ti = v * vi[k];
+            t[ii[k]] = ti;
+            y[ii[k]] = vy[k] + ck * ti;
+        }
     }
 }
 
