@@ -365,6 +365,11 @@ int h1pa_batch_run(h1pa_batch *batch, int max_replays, int *n_running);
  * solve joins at the next launch); wait blocks until the replay is done. */
 int h1pa_batch_launch(h1pa_batch *batch, int *launched);
 int h1pa_batch_wait(h1pa_batch *batch, int *n_running);
+/* *idle = 1 when no replay is in flight or the one in flight has finished,
+ * so that h1pa_batch_wait would return at once; never blocks.  A serving loop
+ * polls it between admissions to start the next replay without delay (no
+ * MFEM counterpart: CGSolver::Mult is synchronous). */
+int h1pa_batch_query(h1pa_batch *batch, int *idle);
 /* Return (and retire) up to `cap` finished solves: tags[i], results[i]. */
 int h1pa_batch_poll(h1pa_batch *batch, int cap, int64_t *tags, h1pa_batch_result *results,
                     int *n);

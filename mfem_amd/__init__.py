@@ -147,6 +147,7 @@ SIGNATURES = {
     "h1pa_batch_run": (_i, [_vp, _i, C.POINTER(_i)]),
     "h1pa_batch_launch": (_i, [_vp, C.POINTER(_i)]),
     "h1pa_batch_wait": (_i, [_vp, C.POINTER(_i)]),
+    "h1pa_batch_query": (_i, [_vp, C.POINTER(_i)]),
     "h1pa_batch_poll": (_i, [_vp, _i, _pi64, C.POINTER(BatchResult), C.POINTER(_i)]),
     "h1pa_batch_info": (_i, [_vp, _pi64]),
     "h1pa_form_device_bytes": (_i, [_vp, _pi64]),
@@ -1014,6 +1015,12 @@ class BatchSolver:
         """Start one replay without waiting; False when nothing runs."""
         v = C.c_int()
         _check(lib().h1pa_batch_launch(self._h, C.byref(v)))
+        return bool(v.value)
+
+    def Idle(self):
+        """True when no replay is in flight or it has finished (never blocks)."""
+        v = C.c_int()
+        _check(lib().h1pa_batch_query(self._h, C.byref(v)))
         return bool(v.value)
 
     def Wait(self):

@@ -1237,6 +1237,17 @@ int batch_wait(Batch &B) {
     return running_slots(B);
 }
 
+bool batch_idle(Batch &B) {
+    if (!B.in_flight) return true;
+    const hipError_t e = hipStreamQuery(B.stream);
+    if (e == hipErrorNotReady) {
+        (void)hipGetLastError();  // not an error: keep it out of later checks
+        return false;
+    }
+    H1PA_HIP(e);
+    return true;
+}
+
 // Runs up to `max_replays` replays (stops early once no slot is running).
 // Returns the number of slots still running.
 int batch_run(Batch &B, int max_replays) {

@@ -931,6 +931,13 @@ int h1pa_batch_wait(h1pa_batch *batch, int *n_running) {
     API_END
 }
 
+int h1pa_batch_query(h1pa_batch *batch, int *idle) {
+    API_BEGIN
+    REQUIRE(batch && idle, "batch_query: null argument");
+    *idle = batch_idle(*batch) ? 1 : 0;
+    API_END
+}
+
 int h1pa_batch_run(h1pa_batch *batch, int max_replays, int *n_running) {
     API_BEGIN
     REQUIRE(batch && max_replays >= 0, "batch_run: bad argument");

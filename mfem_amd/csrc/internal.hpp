@@ -308,6 +308,9 @@ int batch_add(h1pa_batch &B, h1pa_form *form, const uint8_t *mask, const double 
               const h1pa_pcg_opts &po, int64_t tag, hipStream_t cs);
 bool batch_launch(h1pa_batch &B);
 int batch_wait(h1pa_batch &B);
+// True when no replay is in flight or the one in flight has finished (a
+// batch_wait would return at once); never blocks.
+bool batch_idle(h1pa_batch &B);
 int batch_run(h1pa_batch &B, int max_replays);
 int batch_poll(h1pa_batch &B, int cap, int64_t *tags, h1pa_batch_result *res);
 void batch_info(const h1pa_batch &B, int64_t *info);

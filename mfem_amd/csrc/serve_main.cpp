@@ -9,10 +9,12 @@
 // relative measure at rel_tol.
 //
 //   --mode batch   continuous batching: arrived requests are set up and
-//                  admitted while the HBM budget allows, overlapping the replay
-//                  in flight; all admitted solves advance together (one graph
-//                  replay per loop turn, h1pa_batch_launch / _wait); finished
-//                  solves are retired and their memory released.
+//                  admitted one at a time while the HBM budget allows,
+//                  overlapping the replay in flight; all admitted solves
+//                  advance together in graph replays that run back to back
+//                  (between admissions the loop checks h1pa_batch_query,
+//                  retires the finished solves, releases their memory and
+//                  launches the next replay at once).
 //   --mode serial  plain FP64 GPU PCG: requests are set up and solved one at
 //                  a time, in arrival order (h1pa_pcg_solve).
 //
@@ -287,30 +289,11 @@ int main(int argc, char **argv) {
         double t = now_s(t0);
         while (next < R && reqs[next].t_arrive <= t) queue.push_back(next++);
         if (batch) {
-            // admit in arrival order while the budget allows
-            while (!queue.empty()) {
-                const Request &q = reqs[queue.front()];
-                const int64_t est = estimate_bytes(q.n, q.p, prec, true);
-                if (!live.empty() && used + est > budget) break;
-                const double ts = now_s(t0);
-                Live L = setup(q, s);
-                setup_s += now_s(t0) - ts;
-                const int *ess;
-                int64_t nEss;
-                check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
-                h1pa_pcg_opts po{o.rel_tol, 0.0, 200000, 8};
-                int slot = 0;
-                check(h1pa_batch_add(B, L.form, ess, nEss, L.B, L.X, &po, q.id, &slot, s),
-                      "batch_add");
-                const int64_t acc = std::max(est, L.bytes);
-                used += acc;
-                live_est[q.id] = acc;
-                live[q.id] = L;
-                queue.pop_front();
-            }
-            peak_used = std::max(peak_used, used);
-            peak_live = std::max(peak_live, (int)live.size());
-            if (inflight) {  // the admissions above overlapped this replay
+            // Retire a finished replay and start the next one before anything
+            // else, so the GPU runs replays back to back; between checks the
+            // host admits one queued request at a time (its setup overlaps the
+            // replay in flight and it joins at the next launch).
+            auto finish_replay = [&]() {
                 int running = 0;
                 check(h1pa_batch_wait(B, &running), "batch_wait");
                 inflight = false;
@@ -329,17 +312,50 @@ int main(int argc, char **argv) {
                     live_est.erase(tags[i]);
                     ++done;
                 }
+            };
+            if (inflight) {
+                int idle = 0;
+                check(h1pa_batch_query(B, &idle), "batch_query");
+                if (idle) finish_replay();
             }
-            if (live.empty()) {
-                if (next < R) {
-                    const double wait = reqs[next].t_arrive - now_s(t0);
-                    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            if (!inflight && !live.empty()) {
+                int launched = 0;
+                check(h1pa_batch_launch(B, &launched), "batch_launch");
+                inflight = launched != 0;
+                if (!launched) finish_replay();  // every live solve has finished: retire them
+            }
+            if (!queue.empty()) {
+                const Request &q = reqs[queue.front()];
+                const int64_t est = estimate_bytes(q.n, q.p, prec, true);
+                if (live.empty() || used + est <= budget) {
+                    const double ts = now_s(t0);
+                    Live L = setup(q, s);
+                    setup_s += now_s(t0) - ts;
+                    const int *ess;
+                    int64_t nEss;
+                    check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+                    h1pa_pcg_opts po{o.rel_tol, 0.0, 200000, 8};
+                    int slot = 0;
+                    check(h1pa_batch_add(B, L.form, ess, nEss, L.B, L.X, &po, q.id, &slot, s),
+                          "batch_add");
+                    const int64_t acc = std::max(est, L.bytes);
+                    used += acc;
+                    live_est[q.id] = acc;
+                    live[q.id] = L;
+                    queue.pop_front();
+                    peak_used = std::max(peak_used, used);
+                    peak_live = std::max(peak_live, (int)live.size());
+                    continue;
                 }
+            }
+            if (inflight) {  // nothing to admit (or no room): wait for the replay
+                finish_replay();
                 continue;
             }
-            int launched = 0;
-            check(h1pa_batch_launch(B, &launched), "batch_launch");
-            inflight = launched != 0;
+            if (live.empty() && queue.empty() && next < R) {
+                const double wait = reqs[next].t_arrive - now_s(t0);
+                if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+            }
         } else {
             if (queue.empty()) {
                 if (next < R) {

@@ -206,3 +206,33 @@ def test_add_while_replay_in_flight(h1pa, torch_cuda, probs):
         alone = _run_batch(h1pa, [Q], P, rel_tol=1e-10, k=4)
         assert alone[Q.seed][1] == res[Q.seed]
         assert np.array_equal(alone[Q.seed][0], xs[Q.seed].cpu().numpy())
+
+
+def test_batch_query_loop_matches_run(h1pa, torch_cuda):
+    """Replays driven by Launch / Idle (the non-blocking h1pa_batch_query) /
+    Wait, as the serving loop drives them, give bitwise Run's results; Idle
+    is true whenever nothing is in flight."""
+    import time
+    M = h1pa
+    P = Prob(M, torch_cuda, 4, 3, 21)
+    out = []
+    for mode in ("run", "query"):
+        bs = M.BatchSolver(M.PREC_BF16_PA, iters_per_replay=8)
+        X = P.fresh_x()
+        bs.Add(P.A, P.B, X, rel_tol=1e-10, max_iter=20000, tag=7)
+        assert bs.Idle()
+        if mode == "run":
+            assert bs.Run() == 0
+        else:
+            t0 = time.time()
+            while bs.Launch():
+                while not bs.Idle():
+                    assert time.time() - t0 < 120, "replay did not finish"
+                    time.sleep(1e-4)
+                bs.Wait()
+                assert bs.Idle()
+        res = dict(bs.Poll())
+        assert res[7]["converged"]
+        out.append((X.cpu().numpy(), res[7]["iterations"]))
+    assert out[0][1] == out[1][1]
+    assert np.array_equal(out[0][0], out[1][0])

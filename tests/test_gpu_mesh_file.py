@@ -1,7 +1,8 @@
 """GPU: spaces, PA operators and solves on meshes read from MFEM v1.0 files,
 against the oracle on the same file.
 
-Bars: restriction maps and boundary DOFs bit-exact; PA diffusion and mass
+Bars: restriction maps, boundary DOFs and the gather / scatter operations
+bit-exact; PA diffusion and mass
 applies within 1e-13 relative; Jacobi-PCG iteration count within +-1 and
 solution within 1e-9 (rel_tol 1e-12).
 """
@@ -64,6 +65,21 @@ def test_maps_and_boundary_bit_exact(orc, h1pa, torch_cuda, name, p):
     assert np.array_equal(S.host(off), S.P.offsets)
     assert np.array_equal(S.host(ind), S.P.indices)
     assert np.array_equal(S.host(S.fes.GetBoundaryTrueDofs()), S.P.ess)
+    # gather and the transpose family on the file mesh's own numbering
+    D3 = (p + 1) ** 3
+    rng = np.random.default_rng(p)
+    x = rng.uniform(-1, 1, S.P.ndofs)
+    xe = rng.uniform(-1, 1, S.P.gather_map.size)
+    ye = S.torch.empty(xe.size, dtype=S.torch.float64, device="cuda")
+    R.Mult(S.dev(x), ye)
+    assert np.array_equal(S.host(ye), orc.restriction_mult(S.P.gather_map, D3, x))
+    y = S.torch.empty(S.P.ndofs, dtype=S.torch.float64, device="cuda")
+    R.MultTranspose(S.dev(xe), y)
+    assert np.array_equal(S.host(y), orc.restriction_mult_transpose(S.P.offsets, S.P.indices, D3,
+                                                                     xe, 0))
+    R.AbsMultTranspose(S.dev(xe), y)
+    assert np.array_equal(S.host(y), orc.restriction_mult_transpose(S.P.offsets, S.P.indices, D3,
+                                                                     xe, 2))
 
 
 @pytest.mark.parametrize("name", NAMES)
