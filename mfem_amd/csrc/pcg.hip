@@ -70,7 +70,9 @@ inline Chunks chunks(int64_t n) {
 // group at the clamped indices g.lt[k] before the group's first store, so the
 // loads of a group go out together instead of one entry's round trip after
 // another; stores and sums then run over the g.ok entries in k order, which
-// is the H1PA_CHUNK order (dot products come out bit-identical).
+// is the H1PA_CHUNK order (dot products come out bit-identical).  Slots at or
+// past `per` belong to no entry of this thread (small vectors have per < kGrp):
+// kernels issue no loads for them and skip them, as every such slot is not ok.
 constexpr int kGrp = 4;
 struct Grp {
     int64_t t[kGrp], lt[kGrp];
@@ -160,6 +162,7 @@ k_pcg_init(int64_t n, int per, const T *__restrict__ b, const T *__restrict__ di
         T vb[kGrp], vi[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             vb[k] = b[g.lt[k]];
             vi[k] = dinv[g.lt[k]];
         }
@@ -220,12 +223,16 @@ k_pcg_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
         // the group's rows, then their offsets, masks and d values, as batches
         int64_t ii[kGrp];
 #pragma unroll
-        for (int k = 0; k < kGrp; ++k) ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
+            ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        }
         T vd[kGrp];
         int ob[kGrp], oe[kGrp];
         bool m[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             vd[k] = d[ii[k]];
             ob[k] = offsets[ii[k]];
             oe[k] = offsets[ii[k] + 1];
@@ -233,13 +240,13 @@ k_pcg_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
         }
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;  // (every slot past `per` is not ok)
             T v;
             if (g.t[k] >= bb) {  // interior DOF: written by the element kernel
                 v = z[ii[k]];
             } else {
                 v = m[k] ? vd[k] : row_sum(indices, ob[k], oe[k], ye);
             }
-            if (!g.ok[k]) continue;
             if (g.t[k] < bb) z[ii[k]] = v;
             acc[0] += (double)vd[k] * (double)v;
         }
@@ -274,6 +281,7 @@ k_pcg_update(int64_t n, int per, const T *__restrict__ dinv, T *__restrict__ x, 
         T vd[kGrp], vx[kGrp], vr[kGrp], vz[kGrp], vi[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             const int64_t j = g.lt[k];
             vd[k] = d[j];
             vx[k] = x[j];
@@ -333,6 +341,7 @@ k_pcg_dir(int64_t n, int per, const T *__restrict__ z, T *__restrict__ d, const 
         T vz[kGrp], vd[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             vz[k] = z[g.lt[k]];
             vd[k] = d[g.lt[k]];
         }
@@ -450,12 +459,16 @@ k_ir_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
     H1PA_GROUPS(g, n) {
         int64_t ii[kGrp];
 #pragma unroll
-        for (int k = 0; k < kGrp; ++k) ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
+            ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        }
         float vd[kGrp];
         int ob[kGrp], oe[kGrp];
         bool m[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             vd[k] = d[ii[k]];
             ob[k] = offsets[ii[k]];
             oe[k] = offsets[ii[k] + 1];
@@ -463,13 +476,13 @@ k_ir_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
         }
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;  // (every slot past `per` is not ok)
             float v;
             if (g.t[k] >= bb) {  // interior DOF: written by the element kernel
                 v = q[ii[k]];
             } else {
                 v = m[k] ? vd[k] : row_sum(indices, ob[k], oe[k], ye);
             }
-            if (!g.ok[k]) continue;
             if (g.t[k] < bb) q[ii[k]] = v;
             acc[0] += (double)vd[k] * (double)v;
         }
@@ -499,6 +512,7 @@ k_ir_update(int64_t n, int per, const float *__restrict__ dinv, float *__restric
         float vs[kGrp], vd[kGrp], vr[kGrp], vq[kGrp], vi[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             const int64_t j = g.lt[k];
             vs[k] = xs[j];
             vd[k] = d[j];
@@ -622,6 +636,7 @@ k_ir_dir(int64_t n, int per, const float *__restrict__ z, float *__restrict__ d,
         float vz[kGrp], vd[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             vz[k] = z[g.lt[k]];
             vd[k] = d[g.lt[k]];
         }
@@ -644,6 +659,7 @@ k_cheb_first(int64_t n, int per, const double *__restrict__ dinv, const double *
         double vx[kGrp], vi[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             vx[k] = x[g.lt[k]];
             vi[k] = dinv[g.lt[k]];
         }
@@ -669,12 +685,16 @@ k_cheb_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
     H1PA_GROUPS(g, n) {
         int64_t ii[kGrp];
 #pragma unroll
-        for (int k = 0; k < kGrp; ++k) ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
+            ii[k] = g.lt[k] < nperm ? perm[g.lt[k]] : g.lt[k];
+        }
         double vi[kGrp], vy[kGrp], vt[kGrp];
         int ob[kGrp], oe[kGrp];
         bool m[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             vi[k] = dinv[ii[k]];
             vy[k] = y[ii[k]];
             vt[k] = t[ii[k]];
@@ -684,10 +704,10 @@ k_cheb_scatter(int64_t n, int per, int64_t bb, const int *__restrict__ offsets,
         }
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (!g.ok[k]) continue;  // (every slot past `per` is not ok)
             double v;
             if (g.t[k] >= bb) v = u[ii[k]];
             else v = m[k] ? vt[k] : row_sum(indices, ob[k], oe[k], ye);
-            if (!g.ok[k]) continue;
             const double ti = v * vi[k];
             t[ii[k]] = ti;
             y[ii[k]] = vy[k] + ck * ti;
@@ -729,6 +749,7 @@ k_cg_update_xr(int64_t n, int per, double *__restrict__ x, double *__restrict__ 
         double vx[kGrp], vd[kGrp], vr[kGrp], vq[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             const int64_t j = g.lt[k];
             vx[k] = x[j];
             vd[k] = d[j];
@@ -755,6 +776,7 @@ k_dot(int64_t n, int per, const double *__restrict__ a, const double *__restrict
         double va[kGrp], vb[kGrp];
 #pragma unroll
         for (int k = 0; k < kGrp; ++k) {
+            if (k >= per) break;  // past this thread's entries
             va[k] = a[g.lt[k]];
             vb[k] = b[g.lt[k]];
         }
