@@ -350,23 +350,38 @@ kb_scatter(const VecTask *__restrict__ tasks, const SlotDesc *__restrict__ desc,
            const SlotState *__restrict__ states, double *__restrict__ partials) {
     __shared__ double sh[kT];
     H1PA_TASK_PROLOGUE(ph_ == PH_INNER)
-    const T *ye = vp<T>(sd.ye), *d = vp<T>(sd.d);
-    T *z = vp<T>(sd.z);
+    const T *ye = gv<const T>(sd.ye), *d = gv<const T>(sd.d);
+    T *z = gv<T>(sd.z);
+    const int *perm = sd.perm ? global_ptr(sd.perm) : nullptr;
+    const int *off = global_ptr(sd.offsets), *ind = global_ptr(sd.indices);
+    const uint8_t *mask = sd.mask ? global_ptr(sd.mask) : nullptr;
+    // the share's rows, then their offsets, masks and d values, as batches
+    const Share h = share_of(tk.chunk, sd.n);
+    int64_t ii[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) ii[k] = h.lt[k] < sd.nperm ? perm[h.lt[k]] : h.lt[k];
+    T vd[kPer];
+    int ob[kPer], oe[kPer];
+    bool m[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        vd[k] = d[ii[k]];
+        ob[k] = off[ii[k]];
+        oe[k] = off[ii[k] + 1];
+        m[k] = mask && mask[ii[k]];
+    }
     double acc[1] = {0.0};
-    H1PA_CHUNK_LOOP(t) {
-        const int64_t i = t < sd.nperm ? sd.perm[t] : t;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if (!h.ok[k]) continue;
         T v;
-        if (t >= sd.bb) {  // interior DOF: written by the element kernel
-            v = z[i];
+        if (h.t[k] >= sd.bb) {  // interior DOF: written by the element kernel
+            v = z[ii[k]];
         } else {
-            if (sd.mask && sd.mask[i]) {
-                v = d[i];
-            } else {
-                v = row_sum(sd.indices, sd.offsets[i], sd.offsets[i + 1], ye);
-            }
-            z[i] = v;
+            v = m[k] ? vd[k] : row_sum(ind, ob[k], oe[k], ye);
+            z[ii[k]] = v;
         }
-        acc[0] += (double)d[i] * (double)v;
+        acc[0] += (double)vd[k] * (double)v;
     }
     store_chunk<1>(acc, sd, tk.chunk, partials, sh);
 }
