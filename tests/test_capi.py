@@ -36,6 +36,12 @@ def test_errors_are_reported(h1pa):
     assert "must be >= 1" in str(e.value)
     with pytest.raises(h1pa.H1paError):
         h1pa.H1_FECollection(9)
+    # null handles are refused before any device work (no GPU needed)
+    import ctypes as C
+    v = C.c_int()
+    assert h1pa.lib().h1pa_batch_query(None, C.byref(v)) == h1pa.H1PA_EINVAL
+    assert h1pa.lib().h1pa_batch_wait(None, C.byref(v)) == h1pa.H1PA_EINVAL
+    assert h1pa.lib().h1pa_mesh_load(None, 0, None) == h1pa.H1PA_EINVAL
 
 
 @pytest.mark.parametrize("dims", [(1, 1, 1), (2, 2, 2), (3, 3, 3), (4, 3, 5), (7, 5, 6),
