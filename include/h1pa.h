@@ -52,9 +52,11 @@ typedef struct h1pa_form h1pa_form;
 const char *h1pa_last_error(void);
 /* Library version string. */
 const char *h1pa_version(void);
-/* Select the HIP device for subsequent handle creation. */
+/* Select the HIP device for subsequent handle creation (Device::Configure
+ * with "hip:<id>", general/device.cpp:191, which calls hipSetDevice at :460). */
 int h1pa_set_device(int device);
-/* Block until all work the library queued on `stream` is done. */
+/* Block until all work the library queued on `stream` is done
+ * (MFEM_STREAM_SYNC / MFEM_DEVICE_SYNC, general/backends.hpp:48-51). */
 int h1pa_stream_synchronize(void *stream);
 
 /* ---- mesh: Mesh::MakeCartesian3D(nx, ny, nz, HEXAHEDRON, 1, 1, 1, sfc)
@@ -71,7 +73,9 @@ int h1pa_mesh_create_cartesian(int nx, int ny, int nz, int sfc_ordering, h1pa_me
 int h1pa_mesh_load(const char *text, int64_t len, h1pa_mesh **mesh);
 int h1pa_mesh_destroy(h1pa_mesh *mesh);
 /* Perturb interior vertices by U(-amp*h, amp*h) per coordinate (counter-based
- * splitmix64 with `seed`); boundary vertices stay on the cube faces. */
+ * splitmix64 with `seed`); boundary vertices stay on the cube faces.  No MFEM
+ * function does this; MFEM code moves vertices with Mesh::Transform
+ * (mesh/mesh.cpp:14056). */
 int h1pa_mesh_perturb(h1pa_mesh *mesh, double amp, uint64_t seed);
 /* Give the mesh order-`order` nodes (Mesh::SetCurvature, mesh/mesh.cpp:7211)
  * mapped by x' = x + amp sin(2 pi y) sin(2 pi z) and its cyclic permutations
@@ -110,6 +114,7 @@ int h1pa_space_restriction(h1pa_space *space, h1pa_restriction **restriction);
  * entries follow MFEM's -1-i convention.  Host inputs. */
 int h1pa_restriction_create(int ne, int dof, int ndofs, const int *h_elem_dof,
                             const int *h_dof_map, h1pa_restriction **restriction);
+/* Releases what h1pa_restriction_create made (~ElementRestriction). */
 int h1pa_restriction_destroy(h1pa_restriction *restriction);
 /* Device pointers of gather_map [ne*dof], offsets [ndofs+1], indices [ne*dof]
  * (GatherMap/Offsets/Indices, fem/restriction.hpp:105-107). */
@@ -143,7 +148,9 @@ enum {
 };
 enum { H1PA_QI_BY_NODES = 0, H1PA_QI_BY_VDIM = 1 };  /* QVectorLayout */
 /* Evaluate the mesh nodes at the q1d^3 Gauss-Legendre points: X[nq][3][ne],
- * J[nq][3][3][ne] (J(q,r,c,e) = dx_r/dxi_c), detJ[nq][ne]; NULL skips. */
+ * J[nq][3][3][ne] (J(q,r,c,e) = dx_r/dxi_c), detJ[nq][ne]; NULL skips
+ * (Mesh::GetGeometricFactors, mesh/mesh.cpp:958; GeometricFactors::Compute,
+ * :15445-15497). */
 int h1pa_mesh_geometric_factors(const h1pa_mesh *mesh, int q1d, unsigned flags,
                                 double *d_X, double *d_J, double *d_detJ, void *stream);
 
@@ -238,7 +245,8 @@ typedef struct {
     double final_norm;    /* sqrt((B r, r)), CGSolver::GetFinalNorm        */
     double seconds;       /* wall time of the iteration loop               */
 } h1pa_pcg_stats;
-/* OperatorJacobiSmoother(a, ess) + CGSolver on the DIAG_ONE ConstrainedOperator,
+/* OperatorJacobiSmoother(a, ess) (linalg/solvers.cpp:401-453) + CGSolver
+ * (:869-1050) on the DIAG_ONE ConstrainedOperator (linalg/operator.cpp:601-662),
  * iterative_mode = false (x is overwritten; initial guess 0). */
 int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *d_b,
                    double *d_x, const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats,
@@ -262,14 +270,18 @@ int h1pa_chebyshev_create(h1pa_form *form, const double *d_diag, const int *d_es
                           int64_t n_ess, int order, double max_eig_estimate,
                           int power_iterations, double power_tolerance, int power_seed,
                           h1pa_chebyshev **cheb, void *stream);
+/* ~OperatorChebyshevSmoother. */
 int h1pa_chebyshev_destroy(h1pa_chebyshev *cheb);
 /* The eigenvalue estimate and the order polynomial coefficients (coeffs has
- * room for order values; either pointer may be NULL). */
+ * room for order values; either pointer may be NULL) -- the values the
+ * constructor computes, linalg/solvers.cpp:546-617. */
 int h1pa_chebyshev_info(const h1pa_chebyshev *cheb, double *max_eig, double *coeffs);
-/* OperatorChebyshevSmoother::Mult: y = sum_k c_k (diag^-1 A)^k diag^-1 x. */
+/* OperatorChebyshevSmoother::Mult (linalg/solvers.cpp:619-656):
+ * y = sum_k c_k (diag^-1 A)^k diag^-1 x. */
 int h1pa_chebyshev_mult(h1pa_chebyshev *cheb, const double *d_x, double *d_y, void *stream);
 /* CGSolver with SetPreconditioner(smoother) on the smoother's constrained
- * operator, iterative_mode = false; as h1pa_pcg_solve otherwise. */
+ * operator (linalg/solvers.cpp:869-1050), iterative_mode = false; as
+ * h1pa_pcg_solve otherwise. */
 int h1pa_pcg_solve_chebyshev(h1pa_chebyshev *cheb, const double *d_b, double *d_x,
                              const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats, void *stream);
 
@@ -344,6 +356,7 @@ typedef struct {
     double final_norm;     /* FP64: sqrt((B r, r)); mixed: ||b - A x||_2        */
     double rel_residual;   /* mixed: ||b - A x||_2 / ||b||_2; FP64: -1          */
 } h1pa_batch_result;
+/* No MFEM counterpart (MFEM runs one CGSolver::Mult at a time). */
 int h1pa_batch_create(const h1pa_batch_opts *opts, h1pa_batch **batch);
 int h1pa_batch_destroy(h1pa_batch *batch);
 /* Add the solve of the DIAG_ONE constrained system (ess list) A x = b.
@@ -376,7 +389,8 @@ int h1pa_batch_poll(h1pa_batch *batch, int cap, int64_t *tags, h1pa_batch_result
 /* info[0..5] = solves added and not yet polled, solves running, device bytes
  * held by the batch's vectors, replays, graph captures, (p, q1d) groups. */
 int h1pa_batch_info(const h1pa_batch *batch, int64_t *info);
-/* Device bytes held by a form, its space and its mesh (admission control). */
+/* Device bytes held by a form, its space and its mesh (admission control; no
+ * MFEM counterpart). */
 int h1pa_form_device_bytes(const h1pa_form *form, int64_t *bytes);
 
 #ifdef __cplusplus
