@@ -433,7 +433,7 @@ def main():
                     help="run one 1 GiB device copy first (scales rocprof FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the library has no multi-GPU path "
               "(one process, one MI355X); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     legs = set(args.legs.split(","))
