@@ -532,6 +532,12 @@ class Problem:
         op.n_ess, op.ess = int(self.ess.size), _ip(self.ess)
         self._op = op
 
+    def set_ess(self, ess):
+        """Replace the essential (DIAG_ONE) list of the constrained operator
+        (ConstrainedOperator's list, linalg/operator.cpp:601-662)."""
+        self.ess = np.ascontiguousarray(ess, dtype=np.int32)
+        self._op.n_ess, self._op.ess = int(self.ess.size), _ip(self.ess)
+
     def replace_pa(self, Dd=None, Mv=None):
         """Swap in other PA data of the same shape (e.g. rounded copies)."""
         if Dd is not None:
