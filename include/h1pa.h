@@ -111,7 +111,8 @@ int h1pa_space_restriction(h1pa_space *space, h1pa_restriction **restriction);
 /* ---- ElementRestriction (fem/restriction.hpp:40-108) ---------------------- */
 /* Build from MFEM's own element-to-DOF table and lexicographic dof_map, as the
  * ElementRestriction constructor does (fem/restriction.cpp:26-114).  Signed
- * entries follow MFEM's -1-i convention.  Host inputs. */
+ * entries follow MFEM's -1-i convention.  h_dof_map = NULL gives the NATIVE
+ * element ordering (no reordering, restriction.cpp:60-98).  Host inputs. */
 int h1pa_restriction_create(int ne, int dof, int ndofs, const int *h_elem_dof,
                             const int *h_dof_map, h1pa_restriction **restriction);
 /* Releases what h1pa_restriction_create made (~ElementRestriction). */

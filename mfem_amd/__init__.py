@@ -458,9 +458,11 @@ class FiniteElementSpace:
         r = C.c_void_p()
         _check(lib().h1pa_space_restriction(h, C.byref(r)))
         self._restr = ElementRestriction(r, self.ne, self.dof, self.ndofs, owner=self, owned=False)
+        self._restr_native = None
 
     def __del__(self, _d=_destroy):
         self._restr = None
+        self._restr_native = None
         _d("h1pa_space_destroy", getattr(self, "_h", None))
         self._h = None
 
@@ -477,9 +479,17 @@ class FiniteElementSpace:
         return self.ndofs
 
     def GetElementRestriction(self, ordering=ElementDofOrdering.LEXICOGRAPHIC):
-        if ordering != ElementDofOrdering.LEXICOGRAPHIC:
-            raise H1paError(H1PA_EUNSUP, "only LEXICOGRAPHIC element ordering is provided")
-        return self._restr
+        """ElementRestriction in LEXICOGRAPHIC order (the space's own, used by
+        PA) or NATIVE order (built once from the element-to-DOF table without
+        the dof_map reordering, fem/restriction.cpp:60-98)."""
+        if ordering == ElementDofOrdering.LEXICOGRAPHIC:
+            return self._restr
+        if ordering != ElementDofOrdering.NATIVE:
+            raise H1paError(H1PA_EINVAL, "ordering must be NATIVE or LEXICOGRAPHIC")
+        if self._restr_native is None:
+            ed = self.GetElementToDofTable().cpu().numpy()
+            self._restr_native = ElementRestriction.FromTable(ed, None, self.ndofs)
+        return self._restr_native
 
     def GetElementToDofTable(self):
         """Native-order element-to-DOF table as an int32 [ne, dof] device tensor."""

@@ -147,6 +147,37 @@ def test_restriction_from_mfem_table(orc, h1pa, torch_cuda, p):
         assert np.array_equal(y.cpu().numpy(), orc.restriction_mult_transpose(off, ind, D3, xe, 0))
 
 
+@pytest.mark.parametrize("p", [1, 3, 5])
+def test_native_ordering_restriction(orc, h1pa, torch_cuda, p):
+    """GetElementRestriction(NATIVE): the element-to-DOF table without the
+    lexicographic reordering (restriction.cpp:60-98), maps and operations
+    bit-exact against the oracle built with no dof_map."""
+    torch = torch_cuda
+    dims = (3, 2, 2)
+    m = h1pa.Mesh.MakeCartesian3D(*dims)
+    fes = h1pa.FiniteElementSpace(m, h1pa.H1_FECollection(p))
+    R = fes.GetElementRestriction(h1pa.ElementDofOrdering.NATIVE)
+    assert fes.GetElementRestriction(h1pa.ElementDofOrdering.NATIVE) is R
+    om = orc.CartesianMesh(*dims)
+    ed, ndofs = om.elem_dofs(p)
+    g, off, ind = orc.restriction(ed, None, ndofs)
+    assert np.array_equal(R.GatherMap().cpu().numpy(), g)
+    assert np.array_equal(R.Offsets().cpu().numpy(), off)
+    assert np.array_equal(R.Indices().cpu().numpy(), ind)
+    D3 = (p + 1) ** 3
+    rng = np.random.default_rng(p)
+    x = rng.uniform(-1, 1, ndofs)
+    xe = rng.uniform(-1, 1, g.size)
+    ye = torch.empty(g.size, dtype=torch.float64, device="cuda")
+    R.Mult(torch.from_numpy(x).cuda(), ye)
+    assert np.array_equal(ye.cpu().numpy(), orc.restriction_mult(g, D3, x))
+    y = torch.empty(ndofs, dtype=torch.float64, device="cuda")
+    R.MultTranspose(torch.from_numpy(xe).cuda(), y)
+    assert np.array_equal(y.cpu().numpy(), orc.restriction_mult_transpose(off, ind, D3, xe, 0))
+    with pytest.raises(h1pa.H1paError):
+        fes.GetElementRestriction(7)
+
+
 @pytest.mark.parametrize("case", [dict(perturb=0.2, seed=3), dict(curve=0.05)])
 @pytest.mark.parametrize("p", [1, 2, 4, 6])
 def test_geometric_factors(orc, h1pa, torch_cuda, p, case):
