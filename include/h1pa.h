@@ -198,6 +198,14 @@ int h1pa_form_assemble(h1pa_form *form, void *stream);
 /* Operator::Mult: y = A x on L-vectors, y overwritten
  * (PABilinearFormExtension::Mult, fem/bilinearform_ext.cpp:487-560). */
 int h1pa_form_mult(h1pa_form *form, const double *d_x, double *d_y, void *stream);
+/* Operator::AddMult: y += a A x (PABilinearFormExtension inherits
+ * Operator::AddMult, linalg/operator.cpp:51-56: A x into a form-owned
+ * L-vector, then y += a z).  The mass and diffusion operators here are
+ * symmetric, so MultTranspose / AddMultTranspose are h1pa_form_mult /
+ * h1pa_form_add_mult (DiffusionIntegrator::AddMultTransposePA,
+ * fem/integ/bilininteg_diffusion_pa.cpp:95-106; MassIntegrator's,
+ * bilininteg_mass_pa.cpp:230-234). */
+int h1pa_form_add_mult(h1pa_form *form, const double *d_x, double *d_y, double a, void *stream);
 /* AddMultPA on E-vectors: yE += A_E xE (fem/bilininteg.hpp:74-81). */
 int h1pa_form_add_mult_e(h1pa_form *form, const double *d_xE, double *d_yE, void *stream);
 /* Operator::AssembleDiagonal: L-vector diagonal, set

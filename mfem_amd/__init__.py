@@ -119,6 +119,7 @@ SIGNATURES = {
     "h1pa_form_destroy": (_i, [_vp]),
     "h1pa_form_assemble": (_i, [_vp, _vp]),
     "h1pa_form_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_form_add_mult": (_i, [_vp, _vp, _vp, C.c_double, _vp]),
     "h1pa_form_add_mult_e": (_i, [_vp, _vp, _vp, _vp]),
     "h1pa_form_assemble_diagonal": (_i, [_vp, _vp, _vp]),
     "h1pa_form_pa_data": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
@@ -700,9 +701,23 @@ class BilinearForm:
         _check_vec(x, n); _check_vec(y, n)
         _check(lib().h1pa_form_mult(self._forms[0]._h, _ptr(x), _ptr(y), _stream()))
         for f in self._forms[1:]:
-            t = self._scratch()
-            _check(lib().h1pa_form_mult(f._h, _ptr(x), _ptr(t), _stream()))
-            y.add_(t)
+            _check(lib().h1pa_form_add_mult(f._h, _ptr(x), _ptr(y), 1.0, _stream()))
+
+    def AddMult(self, x, y, a=1.0):
+        """y += a A x (Operator::AddMult, linalg/operator.cpp:51-56)."""
+        n = self.fes.ndofs
+        _check_vec(x, n); _check_vec(y, n)
+        for f in self._forms:
+            _check(lib().h1pa_form_add_mult(f._h, _ptr(x), _ptr(y), float(a), _stream()))
+
+    def MultTranspose(self, x, y):
+        """y = A^T x, which is A x: the PA mass and diffusion operators are
+        symmetric (fem/integ/bilininteg_diffusion_pa.cpp:95-106)."""
+        self.Mult(x, y)
+
+    def AddMultTranspose(self, x, y, a=1.0):
+        """y += a A^T x = y + a A x."""
+        self.AddMult(x, y, a)
 
     def MultTimed(self, x, y, steps):
         """`steps` applies with HIP events around every launch; returns

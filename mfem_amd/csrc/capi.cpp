@@ -611,6 +611,22 @@ int h1pa_form_mult(h1pa_form *form, const double *x, double *y, void *stream) {
     API_END
 }
 
+int h1pa_form_add_mult(h1pa_form *form, const double *x, double *y, double a, void *stream) {
+    API_BEGIN
+    REQUIRE(form && x && y, "form_add_mult: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    hipStream_t s = as_stream(stream);
+    if (!f.lwork0.p) f.lwork0.alloc(sp.ndofs);
+    element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, f.lwork0.p, false, s);
+    kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                  sp.restr.indices_nat.p, f.ework0.p, f.lwork0.p, 0,
+                                  sp.restr.perm.p, sp.restr.nperm, s);
+    kern::axpy(sp.ndofs, a, f.lwork0.p, y, s);
+    API_END
+}
+
 int h1pa_form_mult_timed(h1pa_form *form, const double *x, double *y, int steps, double *ms_total,
                          double *ms_element, double *ms_scatter, void *stream) {
     API_BEGIN

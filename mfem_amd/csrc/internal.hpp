@@ -263,6 +263,8 @@ void jacobi_dinv(int64_t n, const double *diag, const uint8_t *mask, double *din
 void jacobi_dinv_f32(int64_t n, const double *diag, const uint8_t *mask, float *dinv,
                      hipStream_t s);
 void form_x(int64_t n, const uint8_t *mask, const double *x, double *X, hipStream_t s);
+// y += a z on n entries
+void axpy(int64_t n, double a, const double *z, double *y, hipStream_t s);
 void form_b(int64_t n, const uint8_t *mask, const double *X, const double *b, const double *z,
             double *B, hipStream_t s);
 }  // namespace kern

@@ -808,6 +808,12 @@ k_jacobi_mult(int64_t n, int per, const double *__restrict__ dinv, const double 
     H1PA_CHUNK(i, n) y[i] = 0.0 + dinv[i] * u[i];
 }
 
+// y += a z (Vector::Add, linalg/vector.cpp:436-475)
+__global__ void k_axpy(int64_t n, double a, const double *__restrict__ z, double *__restrict__ y) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) y[i] += a * z[i];
+}
+
 __global__ void k_form_x(int64_t n, const uint8_t *__restrict__ mask, const double *__restrict__ x,
                          double *__restrict__ X) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -833,6 +839,12 @@ void jacobi_dinv_f32(int64_t n, const double *diag, const uint8_t *mask, float *
                      hipStream_t s) {
     hipLaunchKernelGGL(k_jacobi_dinv<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
                        diag, mask, dinv);
+    H1PA_HIP(hipGetLastError());
+}
+
+void axpy(int64_t n, double a, const double *z, double *y, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_axpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, a, z, y);
     H1PA_HIP(hipGetLastError());
 }
 

@@ -147,6 +147,31 @@ def test_restriction_from_mfem_table(orc, h1pa, torch_cuda, p):
         assert np.array_equal(y.cpu().numpy(), orc.restriction_mult_transpose(off, ind, D3, xe, 0))
 
 
+@pytest.mark.parametrize("p", [2, 4, 7])
+def test_add_mult_and_transpose(orc, h1pa, torch_cuda, p):
+    """Operator::AddMult (y += a A x, linalg/operator.cpp:51-56) within 1e-13 of
+    the oracle; MultTranspose = Mult bitwise (symmetric operators);
+    AddMultTranspose = AddMult bitwise."""
+    torch = torch_cuda
+    S = Pair(orc, h1pa, torch, (3, 2, 2), p, perturb=0.2, seed=p, coeff="sincos",
+             integrators=("diffusion", "mass"))
+    rng = np.random.default_rng(p)
+    x = rng.uniform(-1, 1, S.P.ndofs)
+    y0 = rng.uniform(-1, 1, S.P.ndofs)
+    y = S.dev(y0.copy())
+    S.a.AddMult(S.dev(x), y, -0.7)
+    ref = y0 - 0.7 * S.P.mult(x)
+    assert _rel(S.host(y), ref) < 1e-13
+    y2 = S.dev(y0.copy())
+    S.a.AddMultTranspose(S.dev(x), y2, -0.7)
+    assert torch.equal(y, y2)
+    m1 = torch.empty_like(y)
+    m2 = torch.empty_like(y)
+    S.a.Mult(S.dev(x), m1)
+    S.a.MultTranspose(S.dev(x), m2)
+    assert torch.equal(m1, m2)
+
+
 @pytest.mark.parametrize("p", [1, 3, 5])
 def test_native_ordering_restriction(orc, h1pa, torch_cuda, p):
     """GetElementRestriction(NATIVE): the element-to-DOF table without the
