@@ -233,9 +233,10 @@ void launch_diffusion_v(int64_t ne, const T *basis, const TP *pa, const int *gma
 // Variants of the diffusion apply: (elements per block, line-loop unroll,
 // PA-data pipeline depth: 0, 2 planes or all of them; 6 and 7 are 0 and 4
 // with the registers capped so that as many blocks stay resident as the LDS
-// allows).  Every variant performs the same arithmetic per element in the
-// same order, so they give bit-identical results and differ only in speed.
-constexpr int kDiffVariants = 8;
+// allows; 8 is 4 with half the elements per block).  Every variant performs
+// the same arithmetic per element in the same order, so they give
+// bit-identical results and differ only in speed.
+constexpr int kDiffVariants = 9;
 template <int D, int Q>
 void launch_diffusion(int variant, int64_t ne, const double *basis, const double *pa,
                       const int *gmap, const uint8_t *mask, const double *x, const EOut &out,
@@ -252,6 +253,7 @@ void launch_diffusion(int variant, int64_t ne, const double *basis, const double
         case 5: launch_diffusion_v<T, TP, D, Q, NBmax, 1, kAllPlanes>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         case 6: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 0, lds_waves<T, D, Q, NBmax>()>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         case 7: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 2, lds_waves<T, D, Q, NBmax>()>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
+        case 8: launch_diffusion_v<T, TP, D, Q, NBhalf, 1, 2>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
         default: launch_diffusion_v<T, TP, D, Q, NBmax, 1, 0>(ne, basis, pa, gmap, mask, x, out, acc, s); break;
     }
 }
@@ -271,7 +273,10 @@ void launch_diffusion_lp(int variant, int64_t ne, const float *basis, const TP *
                          const int *gmap, const uint8_t *mask, const float *x,
                          const EOutT<float> &out, bool acc, hipStream_t s) {
     constexpr int NBmax = nb_for(Q, 3 * D * Q * Q, sizeof(float));
-    if (variant == 1)
+    constexpr int NBhalf = NBmax / 2 > 0 ? NBmax / 2 : 1;
+    if (variant == 8)
+        launch_diffusion_v<float, TP, D, Q, NBhalf, 1, 2>(ne, basis, pa, gmap, mask, x, out, acc, s);
+    else if (variant == 1)
         launch_diffusion_v<float, TP, D, Q, NBmax, 2, 0>(ne, basis, pa, gmap, mask, x, out, acc, s);
     else if (variant == 4)
         launch_diffusion_v<float, TP, D, Q, NBmax, 1, 2>(ne, basis, pa, gmap, mask, x, out, acc, s);

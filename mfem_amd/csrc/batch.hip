@@ -642,7 +642,7 @@ constexpr int nb_half() { return nb_max<T, D, Q>() / 2 > 0 ? nb_max<T, D, Q>() /
 // Elements per block of variant v for value type T (same table as the
 // single-problem kernels).
 template <typename T, int D, int Q>
-int nb_of(int v) { return (v == 2 || v == 3) ? nb_half<T, D, Q>() : nb_max<T, D, Q>(); }
+int nb_of(int v) { return (v == 2 || v == 3 || v == 8) ? nb_half<T, D, Q>() : nb_max<T, D, Q>(); }
 
 template <typename T, typename TP, int D, int Q, bool RU>
 void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc *desc,
@@ -658,16 +658,17 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
             case 5: launch_kb<T, TP, D, Q, NM, 1, kAllPlanes, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 6: launch_kb<T, TP, D, Q, NM, 1, 0, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
             case 7: launch_kb<T, TP, D, Q, NM, 1, 2, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 8: launch_kb<T, TP, D, Q, NH, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
         }
     } else {
-        (void)NH;
         switch (v) {
             case 1: launch_kb<T, TP, D, Q, NM, 2, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 4: launch_kb<T, TP, D, Q, NM, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 5: launch_kb<T, TP, D, Q, NM, 1, kAllPlanes, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             case 6: launch_kb<T, TP, D, Q, NM, 1, 0, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
             case 7: launch_kb<T, TP, D, Q, NM, 1, 2, RU, OW>(tasks, ntasks, desc, st, basis, force, s); return;
+            case 8: launch_kb<T, TP, D, Q, NH, 1, 2, RU>(tasks, ntasks, desc, st, basis, force, s); return;
             default: launch_kb<T, TP, D, Q, NM, 1, 0, RU>(tasks, ntasks, desc, st, basis, force, s); return;
         }
     }
@@ -681,9 +682,9 @@ void launch_kb_variant(int v, const ElemTask *tasks, int ntasks, const SlotDesc 
 // ---------------------------------------------------------------------------
 namespace {
 
-// Variants valid for the inner type (FP32 applies have 0, 1, 4, 5, 6, 7).
+// Variants valid for the inner type (FP32 applies have 0, 1, 4, 5, 6, 7, 8).
 bool variant_ok(int prec, int v) {
-    return prec == H1PA_PREC_FP64 ? (v >= 0 && v < 8) : (v == 0 || v == 1 || (v >= 4 && v < 8));
+    return prec == H1PA_PREC_FP64 ? (v >= 0 && v < 9) : (v == 0 || v == 1 || (v >= 4 && v < 9));
 }
 
 #define H1PA_BATCH_DQ(X) X(2, 3) X(3, 4) X(4, 5) X(5, 6) X(6, 7) X(7, 8) X(8, 9) X(9, 10)
@@ -918,7 +919,7 @@ void autotune(Batch &B) {
             G.tuned = true;
             float best = 1e30f;
             int bestv = G.variant;
-            for (int v = 0; v < 8; ++v) {
+            for (int v = 0; v < 9; ++v) {
                 if (!variant_ok(B.o.precision, v)) continue;
                 const int nb = nb_of_dq(G.p + 1, G.q1d, v, B.o.precision == H1PA_PREC_FP64);
                 element_tasks(B, (int)g, nb, et);

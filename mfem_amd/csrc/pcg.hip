@@ -968,15 +968,15 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
 namespace {
 // Pick the reduced-precision diffusion variant for (kind, p, Q) once per
 // process on the first large enough form: 2 warm-up + 5 timed launches each.
-// H1PA_DIFF_VARIANT=<v> forces variant v (0, 1, 4, 5, 6 or 7).
+// H1PA_DIFF_VARIANT=<v> forces variant v (0, 1, 4, 5, 6, 7 or 8).
 bool g_lp_tuned[2][10][11] = {};
 void autotune_lp(Form &f, int kind, hipStream_t s) {
     Space &sp = *f.space;
     const int p = sp.p, Q = f.q1d;
     if (const char *v = std::getenv("H1PA_DIFF_VARIANT")) {
         const int fv = std::atoi(v);
-        H1PA_CHECK(fv == 0 || fv == 1 || (fv >= 4 && fv < 8), H1PA_EINVAL,
-                   "H1PA_DIFF_VARIANT: reduced-precision variants are 0, 1, 4, 5, 6, 7");
+        H1PA_CHECK(fv == 0 || fv == 1 || (fv >= 4 && fv < 9), H1PA_EINVAL,
+                   "H1PA_DIFF_VARIANT: reduced-precision variants are 0, 1, 4, 5, 6, 7, 8");
         kern::set_diffusion_variant_lp(kind, p, Q, fv);
         return;
     }
@@ -994,7 +994,7 @@ void autotune_lp(Form &f, int kind, hipStream_t s) {
     out.l2n = sp.restr.lex2nat.p;
     int best = 0;
     float best_ms = 1e30f;
-    for (int v : {0, 1, 4, 5, 6, 7}) {
+    for (int v : {0, 1, 4, 5, 6, 7, 8}) {
         for (int w = 0; w < 2; ++w)
             kern::diffusion_apply_lp_variant(v, p, Q, sp.ne, f.basis32.p, f.pa_diff_lp.p, kind,
                                              sp.restr.gather_map.p, nullptr, x.p, out, false, s);

@@ -7,9 +7,9 @@ by timing at sizes no other test runs, so each is forced here in turn with
 H1PA_DIFF_VARIANT (read when a form is assembled, and when its reduced-
 precision PA data is made).
 
-Bars: FP64 applies bitwise equal across variants 0..7 and within 1e-13
+Bars: FP64 applies bitwise equal across variants 0..8 and within 1e-13
 relative of the oracle; FP32 and BF16-PA applies bitwise equal across
-variants 0, 1, 4, 5, 6, 7; a FP64 and a mixed batch give bitwise the same
+variants 0, 1, 4, 5, 6, 7, 8; a FP64 and a mixed batch give bitwise the same
 solutions and iteration counts under every variant.
 """
 import numpy as np
@@ -17,8 +17,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-V64 = list(range(8))
-V32 = [0, 1, 4, 5, 6, 7]
+V64 = list(range(9))
+V32 = [0, 1, 4, 5, 6, 7, 8]
 
 
 def _rel(a, b):
@@ -97,7 +97,7 @@ def test_low_precision_variants_bitwise(h1pa, torch_cuda, monkeypatch, p, prec):
 def test_variant_out_of_range_is_an_error(h1pa, torch_cuda, monkeypatch):
     M = h1pa
     mesh = M.Mesh.MakeCartesian3D(2, 2, 2)
-    monkeypatch.setenv("H1PA_DIFF_VARIANT", "9")
+    monkeypatch.setenv("H1PA_DIFF_VARIANT", "9")  # one past the last
     with pytest.raises(M.H1paError):
         _form(M, mesh, 2, 4)
     monkeypatch.setenv("H1PA_DIFF_VARIANT", "2")  # FP64-only variant
