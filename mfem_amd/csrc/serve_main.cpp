@@ -393,12 +393,14 @@ int main(int argc, char **argv) {
 
     std::vector<double> lat;
     int64_t dofs = 0, conv = 0, iters = 0;
+    double dof_iters = 0.0;  // sum of DOFs x iterations: the solver work of the run
     for (const Request &q : reqs) {
         lat.push_back(q.t_done - q.t_arrive);
         const int64_t m = (int64_t)q.n * q.p + 1;
         dofs += m * m * m;
         conv += q.converged;
         iters += q.iterations;
+        dof_iters += (double)(m * m * m) * q.iterations;
     }
     std::printf(
         "{\"mode\": \"%s\", \"precision\": \"%s\", \"rate\": %.1f, \"window_s\": %.2f, "
@@ -406,10 +408,11 @@ int main(int argc, char **argv) {
         "\"latency_p50_s\": %.4f, \"latency_p90_s\": %.4f, \"latency_p99_s\": %.4f, "
         "\"latency_max_s\": %.4f, \"mean_dofs\": %.0f, \"mean_iterations\": %.1f, "
         "\"peak_live\": %d, \"peak_bytes\": %lld, \"budget_bytes\": %lld, "
-        "\"setup_s\": %.3f}\n",
+        "\"setup_s\": %.3f, \"dof_iterations\": %.4e}\n",
         o.mode.c_str(), prec == H1PA_PREC_FP64 ? "fp64" : prec == H1PA_PREC_FP32 ? "fp32" : "bf16",
         o.rate, o.seconds, R, (long long)conv, makespan, R / makespan, pct(lat, 0.5),
         pct(lat, 0.9), pct(lat, 0.99), pct(lat, 1.0), R ? (double)dofs / R : 0.0,
-        R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget, setup_s);
+        R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget, setup_s,
+        dof_iters);
     return 0;
 }
