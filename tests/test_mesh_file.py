@@ -138,3 +138,25 @@ def test_file_mesh_cannot_be_perturbed(h1pa):
     pm = h1pa.Mesh(path)
     with pytest.raises(h1pa.H1paError):
         pm.Perturb(0.1, 1)
+
+
+@pytest.mark.parametrize("variant", ["crlf", "comments", "tabs_in_data", "trailing"])
+def test_library_loader_format_variants(h1pa, tmp_path, variant):
+    """Line endings, '#' comment lines between sections (MFEM skips them),
+    tabs as separators in the data and trailing blank lines read to the same
+    topology."""
+    import os
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = os.path.join(here, MESHES, "fichera.mesh")
+    base = open(path).read()
+    head, rest = base.split("\n", 1)
+    txt = {"crlf": base.replace("\n", "\r\n"),
+           "comments": base.replace("\nelements", "\n# comment\n\n#\nelements")
+                           .replace("\nvertices", "\n# vertices follow\nvertices"),
+           "tabs_in_data": head + "\n" + rest.replace(" ", "\t"),
+           "trailing": base + "\n\n# end\n\n"}[variant]
+    f = tmp_path / "v.mesh"
+    f.write_text(txt)
+    ref, got = h1pa.Mesh(path).Topology(), h1pa.Mesh(str(f)).Topology()
+    for k in ("ev", "ee", "eeo", "ef", "efo"):
+        assert np.array_equal(got[k], ref[k]), k
