@@ -235,6 +235,8 @@ int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess,
 /* Vector::Randomize (linalg/vector.cpp:955-966) on the device with a
  * counter-based generator: d_x[i] = lo + (hi - lo) u_i, u_i =
  * (splitmix64(seed ^ splitmix64(i)) >> 11) 2^-53, reproducible for any n. */
+/* Vector::Add (linalg/vector.cpp:326): d_y += a d_x on n device entries. */
+int h1pa_vector_add(double *d_y, double a, const double *d_x, int64_t n, void *stream);
 int h1pa_vector_randomize(double *d_x, int64_t n, uint64_t seed, double lo, double hi,
                           void *stream);
 /* DomainLFIntegrator(ConstantCoefficient(f)) assembled into an L-vector with

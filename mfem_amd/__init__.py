@@ -144,6 +144,7 @@ SIGNATURES = {
     "h1pa_batch_destroy": (_i, [_vp]),
     "h1pa_batch_add": (_i, [_vp, _vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), _i64,
                             C.POINTER(_i), _vp]),
+    "h1pa_vector_add": (_i, [_vp, C.c_double, _vp, _i64, _vp]),
     "h1pa_vector_randomize": (_i, [_vp, _i64, C.c_uint64, _d, _d, _vp]),
     "h1pa_batch_run": (_i, [_vp, _i, C.POINTER(_i)]),
     "h1pa_batch_launch": (_i, [_vp, C.POINTER(_i)]),
@@ -747,7 +748,7 @@ class BilinearForm:
         for f in self._forms[1:]:
             t = self._scratch()
             _check(lib().h1pa_form_assemble_diagonal(f._h, _ptr(t), _stream()))
-            diag.add_(t)
+            _check(lib().h1pa_vector_add(_ptr(diag), 1.0, _ptr(t), t.numel(), _stream()))
 
     def PAData(self):
         """(diffusion D[q^3*6*ne], mass v[q^3*ne]) copies of the first form's PA data."""

@@ -756,6 +756,13 @@ int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess, co
     API_END
 }
 
+int h1pa_vector_add(double *y, double a, const double *x, int64_t n, void *stream) {
+    API_BEGIN
+    REQUIRE(n >= 0 && (n == 0 || (x && y)), "vector_add: bad argument");
+    kern::axpy(n, a, x, y, as_stream(stream));
+    API_END
+}
+
 int h1pa_vector_randomize(double *x, int64_t n, uint64_t seed, double lo, double hi,
                           void *stream) {
     API_BEGIN

@@ -147,6 +147,36 @@ def test_restriction_from_mfem_table(orc, h1pa, torch_cuda, p):
         assert np.array_equal(y.cpu().numpy(), orc.restriction_mult_transpose(off, ind, D3, xe, 0))
 
 
+@pytest.mark.parametrize("p", [2, 5])
+def test_integrators_on_different_rules(orc, h1pa, torch_cuda, p):
+    """Diffusion on Q = p+2 and mass on Q = p+1 in one BilinearForm (two
+    operators summed on the device): apply and diagonal within 1e-13 of the
+    oracle's two operators added."""
+    torch, M = torch_cuda, h1pa
+    dims = (3, 2, 2)
+    mesh = M.Mesh.MakeCartesian3D(*dims)
+    mesh.Perturb(0.2, 9)
+    fes = M.FiniteElementSpace(mesh, M.H1_FECollection(p, 3))
+    a = M.BilinearForm(fes)
+    a.SetAssemblyLevel(M.AssemblyLevel.PARTIAL)
+    a.AddDomainIntegrator(M.DiffusionIntegrator(M.SinCosCoefficient(), p + 2))
+    a.AddDomainIntegrator(M.MassIntegrator(M.SinCosCoefficient(), p + 1))
+    a.Assemble()
+    Pd = orc.Problem(None, p, Q=p + 2, perturb=0.2, seed=9, coeff="sincos",
+                     integrators=("diffusion",), dims=dims)
+    Pm = orc.Problem(None, p, Q=p + 1, perturb=0.2, seed=9, coeff="sincos",
+                     integrators=("mass",), dims=dims)
+    x = np.random.default_rng(p).uniform(-1, 1, Pd.ndofs)
+    y = torch.empty(Pd.ndofs, dtype=torch.float64, device="cuda")
+    a.Mult(torch.from_numpy(x).cuda(), y)
+    torch.cuda.synchronize()
+    assert _rel(y.cpu().numpy(), Pd.mult(x) + Pm.mult(x)) < 1e-13
+    d = torch.empty_like(y)
+    a.AssembleDiagonal(d)
+    torch.cuda.synchronize()
+    assert _rel(d.cpu().numpy(), Pd.diagonal() + Pm.diagonal()) < 1e-13
+
+
 @pytest.mark.parametrize("p", [2, 4, 7])
 def test_add_mult_and_transpose(orc, h1pa, torch_cuda, p):
     """Operator::AddMult (y += a A x, linalg/operator.cpp:51-56) within 1e-13 of
