@@ -206,10 +206,11 @@ def cpu_baseline(p, target_seconds=12.0):
                        f"{reps} applies in {el:.1f} s, OpenMP over elements")
 
 
-def traffic_from_profiles(p, n):
+def traffic_from_profiles(p, n, variant):
     """HBM bytes per launch of the element kernel measured with rocprofv3 PMC
     counters (FETCH_SIZE, WRITE_SIZE, scaled by the copy-kernel calibration),
-    when profiles/traffic.json holds this workload; else None."""
+    when profiles/traffic.json holds this workload and element-kernel variant;
+    else None."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -217,7 +218,8 @@ def traffic_from_profiles(p, n):
     except (OSError, ValueError):
         return None
     for r in rows:
-        if r.get("p") == p and r.get("n") == n and r.get("kernel") == "element":
+        if (r.get("p") == p and r.get("n") == n and r.get("kernel") == "element"
+                and r.get("variant_index") == variant):
             return r.get("bytes_per_launch")
     return None
 
@@ -516,7 +518,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(r["element_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(args.p, n),
+            "traffic": traffic_from_profiles(args.p, n, r["variant"]),
             "kernel": f"k_diffusion_apply<double,double,{args.p + 1},{args.p + 2},...>",
             "kernel_ms": round(r["ms_element"], 4),
             "bytes_per_launch": r["bytes_element"],

@@ -41,7 +41,7 @@ k_diffusion_apply(int64_t ne, const T *__restrict__ basis, const TP *__restrict_
                   const T *__restrict__ x, const EOutT<T> out, int accumulate) {
     if (out.stop && *out.stop) return;  // solver already finished (block-uniform)
     __shared__ T smem[NB * diffusion_buf<D, Q>()];
-    diffusion_elements<T, TP, D, Q, NB, U, K>(xcd_block(blockIdx.x, gridDim.x) * NB, ne, basis, pa,
+    diffusion_elements<T, TP, D, Q, NB, U, K>((int64_t)blockIdx.x * NB, ne, basis, pa,
                                               gmap, mask, x, out, accumulate, smem);
 }
 
@@ -52,7 +52,7 @@ k_mass_apply(int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
              const T *__restrict__ x, const EOutT<T> out, int accumulate) {
     if (out.stop && *out.stop) return;
     __shared__ T smem[NB * D * Q * Q];
-    mass_elements<T, TP, D, Q, NB>(xcd_block(blockIdx.x, gridDim.x) * NB, ne, basis, pa, gmap, mask,
+    mass_elements<T, TP, D, Q, NB>((int64_t)blockIdx.x * NB, ne, basis, pa, gmap, mask,
                                    x, out, accumulate, smem);
 }
 

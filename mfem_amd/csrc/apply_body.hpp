@@ -58,18 +58,6 @@ constexpr int nb_for(int Q, int buf, int bytes) {
     return nb;
 }
 
-// Work-groups are dispatched to the 8 XCDs round-robin (block b runs on XCD
-// b % 8), and each XCD has its own L2.  xcd_block(b, G) renumbers the blocks
-// so that every XCD processes one contiguous range of the grid: elements that
-// are neighbours in the (space-filling-curve) element order, and so share
-// DOFs, run on the same XCD and find each other's x lines in its L2.  The map
-// is a bijection of [0, G).
-__host__ __device__ inline int64_t xcd_block(int64_t b, int64_t G) {
-    constexpr int64_t X = 8;
-    const int64_t q = G / X, r = G % X, k = b % X, j = b / X;
-    return k * q + (k < r ? k : r) + j;
-}
-
 // Load one PA value as the compute type (bf16 is stored as the upper 16 bits
 // of an FP32).  PA data is streamed once per apply, so with NT it is loaded
 // non-temporally and does not push the gathered x-values (re-read by up to 8

@@ -837,19 +837,15 @@ void drop_graph(Batch &B) {
     }
 }
 
-// Work list of group g: (slot, first element) per block, ordered so that
-// block b (dispatched to XCD b % 8) covers a contiguous range of the list per
-// XCD (xcd_block), keeping each slot's neighbouring elements on one L2.
+// Work list of group g: (slot, first element) per block, in slot and element
+// order (consecutive blocks take neighbouring elements of one slot).
 void element_tasks(const Batch &B, int g, int nb, std::vector<ElemTask> &out) {
-    std::vector<ElemTask> lin;
+    out.clear();
     for (size_t k = 0; k < B.slots.size(); ++k) {
         const BatchSlot &sl = *B.slots[k];
         if (sl.group != g || !sl.live || !running_phase(B.h_state[k].phase)) continue;
-        for (int64_t e = 0; e < sl.ne; e += nb) lin.push_back({(int)k, (int)e});
+        for (int64_t e = 0; e < sl.ne; e += nb) out.push_back({(int)k, (int)e});
     }
-    const int64_t G = (int64_t)lin.size();
-    out.resize(lin.size());
-    for (int64_t b = 0; b < G; ++b) out[b] = lin[kern::xcd_block(b, G)];
 }
 
 // Rebuild descriptors, work lists and partial slices for the running slots;
