@@ -8,15 +8,16 @@
 // the boundary is homogeneous Dirichlet, the stop test is CGSolver's
 // relative measure at rel_tol.
 //
-//   --mode batch   continuous batching: arrived requests are set up and
-//                  admitted one at a time while the HBM budget allows,
-//                  overlapping the replay in flight; all admitted solves
-//                  advance together in graph replays that run back to back
-//                  (between admissions the loop checks h1pa_batch_query,
-//                  retires the finished solves, releases their memory and
-//                  launches the next replay at once).
-//   --mode serial  plain FP64 GPU PCG: requests are set up and solved one at
-//                  a time, in arrival order (h1pa_pcg_solve).
+//   --mode batch   continuous batching: arrived requests are admitted in
+//                  arrival order while the HBM budget allows and set up on a
+//                  host thread of their own (own stream) beside the replays;
+//                  all admitted solves advance together in graph replays that
+//                  run back to back (the loop checks h1pa_batch_query,
+//                  retires the finished solves, releases their memory, adds
+//                  the finished setups and launches the next replay at once).
+//   --mode serial  plain FP64 GPU PCG: requests are solved one at a time, in
+//                  arrival order (h1pa_pcg_solve); the same setup thread
+//                  prepares the next request during each solve.
 //
 // Prints one JSON line: solves/s over the whole run, latency percentiles
 // (arrival to result), and the request mix.
@@ -27,9 +28,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <thread>
@@ -180,6 +183,81 @@ void release(Live &L) {
     L = Live{};
 }
 
+// Sets requests up on a host thread of its own, on its own stream, so that
+// the loop's replays and solves never wait for a setup: give() queues a
+// request, take() returns the finished setups (their device work complete).
+class SetupWorker {
+  public:
+    explicit SetupWorker(const std::vector<Request> &reqs) : reqs_(reqs), th_([this] { run(); }) {}
+    void give(int id) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            todo_.push_back(id);
+        }
+        cv_.notify_all();
+    }
+    std::vector<std::pair<int, Live>> take() {
+        std::lock_guard<std::mutex> g(m_);
+        std::vector<std::pair<int, Live>> out(ready_.begin(), ready_.end());
+        ready_.clear();
+        return out;
+    }
+    void wait_ready(double seconds) {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait_for(g, std::chrono::duration<double>(seconds), [this] { return !ready_.empty(); });
+    }
+    // Joins the thread; returns its total setup time (seconds).
+    double stop() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+        return setup_s_;
+    }
+    ~SetupWorker() { stop(); }
+
+  private:
+    void run() {
+        int dev = 0;
+        hcheck(hipGetDevice(&dev), "get device");
+        hcheck(hipSetDevice(dev), "set device");
+        hipStream_t ws;
+        hcheck(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking), "stream");
+        for (;;) {
+            int id;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [this] { return stop_ || !todo_.empty(); });
+                if (todo_.empty()) break;  // stop_ and nothing left
+                id = todo_.front();
+                todo_.pop_front();
+            }
+            const auto ts = std::chrono::steady_clock::now();
+            Live L = setup(reqs_[id], ws);
+            hcheck(hipStreamSynchronize(ws), "setup sync");
+            const double dt =
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
+            {
+                std::lock_guard<std::mutex> g(m_);
+                setup_s_ += dt;
+                ready_.emplace_back(id, L);
+            }
+            cv_.notify_all();
+        }
+        hcheck(hipStreamDestroy(ws), "stream destroy");
+    }
+    const std::vector<Request> &reqs_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<int> todo_;
+    std::deque<std::pair<int, Live>> ready_;
+    bool stop_ = false;
+    double setup_s_ = 0.0;
+    std::thread th_;
+};
+
 double pct(std::vector<double> v, double q) {
     if (v.empty()) return 0.0;
     std::sort(v.begin(), v.end());
@@ -279,20 +357,21 @@ int main(int argc, char **argv) {
     std::map<int64_t, int64_t> live_est;
     std::deque<int> queue;
     int64_t used = 0, peak_used = 0;
-    int next = 0, done = 0, peak_live = 0;
-    double setup_s = 0.0;  // host time spent setting requests up (GPU idle in serial mode)
+    int next = 0, done = 0, peak_live = 0, pending = 0;
+    double launch_s = 0.0;  // host time inside h1pa_batch_launch (the GPU has no replay queued)
     bool inflight = false;  // batch mode: a replay is running on the GPU
-    const auto t0 = std::chrono::steady_clock::now();
     std::vector<int64_t> tags(4096);
     std::vector<h1pa_batch_result> res(4096);
+    SetupWorker W(reqs);
+    const auto t0 = std::chrono::steady_clock::now();
     while (done < R) {
         double t = now_s(t0);
         while (next < R && reqs[next].t_arrive <= t) queue.push_back(next++);
         if (batch) {
-            // Retire a finished replay and start the next one before anything
-            // else, so the GPU runs replays back to back; between checks the
-            // host admits one queued request at a time (its setup overlaps the
-            // replay in flight and it joins at the next launch).
+            // Retire a finished replay, add the solves the setup thread has
+            // finished, hand queued requests to it while the HBM budget allows,
+            // and start the next replay, so the GPU runs replays back to back
+            // while requests are set up beside them.
             auto finish_replay = [&]() {
                 int running = 0;
                 check(h1pa_batch_wait(B, &running), "batch_wait");
@@ -318,58 +397,81 @@ int main(int argc, char **argv) {
                 check(h1pa_batch_query(B, &idle), "batch_query");
                 if (idle) finish_replay();
             }
-            if (!inflight && !live.empty()) {
-                int launched = 0;
-                check(h1pa_batch_launch(B, &launched), "batch_launch");
-                inflight = launched != 0;
-                if (!launched) finish_replay();  // every live solve has finished: retire them
+            for (auto &r : W.take()) {
+                Live &L = r.second;
+                const int *ess;
+                int64_t nEss;
+                check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+                h1pa_pcg_opts po{o.rel_tol, 0.0, 200000, 8};
+                int slot = 0;
+                check(h1pa_batch_add(B, L.form, ess, nEss, L.B, L.X, &po, r.first, &slot, s),
+                      "batch_add");
+                const int64_t acc = std::max(live_est[r.first], L.bytes);
+                used += acc - live_est[r.first];
+                live_est[r.first] = acc;
+                live[r.first] = L;
+                --pending;
+                peak_used = std::max(peak_used, used);
+                peak_live = std::max(peak_live, (int)live.size());
             }
-            if (!queue.empty()) {
+            while (!queue.empty()) {
                 const Request &q = reqs[queue.front()];
                 const int64_t est = estimate_bytes(q.n, q.p, prec, true);
-                if (live.empty() || used + est <= budget) {
-                    const double ts = now_s(t0);
-                    Live L = setup(q, s);
-                    setup_s += now_s(t0) - ts;
-                    const int *ess;
-                    int64_t nEss;
-                    check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
-                    h1pa_pcg_opts po{o.rel_tol, 0.0, 200000, 8};
-                    int slot = 0;
-                    check(h1pa_batch_add(B, L.form, ess, nEss, L.B, L.X, &po, q.id, &slot, s),
-                          "batch_add");
-                    const int64_t acc = std::max(est, L.bytes);
-                    used += acc;
-                    live_est[q.id] = acc;
-                    live[q.id] = L;
-                    queue.pop_front();
-                    peak_used = std::max(peak_used, used);
-                    peak_live = std::max(peak_live, (int)live.size());
-                    continue;
-                }
+                if (!((live.empty() && pending == 0) || used + est <= budget)) break;
+                used += est;
+                live_est[q.id] = est;
+                ++pending;
+                W.give(queue.front());
+                queue.pop_front();
             }
-            if (inflight) {  // nothing to admit (or no room): wait for the replay
+            if (!inflight && !live.empty()) {
+                int launched = 0;
+                const double tl = now_s(t0);
+                check(h1pa_batch_launch(B, &launched), "batch_launch");
+                launch_s += now_s(t0) - tl;
+                inflight = launched != 0;
+                if (!launched) finish_replay();  // every live solve has finished: retire them
+                continue;
+            }
+            if (inflight) {  // wait for the replay (the setup thread keeps working)
                 finish_replay();
                 continue;
             }
-            if (live.empty() && queue.empty() && next < R) {
+            if (pending > 0) {
+                W.wait_ready(0.001);
+            } else if (live.empty() && queue.empty() && next < R) {
                 const double wait = reqs[next].t_arrive - now_s(t0);
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
             }
         } else {
-            if (queue.empty()) {
+            // One solve at a time; the setup thread prepares the next request
+            // while the current one solves.
+            if (pending == 0 && !queue.empty()) {
+                W.give(queue.front());
+                queue.pop_front();
+                ++pending;
+            }
+            if (pending == 0) {
                 if (next < R) {
                     const double wait = reqs[next].t_arrive - now_s(t0);
                     if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
                 }
                 continue;
             }
-            Request &q = reqs[queue.front()];
-            queue.pop_front();
-            const double ts = now_s(t0);
-            Live L = setup(q, s);
-            hcheck(hipStreamSynchronize(s), "sync");
-            setup_s += now_s(t0) - ts;
+            auto got = W.take();
+            if (got.empty()) {
+                W.wait_ready(0.01);
+                continue;
+            }
+            --pending;
+            const int id = got.front().first;
+            Live L = got.front().second;
+            if (!queue.empty()) {  // set the next request up during this solve
+                W.give(queue.front());
+                queue.pop_front();
+                ++pending;
+            }
+            Request &q = reqs[id];
             peak_used = std::max(peak_used, L.bytes);
             peak_live = 1;
             const int *ess;
@@ -387,6 +489,7 @@ int main(int argc, char **argv) {
             ++done;
         }
     }
+    const double setup_s = W.stop();
     const double makespan = now_s(t0);
     if (B) h1pa_batch_destroy(B);
     hcheck(hipStreamDestroy(s), "stream destroy");
@@ -408,11 +511,11 @@ int main(int argc, char **argv) {
         "\"latency_p50_s\": %.4f, \"latency_p90_s\": %.4f, \"latency_p99_s\": %.4f, "
         "\"latency_max_s\": %.4f, \"mean_dofs\": %.0f, \"mean_iterations\": %.1f, "
         "\"peak_live\": %d, \"peak_bytes\": %lld, \"budget_bytes\": %lld, "
-        "\"setup_s\": %.3f, \"dof_iterations\": %.4e}\n",
+        "\"setup_s\": %.3f, \"launch_s\": %.3f, \"dof_iterations\": %.4e}\n",
         o.mode.c_str(), prec == H1PA_PREC_FP64 ? "fp64" : prec == H1PA_PREC_FP32 ? "fp32" : "bf16",
         o.rate, o.seconds, R, (long long)conv, makespan, R / makespan, pct(lat, 0.5),
         pct(lat, 0.9), pct(lat, 0.99), pct(lat, 1.0), R ? (double)dofs / R : 0.0,
-        R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget, setup_s,
+        R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget, setup_s, launch_s,
         dof_iters);
     return 0;
 }
