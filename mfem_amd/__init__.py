@@ -224,6 +224,23 @@ def _check_vec(t, n, dtype_name="float64"):
         raise ValueError(f"vector must be {dtype_name}[{n}], got {t.dtype}[{t.numel()}]")
 
 
+def _ess(t):
+    """An essential-DOF list as MFEM's Array<int>: an int32 device tensor.
+    Integer tensors of other widths are converted (entries must fit in
+    int32); anything else is rejected, so an int64 list (torch's default
+    index type) is never read as pairs of int32 words."""
+    torch = _torch()
+    if not (isinstance(t, torch.Tensor) and t.is_cuda):
+        raise TypeError("ess_tdof_list must be a CUDA (HIP) integer tensor")
+    if t.dtype == torch.int32:
+        return t.contiguous()
+    if t.dtype in (torch.int64, torch.int16, torch.int8, torch.uint8):
+        if t.numel() and (int(t.min()) < -2**31 or int(t.max()) >= 2**31):
+            raise ValueError("ess_tdof_list entries do not fit in int32")
+        return t.to(torch.int32).contiguous()
+    raise TypeError(f"ess_tdof_list must be an integer tensor, got {t.dtype}")
+
+
 def _wrap_device_int(ptr, n):
     """Copy n int32 values from a device pointer into a new torch tensor."""
     torch = _torch()
@@ -631,6 +648,17 @@ class _PAForm:
     """One C form (a set of integrators sharing a coefficient and q1d)."""
 
     def __init__(self, fes, flags, coeff, q1d):
+        # the device reads the coefficient data at fixed sizes: check them here
+        if coeff.kind in (COEFF_PER_ELEMENT, COEFF_QUADRATURE):
+            q = q1d if q1d > 0 else fes.p + 2
+            want = fes.ne if coeff.kind == COEFF_PER_ELEMENT else q ** 3 * fes.ne
+            what = ("PWConstCoefficient" if coeff.kind == COEFF_PER_ELEMENT
+                    else "QuadratureFunctionCoefficient")
+            _ptr(coeff.data)
+            try:
+                _check_vec(coeff.data, want)
+            except ValueError as e:
+                raise ValueError(f"{what}: {e}") from None
         desc = FormDesc()
         desc.integrators = flags
         desc.coeff_kind = coeff.kind
@@ -681,7 +709,12 @@ class BilinearForm:
             if g[0] & it.flag:
                 raise H1paError(H1PA_EUNSUP, "duplicate integrator with the same coefficient")
             g[0] |= it.flag
-        self._forms = [_PAForm(self.fes, f, c, q) for f, c, q in groups.values()]
+        key = [(f, id(c), q) for f, c, q in groups.values()]
+        # re-assembly keeps the C forms (as MFEM re-assembles in place); only
+        # a changed integrator set makes new ones
+        if not self._forms or getattr(self, "_form_key", None) != key:
+            self._forms = [_PAForm(self.fes, f, c, q) for f, c, q in groups.values()]
+            self._form_key = key
         for f in self._forms:
             _check(lib().h1pa_form_assemble(f._h, _stream()))
 
@@ -790,6 +823,7 @@ class BilinearForm:
         X = torch.empty(n, dtype=torch.float64, device="cuda")
         B = torch.empty(n, dtype=torch.float64, device="cuda")
         f = self._single()
+        ess_tdof_list = _ess(ess_tdof_list)
         _check(lib().h1pa_form_linear_system(f._h, _ptr(ess_tdof_list), ess_tdof_list.numel(),
                                              _ptr(x), _ptr(b), _ptr(X), _ptr(B), _stream()))
         return ConstrainedOperator(self, ess_tdof_list), X, B
@@ -802,7 +836,7 @@ class ConstrainedOperator:
     """DIAG_ONE constrained operator (linalg/operator.cpp:526-662)."""
 
     def __init__(self, form, ess):
-        self.form, self.ess = form, ess
+        self.form, self.ess = form, _ess(ess)
 
     def Height(self):
         return self.form.fes.ndofs
@@ -819,7 +853,7 @@ class OperatorJacobiSmoother:
     def __init__(self, a, ess_tdof_list, damping=1.0):
         if damping != 1.0:
             raise H1paError(H1PA_EUNSUP, "only damping = 1 is provided")
-        self.a, self.ess = a, ess_tdof_list
+        self.a, self.ess = a, _ess(ess_tdof_list)
 
 
 class OperatorChebyshevSmoother:
@@ -840,6 +874,7 @@ class OperatorChebyshevSmoother:
         n = A.form.fes.ndofs
         if diag is not None:
             _check_vec(diag, n)
+        ess_tdof_list = _ess(ess_tdof_list)
         self.A, self.ess, self.order = A, ess_tdof_list, int(order)
         self._form = f  # keeps the form alive
         h = _vp()

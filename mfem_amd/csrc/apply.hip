@@ -312,10 +312,11 @@ void launch_mass_diag(int64_t ne, const double *basis, const double *pa, const E
     X(2, 2) X(2, 3) X(3, 3) X(3, 4) X(4, 4) X(4, 5) X(5, 5) X(5, 6) X(6, 6) X(6, 7) \
     X(7, 7) X(7, 8) X(8, 8) X(8, 9) X(9, 9) X(9, 10)
 
-// Selected diffusion variant per (D, Q); filled by the autotuner.
-int g_diff_variant[11][11] = {};
+// Selected diffusion variant per (D, Q); filled by the autotuner, which may
+// run on another host thread (a setup thread) while launches read the table.
+std::atomic<int> g_diff_variant[11][11] = {};
 // Reduced-precision applies: per PA storage kind (0 FP32, 1 BF16) and (D, Q).
-int g_diff_variant_lp[2][11][11] = {};
+std::atomic<int> g_diff_variant_lp[2][11][11] = {};
 
 }  // namespace
 
@@ -323,10 +324,10 @@ bool apply_supported(int p, int q1d) { return p >= 1 && p <= 8 && (q1d == p + 1 
 
 int diffusion_variant_count() { return kDiffVariants; }
 void set_diffusion_variant(int p, int q1d, int v) {
-    if (p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10) g_diff_variant[p + 1][q1d] = v;
+    if (p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10) g_diff_variant[p + 1][q1d].store(v);
 }
 int get_diffusion_variant(int p, int q1d) {
-    return (p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10) ? g_diff_variant[p + 1][q1d] : 0;
+    return (p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10) ? g_diff_variant[p + 1][q1d].load() : 0;
 }
 
 void diffusion_apply(int p, int q1d, int64_t ne, const double *basis, const double *pa,
@@ -403,11 +404,11 @@ void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double
 
 void set_diffusion_variant_lp(int pa_kind, int p, int q1d, int v) {
     if ((pa_kind == 0 || pa_kind == 1) && p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10)
-        g_diff_variant_lp[pa_kind][p + 1][q1d] = v;
+        g_diff_variant_lp[pa_kind][p + 1][q1d].store(v);
 }
 int get_diffusion_variant_lp(int pa_kind, int p, int q1d) {
     return ((pa_kind == 0 || pa_kind == 1) && p >= 1 && p <= 8 && q1d >= 1 && q1d <= 10)
-               ? g_diff_variant_lp[pa_kind][p + 1][q1d]
+               ? g_diff_variant_lp[pa_kind][p + 1][q1d].load()
                : 0;
 }
 

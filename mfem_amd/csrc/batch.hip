@@ -783,6 +783,10 @@ struct h1pa_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int64_t replays = 0, captures = 0;
+    // Every device address and launch size the captured replay holds in its
+    // kernel nodes, taken at capture; a launch of the cached graph checks
+    // them against the batch's current buffers.
+    std::vector<uintptr_t> baked;
     // solves added since the last launch; merged into `slots` by the next one
     std::vector<std::unique_ptr<h1pa::BatchSlot>> pending;
     bool in_flight = false;
@@ -793,6 +797,11 @@ struct h1pa_batch {
         if (stream) (void)hipStreamSynchronize(stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        // solves never retired by poll release their hold on the forms
+        for (auto &sl : pending)
+            if (sl && sl->live) --sl->form->f.batch_refs;
+        for (auto &sl : slots)
+            if (sl && sl->live) --sl->form->f.batch_refs;
         pending.clear();
         slots.clear();
         if (h_pinned) (void)hipHostFree(h_pinned);
@@ -951,6 +960,24 @@ void autotune(Batch &B) {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+}
+
+// The device addresses and sizes a replay's kernel launches take as
+// arguments (enqueue_replay), in a fixed order.
+std::vector<uintptr_t> replay_arguments(const Batch &B) {
+    std::vector<uintptr_t> a{(uintptr_t)B.vtasks.p, (uintptr_t)B.runlist.p,
+                             (uintptr_t)B.d_desc.p, (uintptr_t)B.d_state.p,
+                             (uintptr_t)B.partials.p, (uintptr_t)B.nvtasks, (uintptr_t)B.nrun};
+    for (const BatchGroup &G : B.groups) {
+        a.push_back((uintptr_t)G.tasks.p);
+        a.push_back((uintptr_t)G.ntasks);
+        a.push_back((uintptr_t)G.tasks64.p);
+        a.push_back((uintptr_t)G.ntasks64);
+        a.push_back((uintptr_t)G.basis);
+        a.push_back((uintptr_t)G.basis64);
+        a.push_back((uintptr_t)G.variant);
+    }
+    return a;
 }
 
 void enqueue_replay(Batch &B, hipStream_t s) {
@@ -1128,6 +1155,7 @@ int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *
     }
     sl->bytes = bytes;
     sl->live = true;
+    ++f.batch_refs;  // the descriptor holds f's PA pointers until poll retires it
     sl->init.phase = PH_INIT;
     sl->init.max_iter = mixed ? B.o.max_inner : po.max_iter;
     H1PA_HIP(hipEventCreateWithFlags(&sl->ready, hipEventDisableTiming));
@@ -1166,11 +1194,15 @@ void merge_pending(Batch &B) {
         }
         B.h_state[k] = up->init;
         B.slots[k] = std::move(up);
-        H1PA_HIP(hipMemcpyAsync(B.d_state.p + k, &B.h_state[k], sizeof(SlotState),
-                                hipMemcpyHostToDevice, s));
     }
     B.pending.clear();
     H1PA_HIP(hipStreamSynchronize(s));
+    // Nothing is in flight, so the device states equal h_state: upload them
+    // all with one synchronous copy.  (An asynchronous copy from h_state,
+    // issued inside the loop above, could still be reading the vector when a
+    // push_back moved it.)
+    H1PA_HIP(hipMemcpy(B.d_state.p, B.h_state.data(), sizeof(SlotState) * B.slots.size(),
+                       hipMemcpyHostToDevice));
     B.dirty = true;
 }
 
@@ -1218,7 +1250,14 @@ bool batch_launch(Batch &B) {
             }
             H1PA_HIP(hipStreamEndCapture(s, &B.graph));
             H1PA_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+            B.baked = replay_arguments(B);
             ++B.captures;
+        } else {
+            // every buffer the captured kernels address must still be the one
+            // they were captured with (buffers move only in rebuild, which
+            // drops the graph): a stale graph is refused, never launched
+            H1PA_CHECK(replay_arguments(B) == B.baked, H1PA_EINVAL,
+                       "batch: the captured replay refers to buffers that have been replaced");
         }
         H1PA_HIP(hipGraphLaunch(B.exec, s));
     } else {
@@ -1293,15 +1332,17 @@ int batch_poll(Batch &B, int cap, int64_t *tags, h1pa_batch_result *res) {
         ++n;
         // retire: free the slot's vectors, keep its position
         sl.live = false;
+        --sl.form->f.batch_refs;
         for (auto &v : sl.v64) v.release();
         for (auto &v : sl.v32) v.release();
         sl.mask.release();
         sl.bytes = 0;
         B.h_state[k].phase = PH_EMPTY;
-        H1PA_HIP(hipMemcpyAsync(B.d_state.p + k, &B.h_state[k], sizeof(SlotState),
-                                hipMemcpyHostToDevice, B.stream));
     }
-    if (n) H1PA_HIP(hipStreamSynchronize(B.stream));
+    // nothing is in flight (batch_wait above): one synchronous state upload
+    if (n)
+        H1PA_HIP(hipMemcpy(B.d_state.p, B.h_state.data(), sizeof(SlotState) * B.slots.size(),
+                           hipMemcpyHostToDevice));
     return n;
 }
 

@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <exception>
 #include <new>
 
@@ -423,6 +424,7 @@ int h1pa_form_create(h1pa_space *space, const h1pa_form_desc *desc, h1pa_form **
 
 int h1pa_form_destroy(h1pa_form *form) {
     API_BEGIN
+    if (form) require_no_batch_refs(form->f, "form_destroy");
     delete form;
     API_END
 }
@@ -434,7 +436,10 @@ namespace {
 // process on the first form large enough to time reliably.  All variants
 // compute bit-identical results.  H1PA_DIFF_VARIANT=<v> forces a variant,
 // H1PA_AUTOTUNE=0 keeps the default.
+// g_tuned and the tuning itself are serialized: forms may be assembled on
+// several host threads at once.
 bool g_tuned[10][11] = {};
+std::mutex g_tune_mutex;
 
 void autotune_diffusion(Form &f, hipStream_t s) {
     Space &sp = *f.space;
@@ -447,7 +452,10 @@ void autotune_diffusion(Form &f, hipStream_t s) {
         return;
     }
     const char *at = std::getenv("H1PA_AUTOTUNE");
-    if ((at && at[0] == '0') || g_tuned[p][Q] || sp.ne < 20000) return;
+    // large enough to time reliably: >= 20000 elements or >= 500k DOFs
+    if ((at && at[0] == '0') || (sp.ne < 20000 && sp.ndofs < 500000)) return;
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    if (g_tuned[p][Q]) return;
     DevBuf<double> x(sp.ndofs);
     H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * sp.ndofs, s));
     hipEvent_t e0, e1;
@@ -484,6 +492,7 @@ int h1pa_form_assemble(h1pa_form *form, void *stream) {
     API_BEGIN
     REQUIRE(form, "form: null");
     Form &f = form->f;
+    require_no_batch_refs(f, "form_assemble");
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
     sp.mesh->ensure_nodes(s);

@@ -51,7 +51,8 @@ __host__ __device__ inline uint64_t splitmix64(uint64_t x) {
 
 Fourier8 make_fourier(uint64_t seed) {
     // 8 modes; integer wave vectors in [-3,3]^3 \ {0}, phases U[0, 2 pi),
-    // amplitude 0.5 / sqrt(8) each (RMS of g = 0.25).
+    // amplitude 0.5 / sqrt(8) each (root sum of squares 0.5; RMS of g
+    // 0.5 / sqrt(2) for distinct modes).
     Fourier8 f{};
     uint64_t ctr = 0;
     auto draw = [&]() { return (double)(splitmix64(seed ^ splitmix64(ctr++)) >> 11) * (1.0 / 9007199254740992.0); };

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -161,7 +162,14 @@ struct Form {
     DevBuf<uint32_t> pa_diff_lp, pa_mass_lp;  // raw storage, 4-byte aligned
     int lp_kind = -1;
     DevBuf<float> ework32;
+    // Batch solves (added, not yet retired by poll) whose descriptors hold
+    // raw pointers to this form's PA data: while nonzero, calls that would
+    // free or move those buffers (Assemble, a reduced-precision shadow of the
+    // other kind, destroy) are refused.
+    std::atomic<int> batch_refs{0};
 };
+// Refuses (H1PA_EINVAL) when live batch solves reference f's device data.
+void require_no_batch_refs(const Form &f, const char *what);
 
 // ---- kernel launchers (HIP translation units) ------------------------------
 namespace kern {

@@ -305,22 +305,24 @@ void Mesh::load_mfem_v10(const char *text, size_t len) {
     tk.expect("elements");
     const long long n_el = tk.integer("element count");
     H1PA_CHECK(n_el >= 1 && n_el < (1LL << 28), H1PA_EINVAL, "mesh file: bad element count");
-    ev.resize(8 * (size_t)n_el);
+    // indices stay 64-bit until the vertex count is known and they are range
+    // checked: a narrowing cast first could wrap 2^32 + v into [0, n_v)
+    std::vector<long long> ev64(8 * (size_t)n_el);
     for (long long e = 0; e < n_el; ++e) {
         (void)tk.integer("attribute");
         H1PA_CHECK(tk.integer("geometry") == 5, H1PA_EUNSUP,
                    "mesh file: only hexahedral elements (geometry 5) are supported");
-        for (int k = 0; k < 8; ++k) ev[8 * e + k] = (int)tk.integer("vertex index");
+        for (int k = 0; k < 8; ++k) ev64[8 * e + k] = tk.integer("vertex index");
     }
     tk.expect("boundary");
     const long long n_bdr = tk.integer("boundary count");
     H1PA_CHECK(n_bdr >= 0 && n_bdr <= 6 * n_el, H1PA_EINVAL, "mesh file: bad boundary count");
-    std::vector<int> bq(4 * (size_t)n_bdr);
+    std::vector<long long> bq64(4 * (size_t)n_bdr);
     for (long long b = 0; b < n_bdr; ++b) {
         (void)tk.integer("attribute");
         H1PA_CHECK(tk.integer("geometry") == 3, H1PA_EUNSUP,
                    "mesh file: only quadrilateral boundary elements (geometry 3) are supported");
-        for (int k = 0; k < 4; ++k) bq[4 * b + k] = (int)tk.integer("vertex index");
+        for (int k = 0; k < 4; ++k) bq64[4 * b + k] = tk.integer("vertex index");
     }
     tk.expect("vertices");
     const long long n_v = tk.integer("vertex count");
@@ -331,8 +333,12 @@ void Mesh::load_mfem_v10(const char *text, size_t len) {
     H1PA_CHECK(sd == "3", H1PA_EUNSUP, "mesh file: vertices must have 3 coordinates");
     xyz.resize(3 * (size_t)n_v);
     for (auto &c : xyz) c = tk.real();
-    for (int v : ev) H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
-    for (int v : bq) H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
+    for (long long v : ev64)
+        H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
+    for (long long v : bq64)
+        H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
+    ev.assign(ev64.begin(), ev64.end());
+    std::vector<int> bq(bq64.begin(), bq64.end());
     nx = ny = nz = 0;
     nv = n_v;
     ne = n_el;

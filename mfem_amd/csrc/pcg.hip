@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <stdlib.h>
 
 #include "internal.hpp"
@@ -970,6 +971,7 @@ namespace {
 // process on the first large enough form: 2 warm-up + 5 timed launches each.
 // H1PA_DIFF_VARIANT=<v> forces variant v (0, 1, 4, 5, 6, 7 or 8).
 bool g_lp_tuned[2][10][11] = {};
+std::mutex g_lp_tune_mutex;  // forms may be set up on several host threads
 void autotune_lp(Form &f, int kind, hipStream_t s) {
     Space &sp = *f.space;
     const int p = sp.p, Q = f.q1d;
@@ -981,8 +983,10 @@ void autotune_lp(Form &f, int kind, hipStream_t s) {
         return;
     }
     const char *at = std::getenv("H1PA_AUTOTUNE");
-    if ((at && at[0] == '0') || sp.ne < 20000 || p < 1 || p > 8 || Q > 10 || g_lp_tuned[kind][p][Q])
+    if ((at && at[0] == '0') || (sp.ne < 20000 && sp.ndofs < 500000) || p < 1 || p > 8 || Q > 10)
         return;
+    std::lock_guard<std::mutex> lock(g_lp_tune_mutex);
+    if (g_lp_tuned[kind][p][Q]) return;
     g_lp_tuned[kind][p][Q] = true;
     DevBuf<float> x(sp.ndofs);
     H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * sp.ndofs, s));
@@ -1017,9 +1021,18 @@ void autotune_lp(Form &f, int kind, hipStream_t s) {
 }
 }  // namespace
 
+void require_no_batch_refs(const Form &f, const char *what) {
+    const int n = f.batch_refs.load();
+    H1PA_CHECK(n == 0, H1PA_EINVAL,
+               std::string(what) + ": the form's device data are in use by " + std::to_string(n) +
+                   " batch solve(s) not yet retired (h1pa_batch_poll)");
+}
+
 void ensure_low_precision(Form &f, int precision, hipStream_t s) {
     const int kind = precision == H1PA_PREC_BF16_PA ? 1 : 0;
     if (f.lp_kind == kind) return;
+    // replacing the shadow frees the buffers live batch slots read
+    if (f.lp_kind >= 0) require_no_batch_refs(f, "reduced-precision shadow of another kind");
     Space &sp = *f.space;
     auto convert = [&](const DevBuf<double> &src, DevBuf<uint32_t> &dst) {
         if (!src.p) {

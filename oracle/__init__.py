@@ -81,6 +81,8 @@ def lib():
         L.orc_nodes_curved.argtypes = [C.POINTER(_Mesh), _pd, _i, _d, _pd]
         L.orc_geom.argtypes = [_i, _i, _i, _pd, _pd, _pd]
         L.orc_coeff_sincos.argtypes = [_i, _i, _pd, _pd]
+        L.orc_fourier_modes.argtypes = [C.c_ulonglong, _pi, _pd, _pd]
+        L.orc_coeff_fourier.argtypes = [_i, _pi, _pd, _d, _i, _i, _pd, _pd]
         L.orc_diffusion_setup.argtypes = [_i, _i, _pd, _pd, _pd, _i, _pd]
         L.orc_mass_setup.argtypes = [_i, _i, _pd, _pd, _pd, _i, _pd]
         L.orc_diffusion_apply.argtypes = [_i, _i, _i, _pd, _pd, _pd, _pd, _pd]
@@ -368,6 +370,27 @@ def qinterp(p, Q, vdim, ne, xE, flags, by_vdim=False, J=None):
     return val, der, det
 
 
+def fourier_modes(seed):
+    """(kv[8][3] int32, phi[8], amplitude) of the seeded log-normal field."""
+    kv = np.zeros(24, np.int32)
+    phi = _da(8)
+    a = C.c_double()
+    lib().orc_fourier_modes(int(seed) & ((1 << 64) - 1), _ip(kv), _dp(phi), C.byref(a))
+    return kv.reshape(8, 3), phi, a.value
+
+
+def coeff_fourier(Q, ne, P, seed=None, modes=None):
+    """k = exp(sum_m a cos(2 pi k_m . x + phi_m)) at the points P[nq][3][ne]:
+    the seeded field (seed) or explicit modes = (kv[m][3], phi[m], a)."""
+    kv, phi, a = fourier_modes(seed) if modes is None else modes
+    kv = np.ascontiguousarray(kv, dtype=np.int32).reshape(-1, 3)
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    C_ = _da(Q ** 3 * ne)
+    lib().orc_coeff_fourier(kv.shape[0], _ip(kv), _dp(phi), float(a), Q ** 3, ne,
+                            _dp(np.ascontiguousarray(P, dtype=np.float64)), _dp(C_))
+    return C_
+
+
 def coeff_sincos(Q, ne, P):
     C_ = _da(Q ** 3 * ne)
     lib().orc_coeff_sincos(Q ** 3, ne, _dp(P), _dp(C_))
@@ -470,7 +493,8 @@ class Problem:
 
     perturb: interior vertex noise amplitude as a fraction of h (cfg2: 0.2).
     curve:   amplitude of the cfg3 curving map (geometry of order p) or 0.
-    coeff:   'one' | 'sincos' | ndarray of per-element constants.
+    coeff:   'one' | 'sincos' | ('fourier', seed) | ndarray of per-element
+             constants.
     integrators: subset of {'diffusion', 'mass'}.
     """
 
@@ -505,7 +529,9 @@ class Problem:
             self.X = m.nodes_trilinear(self.xyz)
         self.J, self.P = geom(self.ne, self.pg, self.Q, self.X)
         nq = self.Q ** 3
-        if isinstance(coeff, str):
+        if isinstance(coeff, tuple) and coeff[0] == "fourier":
+            self.C = coeff_fourier(self.Q, self.ne, self.P, seed=coeff[1])
+        elif isinstance(coeff, str):
             if coeff == "one":
                 self.C = np.ones(1)
             elif coeff == "sincos":

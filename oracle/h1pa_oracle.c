@@ -878,6 +878,48 @@ void orc_geom(int ne, int pg, int Q, const double *X, double *J, double *P)
     }
 }
 
+/* The seeded log-normal coefficient of the batched and served workloads
+ * (no MFEM counterpart; MFEM would project it as a FunctionCoefficient,
+ * fem/coefficient.cpp:52-72): k = exp(g), g(x) = sum_m a cos(2 pi k_m . x +
+ * phi_m) over 8 modes.  Mode m draws its integer wave vector k_m uniformly
+ * from [-3,3]^3 \ {0} (three draws floor(7u) - 3, redrawn while all are
+ * zero), then its phase phi_m = 2 pi u, all from the stream u01(seed, ctr),
+ * ctr = 0, 1, 2, ...; every mode has amplitude a = 0.5 / sqrt(8): the
+ * root sum of squares of the 8 amplitudes is the workload's 0.5, g has RMS
+ * 0.5 / sqrt(2) for distinct modes, and |g| <= 8 a = sqrt(2). */
+void orc_fourier_modes(unsigned long long seed, int *kv, double *phi, double *amp)
+{
+    unsigned long long ctr = 0;
+    for (int m = 0; m < 8; m++) {
+        int k[3];
+        do {
+            for (int c = 0; c < 3; c++) k[c] = (int)floor(u01(seed, ctr++) * 7.0) - 3;
+        } while (k[0] == 0 && k[1] == 0 && k[2] == 0);
+        for (int c = 0; c < 3; c++) kv[3 * m + c] = k[c];
+        phi[m] = 6.283185307179586 * u01(seed, ctr++);
+    }
+    *amp = 0.5 / sqrt(8.0);
+}
+
+/* k = exp(sum_{m < nmodes} amp cos(2 pi kv_m . x + phi_m)) at the physical
+ * points P[nq][3][ne] into C[nq][ne]. */
+void orc_coeff_fourier(int nmodes, const int *kv, const double *phi, double amp, int nq, int ne,
+                       const double *P, double *C)
+{
+    const double twopi = 6.283185307179586476925286766559;
+    for (long e = 0; e < ne; e++)
+        for (int q = 0; q < nq; q++) {
+            const double x = P[q + (long)nq * (0 + 3 * e)];
+            const double y = P[q + (long)nq * (1 + 3 * e)];
+            const double z = P[q + (long)nq * (2 + 3 * e)];
+            double g = 0.0;
+            for (int m = 0; m < nmodes; m++)
+                g += amp * cos(twopi * (kv[3 * m] * x + kv[3 * m + 1] * y + kv[3 * m + 2] * z) +
+                               phi[m]);
+            C[q + (long)nq * e] = exp(g);
+        }
+}
+
 void orc_coeff_sincos(int nq, int ne, const double *P, double *C)
 {
     /* FunctionCoefficient projected at the physical quadrature points

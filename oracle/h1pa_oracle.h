@@ -113,6 +113,9 @@ void orc_geom(int ne, int pg, int Q, const double *X, double *J, double *P);
 /* ---- coefficients (fem/coefficient.cpp:52-72, :2058-2130) --------------- */
 /* k(x) = 1 + 0.5 sin(2 pi x) cos(2 pi y) sin(2 pi z) at P[nq][3][ne] -> C[nq][ne]. */
 void orc_coeff_sincos(int nq, int ne, const double *P, double *C);
+void orc_fourier_modes(unsigned long long seed, int *kv, double *phi, double *amp);
+void orc_coeff_fourier(int nmodes, const int *kv, const double *phi, double amp, int nq, int ne,
+                       const double *P, double *C);
 
 /* ---- PA setup (fem/integ/bilininteg_diffusion_kernels.cpp:262-386;
  *                fem/integ/bilininteg_mass_pa.cpp:25-90) ------------------ */

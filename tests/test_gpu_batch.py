@@ -236,3 +236,35 @@ def test_batch_query_loop_matches_run(h1pa, torch_cuda):
         out.append((X.cpu().numpy(), res[7]["iterations"]))
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][0], out[1][0])
+
+
+def test_form_in_live_batch_is_protected(h1pa, torch_cuda):
+    """A live batch solve holds raw pointers to its form's PA data: until Poll
+    retires it, re-assembling the form, building its reduced-precision shadow
+    of the other kind (another batch's precision, MultLowPrecision) and
+    destroying it are refused; after the poll they work again."""
+    M = h1pa
+    P = Prob(M, torch_cuda, 4, 3, 31)
+    bs = M.BatchSolver(M.PREC_BF16_PA, iters_per_replay=8)
+    X = P.fresh_x()
+    bs.Add(P.A, P.B, X, rel_tol=1e-10, max_iter=20000, tag=1)
+    with pytest.raises(M.H1paError) as ei:
+        P.a.Assemble()
+    assert "batch" in str(ei.value)
+    other = M.BatchSolver(M.PREC_FP32, iters_per_replay=8)
+    with pytest.raises(M.H1paError):
+        other.Add(P.A, P.B, P.fresh_x(), rel_tol=1e-10, max_iter=20000, tag=2)
+    x32 = torch_cuda.zeros(P.N, dtype=torch_cuda.float32, device="cuda")
+    y32 = torch_cuda.empty_like(x32)
+    with pytest.raises(M.H1paError):
+        P.a.MultLowPrecision(x32, y32, M.PREC_FP32)
+    f = P.a._single()
+    assert M.lib().h1pa_form_destroy(f._h) == M.H1PA_EINVAL
+    # the same precision shares the shadow: allowed while the solve is live
+    P.a.MultLowPrecision(x32, y32, M.PREC_BF16_PA)
+    assert bs.Run() == 0
+    res = dict(bs.Poll())
+    assert res[1]["converged"]
+    P.a.Assemble()
+    P.a.MultLowPrecision(x32, y32, M.PREC_FP32)
+    torch_cuda.cuda.synchronize()

@@ -402,3 +402,42 @@ def test_large_size_properties(h1pa, torch_cuda):
     a.Mult(ul, Aul)
     assert abs(torch.dot(ul, Aul).item() - 14.0) < 1e-10
     assert D ** 3 == fes.dof
+
+
+@pytest.mark.parametrize("p", [2, 5])
+def test_quadrature_function_coefficient(orc, h1pa, torch_cuda, p):
+    """QuadratureFunctionCoefficient (MFEM's general coefficient path:
+    CoefficientVector::Project of a QuadratureFunction, fem/coefficient.cpp:
+    2058-2076): the sincos field evaluated on the host side at the mesh's own
+    physical quadrature points, q fastest per element, gives the SinCos
+    form's PA data and the oracle's apply; a wrongly sized array is refused."""
+    M, torch = h1pa, torch_cuda
+    dims = (3, 2, 2)
+    S = Pair(orc, M, torch, dims, p, perturb=0.2, seed=p, coeff="sincos",
+             integrators=("diffusion", "mass"))
+    Q, ne = p + 2, S.fes.GetNE()
+    X, _, _ = S.mesh.GetGeometricFactors(Q, M.QI_VALUES)
+    Xr = X.view(ne, 3, Q ** 3)
+    tp = 2 * np.pi
+    k = (1 + 0.5 * torch.sin(tp * Xr[:, 0]) * torch.cos(tp * Xr[:, 1]) *
+         torch.sin(tp * Xr[:, 2])).contiguous().view(-1)
+    qc = M.QuadratureFunctionCoefficient(k)
+    a = M.BilinearForm(S.fes)
+    a.SetAssemblyLevel(M.AssemblyLevel.PARTIAL)
+    a.AddDomainIntegrator(M.DiffusionIntegrator(qc))
+    a.AddDomainIntegrator(M.MassIntegrator(qc))
+    a.Assemble()
+    dd, dm = a.PAData()
+    dd0, dm0 = S.a.PAData()
+    assert _rel(S.host(dd), S.host(dd0)) < 1e-14
+    assert _rel(S.host(dm), S.host(dm0)) < 1e-14
+    assert _rel(S.host(dd), S.P.Dd) < RTOL
+    x = np.random.default_rng(3 + p).uniform(-1, 1, S.P.ndofs)
+    y = torch.empty(S.P.ndofs, dtype=torch.float64, device="cuda")
+    a.Mult(S.dev(x), y)
+    assert _rel(S.host(y), S.P.mult(x)) < RTOL
+    bad = M.BilinearForm(S.fes)
+    bad.SetAssemblyLevel(M.AssemblyLevel.PARTIAL)
+    bad.AddDomainIntegrator(M.DiffusionIntegrator(M.QuadratureFunctionCoefficient(k[:-1])))
+    with pytest.raises(ValueError, match="QuadratureFunctionCoefficient"):
+        bad.Assemble()

@@ -59,3 +59,70 @@ def test_serve_budget_limits_concurrency(torch_cuda):
                  "--seed", "9")
     assert out["converged"] == out["requests"]
     assert out["peak_live"] == 1
+
+
+def _rhs(seed, n):
+    """h1pa_vector_randomize(seed, -1, 1) on the host (the generator is pinned
+    bit for bit by test_vector_randomize_bit_exact)."""
+    with np.errstate(over="ignore"):
+        def sm(x):
+            x = x + np.uint64(0x9E3779B97F4A7C15)
+            x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            return x ^ (x >> np.uint64(31))
+        i = np.arange(n, dtype=np.uint64)
+        u = (sm(np.uint64(seed) ^ sm(i)) >> np.uint64(11)).astype(np.float64) * 2.0 ** -53
+    return -1.0 + 2.0 * u
+
+
+def _read(d):
+    reqs = {}
+    for line in open(os.path.join(d, "requests.txt")):
+        i, n, p, cs, rs, it, conv = line.split()
+        reqs[int(i)] = dict(n=int(n), p=int(p), cseed=int(cs), rseed=int(rs), it=int(it),
+                            conv=int(conv))
+    return reqs
+
+
+def test_serve_cfg5_mix_equals_alone(orc, torch_cuda, tmp_path):
+    """Continuous batching on a draw of the serving workload (n in [6, 32]
+    with P(n) ~ 1/n, p in 1..8, BF16-PA mixed precision at 1e-10): every
+    served solution equals, bit for bit, the same request solved alone
+    through the batch (--solo), with the same iteration count; and every
+    request up to 4M DOFs meets CGSolver's measure of the FP64 true residual,
+    computed by the oracle from the request's own mesh, coefficient and RHS."""
+    mix, solo = tmp_path / "mix", tmp_path / "solo"
+    mix.mkdir()
+    solo.mkdir()
+    common = ["--mode", "batch", "--precision", "bf16", "--rate", "200", "--seconds", "0.1",
+              "--seed", "11"]
+    out_mix = _serve(*common, "--dump", str(mix))
+    out_solo = _serve(*common, "--solo", "--dump", str(solo))
+    print(out_mix, out_solo, flush=True)
+    assert out_solo["peak_live"] == 1 and out_mix["peak_live"] > 1
+    rm, rs = _read(mix), _read(solo)
+    assert rm.keys() == rs.keys() and len(rm) >= 8
+    for i, q in rm.items():
+        assert q["conv"] == 1 and rs[i]["conv"] == 1
+        assert (q["n"], q["p"], q["cseed"], q["rseed"]) == \
+            (rs[i]["n"], rs[i]["p"], rs[i]["cseed"], rs[i]["rseed"])
+        assert q["it"] == rs[i]["it"], i
+        a = (mix / f"x_{i}.bin").read_bytes()
+        assert a == (solo / f"x_{i}.bin").read_bytes(), f"request {i} differs alone"
+    checked = 0
+    for i, q in sorted(rm.items()):
+        n, p = q["n"], q["p"]
+        if (n * p + 1) ** 3 > 4_000_000:
+            continue
+        P = orc.Problem(n, p, perturb=0.15, seed=q["cseed"], coeff=("fourier", q["cseed"]))
+        b = _rhs(q["rseed"], P.ndofs)
+        _, Bo = P.form_linear_system(np.zeros(P.ndofs), b)
+        x = np.fromfile(mix / f"x_{i}.bin", dtype=np.float64)
+        r = Bo - P.constrained_mult(x)
+        dinv = 1.0 / P.diagonal()
+        dinv[P.ess] = 1.0
+        meas = np.sqrt(r @ (dinv * r)) / np.sqrt(Bo @ (dinv * Bo))
+        print(f"[serve] request {i} n={n} p={p} it={q['it']} measure {meas:.2e}", flush=True)
+        assert meas <= 1.05e-10
+        checked += 1
+    assert checked >= 5

@@ -119,6 +119,12 @@ def test_library_loader_matches_oracle(orc, h1pa, name):
      .replace("0 0 1\n1 0 1\n1 1 1\n0 1 1", "0 0 -1\n1 0 -1\n1 1 -1\n0 1 -1"), "inverted"),
     ("MFEM mesh v1.0\ndimension\n3\nelements\n1\n1 5 0 1 2 3 4 5 6 9\nboundary\n0\n"
      "vertices\n8\n3\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n0 0 1\n1 0 1\n1 1 1\n0 1 1\n", "range"),
+    # 2^32 + 7 and 2^32 + 1 would wrap to valid indices if narrowed to int first
+    ("MFEM mesh v1.0\ndimension\n3\nelements\n1\n1 5 0 1 2 3 4 5 6 4294967303\nboundary\n0\n"
+     "vertices\n8\n3\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n0 0 1\n1 0 1\n1 1 1\n0 1 1\n", "range"),
+    ("MFEM mesh v1.0\ndimension\n3\nelements\n1\n1 5 0 1 2 3 4 5 6 7\nboundary\n1\n"
+     "1 3 0 4294967297 2 3\nvertices\n8\n3\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n0 0 1\n1 0 1\n"
+     "1 1 1\n0 1 1\n", "range"),
     ("MFEM mesh v1.0\ndimension\n3\nelements\n1\n1 5 0 1 2 3 4 5 6 7\nboundary\n1\n"
      "1 3 0 1 2 4\nvertices\n8\n3\n0 0 0\n1 0 0\n1 1 0\n0 1 0\n0 0 1\n1 0 1\n1 1 1\n"
      "0 1 1\n", "not an element face"),
