@@ -21,7 +21,7 @@ SERVE = os.path.join(HERE, "h1pa_serve")
 CEILINGS = os.path.join(HERE, "h1pa_ceilings")
 
 SOURCES = ["basis.cpp", "mesh.cpp", "capi.cpp", "restriction.hip", "geometry.hip",
-           "apply.hip", "pcg.hip", "batch.hip", "qinterp.hip"]
+           "apply.hip", "columns.hip", "pcg.hip", "batch.hip", "qinterp.hip"]
 
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",

@@ -90,10 +90,12 @@ constexpr bool stream_pa() { return Q >= 8 || (Q * Q * sizeof(TP)) % 128 == 0; }
 // constrained L-DOFs read as zero.  All index loads are issued first, then all
 // value and mask loads, with no branch between them: a line costs two memory
 // round trips, not D dependent (index, value) pairs.
-template <int D, typename T>
+// SAVE: the line's D gather indices are also stored to gsave (LDS).
+template <int D, typename T, bool SAVE = false>
 __device__ __forceinline__ void load_line(const int *__restrict__ gmap,
                                           const uint8_t *__restrict__ mask,
-                                          const T *__restrict__ x, int64_t base, T (&u)[D]) {
+                                          const T *__restrict__ x, int64_t base, T (&u)[D],
+                                          int *__restrict__ gsave = nullptr) {
     if (!gmap) {
 #pragma unroll
         for (int d = 0; d < D; ++d) u[d] = x[base + d];
@@ -102,6 +104,10 @@ __device__ __forceinline__ void load_line(const int *__restrict__ gmap,
     int g[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) g[d] = gmap[base + d];
+    if constexpr (SAVE) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) gsave[d] = g[d];
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) u[d] = x[g[d] >= 0 ? g[d] : -1 - g[d]];
     if (mask) {
@@ -209,12 +215,17 @@ constexpr int lds_waves() {
 }
 
 // The elements e0 + (threadIdx.x / Q^2), NB per work-group, of one problem.
-template <typename T, typename TP, int D, int Q, int NB, int U, int K>
+// OM (output mode) 0: the element result goes to `out` (store_line).  OM 1
+// (the column-sweep apply): e0 is this thread's element itself, valid when
+// ne != 0; the element's D^3 results are left in LDS at s + 2 D^2 Q in
+// lexicographic order (a region the last stage does not read) and its gather
+// indices at sg + le D^3; `out` is not used.
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, int OM = 0>
 __device__ __forceinline__ void
 diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
                    const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
                    const T *__restrict__ x, const EOutT<T> &out, int accumulate,
-                   T *__restrict__ smem) {
+                   T *__restrict__ smem, int *__restrict__ sg = nullptr) {
     using E = EO<D, Q>;
     constexpr int HD = E::HD, OD = E::OD, HQ = E::HQ, R = E::R, W = E::W, RB = E::RB;
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
@@ -227,8 +238,8 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
     const int le = threadIdx.x / QQ;
     const int t = threadIdx.x - le * QQ;
     const int a = t % Q, b = t / Q;
-    const int64_t e = e0 + le;
-    const bool valid = e < ne;
+    const int64_t e = OM ? e0 : e0 + le;
+    const bool valid = OM ? ne != 0 : e < ne;
     T *s = smem + le * BUF;
     constexpr int KP = K > 0 ? ((K + 1) / 2 < R ? (K + 1) / 2 : R) : 0;  // pairs ahead
     const TP *pcol = pa + (int64_t)QQQ * 6 * (valid ? e : 0) + a + Q * b;
@@ -255,7 +266,8 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
         {
             T u[D];
             const int64_t base = (int64_t)DDD * e + D * (a + D * b);
-            load_line<D>(gmap, mask, x, base, u);
+            load_line<D, T, OM == 1>(gmap, mask, x, base, u,
+                                     OM == 1 ? sg + le * DDD + D * (a + D * b) : nullptr);
 #pragma unroll
             for (int d = 0; d < HD; ++d) {
                 ue[d] = u[d] + u[D - 1 - d];
@@ -563,7 +575,13 @@ diffusion_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const
                 yv[d] = Ev;
             }
         }
-        store_line<D, T>(out, e, a, b, yv, accumulate);
+        if constexpr (OM == 0) {
+            store_line<D, T>(out, e, a, b, yv, accumulate);
+        } else {
+            T *o = s + 2 * DDQ + D * (a + D * b);
+#pragma unroll
+            for (int dx = 0; dx < D; ++dx) o[dx] = yv[dx];
+        }
     }
 }
 
@@ -773,12 +791,17 @@ mass_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *
 //    (their latency hides behind the x/y stages) and plane qz + K is issued
 //    while plane qz is computed, from a K x 6 register ring.
 // The elements e0 + (threadIdx.x / Q^2), NB per work-group, of one problem.
-template <typename T, typename TP, int D, int Q, int NB, int U, int K>
+// OM (output mode) 0: the element result goes to `out` (store_line).  OM 1
+// (the column-sweep apply): e0 is this thread's element itself, valid when
+// ne != 0; the element's D^3 results are left in LDS at s + 2 D^2 Q in
+// lexicographic order (a region the last stage does not read) and its gather
+// indices at sg + le D^3; `out` is not used.
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, int OM = 0>
 __device__ __forceinline__ void
 diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
                    const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
                    const T *__restrict__ x, const EOutT<T> &out, int accumulate,
-                   T *__restrict__ smem) {
+                   T *__restrict__ smem, int *__restrict__ sg = nullptr) {
     constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
     constexpr int DDQ = D * D * Q, DQQ = D * Q * Q;
     constexpr int BUF = diffusion_buf<D, Q>();
@@ -789,8 +812,8 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
     const int le = threadIdx.x / QQ;
     const int t = threadIdx.x - le * QQ;
     const int a = t % Q, b = t / Q;
-    const int64_t e = e0 + le;
-    const bool valid = e < ne;
+    const int64_t e = OM ? e0 : e0 + le;
+    const bool valid = OM ? ne != 0 : e < ne;
     T *s = smem + le * BUF;
     constexpr int KK = K < Q ? K : Q;
     const TP *pcol = pa + (int64_t)QQQ * 6 * (valid ? e : 0) + a + Q * b;
@@ -808,7 +831,8 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
     if (valid && a < D && b < D) {
         T u[D];
         const int64_t base = (int64_t)DDD * e + D * (a + D * b);
-        load_line<D>(gmap, mask, x, base, u);
+        load_line<D, T, OM == 1>(gmap, mask, x, base, u,
+                                     OM == 1 ? sg + le * DDD + D * (a + D * b) : nullptr);
         T *out = s + Q * (a + D * b);
 #pragma unroll U
         for (int qx = 0; qx < Q; ++qx) {
@@ -1007,7 +1031,13 @@ diffusion_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, c
             }
             yv[dx] = v;
         }
-        store_line<D, T>(out, e, a, b, yv, accumulate);
+        if constexpr (OM == 0) {
+            store_line<D, T>(out, e, a, b, yv, accumulate);
+        } else {
+            T *o = s + 2 * DDQ + D * (a + D * b);
+#pragma unroll
+            for (int dx = 0; dx < D; ++dx) o[dx] = yv[dx];
+        }
     }
 }
 
@@ -1128,18 +1158,18 @@ mass_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const 
 template <typename T, int D>
 constexpr bool use_even_odd() { return sizeof(T) == 4 || D >= 7; }
 
-template <typename T, typename TP, int D, int Q, int NB, int U, int K>
+template <typename T, typename TP, int D, int Q, int NB, int U, int K, int OM = 0>
 __device__ __forceinline__ void
 diffusion_elements(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *__restrict__ pa,
                    const int *__restrict__ gmap, const uint8_t *__restrict__ mask,
                    const T *__restrict__ x, const EOutT<T> &out, int accumulate,
-                   T *__restrict__ smem) {
+                   T *__restrict__ smem, int *__restrict__ sg = nullptr) {
     if constexpr (use_even_odd<T, D>())
-        diffusion_elements_eo<T, TP, D, Q, NB, U, K>(e0, ne, basis, pa, gmap, mask, x, out,
-                                                     accumulate, smem);
+        diffusion_elements_eo<T, TP, D, Q, NB, U, K, OM>(e0, ne, basis, pa, gmap, mask, x, out,
+                                                         accumulate, smem, sg);
     else
-        diffusion_elements_direct<T, TP, D, Q, NB, U, K>(e0, ne, basis, pa, gmap, mask, x, out,
-                                                         accumulate, smem);
+        diffusion_elements_direct<T, TP, D, Q, NB, U, K, OM>(e0, ne, basis, pa, gmap, mask, x,
+                                                             out, accumulate, smem, sg);
 }
 
 template <typename T, typename TP, int D, int Q, int NB>

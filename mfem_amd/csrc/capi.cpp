@@ -556,6 +556,52 @@ void element_apply(Form &f, const int *gmap, const uint8_t *mask, const double *
                          accumulate || diff, s);
 }
 
+// Whether Mult takes the fused column apply (columns.hip): a Cartesian mesh
+// (grid positions known), one diffusion integrator at q1d = p + 2.
+// H1PA_COLUMNS=0 keeps the element kernel + scatter path.  The grid and the
+// interface buffer are set up on the first call.
+bool columns_ready(Form &f, hipStream_t s) {
+    if (f.use_columns == 0) return false;
+    Space &sp = *f.space;
+    Mesh &m = *sp.mesh;
+    if (f.use_columns < 0) {
+        const char *env = std::getenv("H1PA_COLUMNS");
+        const bool ok = !(env && env[0] == '0') && !m.from_file && m.nx > 0 &&
+                        f.desc.integrators == H1PA_INTEG_DIFFUSION &&
+                        kern::columns_supported(sp.p, f.q1d) && sp.ne == (int64_t)m.nx * m.ny * m.nz;
+        if (!ok) {
+            f.use_columns = 0;
+            return false;
+        }
+        if (!m.d_elem_at.p) {
+            std::vector<int> at(m.ne);
+            for (int64_t e = 0; e < m.ne; ++e)
+                at[((int64_t)m.ecoord[3 * e + 2] * m.ny + m.ecoord[3 * e + 1]) * m.nx +
+                   m.ecoord[3 * e]] = (int)e;
+            m.d_elem_at.upload(at.data(), at.size(), s);
+            H1PA_HIP(hipStreamSynchronize(s));
+        }
+        int A = 0, B = 0;
+        kern::column_shape(sp.p, A, B);
+        ColumnGrid &g = f.cols;
+        g.nx = m.nx;
+        g.ny = m.ny;
+        g.nz = m.nz;
+        g.npx = (m.nx + A - 1) / A;
+        g.npy = (m.ny + B - 1) / B;
+        // about 10 work-groups per CU: enough segments per column to fill the
+        // chip, as few as that allows (each segment boundary is shared)
+        const int64_t cols = (int64_t)g.npx * g.npy;
+        int nseg = (int)std::min<int64_t>(m.nz, std::max<int64_t>(1, (2560 + cols - 1) / cols));
+        g.ls = (m.nz + nseg - 1) / nseg;
+        g.nseg = (m.nz + g.ls - 1) / g.ls;
+        g.elem_at = m.d_elem_at.p;
+        f.iface.alloc((size_t)kern::column_interface_size(sp.p, g));
+        f.use_columns = 1;
+    }
+    return true;
+}
+
 const uint8_t *constraint_mask(Form &f, const int *d_ess, int64_t n_ess, hipStream_t s) {
     Space &sp = *f.space;
     if (d_ess == sp.ess.p && n_ess == sp.n_ess) return sp.ess_mask.p;
@@ -591,13 +637,13 @@ int64_t h1pa::form_device_bytes(const Form &f) {
     const Mesh &m = *sp.mesh;
     int64_t b = 0;
     b += f.basis.n * 8 + f.pa_diff.n * 8 + f.pa_mass.n * 8 + f.ework0.n * 8 + f.ework1.n * 8;
-    b += f.lwork0.n * 8 + f.lwork1.n * 8 + f.cmask.n + f.basis32.n * 4;
+    b += f.lwork0.n * 8 + f.lwork1.n * 8 + f.cmask.n + f.basis32.n * 4 + f.iface.n * 8;
     b += f.pa_diff_lp.n * 4 + f.pa_mass_lp.n * 4 + f.ework32.n * 4;
     const Restriction &r = sp.restr;
     b += (r.gather_map.n + r.offsets.n + r.indices.n + r.perm.n + r.perm_lex.n + r.lex2nat.n +
           r.indices_nat.n) * 4;
     b += (sp.elem_dof.n + sp.ess.n) * 4 + sp.ess_mask.n;
-    b += m.d_nodes.n * 8 + m.d_topo.n * 4 + m.d_ori.n;
+    b += m.d_nodes.n * 8 + m.d_topo.n * 4 + m.d_ori.n + m.d_elem_at.n * 4;
     return b;
 }
 
@@ -613,10 +659,15 @@ int h1pa_form_mult(h1pa_form *form, const double *x, double *y, void *stream) {
     require_assembled(f);
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
-    element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, y, false, s);
-    kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
-                                  sp.restr.indices_nat.p, f.ework0.p, y, 0, sp.restr.perm.p,
-                                  sp.restr.nperm, s);
+    if (columns_ready(f, s)) {
+        kern::diffusion_columns(sp.p, f.cols, f.basis.p, f.pa_diff.p, sp.restr.gather_map.p, x, y,
+                                f.iface.p, 3, s);
+    } else {
+        element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, y, false, s);
+        kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                      sp.restr.indices_nat.p, f.ework0.p, y, 0, sp.restr.perm.p,
+                                      sp.restr.nperm, s);
+    }
     API_END
 }
 
@@ -628,10 +679,15 @@ int h1pa_form_add_mult(h1pa_form *form, const double *x, double *y, double a, vo
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
     if (!f.lwork0.p) f.lwork0.alloc(sp.ndofs);
-    element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, f.lwork0.p, false, s);
-    kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
-                                  sp.restr.indices_nat.p, f.ework0.p, f.lwork0.p, 0,
-                                  sp.restr.perm.p, sp.restr.nperm, s);
+    if (columns_ready(f, s)) {
+        kern::diffusion_columns(sp.p, f.cols, f.basis.p, f.pa_diff.p, sp.restr.gather_map.p, x,
+                                f.lwork0.p, f.iface.p, 3, s);
+    } else {
+        element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, f.lwork0.p, false, s);
+        kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                      sp.restr.indices_nat.p, f.ework0.p, f.lwork0.p, 0,
+                                      sp.restr.perm.p, sp.restr.nperm, s);
+    }
     kern::axpy(sp.ndofs, a, f.lwork0.p, y, s);
     API_END
 }
@@ -647,13 +703,22 @@ int h1pa_form_mult_timed(h1pa_form *form, const double *x, double *y, int steps,
     std::vector<hipEvent_t> ev(3 * (size_t)steps);
     for (auto &e : ev) H1PA_HIP(hipEventCreate(&e));
     try {
+        const bool cols = columns_ready(f, s);
         for (int k = 0; k < steps; ++k) {
             H1PA_HIP(hipEventRecord(ev[3 * k], s));
-            element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, y, false, s);
+            if (cols)
+                kern::diffusion_columns(sp.p, f.cols, f.basis.p, f.pa_diff.p,
+                                        sp.restr.gather_map.p, x, y, f.iface.p, 1, s);
+            else
+                element_apply(f, sp.restr.gather_map.p, nullptr, x, f.ework0.p, y, false, s);
             H1PA_HIP(hipEventRecord(ev[3 * k + 1], s));
-            kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
-                                          sp.restr.indices_nat.p, f.ework0.p, y, 0,
-                                          sp.restr.perm.p, sp.restr.nperm, s);
+            if (cols)
+                kern::diffusion_columns(sp.p, f.cols, f.basis.p, f.pa_diff.p,
+                                        sp.restr.gather_map.p, x, y, f.iface.p, 2, s);
+            else
+                kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                              sp.restr.indices_nat.p, f.ework0.p, y, 0,
+                                              sp.restr.perm.p, sp.restr.nperm, s);
             H1PA_HIP(hipEventRecord(ev[3 * k + 2], s));
         }
         H1PA_HIP(hipEventSynchronize(ev.back()));

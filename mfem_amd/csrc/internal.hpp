@@ -98,6 +98,7 @@ struct Mesh {
     bool nodes_valid = false;
     std::vector<uint8_t> bface;              // [ne] bit j: local face j is on the boundary
     bool from_file = false;                  // read from an MFEM mesh file (no grid)
+    DevBuf<int> d_elem_at;  // Cartesian meshes: element at each grid position (built on use)
     DevBuf<int> d_topo;   // [ne][29] vertex, edge, face ids, boundary-face bits, 2 unused
     DevBuf<int8_t> d_ori; // [ne][12+6] edge (+-1) and face (0..7) orientations
     void build_topology(bool sfc);
@@ -106,6 +107,16 @@ struct Mesh {
     void load_mfem_v10(const char *text, size_t len);
     void ensure_device(hipStream_t s);
     void ensure_nodes(hipStream_t s);
+};
+
+// Column decomposition of a Cartesian mesh for the fused apply
+// (columns.hip): npx x npy columns of A x B elements, each swept in nseg
+// segments of ls element layers; elem_at[(k ny + j) nx + i] is the element
+// at grid position (i, j, k).
+struct ColumnGrid {
+    int nx = 0, ny = 0, nz = 0;
+    int npx = 0, npy = 0, nseg = 0, ls = 0;
+    const int *elem_at = nullptr;
 };
 
 // ---- restriction -----------------------------------------------------------
@@ -167,6 +178,10 @@ struct Form {
     // free or move those buffers (Assemble, a reduced-precision shadow of the
     // other kind, destroy) are refused.
     std::atomic<int> batch_refs{0};
+    // fused column apply (Cartesian meshes, diffusion only, q1d = p + 2)
+    ColumnGrid cols;
+    DevBuf<double> iface;
+    int use_columns = -1;  // -1 not decided yet, 0 no, 1 yes
 };
 // Refuses (H1PA_EINVAL) when live batch solves reference f's device data.
 void require_no_batch_refs(const Form &f, const char *what);
@@ -260,6 +275,15 @@ void diffusion_diagonal(int p, int q1d, int64_t ne, const double *basis, const d
 void mass_diagonal(int p, int q1d, int64_t ne, const double *basis, const double *pa,
                    const EOut &out, bool accumulate, hipStream_t s);
 bool apply_supported(int p, int q1d);
+// Fused L->L diffusion apply by column sweeps (Cartesian meshes, q1d = p + 2):
+// y = A x written whole; iface holds column_interface_size(p, g) doubles.
+bool columns_supported(int p, int q1d);
+void column_shape(int p, int &A, int &B);
+int64_t column_interface_size(int p, const ColumnGrid &g);
+// parts: 1 the column sweep, 2 the interface pass, 3 both.
+void diffusion_columns(int p, const ColumnGrid &g, const double *basis, const double *pa,
+                       const int *gmap, const double *x, double *y, double *iface, int parts,
+                       hipStream_t s);
 // Speed variants of the diffusion apply (bit-identical results).
 int diffusion_variant_count();
 void set_diffusion_variant(int p, int q1d, int variant);

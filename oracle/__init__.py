@@ -107,6 +107,7 @@ def lib():
         L.orc_pcg_chebyshev.restype = _i
         L.orc_domain_lf.argtypes = [_i, _i, _i, _pd, _d, _pd]
         L.orc_num_threads.restype = _i
+        L.orc_set_num_threads.argtypes = [_i]
     return _lib
 
 
@@ -470,6 +471,10 @@ def chebyshev_coeffs(order, max_eig):
 
 def num_threads():
     return lib().orc_num_threads()
+
+
+def set_num_threads(n):
+    lib().orc_set_num_threads(int(n))
 
 
 def round_bf16(a):

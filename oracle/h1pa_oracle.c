@@ -29,6 +29,13 @@ int orc_num_threads(void)
 #endif
 }
 
+/* Sets the OpenMP thread count of the following calls (the CPU baseline
+ * times the apply on all cores and on one). */
+void orc_set_num_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+}
+
 /* ======================================================================
  * Generalized Hilbert curve on an nx x ny x nz grid.
  * Follows mesh/ncmesh.cpp:5372-5375 (sgn), :5435-5597 (HilbertSfc3D) and

@@ -199,6 +199,7 @@ void orc_domain_lf(int p, int ne, int pg, const double *X, double f, double *bE)
 
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int  orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 #ifdef __cplusplus
 }

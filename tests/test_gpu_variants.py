@@ -44,6 +44,10 @@ def _dims(p):
 @pytest.mark.parametrize("dq", [1, 2])
 def test_fp64_variants_bitwise(orc, h1pa, torch_cuda, monkeypatch, p, dq):
     torch, M = torch_cuda, h1pa
+    # Mult of a diffusion form at q1d = p + 2 on a Cartesian mesh takes the
+    # fused column apply, which has no element-kernel variants: keep Mult on
+    # the element kernels here
+    monkeypatch.setenv("H1PA_COLUMNS", "0")
     q1d = p + dq
     dims = _dims(p)
     P = orc.Problem(None, p, Q=q1d, perturb=0.2, seed=3, coeff="sincos", dims=dims)
