@@ -22,13 +22,18 @@ Legs (all on by default; --legs selects):
          request loop mfem_amd/h1pa_serve in continuous-batching mixed mode and
          in serial FP64 mode.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 32e6]
-                    [--legs apply,solve,batch,serve] [--no-cpu] [--sweep]
+  sweep  configuration 2 at about 4M and 32M DOFs for p = 1..8: PA diffusion
+         and PA mass Mult, each against its own algorithmic bytes.
 
-Prints one JSON line.  `roofline` is for the dominant kernel of the apply
-(the element kernel), timed with HIP events on its launch stream;
-`cpu_baseline` times the CPU oracle (C + OpenMP restatement of MFEM's PA
-path) on a bounded sample on the host cores.
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--p 4] [--dofs 32e6]
+                    [--legs apply,sweep,solve,batch,serve] [--no-cpu]
+
+Prints one JSON line.  `roofline` is for one whole Operator::Mult (both of
+its kernels) against the fused apply's algorithmic bytes, timed with HIP
+events on the launch stream, with the first kernel (the column sweep, or the
+element kernel) as `roofline.element_kernel`; `cpu_baseline` times the CPU
+oracle (C + OpenMP restatement of MFEM's PA path) on a bounded sample on the
+host cores.
 """
 from __future__ import annotations
 
@@ -75,16 +80,24 @@ def algorithmic_bytes(p, ne, ndofs):
     return apply_, element, scatter
 
 
+def mass_bytes(p, ne, ndofs):
+    """Minimum HBM traffic of one fused L->L mass apply (FP64): PA data Q^3
+    doubles + one int32 index per E-DOF per element, x read and y written."""
+    D, Q = p + 1, p + 2
+    return ne * (Q ** 3 * 8 + D ** 3 * 4) + 2 * ndofs * 8
+
+
 # ---------------------------------------------------------------------------
 # apply
 # ---------------------------------------------------------------------------
-def build_problem(M, torch, p, n, seed=1):
+def build_problem(M, torch, p, n, seed=1, integ="diffusion"):
     mesh = M.Mesh.MakeCartesian3D(n, n, n)
     mesh.Perturb(0.2, seed)
     fes = M.FiniteElementSpace(mesh, M.H1_FECollection(p, 3))
     a = M.BilinearForm(fes)
     a.SetAssemblyLevel(M.AssemblyLevel.PARTIAL)
-    a.AddDomainIntegrator(M.DiffusionIntegrator(M.SinCosCoefficient()))
+    cls = M.DiffusionIntegrator if integ == "diffusion" else M.MassIntegrator
+    a.AddDomainIntegrator(cls(M.SinCosCoefficient()))
     a.Assemble()
     N = fes.GetNDofs()
     g = torch.Generator(device="cuda").manual_seed(seed)
@@ -93,9 +106,15 @@ def build_problem(M, torch, p, n, seed=1):
     return mesh, fes, a, x, y
 
 
-def run_apply(M, torch, p, n, steps, warmup):
-    mesh, fes, a, x, y = build_problem(M, torch, p, n)
+def run_apply(M, torch, p, n, steps, warmup, integ="diffusion"):
+    """`steps` timed Operator::Mult calls; wall clock around the region
+    (value) and HIP events on the launch stream around each kernel."""
+    if integ == "diffusion":
+        mesh, fes, a, x, y = build_problem(M, torch, p, n)
+    else:
+        mesh, fes, a, x, y = build_problem(M, torch, p, n, integ="mass")
     ne, N = fes.GetNE(), fes.GetNDofs()
+    path = a.MultPath()
     for _ in range(warmup):
         a.Mult(x, y)
     torch.cuda.synchronize()
@@ -104,13 +123,22 @@ def run_apply(M, torch, p, n, steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms = wall * 1e3 / steps
-    b_apply, b_elem, b_scat = algorithmic_bytes(p, ne, N)
+    ev_ms = tot_ms / steps  # events: first launch start to last launch end
+    if integ == "diffusion":
+        b_apply, b_elem, b_scat = algorithmic_bytes(p, ne, N)
+    else:
+        b_apply = b_elem = mass_bytes(p, ne, N)
+        b_scat = (N + 1) * 4 + ne * (p + 1) ** 3 * (4 + 8) + N * 8
+    if path == "columns":
+        b_elem = b_apply  # the column sweep moves the whole fused apply's bytes
     variant = a.Info()["variant"]
-    return dict(p=p, n=n, ne=ne, ndofs=N, ms_per_step=ms, event_ms_total=tot_ms / steps,
+    del mesh, fes, a, x, y
+    return dict(p=p, n=n, ne=ne, ndofs=N, path=path, ms_per_step=ms, event_ms=ev_ms,
                 ms_element=el_ms, ms_scatter=sc_ms, mdofs=N / (ms * 1e-3) / 1e6,
-                apply_gbs=b_apply / (ms * 1e-3) / 1e9, element_gbs=b_elem / (el_ms * 1e-3) / 1e9,
-                scatter_gbs=b_scat / (sc_ms * 1e-3) / 1e9, bytes_apply=b_apply,
-                bytes_element=b_elem, variant=variant)
+                apply_gbs=b_apply / (ev_ms * 1e-3) / 1e9,
+                element_gbs=b_elem / (el_ms * 1e-3) / 1e9,
+                scatter_gbs=b_scat / (sc_ms * 1e-3) / 1e9 if sc_ms > 0 else None,
+                bytes_apply=b_apply, bytes_element=b_elem, variant=variant)
 
 
 def _event_ms(torch, fn, reps):
@@ -183,34 +211,54 @@ def run_path_kernels(M, torch, p, n, reps=20, pcg_iters=50):
     return out
 
 
-def cpu_baseline(p, target_seconds=12.0):
-    """The CPU oracle (C + OpenMP restatement of MFEM's PA path) on a bounded
-    sample of the same workload: same p, perturbation and coefficient, on a
-    smaller mesh; MDOF/s of the L->L apply."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(p, reps=10, warmup=3):
+    """The CPU oracle (C + OpenMP restatement of MFEM's PA path, built -O3) on
+    a bounded sample of the same workload: same p, perturbation and
+    coefficient on a 2M-DOF mesh; MDOF/s of the L->L apply as the median of
+    `reps` applies after `warmup`, on every host thread this job has
+    (OMP_NUM_THREADS; 16 on the GPU box) and on one."""
     import numpy as np
     import oracle
     n = n_for_dofs(p, 2.0e6)
     P = oracle.Problem(n, p, perturb=0.2, seed=1, coeff="sincos")
     x = np.random.default_rng(1).uniform(-1, 1, P.ndofs)
-    P.mult(x)  # warm-up
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        P.mult(x)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > target_seconds or reps >= 2000:
-            break
-    return dict(value=round(P.ndofs * reps / el / 1e6, 3), unit="MDOF/s",
-                cores=oracle.num_threads(), kind="port",
-                sample=f"oracle PA diffusion apply (FP64), p={p}, n={n}^3 ({P.ndofs} DOFs), "
-                       f"{reps} applies in {el:.1f} s, OpenMP over elements")
+    all_threads = oracle.num_threads()
+    out = {}
+    for threads in (all_threads, 1):
+        oracle.set_num_threads(threads)
+        for _ in range(warmup):
+            P.mult(x)
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            P.mult(x)
+            ts.append(time.perf_counter() - t0)
+        ts.sort()
+        med = ts[len(ts) // 2] if len(ts) % 2 else 0.5 * (ts[len(ts) // 2 - 1] + ts[len(ts) // 2])
+        out[threads] = P.ndofs / med / 1e6
+    oracle.set_num_threads(all_threads)
+    return dict(value=round(out[all_threads], 3), unit="MDOF/s", cores=all_threads,
+                kind="port", value_1core=round(out[1], 3), cpu=cpu_model(),
+                sample=f"oracle PA diffusion apply (FP64, C + OpenMP over elements), p={p}, "
+                       f"n={n}^3 ({P.ndofs} DOFs): median of {reps} applies after {warmup} "
+                       f"warm-ups, on {all_threads} threads and on 1")
 
 
-def traffic_from_profiles(p, n, variant):
-    """HBM bytes per launch of the element kernel measured with rocprofv3 PMC
-    counters (FETCH_SIZE, WRITE_SIZE, scaled by the copy-kernel calibration),
-    when profiles/traffic.json holds this workload and element-kernel variant;
-    else None."""
+def traffic_from_profiles(p, n, kernel):
+    """HBM bytes of one launch (for kernel "mult_columns": of one Mult, both
+    kernels) measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE,
+    scaled by the copy-kernel calibration) when profiles/traffic.json holds
+    this workload and kernel; else None."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -218,8 +266,7 @@ def traffic_from_profiles(p, n, variant):
     except (OSError, ValueError):
         return None
     for r in rows:
-        if (r.get("p") == p and r.get("n") == n and r.get("kernel") == "element"
-                and r.get("variant_index") == variant):
+        if r.get("p") == p and r.get("n") == n and r.get("kernel") == kernel:
             return r.get("bytes_per_launch")
     return None
 
@@ -362,14 +409,15 @@ def run_batch(M, torch, nprob=512, seed=2024):
         return time.perf_counter() - t0, its, xs
 
     def batched(prec):
+        # timed from the first Add: each Add does the per-solve set-up
+        # (Jacobi diagonal, constraint mask, shadow PA data) that the
+        # sequential CGSolver::Mult does inside its own call
         bs = M.BatchSolver(prec, iters_per_replay=8)
-        xs = []
-        for i, pr in enumerate(probs):
-            x = pr[4].clone()
-            xs.append(x)
-            bs.Add(pr[3], pr[5], x, rel_tol=1e-10, max_iter=100000, tag=i)
+        xs = [pr[4].clone() for pr in probs]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        for i, pr in enumerate(probs):
+            bs.Add(pr[3], pr[5], xs[i], rel_tol=1e-10, max_iter=100000, tag=i)
         left = bs.Run()
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
@@ -403,7 +451,7 @@ def run_serve(rate, seconds, seed=1):
     exe = os.path.join(HERE, "mfem_amd", "h1pa_serve")
     if not os.path.exists(exe):
         raise RuntimeError("mfem_amd/h1pa_serve is not built (run __graft_entry__.build())")
-    out = {}
+    out = {"rate_per_s": rate, "window_s": seconds}
     for mode, prec in (("batch", "bf16"), ("serial", "fp64")):
         cmd = [exe, "--mode", mode, "--precision", prec, "--rate", str(rate), "--seconds",
                str(seconds), "--seed", str(seed)]
@@ -417,6 +465,12 @@ def run_serve(rate, seconds, seed=1):
     return out
 
 
+# Arrival rates of configuration 5 with windows of about 200 requests each:
+# the mix is ~10 s of GPU work per second of arrivals at 200/s, so at every
+# rate the box is overloaded and a longer window only lengthens the queue.
+SERVE_RATES = ((200.0, 1.0), (1000.0, 0.2), (5000.0, 0.04))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -424,11 +478,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--dofs", type=float, default=32e6)
-    ap.add_argument("--legs", default="apply,solve,batch,serve")
-    ap.add_argument("--sweep", action="store_true", help="also run p = 1..8 at --sweep-dofs")
-    ap.add_argument("--sweep-dofs", type=float, default=4e6)
-    ap.add_argument("--serve-rate", type=float, default=200.0)
-    ap.add_argument("--serve-seconds", type=float, default=1.0)
+    ap.add_argument("--legs", default="apply,sweep,solve,batch,serve")
+    ap.add_argument("--sweep-dofs", default="4e6,32e6",
+                    help="DOF targets of the p = 1..8 sweep (PA diffusion and PA mass)")
+    ap.add_argument("--serve-rates", default=",".join(f"{r:g}:{w:g}" for r, w in SERVE_RATES),
+                    help="rate:window pairs of the serving leg")
     ap.add_argument("--batch-problems", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
@@ -456,19 +510,28 @@ def main():
     r = run_apply(M, torch, args.p, n, args.steps, args.warmup)
     log(str(r))
     torch.cuda.empty_cache()
-    log("path kernels: gather, scatter, diagonal, PCG iteration")
-    path_kernels = run_path_kernels(M, torch, args.p, n)
-    log(str(path_kernels))
-    torch.cuda.empty_cache()
+    path_kernels = {}
+    if "apply" in legs:
+        log("path kernels: gather, scatter, diagonal, PCG iteration")
+        path_kernels = run_path_kernels(M, torch, args.p, n)
+        log(str(path_kernels))
+        torch.cuda.empty_cache()
     sweep = []
-    if args.sweep:
-        for p in range(1, 9):
-            ns = n_for_dofs(p, args.sweep_dofs)
-            log(f"sweep p={p} n={ns}^3")
-            s = run_apply(M, torch, p, ns, max(10, args.steps // 2), 3)
-            sweep.append({k: s[k] for k in ("p", "n", "ndofs", "mdofs", "apply_gbs",
-                                             "element_gbs", "ms_per_step")})
-            torch.cuda.empty_cache()
+    if "sweep" in legs:
+        for target in (float(v) for v in args.sweep_dofs.split(",") if v):
+            for p in range(1, 9):
+                ns = n_for_dofs(p, target)
+                row = dict(p=p, n=ns, target_dofs=target)
+                for integ in ("diffusion", "mass"):
+                    log(f"sweep {integ} p={p} n={ns}^3")
+                    s_ = run_apply(M, torch, p, ns, 10, 3, integ=integ)
+                    row["dofs"] = s_["ndofs"]
+                    row[integ] = dict(path=s_["path"], ms=round(s_["event_ms"], 4),
+                                      mdofs=round(s_["ndofs"] / (s_["event_ms"] * 1e-3) / 1e6, 1),
+                                      gbs=round(s_["apply_gbs"], 1),
+                                      frac=round(s_["apply_gbs"] / HBM_PEAK_GBS, 4))
+                    torch.cuda.empty_cache()
+                sweep.append(row)
 
     extra = {}
     if "solve" in legs:
@@ -482,13 +545,17 @@ def main():
         extra["batch"] = run_batch(M, torch, args.batch_problems)
         torch.cuda.empty_cache()
     if "serve" in legs:
-        extra["serve"] = run_serve(args.serve_rate, args.serve_seconds)
+        extra["serve"] = []
+        for pair in args.serve_rates.split(","):
+            rate, window = (float(v) for v in pair.split(":"))
+            extra["serve"].append(run_serve(rate, window))
 
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle)")
         cpu = cpu_baseline(args.p)
 
+    mult_gbs = r["apply_gbs"]
     line = {
         "metric": METRIC,
         "value": round(r["mdofs"], 2),
@@ -503,35 +570,46 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"PA diffusion apply, unit cube {n}^3 hex, vertex perturbation 0.2h, "
-                        f"H1 p={args.p} Q={args.p + 2}, k=1+0.5sin(2pi x)cos(2pi y)sin(2pi z)",
+            "workload": f"PA diffusion apply (Operator::Mult, L-vector to L-vector), unit cube "
+                        f"{n}^3 hex, vertex perturbation 0.2h, H1 p={args.p} Q={args.p + 2}, "
+                        f"k=1+0.5sin(2pi x)cos(2pi y)sin(2pi z)",
             "p": args.p, "q1d": args.p + 2, "elements": r["ne"], "dofs": r["ndofs"],
-            "apply_gbs_algorithmic": round(r["apply_gbs"], 1),
-            "apply_roofline_frac": round(r["apply_gbs"] / HBM_PEAK_GBS, 4),
-            "scatter_ms": round(r["ms_scatter"], 4),
-            "element_variant": r["variant"],
+            "mult_path": r["path"], "element_variant": r["variant"],
         },
-        "kernels": path_kernels,
         "roofline": {
             "bound": "hbm",
-            "achieved": round(r["element_gbs"], 1),
+            "achieved": round(mult_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": round(r["element_gbs"] / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(args.p, n, r["variant"]),
-            "kernel": f"k_diffusion_apply<double,double,{args.p + 1},{args.p + 2},...>",
-            "kernel_ms": round(r["ms_element"], 4),
-            "bytes_per_launch": r["bytes_element"],
+            "frac": round(mult_gbs / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profiles(args.p, n, "mult_" + r["path"]),
+            "kernel": ("k_diffusion_columns + k_columns_interface (one Operator::Mult)"
+                       if r["path"] == "columns" else
+                       "k_diffusion_apply + k_restrict_mult_t (one Operator::Mult)"),
+            "ms": round(r["event_ms"], 4),
+            "bytes_per_launch": r["bytes_apply"],
+            "bytes_formula": "NE (6 Q^3 8 + D^3 4) + 2 N 8",
+            "element_kernel": {
+                "kernel": ("k_diffusion_columns" if r["path"] == "columns" else
+                           f"k_diffusion_apply<double,double,{args.p + 1},{args.p + 2},...>"),
+                "ms": round(r["ms_element"], 4),
+                "bytes": r["bytes_element"],
+                "gbs": round(r["element_gbs"], 1),
+                "frac": round(r["element_gbs"] / HBM_PEAK_GBS, 4),
+                "second_kernel_ms": round(r["ms_scatter"], 4),
+            },
         },
+        "kernels": path_kernels,
         "cpu_baseline": cpu,
     }
     line.update(extra)
-    if "serve" in extra:
-        s = extra["serve"]
-        line["solves_per_s"] = s["batch_bf16"]["solves_per_s"]
-        line["p99_latency_s"] = s["batch_bf16"]["latency_p99_s"]
-        line["fp64_pcg_solves_per_s"] = s["serial_fp64"]["solves_per_s"]
-        line["fp64_pcg_p99_latency_s"] = s["serial_fp64"]["latency_p99_s"]
+    if "serve" in extra and extra["serve"]:
+        s0 = extra["serve"][0]
+        line["solves_per_s"] = s0["batch_bf16"]["solves_per_s"]
+        line["p99_latency_s"] = s0["batch_bf16"]["latency_p99_s"]
+        line["fp64_pcg_solves_per_s"] = s0["serial_fp64"]["solves_per_s"]
+        line["fp64_pcg_p99_latency_s"] = s0["serial_fp64"]["latency_p99_s"]
+        line["solves_per_s_ratio"] = s0["solves_per_s_ratio"]
     if sweep:
         line["sweep"] = sweep
     print(json.dumps(line), flush=True)

@@ -40,7 +40,10 @@ enum {
     H1PA_ENOMEM = 2,    /* host or device allocation failed      */
     H1PA_EHIP = 3,      /* HIP runtime error                     */
     H1PA_ENOTCONV = 4,  /* solver did not converge               */
-    H1PA_EUNSUP = 5     /* unsupported order / shape             */
+    H1PA_EUNSUP = 5,    /* unsupported order / shape             */
+    H1PA_ENONFINITE = 6 /* a solver's (B r, r) or (A d, d) was not finite
+                           (CGSolver's MFEM_VERIFY(IsFinite(...)) aborts,
+                           linalg/solvers.cpp:895, :935, :970, :1011)     */
 };
 
 typedef struct h1pa_mesh h1pa_mesh;
@@ -216,10 +219,18 @@ int h1pa_form_assemble_diagonal(h1pa_form *form, double *d_diag, void *stream);
 int h1pa_form_pa_data(const h1pa_form *form, const double **d_diffusion, const double **d_mass);
 /* info[0..3] = q1d, integrators, ndofs, selected diffusion-apply variant (-1: none) */
 int h1pa_form_info(const h1pa_form *form, int64_t *info);
+/* Which kernels h1pa_form_mult runs (no MFEM counterpart; MFEM's Mult is
+ * gather, element apply, scatter, fem/bilinearform_ext.cpp:487-560):
+ * *path = 1 the fused column apply (a diffusion form on a Cartesian mesh at
+ * q1d = p + 2: element results summed in LDS, no E-vector), 0 the element
+ * kernels followed by the E->L scatter.  Sets the column apply up on first
+ * use, on `stream`. */
+int h1pa_form_mult_path(h1pa_form *form, int *path, void *stream);
 /* Benchmark helper (no MFEM counterpart): `steps` back-to-back h1pa_form_mult
  * calls with HIP events recorded on `stream` around each launch.  Returns the
  * wall time of the whole region and the average duration of the element
- * kernel(s) and of the E->L scatter, in milliseconds. */
+ * kernel(s) (the column sweep on the fused path) and of the E->L scatter (the
+ * interface pass), in milliseconds. */
 int h1pa_form_mult_timed(h1pa_form *form, const double *d_x, double *d_y, int steps,
                          double *ms_total, double *ms_element, double *ms_scatter, void *stream);
 
@@ -366,6 +377,10 @@ typedef struct {
     int outer_iterations;  /* mixed: refinement steps; FP64: 0                  */
     double final_norm;     /* FP64: sqrt((B r, r)); mixed: ||b - A x||_2        */
     double rel_residual;   /* mixed: ||b - A x||_2 / ||b||_2; FP64: -1          */
+    int status;            /* H1PA_OK (converged), H1PA_ENOTCONV (max_iter, or
+                              (B r, r) < 0 or (A d, d) = 0 as CGSolver stops),
+                              H1PA_ENONFINITE (a non-finite product: the solve
+                              is retired, the others go on)                  */
 } h1pa_batch_result;
 /* No MFEM counterpart (MFEM runs one CGSolver::Mult at a time). */
 int h1pa_batch_create(const h1pa_batch_opts *opts, h1pa_batch **batch);

@@ -34,7 +34,8 @@ _lib = None
 _i, _i64, _u, _d, _vp = C.c_int, C.c_int64, C.c_uint, C.c_double, C.c_void_p
 _pi64 = C.POINTER(C.c_int64)
 
-H1PA_OK, H1PA_EINVAL, H1PA_ENOMEM, H1PA_EHIP, H1PA_ENOTCONV, H1PA_EUNSUP = range(6)
+H1PA_OK, H1PA_EINVAL, H1PA_ENOMEM, H1PA_EHIP, H1PA_ENOTCONV, H1PA_EUNSUP, H1PA_ENONFINITE = \
+    range(7)
 INTEG_MASS, INTEG_DIFFUSION = 1, 2
 COEFF_CONSTANT, COEFF_SINCOS, COEFF_PER_ELEMENT, COEFF_QUADRATURE, COEFF_FOURIER = range(5)
 QI_VALUES, QI_DERIVATIVES, QI_DETERMINANTS, QI_PHYSICAL_DERIVATIVES = 1, 2, 4, 8
@@ -82,7 +83,7 @@ class BatchOpts(C.Structure):
 
 class BatchResult(C.Structure):
     _fields_ = [("converged", _i), ("iterations", _i), ("outer_iterations", _i),
-                ("final_norm", _d), ("rel_residual", _d)]
+                ("final_norm", _d), ("rel_residual", _d), ("status", _i)]
 
 
 # name -> (restype, argtypes); every entry point of include/h1pa.h
@@ -124,6 +125,7 @@ SIGNATURES = {
     "h1pa_form_assemble_diagonal": (_i, [_vp, _vp, _vp]),
     "h1pa_form_pa_data": (_i, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
     "h1pa_form_info": (_i, [_vp, _pi64]),
+    "h1pa_form_mult_path": (_i, [_vp, C.POINTER(_i), _vp]),
     "h1pa_form_mult_timed": (_i, [_vp, _vp, _vp, _i, C.POINTER(_d), C.POINTER(_d),
                                   C.POINTER(_d), _vp]),
     "h1pa_form_constrained_mult": (_i, [_vp, _vp, _i64, _vp, _vp, _vp]),
@@ -802,6 +804,13 @@ class BilinearForm:
         _check(lib().h1pa_form_info(self._forms[0]._h, info))
         return dict(q1d=info[0], integrators=info[1], ndofs=info[2], variant=info[3])
 
+    def MultPath(self):
+        """'columns' when Mult runs the fused column apply, 'element+scatter'
+        when it runs the element kernels and the E->L scatter."""
+        v = C.c_int()
+        _check(lib().h1pa_form_mult_path(self._single()._h, C.byref(v), _stream()))
+        return "columns" if v.value else "element+scatter"
+
     def DeviceBytes(self):
         """Device bytes held by the form(s), the space and the mesh."""
         tot = 0
@@ -1092,7 +1101,9 @@ class BatchSolver:
 
     def Poll(self, cap=4096):
         """[(tag, {converged, iterations, outer_iterations, final_norm,
-        rel_residual})] for the solves that finished since the last call."""
+        rel_residual, status})] for the solves that finished since the last
+        call; status is H1PA_OK, H1PA_ENOTCONV or H1PA_ENONFINITE (a
+        non-finite product retired that solve alone)."""
         tags = (C.c_int64 * cap)()
         res = (BatchResult * cap)()
         n = C.c_int()
@@ -1103,7 +1114,7 @@ class BatchSolver:
             out.append((int(tags[i]), dict(converged=bool(r.converged), iterations=r.iterations,
                                            outer_iterations=r.outer_iterations,
                                            final_norm=r.final_norm,
-                                           rel_residual=r.rel_residual)))
+                                           rel_residual=r.rel_residual, status=r.status)))
             self._keep.pop(int(tags[i]), None)
         return out
 

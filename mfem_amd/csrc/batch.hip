@@ -51,6 +51,7 @@ struct SlotState {
     double rho_ref, rho64, rho64_0, r2, b2;    // mixed: refinement bookkeeping
     int iter, in_loop, max_iter, converged, final_iter;
     int phase, outer;
+    int nonfinite;  // a (B r, r) or (A d, d) was not finite: retired with a failed status
 };
 
 struct SlotDesc {
@@ -241,7 +242,10 @@ kb_fin_init(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
     if (sd.abs_tol * sd.abs_tol > r0) r0 = sd.abs_tol * sd.abs_tol;
     st->r0 = r0;
     int next = PH_INNER;
-    if (nom < 0.0) {
+    if (!isfinite(nom)) {  // MFEM_VERIFY(IsFinite(nom)) in CGSolver::Mult
+        st->nonfinite = 1;
+        next = PH_DONE;
+    } else if (nom < 0.0) {
         next = PH_DONE;
     } else if (nom <= r0) {
         st->converged = 1;
@@ -305,7 +309,12 @@ kbm_fin_init(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
     st->iter = 0;
     st->outer = 0;
     st->converged = 0;
-    if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
+    if (!isfinite(tot[0]) || !isfinite(tot[1])) {
+        st->nonfinite = 1;
+        st->phase = PH_DONE;
+    } else if (tot[1] < 0.0) {  // the preconditioner is not positive definite
+        st->phase = PH_DONE;
+    } else if (tot[1] == 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
         st->converged = 1;
         st->phase = PH_DONE;
     } else {
@@ -396,7 +405,11 @@ kb_fin_den(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
     slot_totals<1>(sd, partials, sh, tot);
     if (threadIdx.x != 0) return;
     st->den = tot[0];
-    if (tot[0] == 0.0) {
+    if (!isfinite(tot[0])) {  // MFEM_VERIFY(IsFinite(den))
+        st->nonfinite = 1;
+        st->final_iter = st->in_loop ? st->iter : 0;
+        st->phase = PH_DONE;
+    } else if (tot[0] == 0.0) {
         if (MIXED) {
             st->phase = PH_RU;  // inner residual is zero: check the true one
         } else {            // CGSolver: den == 0 ends the loop
@@ -453,7 +466,11 @@ kb_fin_update(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc
     const int i = st->iter;
     st->in_loop = 1;
     st->betanom = betanom;
-    if (betanom < 0.0) {
+    if (!isfinite(betanom)) {  // MFEM_VERIFY(IsFinite(betanom))
+        st->nonfinite = 1;
+        st->final_iter = i;
+        st->phase = PH_DONE;
+    } else if (betanom < 0.0) {
         st->final_iter = i;
         st->phase = PH_DONE;
     } else if (betanom <= st->r0) {
@@ -515,6 +532,15 @@ kbm_fin_update(const int *__restrict__ runlist, const SlotDesc *__restrict__ des
     if (threadIdx.x != 0) return;
     const double rho = tot[0];
     st->iter += 1;
+    if (!isfinite(rho)) {
+        st->nonfinite = 1;
+        st->phase = PH_DONE;
+        return;
+    }
+    if (rho < 0.0) {  // the preconditioner is not positive definite
+        st->phase = PH_DONE;
+        return;
+    }
     const double tol2 = sd.rel_tol * sd.rel_tol;
     if (rho <= sd.delta * sd.delta * st->rho_ref || rho <= tol2 * st->rho64_0 ||
         st->iter >= st->max_iter) {
@@ -613,7 +639,12 @@ kbm_fin_ru(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
     st->outer += 1;
     st->rho64 = tot[1];
     st->r2 = tot[2];
-    if (tot[1] <= sd.rel_tol * sd.rel_tol * st->rho64_0) {
+    if (!isfinite(tot[0]) || !isfinite(tot[1])) {
+        st->nonfinite = 1;
+        st->phase = PH_DONE;
+    } else if (tot[1] < 0.0) {
+        st->phase = PH_DONE;
+    } else if (tot[1] <= sd.rel_tol * sd.rel_tol * st->rho64_0) {
         st->converged = 1;
         st->phase = PH_DONE;
     } else if (st->outer >= sd.max_outer || st->iter >= st->max_iter) {
@@ -1328,6 +1359,7 @@ int batch_poll(Batch &B, int cap, int64_t *tags, h1pa_batch_result *res) {
             r.final_norm = mixed ? std::sqrt(std::max(st.rho64, 0.0))
                                  : (st.betanom >= 0 ? std::sqrt(st.betanom) : st.betanom);
             r.rel_residual = mixed ? (st.b2 > 0 ? std::sqrt(st.r2 / st.b2) : 0.0) : -1.0;
+            r.status = st.nonfinite ? H1PA_ENONFINITE : (st.converged ? H1PA_OK : H1PA_ENOTCONV);
         }
         ++n;
         // retire: free the slot's vectors, keep its position

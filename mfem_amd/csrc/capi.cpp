@@ -783,6 +783,14 @@ int h1pa_form_info(const h1pa_form *form, int64_t *info) {
     API_END
 }
 
+int h1pa_form_mult_path(h1pa_form *form, int *path, void *stream) {
+    API_BEGIN
+    REQUIRE(form && path, "form_mult_path: null argument");
+    require_assembled(form->f);
+    *path = columns_ready(form->f, as_stream(stream)) ? 1 : 0;
+    API_END
+}
+
 int h1pa_form_constrained_mult(h1pa_form *form, const int *d_ess, int64_t n_ess, const double *x,
                                double *y, void *stream) {
     API_BEGIN

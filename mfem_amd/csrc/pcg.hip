@@ -39,6 +39,7 @@ struct PcgState {
     int done;
     int converged;
     int final_iter;
+    int nonfinite;   // a (B r, r) or (A d, d) was not finite: MFEM_VERIFY fails
 };
 
 namespace kern {
@@ -200,8 +201,12 @@ k_pcg_fin_init(int nb, const double *__restrict__ partials, PcgState *st, double
     st->final_iter = 0;
     st->converged = 0;
     st->done = 0;
+    st->nonfinite = 0;
     st->betanom = nom;
-    if (nom < 0.0) {
+    if (!isfinite(nom)) {  // MFEM_VERIFY(IsFinite(nom)) in CGSolver::Mult
+        st->done = 1;
+        st->nonfinite = 1;
+    } else if (nom < 0.0) {  // the preconditioner is not positive definite
         st->done = 1;
     } else if (nom <= r0) {
         st->done = 1;
@@ -263,7 +268,11 @@ k_pcg_fin_den(int nb, const double *__restrict__ partials, PcgState *st) {
     sum_partials<1>(partials, nb, sh, tot);
     if (threadIdx.x != 0) return;
     st->den = tot[0];
-    if (tot[0] == 0.0) {  // CGSolver: den <= 0 and den == 0 ends the loop
+    if (!isfinite(tot[0])) {  // MFEM_VERIFY(IsFinite(den))
+        st->done = 1;
+        st->nonfinite = 1;
+        st->final_iter = st->in_loop ? st->iter : 0;
+    } else if (tot[0] == 0.0) {  // CGSolver: den <= 0 and den == 0 ends the loop
         st->done = 1;
         st->final_iter = st->in_loop ? st->iter : 0;
     }
@@ -316,7 +325,11 @@ k_pcg_fin_update(int nb, const double *__restrict__ partials, PcgState *st) {
     const int i = st->iter;
     st->in_loop = 1;
     st->betanom = betanom;
-    if (betanom < 0.0) {
+    if (!isfinite(betanom)) {  // MFEM_VERIFY(IsFinite(betanom))
+        st->done = 1;
+        st->nonfinite = 1;
+        st->final_iter = i;
+    } else if (betanom < 0.0) {
         st->done = 1;
         st->final_iter = i;
     } else if (betanom <= st->r0) {
@@ -400,6 +413,7 @@ struct IrState {
     int ru;   // 1: refinement step due this iteration
     int nru;  // !ru: skip flag of the FP64 element kernel
     int done, converged;
+    int nonfinite;  // a dot product was not finite (CGSolver's MFEM_VERIFY)
 };
 
 __global__ void __launch_bounds__(kThreads)
@@ -441,7 +455,13 @@ k_ir_fin_init(int nb, const double *__restrict__ partials, IrState *st) {
     st->nru = 1;
     st->converged = 0;
     st->done = 0;
-    if (tot[1] <= 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
+    st->nonfinite = 0;
+    if (!isfinite(tot[0]) || !isfinite(tot[1])) {
+        st->done = 1;
+        st->nonfinite = 1;
+    } else if (tot[1] < 0.0) {  // the preconditioner is not positive definite
+        st->done = 1;
+    } else if (tot[1] == 0.0) {  // b = 0 on the free DOFs: x = 0 is the solution
         st->done = 1;
         st->converged = 1;
     }
@@ -497,7 +517,12 @@ k_ir_fin_den(int nb, const double *__restrict__ partials, IrState *st) {
     if (st->done) return;
     double tot[1];
     sum_partials<1>(partials, nb, sh, tot);
-    if (threadIdx.x == 0) st->den = tot[0];
+    if (threadIdx.x != 0) return;
+    st->den = tot[0];
+    if (!isfinite(tot[0])) {
+        st->done = 1;
+        st->nonfinite = 1;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -545,6 +570,15 @@ k_ir_fin_update(int nb, const double *__restrict__ partials, IrState *st) {
     if (threadIdx.x != 0) return;
     const double rho_new = tot[0];
     st->iter += 1;
+    if (!isfinite(rho_new)) {
+        st->done = 1;
+        st->nonfinite = 1;
+        return;
+    }
+    if (rho_new < 0.0) {  // the preconditioner is not positive definite
+        st->done = 1;
+        return;
+    }
     const bool trigger = st->den == 0.0 || rho_new <= st->delta2 * st->rho_ref ||
                          rho_new <= st->tol2 * st->rho64_0 || st->iter >= st->max_inner;
     if (trigger) {
@@ -617,7 +651,12 @@ k_ir_fin_ru(int nb, const double *__restrict__ partials, IrState *st) {
     st->r2 = tot[2];
     st->ru = 0;
     st->nru = 1;
-    if (tot[1] <= st->tol2 * st->rho64_0) {
+    if (!isfinite(tot[0]) || !isfinite(tot[1])) {
+        st->done = 1;
+        st->nonfinite = 1;
+    } else if (tot[1] < 0.0) {
+        st->done = 1;
+    } else if (tot[1] <= st->tol2 * st->rho64_0) {
         st->done = 1;
         st->converged = 1;
     } else if (st->outer >= st->max_outer || st->iter >= st->max_inner) {
@@ -962,6 +1001,8 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
     stats.initial_norm = h.nom0 >= 0.0 ? std::sqrt(h.nom0) : h.nom0;
     stats.final_norm = h.betanom >= 0.0 ? std::sqrt(h.betanom) : h.betanom;
     stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    H1PA_CHECK(!h.nonfinite, H1PA_ENONFINITE,
+               "PCG: (B r, r) or (A d, d) is not finite (CGSolver: MFEM_VERIFY(IsFinite))");
 }
 
 }  // namespace
@@ -1315,6 +1356,8 @@ void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts
     stats.initial_norm = h.nom0 >= 0.0 ? std::sqrt(h.nom0) : h.nom0;
     stats.final_norm = h.betanom >= 0.0 ? std::sqrt(h.betanom) : h.betanom;
     stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    H1PA_CHECK(!h.nonfinite, H1PA_ENONFINITE,
+               "PCG: (B r, r) or (A d, d) is not finite (CGSolver: MFEM_VERIFY(IsFinite))");
 }
 
 void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
@@ -1386,6 +1429,8 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     stats.final_norm = std::sqrt(std::max(h.rho64, 0.0));
     stats.rel_residual = h.b2 > 0 ? std::sqrt(h.r2 / h.b2) : 0.0;
     stats.seconds = std::chrono::duration<double>(t1 - t0).count();
+    H1PA_CHECK(!h.nonfinite, H1PA_ENONFINITE,
+               "mixed solve: a residual or search-direction product is not finite");
 }
 
 }  // namespace h1pa
