@@ -74,6 +74,29 @@ int h1pa_mesh_create_cartesian(int nx, int ny, int nz, int sfc_ordering, h1pa_me
  * attributes).  A 'nodes' section, other element types, inverted elements and
  * other dimensions give H1PA_EUNSUP. */
 int h1pa_mesh_load(const char *text, int64_t len, h1pa_mesh **mesh);
+/* Mesh(vertices, nv, element_indices, Geometry::CUBE, attributes, ne,
+ * boundary_indices, Geometry::SQUARE, bdr_attributes, nbdr, 3, 3), which ends
+ * in FinalizeTopology (mesh/mesh.cpp:4891-4930): h_xyz[nv][3] vertex
+ * coordinates, h_ev[ne][8] element vertices in MFEM's hexahedron order
+ * (Geometry::CUBE, fem/geom.cpp), h_bdr[nbdr][4] boundary quadrilaterals
+ * (each must be an element face).  Elements keep the given order; edges and
+ * faces are numbered by first encounter, as FinalizeTopology numbers them;
+ * the boundary DOFs of spaces on the mesh are those of the listed faces (MFEM
+ * marks every boundary attribute essential in the solves here, so the
+ * attributes are not needed).  Host inputs, copied.  Inverted elements give
+ * H1PA_EUNSUP. */
+int h1pa_mesh_create(int64_t nv, const double *h_xyz, int64_t ne, const int *h_ev, int64_t nbdr,
+                     const int *h_bdr, h1pa_mesh **mesh);
+/* High-order mesh nodes (Mesh::SetCurvature(order) then writing the Nodes,
+ * or Mesh::SetNodalGridFunction, mesh/mesh.cpp:7200-7240): the nodes of an
+ * H1_FECollection(order, 3) (Gauss-Lobatto) vector space, given as the
+ * lexicographic E-vector d_nodes[(order+1)^3][3][ne] -- what
+ * GetElementRestriction(LEXICOGRAPHIC)->Mult(*mesh.GetNodes(), Enodes) gives
+ * and what GeometricFactors::Compute interpolates (mesh/mesh.cpp:15445-15497).
+ * Copied on `stream`; forms assembled afterwards use them (MFEM: forms
+ * assembled after Mesh::DeleteGeometricFactors).  h1pa_mesh_set_curvature
+ * or h1pa_mesh_perturb replaces them by nodes computed from the vertices. */
+int h1pa_mesh_set_nodes(h1pa_mesh *mesh, int order, const double *d_nodes, void *stream);
 int h1pa_mesh_destroy(h1pa_mesh *mesh);
 /* Perturb interior vertices by U(-amp*h, amp*h) per coordinate (counter-based
  * splitmix64 with `seed`); boundary vertices stay on the cube faces.  No MFEM
@@ -140,6 +163,12 @@ int h1pa_restriction_abs_mult_transpose(const h1pa_restriction *r, const double 
 /* E->L taking the last E-entry of each L-DOF (restriction.cpp:230-253) */
 int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *d_x,
                                        double *d_y, void *stream);
+
+/* E-vector mask (restriction.cpp:255-290): d_ye[first E-entry of each L-DOF
+ * in indices order] = 1, every other entry 0 -- the weights that make an E->L
+ * sum count each L-DOF once (TensorProductPRefinementTransferOperator,
+ * fem/transfer.cpp:2682). */
+int h1pa_restriction_boolean_mask(const h1pa_restriction *r, double *d_ye, void *stream);
 
 /* ---- quadrature interpolation of the mesh nodes (geometric factors)
  *      GeometricFactors::Compute (mesh/mesh.cpp:15445-15497) via
@@ -243,11 +272,11 @@ int h1pa_form_constrained_mult(h1pa_form *form, const int *d_ess, int64_t n_ess,
 int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess,
                             const double *d_x, const double *d_b, double *d_X, double *d_B,
                             void *stream);
+/* Vector::Add (linalg/vector.cpp:326): d_y += a d_x on n device entries. */
+int h1pa_vector_add(double *d_y, double a, const double *d_x, int64_t n, void *stream);
 /* Vector::Randomize (linalg/vector.cpp:955-966) on the device with a
  * counter-based generator: d_x[i] = lo + (hi - lo) u_i, u_i =
  * (splitmix64(seed ^ splitmix64(i)) >> 11) 2^-53, reproducible for any n. */
-/* Vector::Add (linalg/vector.cpp:326): d_y += a d_x on n device entries. */
-int h1pa_vector_add(double *d_y, double a, const double *d_x, int64_t n, void *stream);
 int h1pa_vector_randomize(double *d_x, int64_t n, uint64_t seed, double lo, double hi,
                           void *stream);
 /* DomainLFIntegrator(ConstantCoefficient(f)) assembled into an L-vector with

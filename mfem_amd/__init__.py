@@ -94,6 +94,8 @@ SIGNATURES = {
     "h1pa_stream_synchronize": (_i, [_vp]),
     "h1pa_mesh_create_cartesian": (_i, [_i, _i, _i, _i, C.POINTER(_vp)]),
     "h1pa_mesh_load": (_i, [C.c_char_p, _i64, C.POINTER(_vp)]),
+    "h1pa_mesh_create": (_i, [_i64, _vp, _i64, _vp, _i64, _vp, C.POINTER(_vp)]),
+    "h1pa_mesh_set_nodes": (_i, [_vp, _i, _vp, _vp]),
     "h1pa_mesh_destroy": (_i, [_vp]),
     "h1pa_mesh_perturb": (_i, [_vp, _d, C.c_uint64]),
     "h1pa_mesh_set_curvature": (_i, [_vp, _i, _d]),
@@ -116,6 +118,7 @@ SIGNATURES = {
     "h1pa_restriction_add_mult_transpose": (_i, [_vp, _vp, _vp, _d, _vp]),
     "h1pa_restriction_abs_mult_transpose": (_i, [_vp, _vp, _vp, _vp]),
     "h1pa_restriction_mult_left_inverse": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_restriction_boolean_mask": (_i, [_vp, _vp, _vp]),
     "h1pa_form_create": (_i, [_vp, C.POINTER(FormDesc), C.POINTER(_vp)]),
     "h1pa_form_destroy": (_i, [_vp]),
     "h1pa_form_assemble": (_i, [_vp, _vp]),
@@ -310,6 +313,22 @@ class Mesh:
         return Mesh(path)
 
     @staticmethod
+    def FromArrays(vertices, elements, boundary):
+        """Mesh(vertices, nv, element_indices, CUBE, ..., boundary_indices,
+        SQUARE, ...) + FinalizeTopology (mesh/mesh.cpp:4891-4930): host arrays
+        vertices [nv][3], elements [ne][8] (MFEM hexahedron vertex order),
+        boundary [nbdr][4] quadrilaterals."""
+        xyz = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        ev = np.ascontiguousarray(elements, dtype=np.int32).reshape(-1, 8)
+        bq = np.ascontiguousarray(boundary, dtype=np.int32).reshape(-1, 4)
+        h = C.c_void_p()
+        _check(lib().h1pa_mesh_create(xyz.shape[0], xyz.ctypes.data_as(_vp), ev.shape[0],
+                                      ev.ctypes.data_as(_vp), bq.shape[0],
+                                      bq.ctypes.data_as(_vp) if bq.shape[0] else None,
+                                      C.byref(h)))
+        return Mesh(h)
+
+    @staticmethod
     def MakeCartesian3D(nx, ny, nz, type=Element.HEXAHEDRON, sx=1.0, sy=1.0, sz=1.0,
                         sfc_ordering=True):
         if type != Element.HEXAHEDRON:
@@ -349,6 +368,14 @@ class Mesh:
     def SetCurvature(self, order, amp=0.0):
         """SetCurvature(order) followed by the curving map of amplitude amp."""
         _check(lib().h1pa_mesh_set_curvature(self._h, int(order), float(amp)))
+
+    def SetNodes(self, order, enodes):
+        """High-order nodes of an H1_FECollection(order, 3) vector space as
+        the lexicographic E-vector [(order+1)^3][3][ne] (FP64, device): what
+        the nodal space's ElementRestriction(LEXICOGRAPHIC) gives from
+        Mesh::GetNodes() (mesh/mesh.cpp:7200-7240, :15445-15497)."""
+        _check_vec(enodes, (order + 1) ** 3 * 3 * self.GetNE())
+        _check(lib().h1pa_mesh_set_nodes(self._h, int(order), _ptr(enodes), _stream()))
 
     def GetVertices(self):
         xyz = np.zeros(3 * self.GetNV())
@@ -463,6 +490,12 @@ class ElementRestriction:
     def MultLeftInverse(self, x, y):
         _check_vec(x, self.ne * self.dof); _check_vec(y, self.ndofs)
         _check(lib().h1pa_restriction_mult_left_inverse(self._h, _ptr(x), _ptr(y), _stream()))
+
+    def BooleanMask(self, y):
+        """y (E-vector) = 1 at each L-DOF's first E-entry, 0 elsewhere
+        (fem/restriction.cpp:255-290)."""
+        _check_vec(y, self.ne * self.dof)
+        _check(lib().h1pa_restriction_boolean_mask(self._h, _ptr(y), _stream()))
 
 
 class FiniteElementSpace:

@@ -18,6 +18,10 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libh1pa.so")
 SERVE = os.path.join(HERE, "h1pa_serve")
+ROOT = os.path.dirname(HERE)
+# C++ callers of the C ABI that the tests run (each a main() over include/h1pa.h)
+CAPI_TESTS = ["capi_mesh_arrays"]
+CAPI_BIN = os.path.join(ROOT, "tests", "_bin")
 CEILINGS = os.path.join(HERE, "h1pa_ceilings")
 
 SOURCES = ["basis.cpp", "mesh.cpp", "capi.cpp", "restriction.hip", "geometry.hip",
@@ -73,6 +77,16 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"h1pa_ceilings build failed:\n{r.stderr}")
     os.replace(tmp, CEILINGS)
+    os.makedirs(CAPI_BIN, exist_ok=True)
+    for name in CAPI_TESTS:
+        exe = os.path.join(CAPI_BIN, name)
+        cmd = [_hipcc(), "-O2", "-std=c++17", "-Wall", f"-I{os.path.join(ROOT, 'include')}",
+               os.path.join(ROOT, "tests", name + ".cpp"), "-o", exe + ".tmp", f"-L{HERE}",
+               "-lh1pa", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../mfem_amd"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"{name} build failed:\n{r.stderr}")
+        os.replace(exe + ".tmp", exe)
     if verbose:
         print(f"built {LIB} and {SERVE}")
     return LIB

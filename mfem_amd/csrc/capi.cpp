@@ -2,6 +2,7 @@
 //
 // Every entry point catches h1pa::Error (and std::exception) and turns it into
 // a status code plus a thread-local message; nothing throws across the ABI.
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -86,6 +87,42 @@ int h1pa_mesh_load(const char *text, int64_t len, h1pa_mesh **mesh) {
     API_END
 }
 
+int h1pa_mesh_create(int64_t nv, const double *xyz, int64_t ne, const int *ev, int64_t nbdr,
+                     const int *bdr, h1pa_mesh **mesh) {
+    API_BEGIN
+    REQUIRE(xyz && ev && mesh && (nbdr == 0 || bdr), "mesh_create: null argument");
+    REQUIRE(nv >= 8 && nv < (1LL << 31), "mesh_create: vertex count must be in [8, 2^31)");
+    REQUIRE(ne >= 1 && ne < (1LL << 28), "mesh_create: element count must be in [1, 2^28)");
+    REQUIRE(nbdr >= 0 && nbdr <= 6 * ne, "mesh_create: boundary count must be in [0, 6 ne]");
+    for (int64_t i = 0; i < 8 * ne; ++i)
+        REQUIRE(ev[i] >= 0 && ev[i] < nv, "mesh_create: element vertex index out of range");
+    for (int64_t i = 0; i < 4 * nbdr; ++i)
+        REQUIRE(bdr[i] >= 0 && bdr[i] < nv, "mesh_create: boundary vertex index out of range");
+    for (int64_t i = 0; i < 3 * nv; ++i)
+        REQUIRE(std::isfinite(xyz[i]), "mesh_create: vertex coordinates must be finite");
+    auto m = std::make_unique<h1pa_mesh>();
+    m->m.from_arrays(nv, xyz, ne, ev, nbdr, bdr);
+    *mesh = m.release();
+    API_END
+}
+
+int h1pa_mesh_set_nodes(h1pa_mesh *mesh, int order, const double *d_nodes, void *stream) {
+    API_BEGIN
+    REQUIRE(mesh && d_nodes, "mesh_set_nodes: null argument");
+    REQUIRE(order >= 1 && order <= 8, "mesh_set_nodes: order must be in [1, 8]");
+    Mesh &m = mesh->m;
+    hipStream_t s = as_stream(stream);
+    const int64_t Dg = order + 1, n = Dg * Dg * Dg * 3 * m.ne;
+    m.ensure_device(s);
+    if (m.d_nodes.n != (size_t)n) m.d_nodes.alloc(n);
+    H1PA_HIP(hipMemcpyAsync(m.d_nodes.p, d_nodes, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    m.node_order = order;
+    m.curve_amp = 0.0;
+    m.nodes_valid = true;
+    m.nodes_external = true;
+    API_END
+}
+
 int h1pa_mesh_destroy(h1pa_mesh *mesh) {
     API_BEGIN
     delete mesh;
@@ -107,6 +144,7 @@ int h1pa_mesh_set_curvature(h1pa_mesh *mesh, int order, double amp) {
     mesh->m.node_order = order;
     mesh->m.curve_amp = amp;
     mesh->m.nodes_valid = false;
+    mesh->m.nodes_external = false;
     API_END
 }
 
@@ -397,6 +435,13 @@ int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *
     kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 3,
                                   r->R().perm_lex.p, r->R().nperm_lex,
                                   as_stream(stream));
+    API_END
+}
+
+int h1pa_restriction_boolean_mask(const h1pa_restriction *r, double *ye, void *stream) {
+    API_BEGIN
+    REQUIRE(r && ye, "restriction_boolean_mask: null argument");
+    kern::boolean_mask(r->R().ndofs, r->R().offsets.p, r->R().indices.p, ye, as_stream(stream));
     API_END
 }
 

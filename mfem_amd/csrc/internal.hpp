@@ -96,6 +96,7 @@ struct Mesh {
     double curve_amp = 0.0;
     DevBuf<double> d_nodes;
     bool nodes_valid = false;
+    bool nodes_external = false;             // d_nodes given by the caller (set_nodes)
     std::vector<uint8_t> bface;              // [ne] bit j: local face j is on the boundary
     bool from_file = false;                  // read from an MFEM mesh file (no grid)
     DevBuf<int> d_elem_at;  // Cartesian meshes: element at each grid position (built on use)
@@ -105,6 +106,9 @@ struct Mesh {
     // ReadMFEMMesh for "MFEM mesh v1.0" hexahedral text; sets ev, xyz and the
     // topology (first-encounter edges/faces, boundary from the file's list).
     void load_mfem_v10(const char *text, size_t len);
+    // Mesh from vertex, element and boundary-quad arrays (+ FinalizeTopology).
+    void from_arrays(int64_t n_v, const double *pts, int64_t n_el, const int *evi,
+                     int64_t n_bdr, const int *bq);
     void ensure_device(hipStream_t s);
     void ensure_nodes(hipStream_t s);
 };
@@ -212,6 +216,9 @@ void first_contribution_order(int64_t nE, const int *gmap, const int *offsets,
 void restrict_mult_transpose(int64_t nrows, const int *offsets, const int *indices,
                              const double *xe, double *y, int mode, const int *perm,
                              int64_t nperm, hipStream_t s);
+// BooleanMask: ye[first E-entry of each row] = 1, every other E-entry 0.
+void boolean_mask(int64_t nrows, const int *offsets, const int *indices, double *ye,
+                  hipStream_t s);
 // E->L set with constraint handling: y[i] = mask[i] ? xin[i] : sum (DIAG_ONE)
 void scatter_constrained(int64_t nrows, const int *offsets, const int *indices,
                          const double *xe, const uint8_t *mask, const double *xin, double *y,

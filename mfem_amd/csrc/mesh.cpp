@@ -331,18 +331,36 @@ void Mesh::load_mfem_v10(const char *text, size_t len) {
     H1PA_CHECK(sd != "nodes", H1PA_EUNSUP,
                "mesh file: curved meshes (a 'nodes' section) are not supported");
     H1PA_CHECK(sd == "3", H1PA_EUNSUP, "mesh file: vertices must have 3 coordinates");
-    xyz.resize(3 * (size_t)n_v);
-    for (auto &c : xyz) c = tk.real();
+    std::vector<double> pts(3 * (size_t)n_v);
+    for (auto &c : pts) c = tk.real();
     for (long long v : ev64)
         H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
     for (long long v : bq64)
         H1PA_CHECK(v >= 0 && v < n_v, H1PA_EINVAL, "mesh file: vertex index out of range");
-    ev.assign(ev64.begin(), ev64.end());
-    std::vector<int> bq(bq64.begin(), bq64.end());
+    const std::vector<int> evi(ev64.begin(), ev64.end()), bq(bq64.begin(), bq64.end());
+    from_arrays(n_v, pts.data(), n_el, evi.data(), n_bdr, bq.data());
+}
+
+// Mesh(vertices, nv, element_indices, HEXAHEDRON, ..., boundary_indices,
+// SQUARE, ...) + FinalizeTopology (mesh/mesh.cpp:4891-4930): elements keep
+// the given order; faces and edges are numbered by first encounter
+// (GetElementToFaceTable mesh/mesh.cpp:8996, GetElementToEdgeTable :8551);
+// the boundary is the given quadrilateral list.  Indices are range checked
+// by the callers.
+void Mesh::from_arrays(int64_t n_v, const double *pts, int64_t n_el, const int *evi,
+                       int64_t n_bdr, const int *bq) {
+    xyz.assign(pts, pts + 3 * n_v);
+    ev.assign(evi, evi + 8 * n_el);
     nx = ny = nz = 0;
     nv = n_v;
     ne = n_el;
     from_file = true;
+    node_order = 1;
+    curve_amp = 0.0;
+    nodes_external = false;
+    d_topo.release();
+    d_ori.release();
+    d_elem_at.release();
     ecoord.assign(3 * ne, 0);
     // an inverted element would flip the sign of every derivative: refuse it
     // (trilinear Jacobian at the element centre)
@@ -356,7 +374,7 @@ void Mesh::load_mfem_v10(const char *text, size_t len) {
         const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                            J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-        H1PA_CHECK(det > 0.0, H1PA_EUNSUP, "mesh file: inverted or degenerate element " +
+        H1PA_CHECK(det > 0.0, H1PA_EUNSUP, "mesh: inverted or degenerate element " +
                                                std::to_string(e));
     }
     // faces
@@ -378,7 +396,7 @@ void Mesh::load_mfem_v10(const char *text, size_t len) {
                 fcount.push_back(1);
                 efo[6 * e + j] = 0;
             } else {
-                H1PA_CHECK(++fcount[id] <= 2, H1PA_EINVAL, "mesh file: face shared by 3 elements");
+                H1PA_CHECK(++fcount[id] <= 2, H1PA_EINVAL, "mesh: face shared by 3 elements");
                 efo[6 * e + j] = quad_orientation(fverts[id].data(), vv.data());
             }
             ef[6 * e + j] = id;
@@ -398,12 +416,12 @@ void Mesh::load_mfem_v10(const char *text, size_t len) {
             eeo[12 * e + j] = va < vb ? 1 : -1;
         }
     nedges = (int64_t)emap.size();
-    // boundary faces: the file's list
+    // boundary faces: the given list
     std::vector<uint8_t> on(nfaces, 0);
     for (long long b = 0; b < n_bdr; ++b) {
         auto it = fmap.find(quad_key(bq[4 * b], bq[4 * b + 1], bq[4 * b + 2], bq[4 * b + 3]));
         H1PA_CHECK(it != fmap.end(), H1PA_EINVAL,
-                   "mesh file: boundary element " + std::to_string(b) + " is not an element face");
+                   "mesh: boundary element " + std::to_string(b) + " is not an element face");
         on[it->second] = 1;
     }
     bface.assign(ne, 0);
@@ -439,6 +457,7 @@ void Mesh::ensure_device(hipStream_t s) {
 
 void Mesh::ensure_nodes(hipStream_t s) {
     if (nodes_valid) return;
+    H1PA_CHECK(!nodes_external, H1PA_EINVAL, "mesh: the nodes given by set_nodes are gone");
     const int64_t Dg = node_order + 1;
     d_nodes.alloc(Dg * Dg * Dg * 3 * ne);
     ensure_device(s);
@@ -475,6 +494,7 @@ void perturb_vertices(Mesh &m, double amp, uint64_t seed) {
                 }
             }
     m.nodes_valid = false;
+    m.nodes_external = false;  // the nodes follow the moved vertices again
 }
 
 }  // namespace kern

@@ -289,6 +289,21 @@ __global__ void k_restrict_mult_t(int64_t nrows, const int *__restrict__ offsets
     else y[i] = v;
 }
 
+// ElementRestriction::BooleanMask (fem/restriction.cpp:255-290): the first
+// E-entry of each L-DOF's row (in indices order) is 1, the others 0.  A signed
+// entry -1-k stands for E-index k (MFEM's loop indexes with the raw value,
+// which its H1 tables never make negative).
+__global__ void k_boolean_mask(int64_t nrows, const int *__restrict__ offsets,
+                               const int *__restrict__ indices, double *__restrict__ ye) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= nrows) return;
+    const int b = offsets[i], e = offsets[i + 1];
+    for (int j = b; j < e; ++j) {
+        const int ix = indices[j];
+        ye[ix >= 0 ? ix : -1 - ix] = j == b ? 1.0 : 0.0;
+    }
+}
+
 __global__ void k_scatter_constrained(int64_t nrows, const int *__restrict__ offsets,
                                       const int *__restrict__ indices,
                                       const double *__restrict__ xe,
@@ -457,6 +472,14 @@ void restrict_mult_transpose(int64_t nrows, const int *offsets, const int *indic
     if (nrows <= 0) return;
     hipLaunchKernelGGL(k_restrict_mult_t, dim3(nblocks(nrows)), dim3(kBlock), 0, s, nrows, offsets,
                        indices, xe, y, mode, perm, perm ? nperm : 0);
+    H1PA_HIP(hipGetLastError());
+}
+
+void boolean_mask(int64_t nrows, const int *offsets, const int *indices, double *ye,
+                  hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(k_boolean_mask, dim3(nblocks(nrows)), dim3(kBlock), 0, s, nrows, offsets,
+                       indices, ye);
     H1PA_HIP(hipGetLastError());
 }
 

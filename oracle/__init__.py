@@ -281,6 +281,18 @@ def restriction_mult_transpose(offsets, indices, nd, xe, mode=0, y=None):
     return y
 
 
+def boolean_mask(offsets, indices, nE):
+    """ElementRestriction::BooleanMask (fem/restriction.cpp:255-290), vdim 1:
+    walking each L-DOF's row of `indices` in order, the first E-entry gets 1
+    and every later one 0."""
+    y = np.full(nE, -1.0)
+    for i in range(offsets.size - 1):
+        for j in range(offsets[i], offsets[i + 1]):
+            ix = int(indices[j])
+            y[ix if ix >= 0 else -1 - ix] = 1.0 if j == offsets[i] else 0.0
+    return y
+
+
 # ---------------------------------------------------------------------------
 # quadrature / basis / geometry / setup / kernels
 # ---------------------------------------------------------------------------
@@ -504,11 +516,14 @@ class Problem:
     """
 
     def __init__(self, n, p, Q=None, perturb=0.0, seed=0, curve=0.0, coeff="one",
-                 integrators=("diffusion",), sfc=True, dims=None, mesh_text=None):
+                 integrators=("diffusion",), sfc=True, dims=None, mesh_text=None,
+                 mesh_arrays=None, nodes=None):
         self.p, self.D = p, p + 1
         self.Q = Q if Q is not None else p + 2
-        if mesh_text is not None:  # a mesh file's text
-            ev, bq, xyz = read_mfem_mesh(mesh_text)
+        if mesh_text is not None or mesh_arrays is not None:
+            # a mesh file's text, or (ev [ne, 8], bquads [nb, 4], xyz [nv, 3])
+            ev, bq, xyz = read_mfem_mesh(mesh_text) if mesh_text is not None else mesh_arrays
+            xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
             self.mesh = CartesianMesh(0, 0, 0, elements=(ev, bq, xyz.shape[0]))
             m = self.mesh
             self.ne = m.ne
@@ -526,7 +541,11 @@ class Problem:
         self.ess = m.boundary_dofs(p, self.elem_dof)
         self.B, self.G = basis(p, self.Q)
         self.W = tensor_weights(self.Q)
-        if curve:
+        if nodes is not None:  # (order, E-vector [(order+1)^3][3][ne]) given
+            self.pg = int(nodes[0])
+            self.X = np.ascontiguousarray(nodes[1], dtype=np.float64).ravel()
+            assert self.X.size == (self.pg + 1) ** 3 * 3 * self.ne
+        elif curve:
             self.pg = p
             self.X = m.nodes_curved(self.xyz, p, curve)
         else:

@@ -93,3 +93,55 @@ def test_serve_driver_rejects_bad_options():
     for args in (["--mode", "queue"], ["--pmax", "9"], ["--bogus"]):
         r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=30)
         assert r.returncode == 2
+
+
+def _grid_arrays(orc, dims, seed):
+    """A lexicographic grid with shuffled elements and jittered interior
+    vertices, as arrays, with its cube-face boundary quadrilaterals."""
+    m = orc.CartesianMesh(*dims, sfc=False)
+    xyz = m.vertices(0.15, seed)
+    ev = m.ev[np.random.default_rng(seed).permutation(m.ne)].copy()
+    faces = [(0, 3, 2, 1), (0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (3, 0, 4, 7), (4, 5, 6, 7)]
+    bq = []
+    for e in range(m.ne):
+        for f in faces:
+            vv = [int(ev[e, k]) for k in f]
+            c = xyz[vv]
+            if any(np.all(np.abs(c[:, a]) < 1e-12) or np.all(np.abs(c[:, a] - 1) < 1e-12)
+                   for a in range(3)):
+                bq.append(vv)
+    return ev, np.array(bq, np.int32), xyz
+
+
+@pytest.mark.parametrize("dims,seed", [((1, 1, 1), 1), ((3, 4, 2), 11), ((5, 3, 4), 2)])
+def test_arrays_mesh_topology_matches_oracle(orc, h1pa, dims, seed):
+    """Mesh from arrays (mesh/mesh.cpp:4891-4930): first-encounter edge and
+    face numbering and orientations equal the oracle's on the same arrays;
+    the vertices are kept as given."""
+    ev, bq, xyz = _grid_arrays(orc, dims, seed)
+    om = orc.CartesianMesh(0, 0, 0, elements=(ev, bq, xyz.shape[0]))
+    pm = h1pa.Mesh.FromArrays(xyz, ev, bq)
+    assert (pm.GetNV(), pm.GetNE(), pm.GetNEdges(), pm.GetNFaces()) == \
+        (om.nv, om.ne, om.nedges, om.nfaces)
+    t = pm.Topology()
+    for k in ("ev", "ee", "eeo", "ef", "efo"):
+        assert np.array_equal(getattr(om, k), t[k]), k
+    assert np.array_equal(pm.GetVertices(), xyz)
+
+
+def test_arrays_mesh_argument_errors(h1pa):
+    ev = np.arange(8, dtype=np.int32).reshape(1, 8)
+    xyz = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                    [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], np.float64)
+    none = np.zeros((0, 4), np.int32)
+    cases = [(xyz, ev + 1, none, "out of range"),
+             (xyz, ev, np.array([[0, 1, 6, 7]], np.int32), "is not an element face"),
+             (xyz[[4, 5, 6, 7, 0, 1, 2, 3]].copy(), ev, none, "inverted"),
+             (np.where(xyz == 1, np.nan, xyz), ev, none, "finite")]
+    for v, e, b, msg in cases:
+        with pytest.raises(h1pa.H1paError) as err:
+            h1pa.Mesh.FromArrays(v, e, b)
+        assert msg in str(err.value)
+    m = h1pa.Mesh.FromArrays(xyz, ev, np.array([[0, 3, 2, 1]], np.int32))
+    assert (m.GetNE(), m.GetNFaces(), m.GetNEdges()) == (1, 6, 12)
+    assert h1pa.lib().h1pa_mesh_set_nodes(None, 2, None, None) == h1pa.H1PA_EINVAL
