@@ -254,9 +254,14 @@ int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space) {
                                        s.nvdofs, r.perm, st);
         r.nperm = s.nvdofs;
         r.bubble_begin = s.nvdofs + s.nedofs + s.nfdofs;
+        {
+            // every row, bubble rows included, in first-contribution order:
+            // the rows that read one element's lines run together
+            const char *env = std::getenv("H1PA_SCATTER_ROWS");
+            r.nperm_lex = (env && env[0] == 'n') ? r.bubble_begin : s.ndofs;
+        }
         kern::first_contribution_order(s.ne * s.dof, r.gather_map.p, r.offsets.p, r.indices.p,
-                                       r.bubble_begin, r.perm_lex, st);
-        r.nperm_lex = r.bubble_begin;
+                                       r.nperm_lex, r.perm_lex, st);
         {
             std::vector<int> dm(s.dof);
             h1_dof_map(order, dm.data());  // lex -> native
