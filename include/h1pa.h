@@ -255,6 +255,17 @@ int h1pa_form_info(const h1pa_form *form, int64_t *info);
  * kernels followed by the E->L scatter.  Sets the column apply up on first
  * use, on `stream`. */
 int h1pa_form_mult_path(h1pa_form *form, int *path, void *stream);
+/* No counterpart in MFEM.  For callers that cannot allocate device memory
+ * during the call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): y = A x as h1pa_form_mult, through the
+ * element kernels and the E->L scatter, with the E-vector in a static device
+ * array of the library (64 MiB per GPU, held until the process ends) instead
+ * of form-owned memory; no allocation of any kind.  Forms whose E-vector has
+ * more than 8388608 entries (ne (p+1)^3) give H1PA_EINVAL.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (other streams or
+ * threads) race and must be serialized by the caller.  The form must be
+ * assembled.  Same operator as h1pa_form_mult to rounding (1e-13 relative). */
+int h1pa_form_mult_noalloc(h1pa_form *form, const double *d_x, double *d_y, void *stream);
 /* Benchmark helper (no MFEM counterpart): `steps` back-to-back h1pa_form_mult
  * calls with HIP events recorded on `stream` around each launch.  Returns the
  * wall time of the whole region and the average duration of the element

@@ -123,6 +123,7 @@ SIGNATURES = {
     "h1pa_form_destroy": (_i, [_vp]),
     "h1pa_form_assemble": (_i, [_vp, _vp]),
     "h1pa_form_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_form_mult_noalloc": (_i, [_vp, _vp, _vp, _vp]),
     "h1pa_form_add_mult": (_i, [_vp, _vp, _vp, C.c_double, _vp]),
     "h1pa_form_add_mult_e": (_i, [_vp, _vp, _vp, _vp]),
     "h1pa_form_assemble_diagonal": (_i, [_vp, _vp, _vp]),

@@ -742,6 +742,61 @@ int h1pa_form_add_mult(h1pa_form *form, const double *x, double *y, double a, vo
     API_END
 }
 
+}  // extern "C"
+
+// ---- the apply that allocates nothing ---------------------------------------
+// Scratch E-vector of h1pa_form_mult_noalloc: a file-scope device array (one
+// copy per GPU, held until the process ends).  Host code never passes the
+// array's name to a kernel: its device address is looked up once per device
+// (hipGetSymbolAddress, with that device current) and kept in g_scratch.
+namespace {
+constexpr int64_t kScratchDoubles = int64_t(8) << 20;  // 64 MiB
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+double *scratch_on_current_device() {
+    int dev = 0;
+    H1PA_HIP(hipGetDevice(&dev));
+    H1PA_CHECK(dev >= 0 && dev < kMaxDevices, H1PA_EINVAL, "form_mult_noalloc: device id too large");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        H1PA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        H1PA_CHECK(ptr != nullptr, H1PA_EHIP, "form_mult_noalloc: no address for the scratch array");
+        g_scratch[dev] = static_cast<double *>(ptr);
+    }
+    return g_scratch[dev];
+}
+}  // namespace
+
+extern "C" {
+
+int h1pa_form_mult_noalloc(h1pa_form *form, const double *x, double *y, void *stream) {
+    API_BEGIN
+    REQUIRE(form && x && y, "form_mult_noalloc: null argument");
+    Form &f = form->f;
+    require_assembled(f);
+    Space &sp = *f.space;
+    REQUIRE(sp.ne * sp.dof <= kScratchDoubles,
+            "form_mult_noalloc: the E-vector (" + std::to_string(sp.ne * sp.dof) +
+                " entries) exceeds the scratch of " + std::to_string(kScratchDoubles) +
+                " entries");
+    hipStream_t s = as_stream(stream);
+    double *ye = scratch_on_current_device();
+    // the element path of h1pa_form_mult, with its E-vector in the scratch:
+    // every E-entry the scatter reads is written first (accumulate = false)
+    element_apply(f, sp.restr.gather_map.p, nullptr, x, ye, y, false, s);
+    kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
+                                  sp.restr.indices_nat.p, ye, y, 0, sp.restr.perm.p,
+                                  sp.restr.nperm, s);
+    API_END
+}
+
 int h1pa_form_mult_timed(h1pa_form *form, const double *x, double *y, int steps, double *ms_total,
                          double *ms_element, double *ms_scatter, void *stream) {
     API_BEGIN
