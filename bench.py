@@ -254,11 +254,12 @@ def cpu_baseline(p, reps=10, warmup=3):
                        f"warm-ups, on {all_threads} threads and on 1")
 
 
-def traffic_from_profiles(p, n, kernel):
-    """HBM bytes of one launch (for kernel "mult_columns": of one Mult, both
-    kernels) measured with rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE,
-    scaled by the copy-kernel calibration) when profiles/traffic.json holds
-    this workload and kernel; else None."""
+def traffic_from_profiles(p, n, kernel, variant=None):
+    """HBM bytes of one launch (kernel "mult_element+scatter": of one Mult,
+    both kernels) measured with rocprofv3 PMC counters (FETCH_SIZE,
+    WRITE_SIZE, scaled by the copy-kernel calibration) when
+    profiles/traffic.json holds this workload, kernel and element-kernel
+    variant; else None."""
     path = os.path.join(HERE, "profiles", "traffic.json")
     try:
         with open(path) as f:
@@ -266,7 +267,8 @@ def traffic_from_profiles(p, n, kernel):
     except (OSError, ValueError):
         return None
     for r in rows:
-        if r.get("p") == p and r.get("n") == n and r.get("kernel") == kernel:
+        if r.get("p") == p and r.get("n") == n and r.get("kernel") == kernel and \
+                (variant is None or r.get("variant_index") == variant):
             return r.get("bytes_per_launch")
     return None
 
@@ -274,6 +276,14 @@ def traffic_from_profiles(p, n, kernel):
 # ---------------------------------------------------------------------------
 # solve (configuration 3)
 # ---------------------------------------------------------------------------
+def mg_orders(p):
+    """p-multigrid levels: the order halved (rounding up) down to 1."""
+    orders = [p]
+    while orders[-1] > 1:
+        orders.append((orders[-1] + 1) // 2)
+    return tuple(reversed(orders))
+
+
 def run_solve(M, torch, p, n=48, seed=7):
     import numpy as np
     mesh = M.Mesh.MakeCartesian3D(n, n, n)
@@ -351,6 +361,36 @@ def run_solve(M, torch, p, n=48, seed=7):
                                   iterations=cg.GetNumIterations(), converged=cg.GetConverged(),
                                   max_eig=cheb.GetMaxEigEstimate(),
                                   true_rel_residual=true_res(Xc))
+    # FP64 CG preconditioned by p-multigrid (ex26's levels: orders halved down
+    # to 1, Chebyshev(2) smoothing, coarse CG to 1e-2); set-up timed apart
+    orders = mg_orders(p)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    mg = M.PMultigrid(a, orders)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    cg = M.CGSolver()
+    cg.SetRelTol(1e-10)
+    cg.SetMaxIter(10000)
+    cg.SetOperator(A)
+    cg.SetPreconditioner(mg)
+    Xm = X.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    cg.Mult(B, Xm)
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    info = mg.Info()
+    out["fp64_pmultigrid"] = dict(seconds=round(tm, 4), setup_seconds=round(t_setup, 4),
+                                  orders=list(orders), iterations=cg.GetNumIterations(),
+                                  converged=cg.GetConverged(),
+                                  coarse_iterations=info["coarse_iterations"],
+                                  true_rel_residual=true_res(Xm),
+                                  rel_diff_vs_fp64=(torch.linalg.norm(Xm - X64)
+                                                    / torch.linalg.norm(X64)).item(),
+                                  speedup_vs_fp64=round(t64 / tm, 3),
+                                  speedup_vs_fp64_with_setup=round(t64 / (tm + t_setup), 3))
+    del mg
     for prec, name in ((M.PREC_FP32, "fp32"), (M.PREC_BF16_PA, "bf16")):
         mixed(prec, 20)
         ir, Xi, ti = mixed(prec, 100000)
@@ -582,7 +622,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(mult_gbs / HBM_PEAK_GBS, 4),
-            "traffic": traffic_from_profiles(args.p, n, "mult_" + r["path"]),
+            "traffic": traffic_from_profiles(args.p, n, "mult_" + r["path"], r["variant"]),
             "kernel": ("k_diffusion_columns + k_columns_interface (one Operator::Mult)"
                        if r["path"] == "columns" else
                        "k_diffusion_apply + k_restrict_mult_t (one Operator::Mult)"),

@@ -251,8 +251,9 @@ int h1pa_form_info(const h1pa_form *form, int64_t *info);
 /* Which kernels h1pa_form_mult runs (no MFEM counterpart; MFEM's Mult is
  * gather, element apply, scatter, fem/bilinearform_ext.cpp:487-560):
  * *path = 1 the fused column apply (a diffusion form on a Cartesian mesh at
- * q1d = p + 2: element results summed in LDS, no E-vector), 0 the element
- * kernels followed by the E->L scatter.  Sets the column apply up on first
+ * q1d = p + 2, opt-in with H1PA_COLUMNS=1 in the environment: element results
+ * summed in LDS, no E-vector), 0 the element kernels followed by the E->L
+ * scatter (the default).  Sets the column apply up on first
  * use, on `stream`. */
 int h1pa_form_mult_path(h1pa_form *form, int *path, void *stream);
 /* No counterpart in MFEM.  For callers that cannot allocate device memory
@@ -345,6 +346,63 @@ int h1pa_chebyshev_mult(h1pa_chebyshev *cheb, const double *d_x, double *d_y, vo
  * operator (linalg/solvers.cpp:869-1050), iterative_mode = false; as
  * h1pa_pcg_solve otherwise. */
 int h1pa_pcg_solve_chebyshev(h1pa_chebyshev *cheb, const double *d_b, double *d_x,
+                             const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats, void *stream);
+
+/* ---- p-refinement transfer and p-multigrid ---------------------------------
+ * TensorProductPRefinementTransferOperator (fem/transfer.cpp:2619-2990)
+ * between two H1 spaces of one mesh (low order below high order). */
+typedef struct h1pa_transfer h1pa_transfer;
+int h1pa_transfer_create(h1pa_space *low, h1pa_space *high, h1pa_transfer **transfer);
+int h1pa_transfer_destroy(h1pa_transfer *transfer);
+/* Mult (transfer.cpp:2941-2965): d_yh = R_h^T (mask_h o (B x B x B) R_l d_xl),
+ * B the low-order basis at the high-order nodes, mask_h the BooleanMask of
+ * the high restriction: every high DOF gets the interpolated value of the
+ * first element holding it.  d_xl: low L-vector, d_yh: high L-vector (set). */
+int h1pa_transfer_mult(h1pa_transfer *transfer, const double *d_xl, double *d_yh, void *stream);
+/* MultTranspose (transfer.cpp:2967-2990): d_yl = R_l^T (Bt x Bt x Bt)(mask_h o R_h d_xh). */
+int h1pa_transfer_mult_transpose(h1pa_transfer *transfer, const double *d_xh, double *d_yl,
+                                 void *stream);
+
+/* Multigrid with order-refined levels of the fine form's mesh: the
+ * GeometricMultigrid of a FiniteElementSpaceHierarchy built with
+ * AddOrderRefinedLevel (fem/multigrid.hpp:135-200,
+ * fem/fespacehierarchy.cpp:86-99), levels set up as ex26's
+ * DiffusionMultigrid (examples/ex26.cpp:42-110): a PA form with the fine
+ * form's integrators and coefficient on every level (default rule
+ * q1d = p + 2), the DIAG_ONE constrained operator of each level's boundary
+ * DOFs, OperatorChebyshevSmoother on levels >= 1 and CGSolver without a
+ * preconditioner (abs_tol 0) on the coarsest. */
+typedef struct h1pa_multigrid h1pa_multigrid;
+typedef struct {
+    int smoother_order;      /* OperatorChebyshevSmoother order, 1..5 (ex26: 2)      */
+    int pre_smoothing;       /* SetCycleType(type, pre, post) (ex26: 1, 1)           */
+    int post_smoothing;
+    int w_cycle;             /* 0: CycleType::VCYCLE, 1: CycleType::WCYCLE           */
+    double coarse_rel_tol;   /* coarse CGSolver::SetRelTol (ex26: sqrt(1e-4))        */
+    int coarse_max_iter;     /* coarse CGSolver::SetMaxIter (ex26: 200)              */
+    int power_iterations;    /* smoothers' eigenvalue estimate (MFEM: 10, 1e-8, 12345) */
+    double power_tolerance;
+    int power_seed;
+} h1pa_mg_opts;
+/* orders[0 .. nlevels-1]: increasing, orders[nlevels-1] = the fine space's
+ * order (2 <= nlevels <= 8).  The fine form must be assembled and stay alive;
+ * a QuadratureFunction coefficient (tied to the fine rule) gives
+ * H1PA_EUNSUP.  The coarser spaces, forms, smoothers and transfers are made
+ * and assembled here, on `stream`. */
+int h1pa_multigrid_create(h1pa_form *fine, int nlevels, const int *orders,
+                          const h1pa_mg_opts *opts, h1pa_multigrid **mg, void *stream);
+int h1pa_multigrid_destroy(h1pa_multigrid *mg);
+/* MultigridBase::Mult (fem/multigrid.cpp:106-145): d_y = one cycle applied to
+ * d_x (fine L-vectors), starting from y = 0. */
+int h1pa_multigrid_mult(h1pa_multigrid *mg, const double *d_x, double *d_y, void *stream);
+/* info[0..3] = levels, cycles run, coarse solves, coarse CG iterations (sum);
+ * max_eig (may be NULL): [nlevels] the smoothers' eigenvalue estimates (0 on
+ * the coarsest level). */
+int h1pa_multigrid_info(const h1pa_multigrid *mg, int64_t *info, double *max_eig);
+/* CGSolver with SetPreconditioner(multigrid) (linalg/solvers.cpp:869-1050)
+ * on the fine form's DIAG_ONE constrained operator of its boundary DOFs,
+ * iterative_mode = false; as h1pa_pcg_solve otherwise. */
+int h1pa_pcg_solve_multigrid(h1pa_multigrid *mg, const double *d_b, double *d_x,
                              const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats, void *stream);
 
 /* ---- mixed precision ------------------------------------------------------

@@ -63,6 +63,12 @@ class PcgStats(C.Structure):
                 ("final_norm", _d), ("seconds", _d)]
 
 
+class MgOpts(C.Structure):
+    _fields_ = [("smoother_order", _i), ("pre_smoothing", _i), ("post_smoothing", _i),
+                ("w_cycle", _i), ("coarse_rel_tol", _d), ("coarse_max_iter", _i),
+                ("power_iterations", _i), ("power_tolerance", _d), ("power_seed", _i)]
+
+
 class IrOpts(C.Structure):
     _fields_ = [("inner_precision", _i), ("rel_tol", _d), ("delta", _d),
                 ("max_outer", _i), ("max_inner", _i)]
@@ -143,6 +149,17 @@ SIGNATURES = {
                                    _vp]),
     "h1pa_chebyshev_destroy": (_i, [_vp]),
     "h1pa_chebyshev_info": (_i, [_vp, C.POINTER(_d), C.POINTER(_d)]),
+    "h1pa_transfer_create": (_i, [_vp, _vp, C.POINTER(_vp)]),
+    "h1pa_transfer_destroy": (_i, [_vp]),
+    "h1pa_transfer_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_transfer_mult_transpose": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_multigrid_create": (_i, [_vp, _i, C.POINTER(_i), C.POINTER(MgOpts), C.POINTER(_vp),
+                                   _vp]),
+    "h1pa_multigrid_destroy": (_i, [_vp]),
+    "h1pa_multigrid_mult": (_i, [_vp, _vp, _vp, _vp]),
+    "h1pa_multigrid_info": (_i, [_vp, _pi64, C.POINTER(_d)]),
+    "h1pa_pcg_solve_multigrid": (_i, [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
+                                      _vp]),
     "h1pa_chebyshev_mult": (_i, [_vp, _vp, _vp, _vp]),
     "h1pa_pcg_solve_chebyshev": (_i, [_vp, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                                       _vp]),
@@ -949,6 +966,82 @@ class OperatorChebyshevSmoother:
         _check(lib().h1pa_chebyshev_mult(self._h, _ptr(x), _ptr(y), _stream()))
 
 
+class TransferOperator:
+    """TransferOperator(lFESpace, hFESpace) between two H1 spaces of one mesh:
+    MFEM picks TensorProductPRefinementTransferOperator for them
+    (fem/transfer.cpp:2619-2990).  Mult: low -> high (prolongation),
+    MultTranspose: high -> low (restriction)."""
+
+    def __init__(self, lfes, hfes):
+        h = _vp()
+        _check(lib().h1pa_transfer_create(lfes._h, hfes._h, C.byref(h)))
+        self._h, self.lfes, self.hfes = h, lfes, hfes
+
+    def __del__(self, _d=_destroy):
+        _d("h1pa_transfer_destroy", getattr(self, "_h", None))
+        self._h = None
+
+    def Height(self):
+        return self.hfes.ndofs
+
+    def Width(self):
+        return self.lfes.ndofs
+
+    def Mult(self, x, y):
+        _check_vec(x, self.lfes.ndofs); _check_vec(y, self.hfes.ndofs)
+        _check(lib().h1pa_transfer_mult(self._h, _ptr(x), _ptr(y), _stream()))
+
+    def MultTranspose(self, x, y):
+        _check_vec(x, self.hfes.ndofs); _check_vec(y, self.lfes.ndofs)
+        _check(lib().h1pa_transfer_mult_transpose(self._h, _ptr(x), _ptr(y), _stream()))
+
+
+class PMultigrid:
+    """Multigrid over order-refined levels of the form's mesh: MFEM's
+    GeometricMultigrid on a FiniteElementSpaceHierarchy built with
+    AddOrderRefinedLevel, with the levels of ex26's DiffusionMultigrid
+    (examples/ex26.cpp:42-110): PA forms with the fine form's integrators and
+    coefficient, OperatorChebyshevSmoother(order) on levels >= 1, CGSolver
+    without a preconditioner (rel_tol coarse_rel_tol, max coarse_max_iter) on
+    the coarsest, every level constrained on its boundary DOFs.
+    orders: increasing, ending with the fine order (ex26: 1, 2, 4, ...).
+    Mult(x, y): one cycle (MultigridBase::Mult, fem/multigrid.cpp:106-145)."""
+
+    def __init__(self, a, orders, smoother_order=2, pre_smoothing=1, post_smoothing=1,
+                 cycle="V", coarse_rel_tol=1e-2, coarse_max_iter=200, power_iterations=10,
+                 power_tolerance=1e-8, power_seed=12345):
+        f = a._single()
+        orders = [int(o) for o in orders]
+        arr = (_i * len(orders))(*orders)
+        o = MgOpts(int(smoother_order), int(pre_smoothing), int(post_smoothing),
+                   1 if cycle.upper().startswith("W") else 0, float(coarse_rel_tol),
+                   int(coarse_max_iter), int(power_iterations), float(power_tolerance),
+                   int(power_seed))
+        h = _vp()
+        _check(lib().h1pa_multigrid_create(f._h, len(orders), arr, C.byref(o), C.byref(h),
+                                           _stream()))
+        self._h, self.form, self.orders = h, a, orders
+
+    def __del__(self, _d=_destroy):
+        _d("h1pa_multigrid_destroy", getattr(self, "_h", None))
+        self._h = None
+
+    def NumLevels(self):
+        return len(self.orders)
+
+    def Mult(self, x, y):
+        n = self.form.fes.ndofs
+        _check_vec(x, n); _check_vec(y, n)
+        _check(lib().h1pa_multigrid_mult(self._h, _ptr(x), _ptr(y), _stream()))
+
+    def Info(self):
+        info = (C.c_int64 * 4)()
+        eig = (_d * len(self.orders))()
+        _check(lib().h1pa_multigrid_info(self._h, info, eig))
+        return dict(levels=info[0], cycles=info[1], coarse_solves=info[2],
+                    coarse_iterations=info[3], max_eig=list(eig))
+
+
 class CGSolver:
     """CGSolver (linalg/solvers.hpp:626, solvers.cpp:869-1050) with the PA
     operator and OperatorJacobiSmoother or OperatorChebyshevSmoother, run
@@ -979,9 +1072,9 @@ class CGSolver:
         self.op = op
 
     def SetPreconditioner(self, prec):
-        if not isinstance(prec, (OperatorJacobiSmoother, OperatorChebyshevSmoother)):
-            raise H1paError(H1PA_EUNSUP,
-                            "OperatorJacobiSmoother and OperatorChebyshevSmoother are provided")
+        if not isinstance(prec, (OperatorJacobiSmoother, OperatorChebyshevSmoother, PMultigrid)):
+            raise H1paError(H1PA_EUNSUP, "OperatorJacobiSmoother, OperatorChebyshevSmoother "
+                                         "and PMultigrid are provided")
         self.prec = prec
 
     def Mult(self, b, x):
@@ -990,7 +1083,16 @@ class CGSolver:
         f = self.op.form._single()
         o = PcgOpts(self.rel_tol, self.abs_tol, self.max_iter, self.check_every)
         st = self.stats
-        if isinstance(self.prec, OperatorChebyshevSmoother):
+        if isinstance(self.prec, PMultigrid):
+            if self.prec.form is not self.op.form:
+                raise H1paError(H1PA_EUNSUP, "the multigrid must be built on this operator's form")
+            ess = self.op.form.fes.GetBoundaryTrueDofs()
+            if self.op.ess.numel() != ess.numel() or not bool((self.op.ess == ess).all()):
+                raise H1paError(H1PA_EUNSUP, "the multigrid constrains the boundary DOFs: "
+                                             "the operator's list must be GetBoundaryTrueDofs()")
+            rc = lib().h1pa_pcg_solve_multigrid(self.prec._h, _ptr(b), _ptr(x), C.byref(o),
+                                                C.byref(st), _stream())
+        elif isinstance(self.prec, OperatorChebyshevSmoother):
             if self.prec.A.form is not self.op.form:
                 raise H1paError(H1PA_EUNSUP, "the smoother must be built on this operator")
             rc = lib().h1pa_pcg_solve_chebyshev(self.prec._h, _ptr(b), _ptr(x), C.byref(o),

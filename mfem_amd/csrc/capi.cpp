@@ -65,6 +65,7 @@ int h1pa_mesh_create_cartesian(int nx, int ny, int nz, int sfc, h1pa_mesh **mesh
     REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, "mesh: nx, ny, nz must be >= 1");
     REQUIRE((int64_t)(nx + 1) * (ny + 1) * (nz + 1) < (1LL << 31), "mesh: too many vertices");
     auto *m = new h1pa_mesh();
+    m->m.handle = m;
     m->m.nx = nx;
     m->m.ny = ny;
     m->m.nz = nz;
@@ -82,6 +83,7 @@ int h1pa_mesh_load(const char *text, int64_t len, h1pa_mesh **mesh) {
     API_BEGIN
     REQUIRE(text && mesh && len > 0, "mesh_load: null argument");
     auto m = std::make_unique<h1pa_mesh>();
+    m->m.handle = m.get();
     m->m.load_mfem_v10(text, (size_t)len);
     *mesh = m.release();
     API_END
@@ -101,6 +103,7 @@ int h1pa_mesh_create(int64_t nv, const double *xyz, int64_t ne, const int *ev, i
     for (int64_t i = 0; i < 3 * nv; ++i)
         REQUIRE(std::isfinite(xyz[i]), "mesh_create: vertex coordinates must be finite");
     auto m = std::make_unique<h1pa_mesh>();
+    m->m.handle = m.get();
     m->m.from_arrays(nv, xyz, ne, ev, nbdr, bdr);
     *mesh = m.release();
     API_END
@@ -254,12 +257,9 @@ int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space) {
                                        s.nvdofs, r.perm, st);
         r.nperm = s.nvdofs;
         r.bubble_begin = s.nvdofs + s.nedofs + s.nfdofs;
-        {
-            // every row, bubble rows included, in first-contribution order:
-            // the rows that read one element's lines run together
-            const char *env = std::getenv("H1PA_SCATTER_ROWS");
-            r.nperm_lex = (env && env[0] == 'n') ? r.bubble_begin : s.ndofs;
-        }
+        // non-bubble rows in first-contribution order (bubble rows too were
+        // measured no faster: 0.375 vs 0.365 ms at p = 4, 32M DOFs)
+        r.nperm_lex = r.bubble_begin;
         kern::first_contribution_order(s.ne * s.dof, r.gather_map.p, r.offsets.p, r.indices.p,
                                        r.nperm_lex, r.perm_lex, st);
         {
@@ -607,16 +607,18 @@ void element_apply(Form &f, const int *gmap, const uint8_t *mask, const double *
 }
 
 // Whether Mult takes the fused column apply (columns.hip): a Cartesian mesh
-// (grid positions known), one diffusion integrator at q1d = p + 2.
-// H1PA_COLUMNS=0 keeps the element kernel + scatter path.  The grid and the
-// interface buffer are set up on the first call.
+// (grid positions known), one diffusion integrator at q1d = p + 2, and
+// H1PA_COLUMNS=1 in the environment.  Off by default: measured on the box it
+// is slower than the element kernel + scatter at every order but p = 2 (32M
+// DOFs, p = 4: 2.73 vs 1.40 ms per Mult).  The grid and the interface buffer
+// are set up on the first call.
 bool columns_ready(Form &f, hipStream_t s) {
     if (f.use_columns == 0) return false;
     Space &sp = *f.space;
     Mesh &m = *sp.mesh;
     if (f.use_columns < 0) {
         const char *env = std::getenv("H1PA_COLUMNS");
-        const bool ok = !(env && env[0] == '0') && !m.from_file && m.nx > 0 &&
+        const bool ok = env && env[0] == '1' && !m.from_file && m.nx > 0 &&
                         f.desc.integrators == H1PA_INTEG_DIFFUSION &&
                         kern::columns_supported(sp.p, f.q1d) && sp.ne == (int64_t)m.nx * m.ny * m.nz;
         if (!ok) {
@@ -862,9 +864,7 @@ int h1pa_form_assemble_diagonal(h1pa_form *form, double *diag, void *stream) {
     REQUIRE(form && diag, "form_assemble_diagonal: null argument");
     Form &f = form->f;
     require_assembled(f);
-    Space &sp = *f.space;
-    hipStream_t s = as_stream(stream);
-    operator_diagonal(f, diag, s);
+    operator_diagonal(f, diag, as_stream(stream));
     API_END
 }
 
@@ -1175,6 +1175,84 @@ int h1pa_form_device_bytes(const h1pa_form *form, int64_t *bytes) {
     API_BEGIN
     REQUIRE(form && bytes, "form_device_bytes: null argument");
     *bytes = form_device_bytes(form->f);
+    API_END
+}
+
+
+// ---- p-refinement transfer and p-multigrid ------------------------------------
+int h1pa_transfer_create(h1pa_space *low, h1pa_space *high, h1pa_transfer **transfer) {
+    API_BEGIN
+    REQUIRE(low && high && transfer, "transfer_create: null argument");
+    auto t = std::make_unique<h1pa_transfer>();
+    t->t.reset(new Transfer(low->s, high->s, nullptr));
+    *transfer = t.release();
+    API_END
+}
+
+int h1pa_transfer_destroy(h1pa_transfer *transfer) {
+    API_BEGIN
+    delete transfer;
+    API_END
+}
+
+int h1pa_transfer_mult(h1pa_transfer *transfer, const double *xl, double *yh, void *stream) {
+    API_BEGIN
+    REQUIRE(transfer && xl && yh, "transfer_mult: null argument");
+    transfer->t->mult(xl, yh, as_stream(stream));
+    API_END
+}
+
+int h1pa_transfer_mult_transpose(h1pa_transfer *transfer, const double *xh, double *yl,
+                                 void *stream) {
+    API_BEGIN
+    REQUIRE(transfer && xh && yl, "transfer_mult_transpose: null argument");
+    transfer->t->mult_transpose(xh, yl, as_stream(stream));
+    API_END
+}
+
+int h1pa_multigrid_create(h1pa_form *fine, int nlevels, const int *orders,
+                          const h1pa_mg_opts *opts, h1pa_multigrid **mg, void *stream) {
+    API_BEGIN
+    REQUIRE(fine && orders && opts && mg, "multigrid_create: null argument");
+    auto m = std::make_unique<h1pa_multigrid>();
+    m->mg.reset(multigrid_create(fine->f, nlevels, orders, *opts, as_stream(stream)));
+    *mg = m.release();
+    API_END
+}
+
+int h1pa_multigrid_destroy(h1pa_multigrid *mg) {
+    API_BEGIN
+    delete mg;
+    API_END
+}
+
+int h1pa_multigrid_mult(h1pa_multigrid *mg, const double *x, double *y, void *stream) {
+    API_BEGIN
+    REQUIRE(mg && x && y, "multigrid_mult: null argument");
+    multigrid_mult(*mg->mg, x, y, as_stream(stream));
+    API_END
+}
+
+int h1pa_multigrid_info(const h1pa_multigrid *mg, int64_t *info, double *max_eig) {
+    API_BEGIN
+    REQUIRE(mg && info, "multigrid_info: null argument");
+    multigrid_info(*mg->mg, info);
+    if (max_eig)
+        for (int l = 0; l < (int)info[0]; ++l) max_eig[l] = multigrid_max_eig(*mg->mg, l);
+    API_END
+}
+
+int h1pa_pcg_solve_multigrid(h1pa_multigrid *mg, const double *b, double *x,
+                             const h1pa_pcg_opts *opts, h1pa_pcg_stats *stats, void *stream) {
+    API_BEGIN
+    REQUIRE(mg && b && x && opts && stats, "pcg_solve_multigrid: null argument");
+    REQUIRE(opts->max_iter >= 1, "pcg_solve_multigrid: max_iter must be >= 1");
+    *stats = h1pa_pcg_stats{};
+    pcg_multigrid(*mg->mg, b, x, *opts, *stats, as_stream(stream));
+    if (!stats->converged) {
+        set_error("PCG: No convergence!");
+        return H1PA_ENOTCONV;
+    }
     API_END
 }
 

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <functional>
+#include <memory>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -100,6 +102,7 @@ struct Mesh {
     std::vector<uint8_t> bface;              // [ne] bit j: local face j is on the boundary
     bool from_file = false;                  // read from an MFEM mesh file (no grid)
     DevBuf<int> d_elem_at;  // Cartesian meshes: element at each grid position (built on use)
+    h1pa_mesh *handle = nullptr;  // the C handle holding this mesh
     DevBuf<int> d_topo;   // [ne][29] vertex, edge, face ids, boundary-face bits, 2 unused
     DevBuf<int8_t> d_ori; // [ne][12+6] edge (+-1) and face (0..7) orientations
     void build_topology(bool sfc);
@@ -216,6 +219,18 @@ void first_contribution_order(int64_t nE, const int *gmap, const int *offsets,
 void restrict_mult_transpose(int64_t nrows, const int *offsets, const int *indices,
                              const double *xe, double *y, int mode, const int *perm,
                              int64_t nperm, hipStream_t s);
+// p-refinement transfer (transfer.hip): emask[E] = 1 at the first E-entry
+// (indices order) of each L-DOF; prolongation low L -> high L (every high
+// L-DOF written); restriction high L -> low E-vector (lexicographic), to be
+// summed by the low restriction's E->L pass.  B[qh + (ph+1) dl]: the order-pl
+// basis at the order-ph GLL nodes.
+void first_entry_mask(int64_t nE, const int *gmap, const int *offsets, const int *indices,
+                      uint8_t *emask, hipStream_t s);
+void transfer_prolong(int64_t ne, int pl, int ph, const double *B, const int *gmap_l,
+                      const int *gmap_h, const uint8_t *emask_h, const double *xl, double *yh,
+                      hipStream_t s);
+void transfer_restrict_e(int64_t ne, int pl, int ph, const double *B, const int *gmap_h,
+                         const uint8_t *emask_h, const double *xh, double *yle, hipStream_t s);
 // BooleanMask: ye[first E-entry of each row] = 1, every other E-entry 0.
 void boolean_mask(int64_t nrows, const int *offsets, const int *indices, double *ye,
                   hipStream_t s);
@@ -328,6 +343,27 @@ void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts
 
 void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
                const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s);
+// Work vectors of a CG solve kept by the caller between solves (multigrid's
+// coarse solves allocate nothing once they have the size).
+struct PcgWork {
+    DevBuf<double> r, z, d, partials;
+    DevBuf<uint8_t> state;
+};
+// Jacobi-PCG with the inverse diagonal given and the caller's work vectors.
+void pcg_solve_dinv(Form &f, const uint8_t *mask, const double *dinv, const double *b, double *x,
+                    const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, PcgWork &w, hipStream_t s);
+// u = A v on f's DIAG_ONE constrained operator (mask: the constrained DOFs;
+// stop: a device flag that skips the work when set, or null).
+void constrained_apply(Form &f, const uint8_t *mask, const double *v, double *u,
+                       const int *stop, hipStream_t s);
+// A preconditioner z = M r on the device (stop as above).
+using Precond = std::function<void(const double *r, double *z, const int *stop, hipStream_t s)>;
+// CGSolver::Mult (linalg/solvers.cpp:869-1050, iterative_mode = false) with
+// SetPreconditioner(M) on f's DIAG_ONE constrained operator; the host reads
+// the solver state every `check` iterations.
+void pcg_preconditioned(Form &f, const uint8_t *mask, const Precond &M, const double *b,
+                        double *x, const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, int check,
+                        hipStream_t s);
 // Builds f's reduced-precision shadows for `precision` (H1PA_PREC_FP32 or
 // H1PA_PREC_BF16_PA) unless they already match.
 void ensure_low_precision(Form &f, int precision, hipStream_t s);
@@ -337,6 +373,43 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
 void form_mult_lp(Form &f, int precision, const float *x, float *y, hipStream_t s);
 // Jacobi diagonal of the unconstrained operator (L-vector, set).
 void operator_diagonal(Form &f, double *diag, hipStream_t s);
+
+// ---- p-multigrid (transfer.hip, multigrid.cpp) -----------------------------
+// TensorProductPRefinementTransferOperator between two spaces of one mesh.
+struct Transfer {
+    Space *lo = nullptr, *hi = nullptr;
+    DevBuf<double> B;         // low-order basis at the high-order GLL nodes
+    DevBuf<uint8_t> emask_h;  // BooleanMask of the high restriction, one byte per E-entry
+    DevBuf<double> eL;        // low E-vector of the restriction
+    Transfer(Space &lo, Space &hi, hipStream_t s);
+    void mult(const double *xl, double *yh, hipStream_t s);             // prolongation
+    void mult_transpose(const double *xh, double *yl, hipStream_t s);   // restriction
+};
+struct MgLevel {
+    h1pa_space *space_own = nullptr;  // levels below the finest: made here
+    h1pa_form *form_own = nullptr;
+    Form *f = nullptr;
+    Space *sp = nullptr;
+    const uint8_t *mask = nullptr;        // the level's boundary DOFs
+    std::unique_ptr<Chebyshev> smoother;  // levels >= 1
+    DevBuf<double> X, Y, R, Z;
+};
+struct Multigrid {
+    h1pa_mg_opts o{};
+    std::vector<MgLevel> lv;                   // 0 = coarsest
+    std::vector<std::unique_ptr<Transfer>> P;  // P[l]: level l -> l + 1
+    DevBuf<double> coarse_dinv;                // ones: CG without a preconditioner
+    PcgWork coarse_work;
+    int64_t cycles = 0, coarse_solves = 0, coarse_iterations = 0;
+    ~Multigrid();
+};
+Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1pa_mg_opts &o,
+                            hipStream_t s);
+void multigrid_mult(Multigrid &mg, const double *x, double *y, hipStream_t s);
+void pcg_multigrid(Multigrid &mg, const double *b, double *x, const h1pa_pcg_opts &o,
+                   h1pa_pcg_stats &stats, hipStream_t s);
+void multigrid_info(const Multigrid &mg, int64_t *info);
+double multigrid_max_eig(const Multigrid &mg, int level);
 
 // ---- batched solves (batch.hip) --------------------------------------------
 h1pa_batch *batch_create(const h1pa_batch_opts &o);
@@ -370,3 +443,5 @@ struct h1pa_restriction {
 };
 struct h1pa_form { h1pa::Form f; };
 struct h1pa_chebyshev { h1pa::Chebyshev c; };
+struct h1pa_transfer { std::unique_ptr<h1pa::Transfer> t; };
+struct h1pa_multigrid { std::unique_ptr<h1pa::Multigrid> mg; };

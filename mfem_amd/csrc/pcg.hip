@@ -949,51 +949,84 @@ void pcg_element_apply(Form &f, int prec, const uint8_t *mask, const T *d, T *ye
 // Jacobi-PCG on the constrained operator with vectors of type T; dinv in T.
 template <typename T>
 void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, T *x, T *ye,
-             const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s) {
+             const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s,
+             PcgWork *ws = nullptr) {
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
-    DevBuf<T> r(n), z(n), d(n);
     const kern::Chunks ch = kern::chunks(n);
     const unsigned nb = ch.blocks;
     const int per = ch.per;
-    DevBuf<double> partials(nb);
-    DevBuf<PcgState> st(1);
+    // work vectors: the caller's (FP64, kept between solves) or this solve's own
+    DevBuf<T> r_own, z_own, d_own;
+    DevBuf<double> part_own;
+    DevBuf<PcgState> st_own;
+    T *r = nullptr, *z = nullptr, *d = nullptr;
+    double *partials = nullptr;
+    PcgState *st = nullptr;
+    if constexpr (sizeof(T) == 8) {
+        if (ws) {
+            if (ws->r.n < (size_t)n) {
+                ws->r.alloc(n);
+                ws->z.alloc(n);
+                ws->d.alloc(n);
+            }
+            if (ws->partials.n < nb) ws->partials.alloc(nb);
+            if (ws->state.n < sizeof(PcgState)) ws->state.alloc(sizeof(PcgState));
+            r = ws->r.p;
+            z = ws->z.p;
+            d = ws->d.p;
+            partials = ws->partials.p;
+            st = reinterpret_cast<PcgState *>(ws->state.p);
+        }
+    }
+    if (!r) {
+        r_own.alloc(n);
+        z_own.alloc(n);
+        d_own.alloc(n);
+        part_own.alloc(nb);
+        st_own.alloc(1);
+        r = r_own.p;
+        z = z_own.p;
+        d = d_own.p;
+        partials = part_own.p;
+        st = st_own.p;
+    }
     PcgState h{};
     h.max_iter = o.max_iter;
-    H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    H1PA_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
     const int check = o.check_every > 0 ? o.check_every : 8;
-    const int *stop = &st.p->done;
+    const int *stop = &st->done;
     const dim3 grid(nb), blk(kern::kThreads), one(1);
 
     auto t0 = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(kern::k_pcg_init<T>, grid, blk, 0, s, n, per, b, dinv, x, r.p, z.p, d.p,
-                       partials.p);
-    hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p,
+    hipLaunchKernelGGL(kern::k_pcg_init<T>, grid, blk, 0, s, n, per, b, dinv, x, r, z, d,
+                       partials);
+    hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials, st,
                        o.rel_tol, o.abs_tol);
     const int64_t bb = direct_rows(sp, mask);
     auto apply = [&]() {
-        pcg_element_apply<T>(f, prec, mask, d.p, ye, bb < n ? z.p : nullptr, stop, s);
+        pcg_element_apply<T>(f, prec, mask, d, ye, bb < n ? z : nullptr, stop, s);
         hipLaunchKernelGGL(kern::k_pcg_scatter<T>, grid, blk, 0, s, n, per, bb, sp.restr.offsets.p,
-                           sp.restr.indices_nat.p, ye, mask, d.p, z.p, partials.p, st.p,
+                           sp.restr.indices_nat.p, ye, mask, d, z, partials, st,
                            sp.restr.perm.p, sp.restr.nperm);
-        hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
+        hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials, st);
     };
     apply();
     H1PA_HIP(hipGetLastError());
     for (int it = 1; it <= o.max_iter; ++it) {
-        hipLaunchKernelGGL(kern::k_pcg_update<T>, grid, blk, 0, s, n, per, dinv, x, r.p, z.p, d.p,
-                           partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_pcg_dir<T>, grid, blk, 0, s, n, per, z.p, d.p, st.p);
+        hipLaunchKernelGGL(kern::k_pcg_update<T>, grid, blk, 0, s, n, per, dinv, x, r, z, d,
+                           partials, st);
+        hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials, st);
+        hipLaunchKernelGGL(kern::k_pcg_dir<T>, grid, blk, 0, s, n, per, z, d, st);
         apply();
         H1PA_HIP(hipGetLastError());
         if (it % check == 0 || it == o.max_iter) {
-            H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            H1PA_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
             H1PA_HIP(hipStreamSynchronize(s));
             if (h.done) break;
         }
     }
-    H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    H1PA_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
     H1PA_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     stats.converged = h.converged;
@@ -1126,6 +1159,23 @@ void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b
     pcg_run<double>(f, H1PA_PREC_FP64, mask, dinv.p, b, x, f.ework0.p, o, stats, s);
 }
 
+void pcg_solve_dinv(Form &f, const uint8_t *mask, const double *dinv, const double *b, double *x,
+                    const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, PcgWork &w, hipStream_t s) {
+    pcg_run<double>(f, H1PA_PREC_FP64, mask, dinv, b, x, f.ework0.p, o, stats, s, &w);
+}
+
+// u = A v on the DIAG_ONE constrained operator; interior rows go straight
+// into u when the mask allows it.
+void constrained_apply(Form &f, const uint8_t *mask, const double *v, double *u,
+                       const int *stop, hipStream_t s) {
+    Space &sp = *f.space;
+    const int64_t bb = direct_rows(sp, mask);
+    pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, v, f.ework0.p,
+                              bb < sp.ndofs ? u : nullptr, stop, s);
+    kern::scatter_constrained(bb, sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask,
+                              v, u, sp.restr.perm.p, sp.restr.nperm, s);
+}
+
 // ---- OperatorChebyshevSmoother ----------------------------------------------
 namespace {
 
@@ -1210,18 +1260,6 @@ double device_dot(int64_t n, const double *a, const double *b, DevBuf<double> &p
     return h;
 }
 
-// u = A v on the DIAG_ONE constrained operator; interior rows go straight
-// into u when the mask allows it.
-void constrained_apply(Chebyshev &c, const double *v, double *u, const int *stop, hipStream_t s) {
-    Form &f = *c.f;
-    Space &sp = *f.space;
-    const int64_t bb = direct_rows(sp, c.mask);
-    pcg_element_apply<double>(f, H1PA_PREC_FP64, c.mask, v, f.ework0.p,
-                              bb < sp.ndofs ? u : nullptr, stop, s);
-    kern::scatter_constrained(bb, sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, c.mask,
-                              v, u, sp.restr.perm.p, sp.restr.nperm, s);
-}
-
 // y = sum_k c_k (D^-1 A)^k D^-1 x
 void chebyshev_apply(Chebyshev &c, const double *x, double *y, const int *stop, hipStream_t s) {
     Form &f = *c.f;
@@ -1278,7 +1316,7 @@ void chebyshev_setup(Chebyshev &c, const double *d_diag, double max_eig, int pow
             const double nrm = device_dot(n, a, a, partials, out, s);
             hipLaunchKernelGGL(kern::k_scale, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n,
                                ch.per, a, 1.0 / std::sqrt(nrm));
-            constrained_apply(c, a, c.u.p, nullptr, s);
+            constrained_apply(*c.f, c.mask, a, c.u.p, nullptr, s);
             hipLaunchKernelGGL(kern::k_jacobi_mult, dim3(ch.blocks), dim3(kern::kThreads), 0, s,
                                n, ch.per, c.dinv.p, c.u.p, b);
             const double enew = device_dot(n, a, b, partials, out, s);
@@ -1296,12 +1334,13 @@ void chebyshev_mult(Chebyshev &c, const double *x, double *y, hipStream_t s) {
     chebyshev_apply(c, x, y, nullptr, s);
 }
 
-// CGSolver::Mult (solvers.cpp:869-1050, iterative_mode = false) with the
-// Chebyshev smoother as preconditioner: the Jacobi-PCG loop with z = M r
-// computed by the smoother instead of fused into the update.
-void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts &o,
-                   h1pa_pcg_stats &stats, hipStream_t s) {
-    Form &f = *c.f;
+// CGSolver::Mult (solvers.cpp:869-1050, iterative_mode = false) with a
+// preconditioner z = M r on f's DIAG_ONE constrained operator: the Jacobi-PCG
+// loop with z = M r computed by M instead of fused into the update.  The host
+// reads the state every `check` iterations.
+void pcg_preconditioned(Form &f, const uint8_t *mask, const Precond &M, const double *b,
+                        double *x, const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, int check,
+                        hipStream_t s) {
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
     DevBuf<double> r(n), q(n), d(n), z(n);
@@ -1313,22 +1352,22 @@ void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts
     PcgState h{};
     h.max_iter = o.max_iter;
     H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
-    const int check = o.check_every > 0 ? o.check_every : 8;
+    check = check > 0 ? check : 8;
     const int *stop = &st.p->done;
     const dim3 grid(nb), blk(kern::kThreads), one(1);
-    const int64_t bb = direct_rows(sp, c.mask);
+    const int64_t bb = direct_rows(sp, mask);
 
     auto t0 = std::chrono::steady_clock::now();
     hipLaunchKernelGGL(kern::k_cg_start, grid, blk, 0, s, n, per, b, x, r.p);
-    chebyshev_apply(c, r.p, z.p, stop, s);
+    M(r.p, z.p, stop, s);
     hipLaunchKernelGGL(kern::k_cg_first_dir, grid, blk, 0, s, n, per, z.p, r.p, d.p, partials.p);
     hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p, o.rel_tol,
                        o.abs_tol);
     auto apply = [&]() {  // q = A d, den = (d, q)
-        pcg_element_apply<double>(f, H1PA_PREC_FP64, c.mask, d.p, f.ework0.p,
+        pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, d.p, f.ework0.p,
                                   bb < n ? q.p : nullptr, stop, s);
         hipLaunchKernelGGL(kern::k_pcg_scatter<double>, grid, blk, 0, s, n, per, bb,
-                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, c.mask, d.p,
+                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask, d.p,
                            q.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
         hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
     };
@@ -1336,7 +1375,7 @@ void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts
     H1PA_HIP(hipGetLastError());
     for (int it = 1; it <= o.max_iter; ++it) {
         hipLaunchKernelGGL(kern::k_cg_update_xr, grid, blk, 0, s, n, per, x, r.p, q.p, d.p, st.p);
-        chebyshev_apply(c, r.p, z.p, stop, s);
+        M(r.p, z.p, stop, s);
         hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, per, r.p, z.p, partials.p, stop);
         hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
         hipLaunchKernelGGL(kern::k_pcg_dir<double>, grid, blk, 0, s, n, per, z.p, d.p, st.p);
@@ -1358,6 +1397,15 @@ void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts
     stats.seconds = std::chrono::duration<double>(t1 - t0).count();
     H1PA_CHECK(!h.nonfinite, H1PA_ENONFINITE,
                "PCG: (B r, r) or (A d, d) is not finite (CGSolver: MFEM_VERIFY(IsFinite))");
+}
+
+// CGSolver with the Chebyshev smoother as preconditioner.
+void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts &o,
+                   h1pa_pcg_stats &stats, hipStream_t s) {
+    const Precond M = [&c](const double *r, double *z, const int *stop, hipStream_t st) {
+        chebyshev_apply(c, r, z, stop, st);
+    };
+    pcg_preconditioned(*c.f, c.mask, M, b, x, o, stats, o.check_every, s);
 }
 
 void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,

@@ -106,6 +106,9 @@ def lib():
                                         _pd, _pd]
         L.orc_pcg_chebyshev.restype = _i
         L.orc_domain_lf.argtypes = [_i, _i, _i, _pd, _d, _pd]
+        L.orc_transfer_basis.argtypes = [_i, _i, _pd]
+        L.orc_transfer_prolong.argtypes = [_i, _i, _i, _pi, _pi, _pi, _i, _pd, _pd, _pd]
+        L.orc_transfer_restrict.argtypes = [_i, _i, _i, _pi, _pi, _pi, _i, _pd, _pd, _pd]
         L.orc_num_threads.restype = _i
         L.orc_set_num_threads.argtypes = [_i]
     return _lib
@@ -291,6 +294,142 @@ def boolean_mask(offsets, indices, nE):
             ix = int(indices[j])
             y[ix if ix >= 0 else -1 - ix] = 1.0 if j == offsets[i] else 0.0
     return y
+
+
+def first_entry_mask(offsets, indices, nE):
+    """BooleanMask (fem/restriction.cpp:255-290) vectorized: 1.0 at the first
+    entry of every non-empty row of the transpose table."""
+    m = np.zeros(nE)
+    rows = np.nonzero(offsets[1:] > offsets[:-1])[0]
+    first = indices[offsets[rows]]
+    m[np.where(first >= 0, first, -1 - first)] = 1.0
+    return m
+
+
+class Transfer:
+    """TensorProductPRefinementTransferOperator (fem/transfer.cpp:2619-2990)
+    between two Problems of one mesh (lo.p < hi.p)."""
+
+    def __init__(self, lo, hi):
+        assert lo.ne == hi.ne and lo.p < hi.p
+        self.lo, self.hi = lo, hi
+        self.mask_h = first_entry_mask(hi.offsets, hi.indices, hi.ne * hi.D ** 3)
+
+    def mult(self, xl):
+        lo, hi = self.lo, self.hi
+        y = _da(hi.ndofs)
+        lib().orc_transfer_prolong(lo.ne, lo.p, hi.p, _ip(lo.gather_map), _ip(hi.offsets),
+                                   _ip(hi.indices), hi.ndofs, _dp(self.mask_h),
+                                   _dp(np.ascontiguousarray(xl, dtype=np.float64)), _dp(y))
+        return y
+
+    def mult_transpose(self, xh):
+        lo, hi = self.lo, self.hi
+        y = _da(lo.ndofs)
+        lib().orc_transfer_restrict(lo.ne, lo.p, hi.p, _ip(hi.gather_map), _ip(lo.offsets),
+                                    _ip(lo.indices), lo.ndofs, _dp(self.mask_h),
+                                    _dp(np.ascontiguousarray(xh, dtype=np.float64)), _dp(y))
+        return y
+
+
+def cg_preconditioned(P, M, b, rel_tol, abs_tol=0.0, max_iter=2000):
+    """CGSolver::Mult (linalg/solvers.cpp:869-1050), iterative_mode = false,
+    with preconditioner z = M(r) on P's DIAG_ONE constrained operator (M =
+    None: no preconditioner).  Returns (x, iterations, converged)."""
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    prec = M if M is not None else (lambda v: v.copy())
+    r = b.copy()
+    x = np.zeros_like(b)
+    z = prec(r)
+    d = z.copy()
+    nom = float(d @ r)
+    if not np.isfinite(nom) or nom < 0.0:
+        return x, 0, False
+    r0 = max(nom * rel_tol * rel_tol, abs_tol * abs_tol)
+    if nom <= r0:
+        return x, 0, True
+    z = P.constrained_mult(d)
+    den = float(z @ d)
+    if den <= 0.0 and den == 0.0:
+        return x, 0, False
+    i = 1
+    while True:
+        alpha = nom / den
+        x += alpha * d
+        r -= alpha * z
+        z = prec(r)
+        betanom = float(r @ z)
+        if not np.isfinite(betanom) or betanom < 0.0:
+            return x, i, False
+        if betanom <= r0:
+            return x, i, True
+        i += 1
+        if i > max_iter:
+            return x, max_iter, False
+        beta = betanom / nom
+        d = z + beta * d
+        z = P.constrained_mult(d)
+        den = float(d @ z)
+        if den <= 0.0 and den == 0.0:
+            return x, i, False
+        nom = betanom
+
+
+class PMultigrid:
+    """MultigridBase::Mult / Cycle / SmoothingStep (fem/multigrid.cpp:106-240)
+    over Problems of one mesh with increasing orders, set up as ex26's
+    DiffusionMultigrid (examples/ex26.cpp:42-110): OperatorChebyshevSmoother
+    on levels >= 1 (diagonal = AssembleDiagonal, eigenvalue by the power
+    method), CGSolver without a preconditioner on the coarsest level, every
+    level constrained on its boundary DOFs (the Problems' ess lists)."""
+
+    def __init__(self, levels, smoother_order=2, pre=1, post=1, w_cycle=False,
+                 coarse_rel_tol=1e-2, coarse_max_iter=200, power_iterations=10,
+                 power_tolerance=1e-8, power_seed=12345):
+        self.levels = levels
+        self.order, self.pre, self.post, self.w_cycle = smoother_order, pre, post, w_cycle
+        self.coarse_rel_tol, self.coarse_max_iter = coarse_rel_tol, coarse_max_iter
+        self.diag, self.coeffs, self.max_eig = [None], [None], [0.0]
+        for P in levels[1:]:
+            dg = P.diagonal()
+            e, c = P.chebyshev(smoother_order, power_iterations, power_tolerance, power_seed)
+            self.diag.append(dg)
+            self.coeffs.append(c)
+            self.max_eig.append(e)
+        self.P = [Transfer(levels[l], levels[l + 1]) for l in range(len(levels) - 1)]
+        self.coarse_iterations = 0
+
+    def _smooth(self, l, v):
+        return self.levels[l].chebyshev_mult(self.coeffs[l], v, self.diag[l])
+
+    def _cycle(self, l, x, y):
+        P = self.levels[l]
+        if l == 0:  # SmoothingStep(0, zero = true): the coarse CGSolver
+            yc, it, _ = cg_preconditioned(P, None, x, self.coarse_rel_tol, 0.0,
+                                          self.coarse_max_iter)
+            self.coarse_iterations += it
+            return yc
+        for i in range(self.pre):
+            if not self.w_cycle and i == 0:
+                y = self._smooth(l, x)
+            else:
+                y = y + self._smooth(l, x - P.constrained_mult(y))
+        r = x - P.constrained_mult(y)
+        xc = self.P[l - 1].mult_transpose(r)
+        yc = self._cycle(l - 1, xc, np.zeros_like(xc))
+        if self.w_cycle:
+            yc = self._cycle(l - 1, xc, yc)
+        y = y + self.P[l - 1].mult(yc)
+        for _ in range(self.post):
+            y = y + self._smooth(l, x - P.constrained_mult(y))
+        return y
+
+    def mult(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        return self._cycle(len(self.levels) - 1, x, np.zeros_like(x))
+
+    def pcg(self, b, rel_tol=1e-12, abs_tol=0.0, max_iter=2000):
+        return cg_preconditioned(self.levels[-1], self.mult, b, rel_tol, abs_tol, max_iter)
 
 
 # ---------------------------------------------------------------------------

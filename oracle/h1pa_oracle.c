@@ -659,6 +659,113 @@ void orc_restriction_mult_transpose(int ndofs, int dof, const int *offsets,
 }
 
 /* ======================================================================
+ * p-refinement transfer (fem/transfer.cpp:2619-2990).
+ * ====================================================================== */
+void orc_transfer_basis(int pl, int ph, double *B)
+{
+    /* the high-order space's nodes in lexicographic order are the tensor
+     * GLL points (transfer.cpp:2641-2650); the low-order basis evaluated
+     * there gives the DofToQuad B of :2653-2658 */
+    double xh[16], wh[16], G[16 * 16];
+    orc_gauss_lobatto(ph + 1, xh, wh);
+    orc_basis_at(pl, ph + 1, xh, B, G);
+}
+
+void orc_transfer_prolong(int ne, int pl, int ph, const int *gmap_l, const int *offsets_h,
+                          const int *indices_h, int ndofs_h, const double *mask_h,
+                          const double *xl, double *yh)
+{
+    /* Prolongation3D (transfer.cpp:2741-2818), between the two restrictions */
+    const int D = pl + 1, Q = ph + 1;
+    const long nl = (long)ne * D * D * D, nh = (long)ne * Q * Q * Q;
+    double B[16 * 16];
+    orc_transfer_basis(pl, ph, B);
+    double *xe = (double *)malloc(sizeof(double) * (size_t)nl);
+    double *ye = (double *)malloc(sizeof(double) * (size_t)nh);
+    orc_restriction_mult(ne, D * D * D, gmap_l, xl, xe);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; e++) {
+        const double *x = xe + (long)e * D * D * D;
+        double *y = ye + (long)e * Q * Q * Q;
+        for (int i = 0; i < Q * Q * Q; i++) y[i] = 0.0;
+        for (int dz = 0; dz < D; dz++) {
+            double sxy[16][16];
+            for (int qy = 0; qy < Q; qy++)
+                for (int qx = 0; qx < Q; qx++) sxy[qy][qx] = 0.0;
+            for (int dy = 0; dy < D; dy++) {
+                double sx[16];
+                for (int qx = 0; qx < Q; qx++) sx[qx] = 0.0;
+                for (int dx = 0; dx < D; dx++) {
+                    const double v = x[dx + D * (dy + D * dz)];
+                    for (int qx = 0; qx < Q; qx++) sx[qx] += B[qx + Q * dx] * v;
+                }
+                for (int qy = 0; qy < Q; qy++) {
+                    const double wy = B[qy + Q * dy];
+                    for (int qx = 0; qx < Q; qx++) sxy[qy][qx] += wy * sx[qx];
+                }
+            }
+            for (int qz = 0; qz < Q; qz++) {
+                const double wz = B[qz + Q * dz];
+                for (int qy = 0; qy < Q; qy++)
+                    for (int qx = 0; qx < Q; qx++) y[qx + Q * (qy + Q * qz)] += wz * sxy[qy][qx];
+            }
+        }
+        const double *m = mask_h + (long)e * Q * Q * Q;
+        for (int i = 0; i < Q * Q * Q; i++) y[i] *= m[i];
+    }
+    orc_restriction_mult_transpose(ndofs_h, Q * Q * Q, offsets_h, indices_h, ye, yh, 0);
+    free(xe);
+    free(ye);
+}
+
+void orc_transfer_restrict(int ne, int pl, int ph, const int *gmap_h, const int *offsets_l,
+                           const int *indices_l, int ndofs_l, const double *mask_h,
+                           const double *xh, double *yl)
+{
+    /* Restriction3D (transfer.cpp:2867-2938), between the two restrictions */
+    const int D = pl + 1, Q = ph + 1;
+    const long nl = (long)ne * D * D * D, nh = (long)ne * Q * Q * Q;
+    double B[16 * 16];
+    orc_transfer_basis(pl, ph, B);  /* Bt(d, q) = B(q, d) */
+    double *xe = (double *)malloc(sizeof(double) * (size_t)nh);
+    double *ye = (double *)malloc(sizeof(double) * (size_t)nl);
+    orc_restriction_mult(ne, Q * Q * Q, gmap_h, xh, xe);
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < ne; e++) {
+        const double *x = xe + (long)e * Q * Q * Q;
+        const double *m = mask_h + (long)e * Q * Q * Q;
+        double *y = ye + (long)e * D * D * D;
+        for (int i = 0; i < D * D * D; i++) y[i] = 0.0;
+        for (int qz = 0; qz < Q; qz++) {
+            double sxy[16][16];
+            for (int dy = 0; dy < D; dy++)
+                for (int dx = 0; dx < D; dx++) sxy[dy][dx] = 0.0;
+            for (int qy = 0; qy < Q; qy++) {
+                double sx[16];
+                for (int dx = 0; dx < D; dx++) sx[dx] = 0.0;
+                for (int qx = 0; qx < Q; qx++) {
+                    const int i = qx + Q * (qy + Q * qz);
+                    const double v = m[i] * x[i];
+                    for (int dx = 0; dx < D; dx++) sx[dx] += B[qx + Q * dx] * v;
+                }
+                for (int dy = 0; dy < D; dy++) {
+                    const double wy = B[qy + Q * dy];
+                    for (int dx = 0; dx < D; dx++) sxy[dy][dx] += wy * sx[dx];
+                }
+            }
+            for (int dz = 0; dz < D; dz++) {
+                const double wz = B[qz + Q * dz];
+                for (int dy = 0; dy < D; dy++)
+                    for (int dx = 0; dx < D; dx++) y[dx + D * (dy + D * dz)] += wz * sxy[dy][dx];
+            }
+        }
+    }
+    orc_restriction_mult_transpose(ndofs_l, D * D * D, offsets_l, indices_l, ye, yl, 0);
+    free(xe);
+    free(ye);
+}
+
+/* ======================================================================
  * 1D quadrature and basis.
  * ====================================================================== */
 static void legendre_ld(int n, long double x, long double *pn, long double *dpn, long double *pn1)

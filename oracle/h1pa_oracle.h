@@ -192,6 +192,21 @@ int  orc_pcg_chebyshev(const orc_pa_op *op, const double *diag, int order, const
                        const double *b, double *x, double rel_tol, double abs_tol, int max_iter,
                        int *iters, double *final_norm, double *initial_norm);
 
+/* ---- p-refinement transfer between two H1 spaces on one mesh
+ *      (TensorProductPRefinementTransferOperator, fem/transfer.cpp:2619-2990) -- */
+/* B[qh + (ph+1) dl] = l_dl(xi_qh): the order-pl GLL Lagrange basis at the
+ * order-ph GLL nodes (lexicographic), transfer.cpp:2641-2660. */
+void orc_transfer_basis(int pl, int ph, double *B);
+/* Mult (transfer.cpp:2941-2965): y_h = R_h^T (mask_h o (B x B x B) R_l x_l);
+ * mask_h is ElementRestriction::BooleanMask of R_h (restriction.cpp:255-290). */
+void orc_transfer_prolong(int ne, int pl, int ph, const int *gmap_l, const int *offsets_h,
+                          const int *indices_h, int ndofs_h, const double *mask_h,
+                          const double *xl, double *yh);
+/* MultTranspose (transfer.cpp:2967-2990): y_l = R_l^T (Bt x Bt x Bt)(mask_h o R_h x_h). */
+void orc_transfer_restrict(int ne, int pl, int ph, const int *gmap_h, const int *offsets_l,
+                           const int *indices_l, int ndofs_l, const double *mask_h,
+                           const double *xh, double *yl);
+
 /* ---- RHS: DomainLFIntegrator (fem/lininteg.cpp:51-79) ------------------ */
 /* b_E[D^3][ne] += sum_q w_q detJ f(x_q) phi(x_q), Q = p+1 GL points,
  * constant f. */

@@ -445,15 +445,17 @@ def test_quadrature_function_coefficient(orc, h1pa, torch_cuda, p):
 
 @pytest.mark.parametrize("p", range(1, 9))
 def test_column_apply(orc, h1pa, torch_cuda, monkeypatch, p):
-    """The fused column apply (Mult of a diffusion form on a Cartesian mesh)
+    """The fused column apply (opt-in with H1PA_COLUMNS=1: Mult of a diffusion form on a Cartesian mesh)
     on a mesh whose sizes are no multiple of the column cross-section, so
     columns are clipped in x and y and swept in several segments: within
     1e-13 of the oracle, within 1e-14 of the element-kernel + scatter path
     (H1PA_COLUMNS=0), the same bits on a second call, and AddMult adding the
     same result."""
     torch = torch_cuda
+    monkeypatch.setenv("H1PA_COLUMNS", "1")  # opt-in path
     dims = (5, 7, 6) if p <= 4 else (3, 4, 5)
     S = Pair(orc, h1pa, torch, dims, p, perturb=0.2, seed=p, coeff="sincos")
+    assert S.a.MultPath() == "columns"
     x = np.random.default_rng(20 + p).uniform(-1, 1, S.P.ndofs)
     xd = S.dev(x)
     y1 = torch.empty_like(xd)
