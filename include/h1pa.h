@@ -383,6 +383,12 @@ typedef struct {
     int power_iterations;    /* smoothers' eigenvalue estimate (MFEM: 10, 1e-8, 12345) */
     double power_tolerance;
     int power_seed;
+    int cycle_precision;     /* H1PA_PREC_FP64, or H1PA_PREC_BF16_PA: the cycle's
+                                operator applies (smoothing, residuals, coarse CG)
+                                read BF16 copies of each level's PA data with FP64
+                                vectors and arithmetic -- a mixed-precision
+                                preconditioner inside the FP64 CG (no MFEM
+                                counterpart; diffusion forms at q1d = p + 2)    */
 } h1pa_mg_opts;
 /* orders[0 .. nlevels-1]: increasing, orders[nlevels-1] = the fine space's
  * order (2 <= nlevels <= 8).  The fine form must be assembled and stay alive;

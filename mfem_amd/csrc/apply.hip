@@ -462,5 +462,26 @@ void mass_apply_lp(int p, int q1d, int64_t ne, const float *basis, const void *p
     throw Error{H1PA_EUNSUP, "reduced-precision mass apply needs q1d = p + 2"};
 }
 
+// FP64 vectors and arithmetic with BF16-stored PA data (the applies of a
+// mixed-precision multigrid cycle): one variant, the PA planes requested two
+// ahead, compiled for the default Q = D + 1.
+void diffusion_apply_d_bf16(int p, int q1d, int64_t ne, const double *basis, const void *pa,
+                            const int *gmap, const uint8_t *mask, const double *x,
+                            const EOut &out, bool accumulate, hipStream_t s) {
+    if (ne <= 0) return;
+    const int D = p + 1;
+#define X(DD, QQ)                                                                          \
+    if (D == DD && q1d == QQ) {                                                            \
+        constexpr int NB = nb_for(QQ, 3 * DD * QQ * QQ, sizeof(double));                   \
+        launch_diffusion_v<double, bf16_t, DD, QQ, NB, 1, 2>(ne, basis, (const bf16_t *)pa, \
+                                                             gmap, mask, x, out, accumulate, s); \
+        H1PA_HIP(hipGetLastError());                                                       \
+        return;                                                                            \
+    }
+    H1PA_FOR_EACH_DQ_DEFAULT(X)
+#undef X
+    throw Error{H1PA_EUNSUP, "BF16-PA apply needs q1d = p + 2"};
+}
+
 }  // namespace kern
 }  // namespace h1pa

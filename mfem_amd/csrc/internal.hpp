@@ -282,6 +282,10 @@ void diffusion_apply_lp_variant(int variant, int p, int q1d, int64_t ne, const f
                                 bool accumulate, hipStream_t s);
 void set_diffusion_variant_lp(int pa_kind, int p, int q1d, int variant);
 int get_diffusion_variant_lp(int pa_kind, int p, int q1d);
+// FP64 vectors with BF16-stored diffusion PA data (q1d = p + 2).
+void diffusion_apply_d_bf16(int p, int q1d, int64_t ne, const double *basis, const void *pa,
+                            const int *gmap, const uint8_t *mask, const double *x,
+                            const EOut &out, bool accumulate, hipStream_t s);
 void mass_apply_lp(int p, int q1d, int64_t ne, const float *basis, const void *pa, int pa_kind,
                    const int *gmap, const uint8_t *mask, const float *x, const EOutT<float> &out,
                    bool accumulate, hipStream_t s);
@@ -329,6 +333,7 @@ struct Chebyshev {
     const uint8_t *mask = nullptr;  // constraint mask (the space's or own.p)
     DevBuf<uint8_t> own;
     int order = 0;
+    int prec = H1PA_PREC_FP64;  // BF16_PA: applies with the form's BF16 PA shadow
     double max_eig = 0.0;
     double coeffs[5] = {0, 0, 0, 0, 0};
     DevBuf<double> dinv, t, u;
@@ -346,24 +351,28 @@ void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b
 // Work vectors of a CG solve kept by the caller between solves (multigrid's
 // coarse solves allocate nothing once they have the size).
 struct PcgWork {
-    DevBuf<double> r, z, d, partials;
+    DevBuf<double> r, z, d, q, partials;
     DevBuf<uint8_t> state;
 };
 // Jacobi-PCG with the inverse diagonal given and the caller's work vectors.
+// prec = H1PA_PREC_BF16_PA applies the operator with the form's BF16 PA shadow
+// (FP64 vectors and arithmetic).
 void pcg_solve_dinv(Form &f, const uint8_t *mask, const double *dinv, const double *b, double *x,
-                    const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, PcgWork &w, hipStream_t s);
+                    const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, PcgWork &w, hipStream_t s,
+                    int prec = H1PA_PREC_FP64);
 // u = A v on f's DIAG_ONE constrained operator (mask: the constrained DOFs;
-// stop: a device flag that skips the work when set, or null).
+// stop: a device flag that skips the work when set, or null; prec as above).
 void constrained_apply(Form &f, const uint8_t *mask, const double *v, double *u,
-                       const int *stop, hipStream_t s);
+                       const int *stop, hipStream_t s, int prec = H1PA_PREC_FP64);
 // A preconditioner z = M r on the device (stop as above).
 using Precond = std::function<void(const double *r, double *z, const int *stop, hipStream_t s)>;
 // CGSolver::Mult (linalg/solvers.cpp:869-1050, iterative_mode = false) with
 // SetPreconditioner(M) on f's DIAG_ONE constrained operator; the host reads
 // the solver state every `check` iterations.
+// ws: work vectors kept by the caller (null: allocated for this solve).
 void pcg_preconditioned(Form &f, const uint8_t *mask, const Precond &M, const double *b,
                         double *x, const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, int check,
-                        hipStream_t s);
+                        hipStream_t s, PcgWork *ws);
 // Builds f's reduced-precision shadows for `precision` (H1PA_PREC_FP32 or
 // H1PA_PREC_BF16_PA) unless they already match.
 void ensure_low_precision(Form &f, int precision, hipStream_t s);
@@ -386,6 +395,7 @@ struct Transfer {
     void mult_transpose(const double *xh, double *yl, hipStream_t s);   // restriction
 };
 struct MgLevel {
+    int prec = H1PA_PREC_FP64;        // PA data of the cycle's applies on this level
     h1pa_space *space_own = nullptr;  // levels below the finest: made here
     h1pa_form *form_own = nullptr;
     Form *f = nullptr;
@@ -399,7 +409,7 @@ struct Multigrid {
     std::vector<MgLevel> lv;                   // 0 = coarsest
     std::vector<std::unique_ptr<Transfer>> P;  // P[l]: level l -> l + 1
     DevBuf<double> coarse_dinv;                // ones: CG without a preconditioner
-    PcgWork coarse_work;
+    PcgWork coarse_work, outer_work;           // kept between cycles and solves
     int64_t cycles = 0, coarse_solves = 0, coarse_iterations = 0;
     ~Multigrid();
 };

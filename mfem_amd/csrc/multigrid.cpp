@@ -110,6 +110,11 @@ Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1
     H1PA_CHECK(o.coarse_max_iter >= 1 && o.coarse_rel_tol >= 0.0, H1PA_EINVAL,
                "multigrid: coarse solver settings");
     H1PA_CHECK(fs.mesh->handle, H1PA_EINVAL, "multigrid: the mesh has no handle");
+    H1PA_CHECK(o.cycle_precision == H1PA_PREC_FP64 || o.cycle_precision == H1PA_PREC_BF16_PA,
+               H1PA_EINVAL, "multigrid: cycle precision must be FP64 or BF16_PA");
+    if (o.cycle_precision == H1PA_PREC_BF16_PA)
+        H1PA_CHECK((fine.desc.integrators & H1PA_INTEG_DIFFUSION) && fine.q1d == fs.p + 2,
+                   H1PA_EUNSUP, "multigrid: a BF16 cycle needs a diffusion form at q1d = p + 2");
     std::unique_ptr<Multigrid> mg(new Multigrid);
     mg->o = o;
     mg->lv.resize(nlevels);
@@ -129,6 +134,8 @@ Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1
         }
         L.sp = L.f->space;
         L.mask = L.sp->ess_mask.p;
+        L.prec = o.cycle_precision;
+        if (L.prec == H1PA_PREC_BF16_PA) ensure_low_precision(*L.f, H1PA_PREC_BF16_PA, s);
         const int64_t n = L.sp->ndofs;
         if (l > 0) {
             // OperatorChebyshevSmoother(A_l, diag, ess, order) (ex26.cpp:99-104)
@@ -136,6 +143,7 @@ Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1
             L.smoother->f = L.f;
             L.smoother->mask = L.mask;
             L.smoother->order = o.smoother_order;
+            L.smoother->prec = L.prec;
             chebyshev_setup(*L.smoother, nullptr, 0.0, o.power_iterations, o.power_tolerance,
                             o.power_seed, s);
             L.R.alloc(n);
@@ -160,7 +168,7 @@ namespace {
 
 // r = x - A y on level l
 void residual(MgLevel &L, const double *x, const double *y, double *r, hipStream_t s) {
-    constrained_apply(*L.f, L.mask, y, r, nullptr, s);
+    constrained_apply(*L.f, L.mask, y, r, nullptr, s, L.prec);
     const int64_t n = L.sp->ndofs;
     hipLaunchKernelGGL(k_sub_from, dim3(nblocks(n)), dim3(kBlock), 0, s, n, x, r);
     H1PA_HIP(hipGetLastError());
@@ -179,7 +187,7 @@ void cycle(Multigrid &mg, int l, const double *x, double *y, hipStream_t s) {
     if (l == 0) {  // coarse solve: SmoothingStep(0, zero = true)
         h1pa_pcg_opts co{mg.o.coarse_rel_tol, 0.0, mg.o.coarse_max_iter, 4};
         h1pa_pcg_stats st{};
-        pcg_solve_dinv(*L.f, L.mask, mg.coarse_dinv.p, x, y, co, st, mg.coarse_work, s);
+        pcg_solve_dinv(*L.f, L.mask, mg.coarse_dinv.p, x, y, co, st, mg.coarse_work, s, L.prec);
         mg.coarse_iterations += st.iterations;
         ++mg.coarse_solves;
         return;
@@ -218,7 +226,7 @@ void pcg_multigrid(Multigrid &mg, const double *b, double *x, const h1pa_pcg_opt
     const Precond M = [&mg](const double *r, double *z, const int *, hipStream_t st) {
         multigrid_mult(mg, r, z, st);
     };
-    pcg_preconditioned(*F.f, F.mask, M, b, x, o, stats, 1, s);
+    pcg_preconditioned(*F.f, F.mask, M, b, x, o, stats, 1, s, &mg.outer_work);
 }
 
 void multigrid_info(const Multigrid &mg, int64_t *info) {

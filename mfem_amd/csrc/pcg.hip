@@ -928,10 +928,17 @@ void pcg_element_apply(Form &f, int prec, const uint8_t *mask, const T *d, T *ye
         out.bb = sp.restr.bubble_begin;
     }
     if constexpr (sizeof(T) == 8) {
-        (void)prec;
-        if (diff)
+        // FP64 vectors; prec = BF16_PA takes the diffusion PA data from the
+        // form's BF16 shadow (the applies of a mixed-precision cycle)
+        if (diff && prec == H1PA_PREC_BF16_PA) {
+            H1PA_CHECK(f.lp_kind == 1 && f.pa_diff_lp.p, H1PA_EINVAL,
+                       "BF16-PA apply: the form has no BF16 shadow");
+            kern::diffusion_apply_d_bf16(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff_lp.p,
+                                         sp.restr.gather_map.p, mask, d, out, false, s);
+        } else if (diff) {
             kern::diffusion_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_diff.p,
                                   sp.restr.gather_map.p, mask, d, out, false, s);
+        }
         if (mass)
             kern::mass_apply(sp.p, f.q1d, sp.ne, f.basis.p, f.pa_mass.p, sp.restr.gather_map.p,
                              mask, d, out, diff, s);
@@ -1160,18 +1167,19 @@ void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b
 }
 
 void pcg_solve_dinv(Form &f, const uint8_t *mask, const double *dinv, const double *b, double *x,
-                    const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, PcgWork &w, hipStream_t s) {
-    pcg_run<double>(f, H1PA_PREC_FP64, mask, dinv, b, x, f.ework0.p, o, stats, s, &w);
+                    const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, PcgWork &w, hipStream_t s,
+                    int prec) {
+    pcg_run<double>(f, prec, mask, dinv, b, x, f.ework0.p, o, stats, s, &w);
 }
 
 // u = A v on the DIAG_ONE constrained operator; interior rows go straight
 // into u when the mask allows it.
 void constrained_apply(Form &f, const uint8_t *mask, const double *v, double *u,
-                       const int *stop, hipStream_t s) {
+                       const int *stop, hipStream_t s, int prec) {
     Space &sp = *f.space;
     const int64_t bb = direct_rows(sp, mask);
-    pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, v, f.ework0.p,
-                              bb < sp.ndofs ? u : nullptr, stop, s);
+    pcg_element_apply<double>(f, prec, mask, v, f.ework0.p, bb < sp.ndofs ? u : nullptr, stop,
+                              s);
     kern::scatter_constrained(bb, sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask,
                               v, u, sp.restr.perm.p, sp.restr.nperm, s);
 }
@@ -1270,7 +1278,7 @@ void chebyshev_apply(Chebyshev &c, const double *x, double *y, const int *stop, 
     hipLaunchKernelGGL(kern::k_cheb_first, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n, ch.per,
                        c.dinv.p, x, c.t.p, y, c.coeffs[0], stop);
     for (int k = 1; k < c.order; ++k) {
-        pcg_element_apply<double>(f, H1PA_PREC_FP64, c.mask, c.t.p, f.ework0.p,
+        pcg_element_apply<double>(f, c.prec, c.mask, c.t.p, f.ework0.p,
                                   bb < n ? c.u.p : nullptr, stop, s);
         hipLaunchKernelGGL(kern::k_cheb_scatter, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n,
                            ch.per, bb, sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p,
@@ -1316,7 +1324,7 @@ void chebyshev_setup(Chebyshev &c, const double *d_diag, double max_eig, int pow
             const double nrm = device_dot(n, a, a, partials, out, s);
             hipLaunchKernelGGL(kern::k_scale, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n,
                                ch.per, a, 1.0 / std::sqrt(nrm));
-            constrained_apply(*c.f, c.mask, a, c.u.p, nullptr, s);
+            constrained_apply(*c.f, c.mask, a, c.u.p, nullptr, s, c.prec);
             hipLaunchKernelGGL(kern::k_jacobi_mult, dim3(ch.blocks), dim3(kern::kThreads), 0, s,
                                n, ch.per, c.dinv.p, c.u.p, b);
             const double enew = device_dot(n, a, b, partials, out, s);
@@ -1340,54 +1348,65 @@ void chebyshev_mult(Chebyshev &c, const double *x, double *y, hipStream_t s) {
 // reads the state every `check` iterations.
 void pcg_preconditioned(Form &f, const uint8_t *mask, const Precond &M, const double *b,
                         double *x, const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, int check,
-                        hipStream_t s) {
+                        hipStream_t s, PcgWork *ws) {
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
-    DevBuf<double> r(n), q(n), d(n), z(n);
     const kern::Chunks ch = kern::chunks(n);
     const unsigned nb = ch.blocks;
     const int per = ch.per;
-    DevBuf<double> partials(nb);
-    DevBuf<PcgState> st(1);
+    // work vectors: the caller's (kept between solves: no allocation, so no
+    // device-wide wait in a free) or this solve's own
+    PcgWork own;
+    PcgWork &w = ws ? *ws : own;
+    if (w.r.n < (size_t)n) {
+        w.r.alloc(n);
+        w.q.alloc(n);
+        w.d.alloc(n);
+        w.z.alloc(n);
+    }
+    if (w.partials.n < nb) w.partials.alloc(nb);
+    if (w.state.n < sizeof(PcgState)) w.state.alloc(sizeof(PcgState));
+    double *r = w.r.p, *q = w.q.p, *d = w.d.p, *z = w.z.p, *partials = w.partials.p;
+    PcgState *st = reinterpret_cast<PcgState *>(w.state.p);
     PcgState h{};
     h.max_iter = o.max_iter;
-    H1PA_HIP(hipMemcpyAsync(st.p, &h, sizeof(h), hipMemcpyHostToDevice, s));
+    H1PA_HIP(hipMemcpyAsync(st, &h, sizeof(h), hipMemcpyHostToDevice, s));
     check = check > 0 ? check : 8;
-    const int *stop = &st.p->done;
+    const int *stop = &st->done;
     const dim3 grid(nb), blk(kern::kThreads), one(1);
     const int64_t bb = direct_rows(sp, mask);
 
     auto t0 = std::chrono::steady_clock::now();
-    hipLaunchKernelGGL(kern::k_cg_start, grid, blk, 0, s, n, per, b, x, r.p);
-    M(r.p, z.p, stop, s);
-    hipLaunchKernelGGL(kern::k_cg_first_dir, grid, blk, 0, s, n, per, z.p, r.p, d.p, partials.p);
-    hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials.p, st.p, o.rel_tol,
+    hipLaunchKernelGGL(kern::k_cg_start, grid, blk, 0, s, n, per, b, x, r);
+    M(r, z, stop, s);
+    hipLaunchKernelGGL(kern::k_cg_first_dir, grid, blk, 0, s, n, per, z, r, d, partials);
+    hipLaunchKernelGGL(kern::k_pcg_fin_init, one, blk, 0, s, (int)nb, partials, st, o.rel_tol,
                        o.abs_tol);
     auto apply = [&]() {  // q = A d, den = (d, q)
-        pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, d.p, f.ework0.p,
-                                  bb < n ? q.p : nullptr, stop, s);
+        pcg_element_apply<double>(f, H1PA_PREC_FP64, mask, d, f.ework0.p,
+                                  bb < n ? q : nullptr, stop, s);
         hipLaunchKernelGGL(kern::k_pcg_scatter<double>, grid, blk, 0, s, n, per, bb,
-                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask, d.p,
-                           q.p, partials.p, st.p, sp.restr.perm.p, sp.restr.nperm);
-        hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials.p, st.p);
+                           sp.restr.offsets.p, sp.restr.indices_nat.p, f.ework0.p, mask, d,
+                           q, partials, st, sp.restr.perm.p, sp.restr.nperm);
+        hipLaunchKernelGGL(kern::k_pcg_fin_den, one, blk, 0, s, (int)nb, partials, st);
     };
     apply();
     H1PA_HIP(hipGetLastError());
     for (int it = 1; it <= o.max_iter; ++it) {
-        hipLaunchKernelGGL(kern::k_cg_update_xr, grid, blk, 0, s, n, per, x, r.p, q.p, d.p, st.p);
-        M(r.p, z.p, stop, s);
-        hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, per, r.p, z.p, partials.p, stop);
-        hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials.p, st.p);
-        hipLaunchKernelGGL(kern::k_pcg_dir<double>, grid, blk, 0, s, n, per, z.p, d.p, st.p);
+        hipLaunchKernelGGL(kern::k_cg_update_xr, grid, blk, 0, s, n, per, x, r, q, d, st);
+        M(r, z, stop, s);
+        hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, per, r, z, partials, stop);
+        hipLaunchKernelGGL(kern::k_pcg_fin_update, one, blk, 0, s, (int)nb, partials, st);
+        hipLaunchKernelGGL(kern::k_pcg_dir<double>, grid, blk, 0, s, n, per, z, d, st);
         apply();
         H1PA_HIP(hipGetLastError());
         if (it % check == 0 || it == o.max_iter) {
-            H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+            H1PA_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
             H1PA_HIP(hipStreamSynchronize(s));
             if (h.done) break;
         }
     }
-    H1PA_HIP(hipMemcpyAsync(&h, st.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    H1PA_HIP(hipMemcpyAsync(&h, st, sizeof(h), hipMemcpyDeviceToHost, s));
     H1PA_HIP(hipStreamSynchronize(s));
     auto t1 = std::chrono::steady_clock::now();
     stats.converged = h.converged;
@@ -1405,7 +1424,7 @@ void pcg_chebyshev(Chebyshev &c, const double *b, double *x, const h1pa_pcg_opts
     const Precond M = [&c](const double *r, double *z, const int *stop, hipStream_t st) {
         chebyshev_apply(c, r, z, stop, st);
     };
-    pcg_preconditioned(*c.f, c.mask, M, b, x, o, stats, o.check_every, s);
+    pcg_preconditioned(*c.f, c.mask, M, b, x, o, stats, o.check_every, s, nullptr);
 }
 
 void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,

@@ -18,6 +18,20 @@
 //   --mode serial  plain FP64 GPU PCG: requests are solved one at a time, in
 //                  arrival order (h1pa_pcg_solve); the same setup thread
 //                  prepares the next request during each solve.
+//   --mode mg      continuous serving with p-multigrid: arrived requests are
+//                  admitted in arrival order while the HBM budget allows, set
+//                  up (with their multigrid levels: orders halved down to 1,
+//                  Chebyshev(2) smoothing, coarse CG) on --setup-threads host
+//                  threads, and solved by --workers solver threads, each on a
+//                  stream of its own, so the solves of several requests run on
+//                  the GPU at once (CG preconditioned by the V-cycle, FP64;
+//                  p = 1 requests, which have no coarser order, take
+//                  Jacobi-PCG).  --precision bf16: the cycle's applies read
+//                  BF16 copies of the PA data (FP64 vectors), the outer CG
+//                  stays FP64.  The stop test is CGSolver's at rel_tol on the
+//                  solver's own (multigrid) measure; each result's Jacobi
+//                  measure -- the serial mode's stop measure -- is checked
+//                  after the solve and reported (max_jacobi_rel_residual).
 //   --solo         (batch mode) admit a request only when no other is live:
 //                  every request is solved alone through the batch.
 //   --dump DIR     write each request's solution X (the constrained system's,
@@ -38,6 +52,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <random>
 #include <string>
@@ -62,6 +77,8 @@ struct Opts {
     int max_requests = 1 << 30;
     bool solo = false;
     std::string dump;
+    int workers = 4;          // mg mode: solver threads (one stream each)
+    int setup_threads = 2;    // mg mode: set-up threads
 };
 
 struct Request {
@@ -72,12 +89,14 @@ struct Request {
     double t_done = -1.0;
     int iterations = 0;
     int converged = 0;
+    double jacobi_rel = -1.0;  // mg mode: the Jacobi measure of the result
 };
 
 struct Live {
     h1pa_mesh *mesh = nullptr;
     h1pa_space *space = nullptr;
     h1pa_form *form = nullptr;
+    h1pa_multigrid *mg = nullptr;
     double *b = nullptr, *x = nullptr, *B = nullptr, *X = nullptr;
     int64_t bytes = 0;
 };
@@ -152,8 +171,18 @@ std::vector<Request> make_requests(const Opts &o) {
     return r;
 }
 
-// Mesh, space, form (assembled), RHS and the constrained system B, X.
-Live setup(const Request &q, hipStream_t s) {
+// Multigrid orders of a p-request: p halved (rounding up) down to 1.
+std::vector<int> mg_orders(int p) {
+    std::vector<int> o{p};
+    while (o.back() > 1) o.push_back((o.back() + 1) / 2);
+    std::reverse(o.begin(), o.end());
+    return o;
+}
+
+// Mesh, space, form (assembled), RHS and the constrained system B, X; with
+// `mg`, also the request's multigrid (p >= 2).
+Live setup(const Request &q, hipStream_t s, bool mg = false,
+           int mg_precision = H1PA_PREC_FP64) {
     Live L;
     check(h1pa_mesh_create_cartesian(q.n, q.n, q.n, 1, &L.mesh), "mesh");
     check(h1pa_mesh_perturb(L.mesh, 0.15, q.coeff_seed), "perturb");
@@ -180,6 +209,14 @@ Live setup(const Request &q, hipStream_t s) {
     int64_t fb = 0;
     check(h1pa_form_device_bytes(L.form, &fb), "device_bytes");
     L.bytes = fb + 4 * N * 8;
+    if (mg && q.p >= 2) {
+        const std::vector<int> ord = mg_orders(q.p);
+        // ex26's settings: Chebyshev(2), V(1, 1), coarse CG to sqrt(1e-4)
+        const h1pa_mg_opts mo{2, 1, 1, 0, 1e-2, 200, 10, 1e-8, 12345, mg_precision};
+        check(h1pa_multigrid_create(L.form, (int)ord.size(), ord.data(), &mo, &L.mg, s),
+              "multigrid_create");
+        L.bytes += L.bytes / 2;  // coarser levels, smoothers, transfers, work vectors
+    }
     return L;
 }
 
@@ -200,6 +237,7 @@ void dump_solution(const std::string &dir, int id, const Live &L) {
 }
 
 void release(Live &L) {
+    if (L.mg) h1pa_multigrid_destroy(L.mg);
     if (L.form) h1pa_form_destroy(L.form);
     if (L.space) h1pa_space_destroy(L.space);
     if (L.mesh) h1pa_mesh_destroy(L.mesh);
@@ -212,7 +250,11 @@ void release(Live &L) {
 // request, take() returns the finished setups (their device work complete).
 class SetupWorker {
   public:
-    explicit SetupWorker(const std::vector<Request> &reqs) : reqs_(reqs), th_([this] { run(); }) {}
+    explicit SetupWorker(const std::vector<Request> &reqs, bool mg = false, int threads = 1,
+                         int mg_precision = H1PA_PREC_FP64)
+        : reqs_(reqs), mg_(mg), mg_precision_(mg_precision) {
+        for (int i = 0; i < std::max(1, threads); ++i) th_.emplace_back([this] { run(); });
+    }
     void give(int id) {
         {
             std::lock_guard<std::mutex> g(m_);
@@ -237,7 +279,8 @@ class SetupWorker {
             stop_ = true;
         }
         cv_.notify_all();
-        if (th_.joinable()) th_.join();
+        for (auto &t : th_)
+            if (t.joinable()) t.join();
         return setup_s_;
     }
     ~SetupWorker() { stop(); }
@@ -259,7 +302,7 @@ class SetupWorker {
                 todo_.pop_front();
             }
             const auto ts = std::chrono::steady_clock::now();
-            Live L = setup(reqs_[id], ws);
+            Live L = setup(reqs_[id], ws, mg_, mg_precision_);
             hcheck(hipStreamSynchronize(ws), "setup sync");
             const double dt =
                 std::chrono::duration<double>(std::chrono::steady_clock::now() - ts).count();
@@ -273,13 +316,138 @@ class SetupWorker {
         hcheck(hipStreamDestroy(ws), "stream destroy");
     }
     const std::vector<Request> &reqs_;
+    const bool mg_;
+    const int mg_precision_;
     std::mutex m_;
     std::condition_variable cv_;
     std::deque<int> todo_;
     std::deque<std::pair<int, Live>> ready_;
     bool stop_ = false;
     double setup_s_ = 0.0;
-    std::thread th_;
+    std::vector<std::thread> th_;
+};
+
+// mg mode: solver threads, each with a stream of its own, take set-up
+// requests and solve them; finished ones go back to the loop (which releases
+// their memory, so no solver thread waits in a free).
+struct Solved {
+    int id;
+    Live L;
+    double t_done;
+    int iterations, converged;
+};
+
+// The serial mode's stop measure on a result: sqrt((B r, r) / (B b, b)),
+// r = B - A X on the constrained operator, B = Jacobi (ess rows 1).
+double jacobi_measure(const Live &L, hipStream_t s) {
+    const int *ess;
+    int64_t nEss;
+    check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+    int64_t info[4];
+    check(h1pa_space_info(L.space, info), "space_info");
+    const int64_t N = info[3];
+    double *r = nullptr;
+    hcheck(hipMalloc(&r, 2 * N * sizeof(double)), "hipMalloc");
+    double *dg = r + N;
+    check(h1pa_form_constrained_mult(L.form, ess, nEss, L.X, r, s), "residual");
+    check(h1pa_vector_add(r, -1.0, L.B, N, s), "residual");
+    check(h1pa_form_assemble_diagonal(L.form, dg, s), "diagonal");
+    hcheck(hipStreamSynchronize(s), "sync");
+    std::vector<double> hr(N), hd(N), hb(N);
+    std::vector<int> he(nEss);
+    hcheck(hipMemcpy(hr.data(), r, N * 8, hipMemcpyDeviceToHost), "copy");
+    hcheck(hipMemcpy(hd.data(), dg, N * 8, hipMemcpyDeviceToHost), "copy");
+    hcheck(hipMemcpy(hb.data(), L.B, N * 8, hipMemcpyDeviceToHost), "copy");
+    if (nEss) hcheck(hipMemcpy(he.data(), ess, nEss * 4, hipMemcpyDeviceToHost), "copy");
+    hcheck(hipFree(r), "hipFree");
+    for (int i : he) hd[i] = 1.0;
+    double rr = 0.0, bb = 0.0;
+    for (int64_t i = 0; i < N; ++i) {
+        rr += hr[i] * hr[i] / hd[i];
+        bb += hb[i] * hb[i] / hd[i];
+    }
+    return bb > 0 ? std::sqrt(rr / bb) : 0.0;
+}
+class SolverPool {
+  public:
+    SolverPool(int workers, double rel_tol, std::chrono::steady_clock::time_point t0)
+        : rel_tol_(rel_tol), t0_(t0) {
+        for (int i = 0; i < std::max(1, workers); ++i) th_.emplace_back([this] { run(); });
+    }
+    void give(int id, const Live &L) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            todo_.emplace_back(id, L);
+        }
+        cv_.notify_all();
+    }
+    std::vector<Solved> take() {
+        std::lock_guard<std::mutex> g(m_);
+        std::vector<Solved> out(done_.begin(), done_.end());
+        done_.clear();
+        return out;
+    }
+    void wait_done(double seconds) {
+        std::unique_lock<std::mutex> g(m_);
+        done_cv_.wait_for(g, std::chrono::duration<double>(seconds),
+                          [this] { return !done_.empty(); });
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto &t : th_)
+            if (t.joinable()) t.join();
+    }
+    ~SolverPool() { stop(); }
+
+  private:
+    void run() {
+        int dev = 0;
+        hcheck(hipGetDevice(&dev), "get device");
+        hcheck(hipSetDevice(dev), "set device");
+        hipStream_t ws;
+        hcheck(hipStreamCreateWithFlags(&ws, hipStreamNonBlocking), "stream");
+        for (;;) {
+            std::pair<int, Live> job;
+            {
+                std::unique_lock<std::mutex> g(m_);
+                cv_.wait(g, [this] { return stop_ || !todo_.empty(); });
+                if (todo_.empty()) break;
+                job = todo_.front();
+                todo_.pop_front();
+            }
+            Live &L = job.second;
+            const int *ess;
+            int64_t nEss;
+            check(h1pa_space_boundary_dofs(L.space, &ess, &nEss), "boundary");
+            h1pa_pcg_opts po{rel_tol_, 0.0, 200000, 8};
+            h1pa_pcg_stats st{};
+            const int rc = L.mg ? h1pa_pcg_solve_multigrid(L.mg, L.B, L.X, &po, &st, ws)
+                                : h1pa_pcg_solve(L.form, ess, nEss, L.B, L.X, &po, &st, ws);
+            if (rc != H1PA_OK && rc != H1PA_ENOTCONV) check(rc, "solve");
+            hcheck(hipStreamSynchronize(ws), "sync");
+            const double t_done = std::chrono::duration<double>(
+                                      std::chrono::steady_clock::now() - t0_).count();
+            Solved sv{job.first, L, t_done, st.iterations, st.converged};
+            {
+                std::lock_guard<std::mutex> g(m_);
+                done_.push_back(sv);
+            }
+            done_cv_.notify_all();
+        }
+        hcheck(hipStreamDestroy(ws), "stream destroy");
+    }
+    const double rel_tol_;
+    const std::chrono::steady_clock::time_point t0_;
+    std::mutex m_;
+    std::condition_variable cv_, done_cv_;
+    std::deque<std::pair<int, Live>> todo_;
+    std::deque<Solved> done_;
+    bool stop_ = false;
+    std::vector<std::thread> th_;
 };
 
 double pct(std::vector<double> v, double q) {
@@ -321,13 +489,19 @@ Opts parse(int argc, char **argv) {
         else if (a == "--max-requests") o.max_requests = std::atoi(val().c_str());
         else if (a == "--solo") o.solo = true;
         else if (a == "--dump") o.dump = val();
+        else if (a == "--workers") o.workers = std::atoi(val().c_str());
+        else if (a == "--setup-threads") o.setup_threads = std::atoi(val().c_str());
         else {
             std::fprintf(stderr, "unknown option %s\n", a.c_str());
             std::exit(2);
         }
     }
-    if (o.mode != "batch" && o.mode != "serial") {
-        std::fprintf(stderr, "--mode must be batch or serial\n");
+    if (o.mode != "batch" && o.mode != "serial" && o.mode != "mg") {
+        std::fprintf(stderr, "--mode must be batch, serial or mg\n");
+        std::exit(2);
+    }
+    if (o.workers < 1 || o.workers > 16 || o.setup_threads < 1 || o.setup_threads > 8) {
+        std::fprintf(stderr, "--workers must be in [1, 16], --setup-threads in [1, 8]\n");
         std::exit(2);
     }
     if (o.nmin < 1 || o.nmax < o.nmin || o.pmin < 1 || o.pmax > 8 || o.pmax < o.pmin ||
@@ -348,7 +522,13 @@ int main(int argc, char **argv) {
     hcheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
     const int64_t budget = (int64_t)(o.budget_gb * 1e9);
     const bool batch = o.mode == "batch";
-    const int prec = batch ? o.precision : H1PA_PREC_FP64;
+    const bool mgmode = o.mode == "mg";
+    // batch: the batch's precision; mg: the cycle's (FP64, or BF16-PA applies)
+    const int prec = (batch || mgmode) ? o.precision : H1PA_PREC_FP64;
+    if (mgmode && prec == H1PA_PREC_FP32) {
+        std::fprintf(stderr, "--mode mg takes --precision fp64 or bf16\n");
+        return 2;
+    }
 
     // warm-up: one small solve (first-use costs, code object loading)
     {
@@ -373,7 +553,7 @@ int main(int argc, char **argv) {
         Request w;
         w.n = 28;  // 21952 elements
         w.p = p;
-        Live L = setup(w, s);
+        Live L = setup(w, s, mgmode, prec);
         if (batch && prec != H1PA_PREC_FP64) {
             int64_t info[4];
             check(h1pa_space_info(L.space, info), "space_info");
@@ -409,8 +589,11 @@ int main(int argc, char **argv) {
     bool inflight = false;  // batch mode: a replay is running on the GPU
     std::vector<int64_t> tags(4096);
     std::vector<h1pa_batch_result> res(4096);
-    SetupWorker W(reqs);
+    SetupWorker W(reqs, mgmode, mgmode ? o.setup_threads : 1, prec);
     const auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<SolverPool> pool;
+    if (mgmode) pool.reset(new SolverPool(o.workers, o.rel_tol, t0));
+    int in_solve = 0;  // mg mode: requests handed to the solver threads
     while (done < R) {
         double t = now_s(t0);
         while (next < R && reqs[next].t_arrive <= t) queue.push_back(next++);
@@ -493,6 +676,55 @@ int main(int argc, char **argv) {
                 const double wait = reqs[next].t_arrive - now_s(t0);
                 if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
             }
+        } else if (mgmode) {
+            // Hand finished setups to the solver threads, retire finished
+            // solves (their memory is released here, not on a solver
+            // thread), and admit queued requests while the HBM budget allows.
+            bool progress = false;
+            for (auto &r : W.take()) {
+                pool->give(r.first, r.second);
+                live[r.first] = r.second;
+                --pending;
+                ++in_solve;
+                progress = true;
+            }
+            for (auto &sv : pool->take()) {
+                Request &q = reqs[sv.id];
+                q.t_done = sv.t_done;
+                q.iterations = sv.iterations;
+                q.converged = sv.converged;
+                q.jacobi_rel = jacobi_measure(sv.L, s);
+                if (!o.dump.empty()) dump_solution(o.dump, sv.id, sv.L);
+                release(sv.L);
+                live.erase(sv.id);
+                used -= live_est[sv.id];
+                live_est.erase(sv.id);
+                --in_solve;
+                ++done;
+                progress = true;
+            }
+            while (!queue.empty()) {
+                const Request &q = reqs[queue.front()];
+                const int64_t est = estimate_bytes(q.n, q.p, H1PA_PREC_FP64, false) * 3 / 2;
+                const bool idle = pending == 0 && in_solve == 0;
+                if (!(idle || used + est <= budget)) break;
+                used += est;
+                live_est[q.id] = est;
+                ++pending;
+                W.give(queue.front());
+                queue.pop_front();
+                progress = true;
+            }
+            peak_used = std::max(peak_used, used);
+            peak_live = std::max(peak_live, pending + in_solve);
+            if (!progress) {
+                if (pending + in_solve > 0) {
+                    pool->wait_done(0.0005);
+                } else if (next < R) {
+                    const double wait = reqs[next].t_arrive - now_s(t0);
+                    if (wait > 0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+                }
+            }
         } else {
             // One solve at a time; the setup thread prepares the next request
             // while the current one solves.
@@ -540,6 +772,7 @@ int main(int argc, char **argv) {
             ++done;
         }
     }
+    if (pool) pool->stop();
     const double setup_s = W.stop();
     const double makespan = now_s(t0);
     if (B) h1pa_batch_destroy(B);
@@ -560,6 +793,8 @@ int main(int argc, char **argv) {
     }
     std::vector<double> lat;
     int64_t dofs = 0, conv = 0, iters = 0;
+    double max_jacobi = -1.0;  // mg mode: worst result on the serial mode's measure
+    for (const Request &q : reqs) max_jacobi = std::max(max_jacobi, q.jacobi_rel);
     double dof_iters = 0.0;  // sum of DOFs x iterations: the solver work of the run
     for (const Request &q : reqs) {
         lat.push_back(q.t_done - q.t_arrive);
@@ -575,11 +810,12 @@ int main(int argc, char **argv) {
         "\"latency_p50_s\": %.4f, \"latency_p90_s\": %.4f, \"latency_p99_s\": %.4f, "
         "\"latency_max_s\": %.4f, \"mean_dofs\": %.0f, \"mean_iterations\": %.1f, "
         "\"peak_live\": %d, \"peak_bytes\": %lld, \"budget_bytes\": %lld, "
-        "\"setup_s\": %.3f, \"launch_s\": %.3f, \"dof_iterations\": %.4e}\n",
+        "\"setup_s\": %.3f, \"launch_s\": %.3f, \"dof_iterations\": %.4e, "
+        "\"max_jacobi_rel_residual\": %.3e, \"workers\": %d}\n",
         o.mode.c_str(), prec == H1PA_PREC_FP64 ? "fp64" : prec == H1PA_PREC_FP32 ? "fp32" : "bf16",
         o.rate, o.seconds, R, (long long)conv, makespan, R / makespan, pct(lat, 0.5),
         pct(lat, 0.9), pct(lat, 0.99), pct(lat, 1.0), R ? (double)dofs / R : 0.0,
         R ? (double)iters / R : 0.0, peak_live, (long long)peak_used, (long long)budget, setup_s, launch_s,
-        dof_iters);
+        dof_iters, max_jacobi, mgmode ? o.workers : 1);
     return 0;
 }

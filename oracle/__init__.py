@@ -385,14 +385,16 @@ class PMultigrid:
 
     def __init__(self, levels, smoother_order=2, pre=1, post=1, w_cycle=False,
                  coarse_rel_tol=1e-2, coarse_max_iter=200, power_iterations=10,
-                 power_tolerance=1e-8, power_seed=12345):
+                 power_tolerance=1e-8, power_seed=12345, diags=None):
+        """diags: the smoothers' diagonals per level (None: each level's own)."""
         self.levels = levels
         self.order, self.pre, self.post, self.w_cycle = smoother_order, pre, post, w_cycle
         self.coarse_rel_tol, self.coarse_max_iter = coarse_rel_tol, coarse_max_iter
         self.diag, self.coeffs, self.max_eig = [None], [None], [0.0]
-        for P in levels[1:]:
-            dg = P.diagonal()
-            e, c = P.chebyshev(smoother_order, power_iterations, power_tolerance, power_seed)
+        for l, P in enumerate(levels[1:], 1):
+            dg = P.diagonal() if diags is None else diags[l]
+            e, c = P.chebyshev(smoother_order, power_iterations, power_tolerance, power_seed,
+                               diag=dg)
             self.diag.append(dg)
             self.coeffs.append(c)
             self.max_eig.append(e)
@@ -776,11 +778,12 @@ class Problem:
         return e, v
 
     def chebyshev(self, order, power_iterations=10, power_tolerance=1e-8, power_seed=12345,
-                  max_eig=None):
-        """OperatorChebyshevSmoother set-up: (max_eig, coeffs)."""
+                  max_eig=None, diag=None):
+        """OperatorChebyshevSmoother set-up: (max_eig, coeffs); diag: the
+        smoother's diagonal (None: this operator's)."""
         if max_eig is None:
             max_eig, _ = self.power_method(randomize(power_seed, self.ndofs), power_iterations,
-                                           power_tolerance)
+                                           power_tolerance, diag=diag)
         return max_eig, chebyshev_coeffs(order, max_eig)
 
     def chebyshev_mult(self, coeffs, x, diag=None):

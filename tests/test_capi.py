@@ -90,7 +90,8 @@ def test_splitmix64_known_answer():
 def test_serve_driver_rejects_bad_options():
     import subprocess
     exe = os.path.join(ROOT, "mfem_amd", "h1pa_serve")
-    for args in (["--mode", "queue"], ["--pmax", "9"], ["--bogus"]):
+    for args in (["--mode", "queue"], ["--pmax", "9"], ["--bogus"], ["--workers", "0"],
+                 ["--setup-threads", "9"]):
         r = subprocess.run([exe, *args], capture_output=True, text=True, timeout=30)
         assert r.returncode == 2
 

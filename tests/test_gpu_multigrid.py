@@ -141,3 +141,48 @@ def test_multigrid_argument_errors(h1pa, torch_cuda):
     with pytest.raises(M.H1paError) as e:
         M.PMultigrid(aq, (1, 3))
     assert e.value.code == M.H1PA_EUNSUP
+
+
+def test_bf16_cycle_matches_oracle(orc, h1pa, torch_cuda):
+    """A mixed-precision cycle (every level's applies read BF16 copies of its
+    PA data; FP64 vectors, diagonals from the FP64 data) inside the FP64 CG:
+    the oracle runs the same cycle on BF16-rounded PA data; iterations within
+    +-1, solution within 1e-8, and the FP64 true residual meets the stop."""
+    torch, M = torch_cuda, h1pa
+    dims, p, orders = (4, 4, 4), 4, (1, 2, 4)
+    ne = 64
+    k = _log_uniform(ne, 77)
+    mesh = _mesh(M, dims, 0.0, 0, 0.05, p)
+    fes = M.FiniteElementSpace(mesh, M.H1_FECollection(p, 3))
+    kd = torch.from_numpy(k).cuda()
+    a = M.BilinearForm(fes)
+    a.SetAssemblyLevel(M.AssemblyLevel.PARTIAL)
+    a.AddDomainIntegrator(M.DiffusionIntegrator(M.PWConstCoefficient(kd)))
+    a.Assemble()
+    mg = M.PMultigrid(a, orders, cycle_precision=M.PREC_BF16_PA)
+    fine = orc.Problem(None, p, curve=0.05, coeff=k, dims=dims)
+    lv = [orc.Problem(None, q, coeff=k, dims=dims, nodes=(p, fine.X)) for q in orders]
+    diags = [P.diagonal() for P in lv]
+    for P in lv:
+        P.replace_pa(Dd=orc.round_bf16(P.Dd))
+    omg = orc.PMultigrid(lv, diags=diags)
+    b = np.random.default_rng(3).uniform(-1, 1, fine.ndofs)
+    ess = fes.GetBoundaryTrueDofs()
+    A, X, B = a.FormLinearSystem(ess, torch.zeros(fine.ndofs, dtype=torch.float64,
+                                                  device="cuda"), torch.from_numpy(b).cuda())
+    _, Bo = fine.form_linear_system(np.zeros(fine.ndofs), b)
+    cg = M.CGSolver()
+    cg.SetRelTol(1e-10)
+    cg.SetMaxIter(500)
+    cg.SetOperator(A)
+    cg.SetPreconditioner(mg)
+    cg.Mult(B, X)
+    xo, it_o, conv_o = orc.cg_preconditioned(fine, omg.mult, Bo, 1e-10, 0.0, 500)
+    it = cg.GetNumIterations()
+    x = X.cpu().numpy()
+    print(f"[mg-bf16] iterations {it} (oracle {it_o}), rel diff {_rel(x, xo):.3e}", flush=True)
+    assert cg.GetConverged() and conv_o
+    assert abs(it - it_o) <= 1
+    assert _rel(x, xo) < 1e-8
+    r = Bo - fine.constrained_mult(x)
+    assert np.linalg.norm(r) <= 1e-8 * np.linalg.norm(Bo)

@@ -41,7 +41,8 @@ def _serve(*args):
     return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-@pytest.mark.parametrize("mode,prec", [("batch", "bf16"), ("batch", "fp64"), ("serial", "fp64")])
+@pytest.mark.parametrize("mode,prec", [("batch", "bf16"), ("batch", "fp64"), ("serial", "fp64"),
+                                       ("mg", "fp64")])
 def test_serve_small_mix(mode, prec, torch_cuda):
     out = _serve("--mode", mode, "--precision", prec, "--rate", "60", "--seconds", "0.3",
                  "--nmin", "3", "--nmax", "8", "--pmin", "1", "--pmax", "4", "--seed", "5")
@@ -123,6 +124,47 @@ def test_serve_cfg5_mix_equals_alone(orc, torch_cuda, tmp_path):
         dinv[P.ess] = 1.0
         meas = np.sqrt(r @ (dinv * r)) / np.sqrt(Bo @ (dinv * Bo))
         print(f"[serve] request {i} n={n} p={p} it={q['it']} measure {meas:.2e}", flush=True)
+        assert meas <= 1.05e-10
+        checked += 1
+    assert checked >= 5
+
+
+def test_serve_mg_concurrent_equals_one_worker(orc, torch_cuda, tmp_path):
+    """Multigrid serving on a draw of the serving workload: the results of 4
+    solver threads running requests side by side equal, bit for bit, those of
+    one solver thread, with the same iteration counts; every result meets the
+    serial mode's stop measure (Jacobi-weighted FP64 true residual <= 1e-10,
+    checked by the driver on every request and by the oracle on those up to
+    4M DOFs)."""
+    four, one = tmp_path / "four", tmp_path / "one"
+    four.mkdir()
+    one.mkdir()
+    common = ["--mode", "mg", "--precision", "fp64", "--rate", "200", "--seconds", "0.1",
+              "--seed", "11"]
+    out4 = _serve(*common, "--workers", "4", "--dump", str(four))
+    out1 = _serve(*common, "--workers", "1", "--dump", str(one))
+    print(out4, out1, flush=True)
+    assert out4["converged"] == out4["requests"] == out1["requests"]
+    assert out4["max_jacobi_rel_residual"] <= 1.05e-10
+    r4, r1 = _read(four), _read(one)
+    assert r4.keys() == r1.keys() and len(r4) >= 8
+    for i, q in r4.items():
+        assert q["conv"] == 1 and q["it"] == r1[i]["it"], i
+        assert (four / f"x_{i}.bin").read_bytes() == (one / f"x_{i}.bin").read_bytes(), i
+    checked = 0
+    for i, q in sorted(r4.items()):
+        n, p = q["n"], q["p"]
+        if (n * p + 1) ** 3 > 4_000_000:
+            continue
+        P = orc.Problem(n, p, perturb=0.15, seed=q["cseed"], coeff=("fourier", q["cseed"]))
+        b = _rhs(q["rseed"], P.ndofs)
+        _, Bo = P.form_linear_system(np.zeros(P.ndofs), b)
+        x = np.fromfile(four / f"x_{i}.bin", dtype=np.float64)
+        r = Bo - P.constrained_mult(x)
+        dinv = 1.0 / P.diagonal()
+        dinv[P.ess] = 1.0
+        meas = np.sqrt(r @ (dinv * r)) / np.sqrt(Bo @ (dinv * Bo))
+        print(f"[serve-mg] request {i} n={n} p={p} it={q['it']} measure {meas:.2e}", flush=True)
         assert meas <= 1.05e-10
         checked += 1
     assert checked >= 5
