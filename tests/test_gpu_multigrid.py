@@ -71,7 +71,9 @@ class MgCase:
                   for q in orders[:-1]] + [fine]
         omg = {"smoother_order": mg.get("smoother_order", 2),
                "coarse_rel_tol": mg.get("coarse_rel_tol", 1e-2),
-               "coarse_max_iter": mg.get("coarse_max_iter", 200)}
+               "coarse_max_iter": mg.get("coarse_max_iter", 200),
+               "pre": mg.get("pre_smoothing", 1), "post": mg.get("post_smoothing", 1),
+               "w_cycle": mg.get("cycle", "V").upper().startswith("W")}
         self.omg = orc.PMultigrid(levels, **omg)
         self.P = fine
         assert self.fes.GetNDofs() == fine.ndofs
@@ -90,6 +92,23 @@ def test_cycle_matches_oracle(orc, h1pa, torch_cuda):
     assert _rel(y.cpu().numpy(), S.omg.mult(x)) < 1e-10
     info = S.mg.Info()
     assert info["cycles"] == 1 and info["coarse_solves"] == 1 and info["coarse_iterations"] == 6
+
+
+def test_w_cycle_matches_oracle(orc, h1pa, torch_cuda):
+    """W-cycle, two pre- and one post-smoothing step, order-3 Chebyshev, four
+    levels; fixed coarse iterations."""
+    torch = torch_cuda
+    S = MgCase(orc, h1pa, torch, (3, 2, 3), 6, (1, 2, 3, 6), 17, coarse_rel_tol=0.0,
+               coarse_max_iter=5, cycle="W", pre_smoothing=2, post_smoothing=1,
+               smoother_order=3)
+    x = np.random.default_rng(8).uniform(-1, 1, S.P.ndofs)
+    y = torch.empty(S.P.ndofs, dtype=torch.float64, device="cuda")
+    S.mg.Mult(torch.from_numpy(x).cuda(), y)
+    torch.cuda.synchronize()
+    assert _rel(y.cpu().numpy(), S.omg.mult(x)) < 1e-10
+    info = S.mg.Info()
+    # a W-cycle visits level l 2^(top - l) times: the coarsest 2^3 = 8 times
+    assert info["coarse_solves"] == 8 and info["coarse_iterations"] == 40
 
 
 @pytest.mark.parametrize("p,orders", [(4, (1, 2, 4)), (6, (1, 3, 6)), (3, (1, 3))])
