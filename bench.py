@@ -370,11 +370,12 @@ def run_solve(M, torch, p, n=48, seed=7):
     # to 1, Chebyshev(2) smoothing, coarse CG to 1e-2); set-up timed apart
     # (and the same with the cycle's applies reading BF16 PA data: mixed)
     orders = mg_orders(p)
-    for name, cprec in (("fp64_pmultigrid", M.PREC_FP64),
-                        ("fp64_pmultigrid_bf16cycle", M.PREC_BF16_PA)):
+    for name, cprec, cj in (("fp64_pmultigrid", M.PREC_FP64, False),
+                            ("fp64_pmultigrid_bf16cycle", M.PREC_BF16_PA, False),
+                            ("fp64_pmultigrid_bf16cycle_jacobi_coarse", M.PREC_BF16_PA, True)):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        mg = M.PMultigrid(a, orders, cycle_precision=cprec)
+        mg = M.PMultigrid(a, orders, cycle_precision=cprec, coarse_jacobi=cj)
         torch.cuda.synchronize()
         t_setup = time.perf_counter() - t0
         cg = M.CGSolver()

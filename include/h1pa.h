@@ -389,6 +389,8 @@ typedef struct {
                                 vectors and arithmetic -- a mixed-precision
                                 preconditioner inside the FP64 CG (no MFEM
                                 counterpart; diffusion forms at q1d = p + 2)    */
+    int coarse_jacobi;       /* 0: the coarse CG has no preconditioner (ex26);
+                                1: OperatorJacobiSmoother on the coarse level    */
 } h1pa_mg_opts;
 /* orders[0 .. nlevels-1]: increasing, orders[nlevels-1] = the fine space's
  * order (2 <= nlevels <= 8).  The fine form must be assembled and stay alive;

@@ -67,7 +67,7 @@ class MgOpts(C.Structure):
     _fields_ = [("smoother_order", _i), ("pre_smoothing", _i), ("post_smoothing", _i),
                 ("w_cycle", _i), ("coarse_rel_tol", _d), ("coarse_max_iter", _i),
                 ("power_iterations", _i), ("power_tolerance", _d), ("power_seed", _i),
-                ("cycle_precision", _i)]
+                ("cycle_precision", _i), ("coarse_jacobi", _i)]
 
 
 class IrOpts(C.Structure):
@@ -1008,18 +1008,20 @@ class PMultigrid:
     orders: increasing, ending with the fine order (ex26: 1, 2, 4, ...).
     Mult(x, y): one cycle (MultigridBase::Mult, fem/multigrid.cpp:106-145).
     cycle_precision=PREC_BF16_PA: the cycle's applies read BF16 copies of the
-    levels' PA data (FP64 vectors): a mixed-precision preconditioner."""
+    levels' PA data (FP64 vectors): a mixed-precision preconditioner.
+    coarse_jacobi=True: Jacobi-preconditioned coarse CG (ex26: none)."""
 
     def __init__(self, a, orders, smoother_order=2, pre_smoothing=1, post_smoothing=1,
                  cycle="V", coarse_rel_tol=1e-2, coarse_max_iter=200, power_iterations=10,
-                 power_tolerance=1e-8, power_seed=12345, cycle_precision=PREC_FP64):
+                 power_tolerance=1e-8, power_seed=12345, cycle_precision=PREC_FP64,
+                 coarse_jacobi=False):
         f = a._single()
         orders = [int(o) for o in orders]
         arr = (_i * len(orders))(*orders)
         o = MgOpts(int(smoother_order), int(pre_smoothing), int(post_smoothing),
                    1 if cycle.upper().startswith("W") else 0, float(coarse_rel_tol),
                    int(coarse_max_iter), int(power_iterations), float(power_tolerance),
-                   int(power_seed), int(cycle_precision))
+                   int(power_seed), int(cycle_precision), 1 if coarse_jacobi else 0)
         h = _vp()
         _check(lib().h1pa_multigrid_create(f._h, len(orders), arr, C.byref(o), C.byref(h),
                                            _stream()))

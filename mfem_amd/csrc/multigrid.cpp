@@ -156,10 +156,18 @@ Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1
     }
     for (int l = 0; l + 1 < nlevels; ++l)
         mg->P.emplace_back(new Transfer(*mg->lv[l].sp, *mg->lv[l + 1].sp, s));
-    // CGSolver without a preconditioner on the coarsest level: Jacobi with
-    // dinv = 1 gives z = r, the same iteration
-    mg->coarse_dinv.alloc(mg->lv[0].sp->ndofs);
-    fill(mg->lv[0].sp->ndofs, 1.0, mg->coarse_dinv.p, s);
+    // CGSolver on the coarsest level: without a preconditioner (Jacobi with
+    // dinv = 1 gives z = r, the same iteration), or with OperatorJacobiSmoother
+    MgLevel &C = mg->lv[0];
+    mg->coarse_dinv.alloc(C.sp->ndofs);
+    if (o.coarse_jacobi) {
+        DevBuf<double> diag(C.sp->ndofs);
+        operator_diagonal(*C.f, diag.p, s);
+        kern::jacobi_dinv(C.sp->ndofs, diag.p, C.mask, mg->coarse_dinv.p, s);
+        H1PA_HIP(hipStreamSynchronize(s));
+    } else {
+        fill(C.sp->ndofs, 1.0, mg->coarse_dinv.p, s);
+    }
     H1PA_HIP(hipStreamSynchronize(s));
     return mg.release();
 }

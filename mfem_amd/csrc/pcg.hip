@@ -841,6 +841,45 @@ k_scale(int64_t n, int per, double *__restrict__ v, double m) {
     H1PA_CHUNK(i, n) v[i] *= m;
 }
 
+// ---- PowerMethod::EstimateLargestEigenvalue (linalg/operator.cpp:886-935)
+// with its state on the device, so the steps run without host round trips.
+struct PowerState {
+    double nrm, eig;
+    int done;
+};
+
+// nrm = (v, v) from the partials (first dot of a step)
+__global__ void __launch_bounds__(kThreads)
+k_power_nrm(int nb, const double *__restrict__ partials, PowerState *st) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x == 0) st->nrm = tot[0];
+}
+
+// v *= 1 / sqrt(nrm)
+__global__ void __launch_bounds__(kThreads)
+k_power_normalize(int64_t n, int per, double *__restrict__ v, const PowerState *st) {
+    if (st->done) return;
+    const double m = 1.0 / sqrt(st->nrm);
+    H1PA_CHUNK(i, n) v[i] *= m;
+}
+
+// eigenvalue = (v, D^-1 A v); stop when it changed by less than tol (relative)
+__global__ void __launch_bounds__(kThreads)
+k_power_eig(int nb, const double *__restrict__ partials, PowerState *st, double tol) {
+    __shared__ double sh[kThreads];
+    if (st->done) return;
+    double tot[1];
+    sum_partials<1>(partials, nb, sh, tot);
+    if (threadIdx.x != 0) return;
+    const double enew = tot[0];
+    const double diff = fabs((enew - st->eig) / st->eig);
+    st->eig = enew;
+    if (diff < tol) st->done = 1;
+}
+
 // y = 0 + dinv u (OperatorJacobiSmoother::Mult with iterative_mode off)
 __global__ void __launch_bounds__(kThreads)
 k_jacobi_mult(int64_t n, int per, const double *__restrict__ dinv, const double *__restrict__ u,
@@ -1253,21 +1292,6 @@ void chebyshev_coeffs(int order, double max_eig, double *c) {
     }
 }
 
-// (a, b) on the host (used by the power method's set-up iterations).
-double device_dot(int64_t n, const double *a, const double *b, DevBuf<double> &partials,
-                  DevBuf<double> &out, hipStream_t s) {
-    const kern::Chunks ch = kern::chunks(n);
-    hipLaunchKernelGGL(kern::k_dot, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n, ch.per, a, b,
-                       partials.p, (const int *)nullptr);
-    hipLaunchKernelGGL(kern::k_dot_fin, dim3(1), dim3(kern::kThreads), 0, s, (int)ch.blocks,
-                       partials.p, out.p);
-    H1PA_HIP(hipGetLastError());
-    double h = 0.0;
-    H1PA_HIP(hipMemcpyAsync(&h, out.p, sizeof(double), hipMemcpyDeviceToHost, s));
-    H1PA_HIP(hipStreamSynchronize(s));
-    return h;
-}
-
 // y = sum_k c_k (D^-1 A)^k D^-1 x
 void chebyshev_apply(Chebyshev &c, const double *x, double *y, const int *stop, hipStream_t s) {
     Form &f = *c.f;
@@ -1313,27 +1337,38 @@ void chebyshev_setup(Chebyshev &c, const double *d_diag, double max_eig, int pow
         // on diag^-1 A from Vector::Randomize(power_seed)
         H1PA_CHECK(power_seed != 0, H1PA_EINVAL, "chebyshev: invalid seed");
         H1PA_CHECK(power_iterations >= 1, H1PA_EINVAL, "chebyshev: power_iterations must be >= 1");
+        // each step: v /= |v|, u = A v, w = D^-1 u, eig = (v, w), v <- w; the
+        // state (norm, estimate, stop) stays on the device and every launch
+        // after the stop does nothing, so the host waits once, at the end
         std::vector<double> h(n);
         randomize_like_mfem(power_seed, h);
-        DevBuf<double> v0(n), v1(n), partials(kern::chunks(n).blocks), out(1);
-        v0.upload(h.data(), n, s);
         const kern::Chunks ch = kern::chunks(n);
+        DevBuf<double> v0(n), v1(n), partials(ch.blocks);
+        DevBuf<kern::PowerState> st(1);
+        v0.upload(h.data(), n, s);
+        const kern::PowerState h0{0.0, 1.0, 0};
+        H1PA_HIP(hipMemcpyAsync(st.p, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
+        const int *stop = &st.p->done;
+        const dim3 grid(ch.blocks), blk(kern::kThreads), one(1);
         double *a = v0.p, *b = v1.p;
-        double eig = 1.0;
         for (int it = 0; it < power_iterations; ++it) {
-            const double nrm = device_dot(n, a, a, partials, out, s);
-            hipLaunchKernelGGL(kern::k_scale, dim3(ch.blocks), dim3(kern::kThreads), 0, s, n,
-                               ch.per, a, 1.0 / std::sqrt(nrm));
-            constrained_apply(*c.f, c.mask, a, c.u.p, nullptr, s, c.prec);
-            hipLaunchKernelGGL(kern::k_jacobi_mult, dim3(ch.blocks), dim3(kern::kThreads), 0, s,
-                               n, ch.per, c.dinv.p, c.u.p, b);
-            const double enew = device_dot(n, a, b, partials, out, s);
-            const double diff = std::abs((enew - eig) / eig);
-            eig = enew;
+            hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, ch.per, a, a, partials.p, stop);
+            hipLaunchKernelGGL(kern::k_power_nrm, one, blk, 0, s, (int)ch.blocks, partials.p,
+                               st.p);
+            hipLaunchKernelGGL(kern::k_power_normalize, grid, blk, 0, s, n, ch.per, a, st.p);
+            constrained_apply(*c.f, c.mask, a, c.u.p, stop, s, c.prec);
+            hipLaunchKernelGGL(kern::k_jacobi_mult, grid, blk, 0, s, n, ch.per, c.dinv.p, c.u.p,
+                               b);
+            hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, ch.per, a, b, partials.p, stop);
+            hipLaunchKernelGGL(kern::k_power_eig, one, blk, 0, s, (int)ch.blocks, partials.p,
+                               st.p, power_tolerance);
+            H1PA_HIP(hipGetLastError());
             std::swap(a, b);
-            if (diff < power_tolerance) break;
         }
-        c.max_eig = eig;
+        kern::PowerState hs{};
+        H1PA_HIP(hipMemcpyAsync(&hs, st.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+        H1PA_HIP(hipStreamSynchronize(s));
+        c.max_eig = hs.eig;
     }
     chebyshev_coeffs(c.order, c.max_eig, c.coeffs);
 }

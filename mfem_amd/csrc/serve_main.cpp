@@ -26,7 +26,8 @@
 //                  stream of its own, so the solves of several requests run on
 //                  the GPU at once (CG preconditioned by the V-cycle, FP64;
 //                  p = 1 requests, which have no coarser order, take
-//                  Jacobi-PCG).  --precision bf16: the cycle's applies read
+//                  Jacobi-PCG; the coarse CG is Jacobi-preconditioned).
+//                  --precision bf16: the cycle's applies read
 //                  BF16 copies of the PA data (FP64 vectors), the outer CG
 //                  stays FP64.  The stop test is CGSolver's at rel_tol on the
 //                  solver's own (multigrid) measure; each result's Jacobi
@@ -212,7 +213,8 @@ Live setup(const Request &q, hipStream_t s, bool mg = false,
     if (mg && q.p >= 2) {
         const std::vector<int> ord = mg_orders(q.p);
         // ex26's settings: Chebyshev(2), V(1, 1), coarse CG to sqrt(1e-4)
-        const h1pa_mg_opts mo{2, 1, 1, 0, 1e-2, 200, 10, 1e-8, 12345, mg_precision};
+        // Jacobi on the coarse level: fewer coarse CG iterations (launch-bound)
+        const h1pa_mg_opts mo{2, 1, 1, 0, 1e-2, 200, 10, 1e-8, 12345, mg_precision, 1};
         check(h1pa_multigrid_create(L.form, (int)ord.size(), ord.data(), &mo, &L.mg, s),
               "multigrid_create");
         L.bytes += L.bytes / 2;  // coarser levels, smoothers, transfers, work vectors

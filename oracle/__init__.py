@@ -385,11 +385,13 @@ class PMultigrid:
 
     def __init__(self, levels, smoother_order=2, pre=1, post=1, w_cycle=False,
                  coarse_rel_tol=1e-2, coarse_max_iter=200, power_iterations=10,
-                 power_tolerance=1e-8, power_seed=12345, diags=None):
-        """diags: the smoothers' diagonals per level (None: each level's own)."""
+                 power_tolerance=1e-8, power_seed=12345, diags=None, coarse_jacobi=False):
+        """diags: the smoothers' diagonals per level (None: each level's own);
+        coarse_jacobi: OperatorJacobiSmoother on the coarse CG (ex26: none)."""
         self.levels = levels
         self.order, self.pre, self.post, self.w_cycle = smoother_order, pre, post, w_cycle
         self.coarse_rel_tol, self.coarse_max_iter = coarse_rel_tol, coarse_max_iter
+        self.coarse_jacobi = coarse_jacobi
         self.diag, self.coeffs, self.max_eig = [None], [None], [0.0]
         for l, P in enumerate(levels[1:], 1):
             dg = P.diagonal() if diags is None else diags[l]
@@ -407,8 +409,12 @@ class PMultigrid:
     def _cycle(self, l, x, y):
         P = self.levels[l]
         if l == 0:  # SmoothingStep(0, zero = true): the coarse CGSolver
-            yc, it, _ = cg_preconditioned(P, None, x, self.coarse_rel_tol, 0.0,
-                                          self.coarse_max_iter)
+            if self.coarse_jacobi:
+                yc, info = P.pcg(x, self.coarse_rel_tol, 0.0, self.coarse_max_iter)
+                it = info["iterations"]
+            else:
+                yc, it, _ = cg_preconditioned(P, None, x, self.coarse_rel_tol, 0.0,
+                                              self.coarse_max_iter)
             self.coarse_iterations += it
             return yc
         for i in range(self.pre):

@@ -70,6 +70,7 @@ class MgCase:
         levels = [orc.Problem(None, q, coeff=k, dims=dims, nodes=(p, fine.X))
                   for q in orders[:-1]] + [fine]
         omg = {"smoother_order": mg.get("smoother_order", 2),
+               "coarse_jacobi": mg.get("coarse_jacobi", False),
                "coarse_rel_tol": mg.get("coarse_rel_tol", 1e-2),
                "coarse_max_iter": mg.get("coarse_max_iter", 200),
                "pre": mg.get("pre_smoothing", 1), "post": mg.get("post_smoothing", 1),
@@ -111,11 +112,12 @@ def test_w_cycle_matches_oracle(orc, h1pa, torch_cuda):
     assert info["coarse_solves"] == 8 and info["coarse_iterations"] == 40
 
 
-@pytest.mark.parametrize("p,orders", [(4, (1, 2, 4)), (6, (1, 3, 6)), (3, (1, 3))])
-def test_multigrid_pcg_matches_oracle(orc, h1pa, torch_cuda, p, orders):
+@pytest.mark.parametrize("p,orders,cj", [(4, (1, 2, 4), False), (6, (1, 3, 6), False),
+                                         (3, (1, 3), False), (4, (1, 2, 4), True)])
+def test_multigrid_pcg_matches_oracle(orc, h1pa, torch_cuda, p, orders, cj):
     torch, M = torch_cuda, h1pa
     dims = (4, 4, 4) if p <= 4 else (3, 3, 3)
-    S = MgCase(orc, M, torch, dims, p, orders, 40 + p)
+    S = MgCase(orc, M, torch, dims, p, orders, 40 + p, coarse_jacobi=cj)
     b = np.random.default_rng(p).uniform(-1, 1, S.P.ndofs)
     ess = S.fes.GetBoundaryTrueDofs()
     x0 = torch.zeros(S.P.ndofs, dtype=torch.float64, device="cuda")
