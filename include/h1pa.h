@@ -294,6 +294,16 @@ int h1pa_vector_randomize(double *d_x, int64_t n, uint64_t seed, double lo, doub
 /* DomainLFIntegrator(ConstantCoefficient(f)) assembled into an L-vector with
  * q1d = p+1 (fem/lininteg.cpp:51-79, LinearForm::Assemble). */
 int h1pa_space_domain_lf(h1pa_space *space, double f, double *d_b, void *stream);
+/* DomainLFIntegrator(QuadratureFunctionCoefficient) on the device path
+ * (DomainLFIntegrator::AssembleDevice, fem/integ/lininteg_domain.cpp:48-59:
+ * default rule of order 2p, CoefficientVector::Project referencing the
+ * QuadratureFunction, fem/coefficient.cpp:2058-2075): the same with f given at
+ * the (p+1)^3 Gauss-Legendre points of every element, d_fq[q + (p+1)^3 e],
+ * points x fastest -- the points of h1pa_mesh_geometric_factors at
+ * q1d = p + 1, whose X[q + nq (c + 3 e)] gives their coordinates, so a field
+ * evaluated there can be passed; d_fq = NULL uses the constant f. */
+int h1pa_space_domain_lf_qf(h1pa_space *space, double f, const double *d_fq, double *d_b,
+                            void *stream);
 
 typedef struct {
     double rel_tol;     /* CGSolver::SetRelTol (preconditioned-norm criterion) */

@@ -142,6 +142,7 @@ SIGNATURES = {
     "h1pa_form_constrained_mult": (_i, [_vp, _vp, _i64, _vp, _vp, _vp]),
     "h1pa_form_linear_system": (_i, [_vp, _vp, _i64, _vp, _vp, _vp, _vp, _vp]),
     "h1pa_space_domain_lf": (_i, [_vp, _d, _vp, _vp]),
+    "h1pa_space_domain_lf_qf": (_i, [_vp, _d, _vp, _vp, _vp]),
     "h1pa_pcg_solve": (_i, [_vp, _vp, _i64, _vp, _vp, C.POINTER(PcgOpts), C.POINTER(PcgStats),
                             _vp]),
     "h1pa_form_mult_f32": (_i, [_vp, _i, _vp, _vp, _vp]),
@@ -1267,22 +1268,36 @@ class BatchSolver:
 
 
 class LinearForm:
-    """LinearForm with one DomainLFIntegrator(ConstantCoefficient(f))."""
+    """LinearForm with one DomainLFIntegrator, whose coefficient is a
+    ConstantCoefficient or a QuadratureFunctionCoefficient holding f at the
+    (p+1)^3 Gauss-Legendre points of every element ([ne][q1d^3], x fastest:
+    the integrator's default rule, order 2p)."""
 
     def __init__(self, fes):
-        self.fes, self.f = fes, None
+        self.fes, self.coeff = fes, None
 
     def AddDomainIntegrator(self, integ):
         if not isinstance(integ, DomainLFIntegrator):
             raise H1paError(H1PA_EUNSUP, "only DomainLFIntegrator is provided")
-        if not isinstance(integ.coeff, ConstantCoefficient):
-            raise H1paError(H1PA_EUNSUP, "only a constant source is provided")
-        self.f = integ.coeff.value
+        if not isinstance(integ.coeff, (ConstantCoefficient, QuadratureFunctionCoefficient)):
+            raise H1paError(H1PA_EUNSUP, "the source is a constant or a quadrature function")
+        self.coeff = integ.coeff
 
     def Assemble(self):
         torch = _torch()
         b = torch.empty(self.fes.ndofs, dtype=torch.float64, device="cuda")
-        _check(lib().h1pa_space_domain_lf(self.fes._h, float(self.f), _ptr(b), _stream()))
+        c = self.coeff
+        if isinstance(c, QuadratureFunctionCoefficient):
+            # the device reads (p+1)^3 values per element: check the size here
+            _ptr(c.data)
+            try:
+                _check_vec(c.data, (self.fes.p + 1) ** 3 * self.fes.GetNE())
+            except ValueError as e:
+                raise ValueError(f"QuadratureFunctionCoefficient: {e}") from None
+            _check(lib().h1pa_space_domain_lf_qf(self.fes._h, 0.0, _ptr(c.data), _ptr(b),
+                                                 _stream()))
+        else:
+            _check(lib().h1pa_space_domain_lf(self.fes._h, float(c.value), _ptr(b), _stream()))
         return b
 
 

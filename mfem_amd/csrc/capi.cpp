@@ -959,13 +959,18 @@ int h1pa_vector_randomize(double *x, int64_t n, uint64_t seed, double lo, double
 }
 
 int h1pa_space_domain_lf(h1pa_space *space, double fval, double *b, void *stream) {
+    return h1pa_space_domain_lf_qf(space, fval, nullptr, b, stream);
+}
+
+int h1pa_space_domain_lf_qf(h1pa_space *space, double fval, const double *d_fq, double *b,
+                            void *stream) {
     API_BEGIN
     REQUIRE(space && b, "space_domain_lf: null argument");
     Space &sp = space->s;
     hipStream_t s = as_stream(stream);
     sp.mesh->ensure_nodes(s);
     DevBuf<double> bE(sp.ne * sp.dof);
-    kern::domain_lf(*sp.mesh, sp.p, fval, bE.p, s);
+    kern::domain_lf(*sp.mesh, sp.p, fval, d_fq, bE.p, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, bE.p, b, 0,
                                   sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
     H1PA_HIP(hipStreamSynchronize(s));

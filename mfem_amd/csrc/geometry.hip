@@ -228,8 +228,11 @@ __global__ void k_pa_setup(int64_t ne, GeomBasis gb, const double *__restrict__ 
 
 // DomainLFIntegrator with constant f, Q = p + 1 (fem/lininteg.cpp:51-79):
 // one block per element; detJ at the Q^3 points staged in LDS first.
+// f at quadrature point q of element e: fq[q + nq e] when fq is given (a
+// QuadratureFunction, points x fastest), else the constant f.
 __global__ void k_domain_lf(int64_t ne, int D, GeomBasis gb, const double *__restrict__ Bs,
-                            const double *__restrict__ X, double f, double *__restrict__ bE) {
+                            const double *__restrict__ X, double f,
+                            const double *__restrict__ fq, double *__restrict__ bE) {
     __shared__ double wdet[1000];
     const int Q = gb.Q, nq = Q * Q * Q, nd = D * D * D;
     const int64_t e = blockIdx.x;
@@ -240,7 +243,7 @@ __global__ void k_domain_lf(int64_t ne, int D, GeomBasis gb, const double *__res
         const double detJ = J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
                             J[1][0] * (J[0][1] * J[2][2] - J[2][1] * J[0][2]) +
                             J[2][0] * (J[0][1] * J[1][2] - J[1][1] * J[0][2]);
-        wdet[q] = gb.w[qx] * gb.w[qy] * gb.w[qz] * detJ * f;
+        wdet[q] = gb.w[qx] * gb.w[qy] * gb.w[qz] * detJ * (fq ? fq[q + (int64_t)nq * e] : f);
     }
     __syncthreads();
     for (int l = threadIdx.x; l < nd; l += blockDim.x) {
@@ -307,7 +310,7 @@ void randomize(int64_t n, double *x, uint64_t seed, double lo, double hi, hipStr
     H1PA_HIP(hipGetLastError());
 }
 
-void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s) {
+void domain_lf(const Mesh &m, int p, double f, const double *fq, double *bE, hipStream_t s) {
     const int Q = p + 1, D = p + 1;
     const GeomBasis gb = make_geom_basis(Q, m.node_order);
     double x[16], w[16], B[256], G[256];
@@ -316,7 +319,7 @@ void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s) {
     DevBuf<double> Bs;
     Bs.upload(B, Q * D, s);
     hipLaunchKernelGGL(k_domain_lf, dim3((unsigned)m.ne), dim3(256), 0, s, m.ne, D, gb, Bs.p,
-                       m.d_nodes.p, f, bE);
+                       m.d_nodes.p, f, fq, bE);
     H1PA_HIP(hipGetLastError());
     H1PA_HIP(hipStreamSynchronize(s));
 }

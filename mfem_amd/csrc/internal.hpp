@@ -251,7 +251,9 @@ void qinterp(Space &sp, int q1d, int vdim, unsigned flags, int by_vdim, const do
 void geometric_factors(const Mesh &m, int q1d, double *X, double *J, double *detJ,
                        hipStream_t s);
 void pa_setup(const Mesh &m, const Form &f, hipStream_t s);
-void domain_lf(const Mesh &m, int p, double f, double *bE, hipStream_t s);
+// DomainLFIntegrator element vectors at q1d = p + 1: constant f, or f given
+// per quadrature point (fq[q + (p+1)^3 e]) when fq is not null.
+void domain_lf(const Mesh &m, int p, double f, const double *fq, double *bE, hipStream_t s);
 void randomize(int64_t n, double *x, uint64_t seed, double lo, double hi, hipStream_t s);
 
 // E->E: ye = A_E xe (overwrite) or += ; x may be gathered from an L-vector

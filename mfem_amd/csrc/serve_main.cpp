@@ -339,6 +339,47 @@ struct Solved {
     int iterations, converged;
 };
 
+// mg mode: finished solves are checked on the serial mode's measure, dumped
+// and released on a host thread of their own (a free waits for the whole
+// device, which must stall neither the loop nor the solver threads).
+class Reaper {
+  public:
+    Reaper(const std::string &dump) : dump_(dump), th_([this] { run(); }) {}
+    void give(const Solved &sv) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            todo_.push_back(sv);
+        }
+        cv_.notify_all();
+    }
+    // ids released since the last call, with their Jacobi measure
+    std::vector<std::pair<int, double>> take() {
+        std::lock_guard<std::mutex> g(m_);
+        std::vector<std::pair<int, double>> out(done_.begin(), done_.end());
+        done_.clear();
+        return out;
+    }
+    void stop() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        if (th_.joinable()) th_.join();
+    }
+    ~Reaper() { stop(); }
+
+  private:
+    void run();
+    const std::string dump_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<Solved> todo_;
+    std::deque<std::pair<int, double>> done_;
+    bool stop_ = false;
+    std::thread th_;
+};
+
 // The serial mode's stop measure on a result: sqrt((B r, r) / (B b, b)),
 // r = B - A X on the constrained operator, B = Jacobi (ess rows 1).
 double jacobi_measure(const Live &L, hipStream_t s) {
@@ -369,6 +410,30 @@ double jacobi_measure(const Live &L, hipStream_t s) {
         bb += hb[i] * hb[i] / hd[i];
     }
     return bb > 0 ? std::sqrt(rr / bb) : 0.0;
+}
+
+void Reaper::run() {
+    int dev = 0;
+    hcheck(hipGetDevice(&dev), "get device");
+    hcheck(hipSetDevice(dev), "set device");
+    hipStream_t rs;
+    hcheck(hipStreamCreateWithFlags(&rs, hipStreamNonBlocking), "stream");
+    for (;;) {
+        Solved sv;
+        {
+            std::unique_lock<std::mutex> g(m_);
+            cv_.wait(g, [this] { return stop_ || !todo_.empty(); });
+            if (todo_.empty()) break;
+            sv = todo_.front();
+            todo_.pop_front();
+        }
+        const double meas = jacobi_measure(sv.L, rs);
+        if (!dump_.empty()) dump_solution(dump_, sv.id, sv.L);
+        release(sv.L);
+        std::lock_guard<std::mutex> g(m_);
+        done_.emplace_back(sv.id, meas);
+    }
+    hcheck(hipStreamDestroy(rs), "stream destroy");
 }
 class SolverPool {
   public:
@@ -594,7 +659,11 @@ int main(int argc, char **argv) {
     SetupWorker W(reqs, mgmode, mgmode ? o.setup_threads : 1, prec);
     const auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<SolverPool> pool;
-    if (mgmode) pool.reset(new SolverPool(o.workers, o.rel_tol, t0));
+    std::unique_ptr<Reaper> reaper;
+    if (mgmode) {
+        pool.reset(new SolverPool(o.workers, o.rel_tol, t0));
+        reaper.reset(new Reaper(o.dump));
+    }
     int in_solve = 0;  // mg mode: requests handed to the solver threads
     while (done < R) {
         double t = now_s(t0);
@@ -695,13 +764,15 @@ int main(int argc, char **argv) {
                 q.t_done = sv.t_done;
                 q.iterations = sv.iterations;
                 q.converged = sv.converged;
-                q.jacobi_rel = jacobi_measure(sv.L, s);
-                if (!o.dump.empty()) dump_solution(o.dump, sv.id, sv.L);
-                release(sv.L);
                 live.erase(sv.id);
-                used -= live_est[sv.id];
-                live_est.erase(sv.id);
+                reaper->give(sv);
                 --in_solve;
+                progress = true;
+            }
+            for (auto &r : reaper->take()) {  // released: the budget frees up
+                reqs[r.first].jacobi_rel = r.second;
+                used -= live_est[r.first];
+                live_est.erase(r.first);
                 ++done;
                 progress = true;
             }
@@ -720,7 +791,7 @@ int main(int argc, char **argv) {
             peak_used = std::max(peak_used, used);
             peak_live = std::max(peak_live, pending + in_solve);
             if (!progress) {
-                if (pending + in_solve > 0) {
+                if (pending + in_solve > 0 || !live_est.empty()) {
                     pool->wait_done(0.0005);
                 } else if (next < R) {
                     const double wait = reqs[next].t_arrive - now_s(t0);
@@ -775,6 +846,7 @@ int main(int argc, char **argv) {
         }
     }
     if (pool) pool->stop();
+    if (reaper) reaper->stop();
     const double setup_s = W.stop();
     const double makespan = now_s(t0);
     if (B) h1pa_batch_destroy(B);

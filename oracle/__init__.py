@@ -106,6 +106,7 @@ def lib():
                                         _pd, _pd]
         L.orc_pcg_chebyshev.restype = _i
         L.orc_domain_lf.argtypes = [_i, _i, _i, _pd, _d, _pd]
+        L.orc_domain_lf_qf.argtypes = [_i, _i, _i, _pd, _d, _pd, _pd]
         L.orc_transfer_basis.argtypes = [_i, _i, _pd]
         L.orc_transfer_prolong.argtypes = [_i, _i, _i, _pi, _pi, _pi, _i, _pd, _pd, _pd]
         L.orc_transfer_restrict.argtypes = [_i, _i, _i, _pi, _pi, _pi, _i, _pd, _pd, _pd]
@@ -600,9 +601,16 @@ def mass_diag(p, Q, ne, B, Mv):
     return ye
 
 
-def domain_lf(p, ne, pg, X, f=1.0):
+def domain_lf(p, ne, pg, X, f=1.0, fq=None):
+    """DomainLFIntegrator element vectors with the constant f, or with f
+    given at the (p+1)^3 Gauss-Legendre points of every element (fq[q + nq e])."""
     bE = _da((p + 1) ** 3 * ne)
-    lib().orc_domain_lf(p, ne, pg, _dp(X), float(f), _dp(bE))
+    if fq is None:
+        lib().orc_domain_lf(p, ne, pg, _dp(X), float(f), _dp(bE))
+    else:
+        fq = np.ascontiguousarray(fq, dtype=np.float64)
+        assert fq.size == (p + 1) ** 3 * ne
+        lib().orc_domain_lf_qf(p, ne, pg, _dp(X), 0.0, _dp(fq), _dp(bE))
     return bE
 
 
@@ -763,8 +771,8 @@ class Problem:
         lib().orc_pa_diagonal(C.byref(self._op), _dp(d))
         return d
 
-    def rhs(self, f=1.0):
-        bE = domain_lf(self.p, self.ne, self.pg, self.X, f)
+    def rhs(self, f=1.0, fq=None):
+        bE = domain_lf(self.p, self.ne, self.pg, self.X, f, fq)
         return restriction_mult_transpose(self.offsets, self.indices, self.D ** 3, bE, 0)
 
     def form_linear_system(self, x, b):

@@ -1489,6 +1489,15 @@ done:
  * ====================================================================== */
 void orc_domain_lf(int p, int ne, int pg, const double *X, double f, double *bE)
 {
+    orc_domain_lf_qf(p, ne, pg, X, f, NULL, bE);
+}
+
+void orc_domain_lf_qf(int p, int ne, int pg, const double *X, double f, const double *fq,
+                      double *bE)
+{
+    /* DomainLFIntegrator::AssembleRHSElementVect (fem/lininteg.cpp:51-81) with
+     * the coefficient's value at quadrature point q of element e: fq[q + nq e]
+     * (a QuadratureFunction, points x fastest), or the constant f (fq NULL) */
     const int D = p + 1, Q = p + 1, nd = D * D * D, nq = Q * Q * Q;
     double B[256], G[256], W[1000];
     double *J = (double *)malloc(sizeof(double) * (size_t)nq * 9);
@@ -1505,7 +1514,7 @@ void orc_domain_lf(int p, int ne, int pg, const double *X, double f, double *bE)
                                 JJ(2, 0) * (JJ(0, 1) * JJ(1, 2) - JJ(1, 1) * JJ(0, 2));
 #undef JJ
             const int qx = q % Q, qy = (q / Q) % Q, qz = q / (Q * Q);
-            const double val = W[q] * detJ * f;
+            const double val = W[q] * detJ * (fq ? fq[q + (long)nq * e] : f);
             for (int dz = 0; dz < D; dz++)
                 for (int dy = 0; dy < D; dy++)
                     for (int dx = 0; dx < D; dx++)

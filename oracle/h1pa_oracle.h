@@ -211,6 +211,10 @@ void orc_transfer_restrict(int ne, int pl, int ph, const int *gmap_h, const int 
 /* b_E[D^3][ne] += sum_q w_q detJ f(x_q) phi(x_q), Q = p+1 GL points,
  * constant f. */
 void orc_domain_lf(int p, int ne, int pg, const double *X, double f, double *bE);
+/* The same with f given at the (p+1)^3 points of every element, fq[q + nq e]
+ * (f used when fq is NULL). */
+void orc_domain_lf_qf(int p, int ne, int pg, const double *X, double f, const double *fq,
+                      double *bE);
 
 /* Number of OpenMP threads the oracle uses (1 when built without OpenMP). */
 int  orc_num_threads(void);

@@ -311,6 +311,37 @@ def test_constrained_system_and_rhs(orc, h1pa, torch_cuda, p):
     assert _rel(S.host(y), S.P.constrained_mult(v)) < RTOL
 
 
+@pytest.mark.parametrize("p", [2, 5])
+def test_rhs_quadrature_function(orc, h1pa, torch_cuda, p):
+    """DomainLFIntegrator(QuadratureFunctionCoefficient) on a curved mesh: f
+    evaluated at the points the product reports (geometric factors at
+    q1d = p + 1), the same values handed to the oracle."""
+    torch = torch_cuda
+    M = h1pa
+    S = Pair(orc, h1pa, torch, (3, 2, 3), p, curve=0.05)
+    Q = p + 1
+    nq, ne = Q ** 3, S.P.ne
+    X, _, _ = S.mesh.GetGeometricFactors(Q, M.QI_VALUES)
+    Xh = S.host(X).reshape(ne, 3, nq)
+    x, y, z = Xh[:, 0], Xh[:, 1], Xh[:, 2]
+    fq = (1.0 + x * y - z * z + np.sin(3 * x)).ravel()  # [ne][nq]
+    lf = M.LinearForm(S.fes)
+    lf.AddDomainIntegrator(M.DomainLFIntegrator(M.QuadratureFunctionCoefficient(S.dev(fq))))
+    b = lf.Assemble()
+    assert _rel(S.host(b), S.P.rhs(fq=fq)) < RTOL
+    # a constant quadrature function is the constant source
+    lf = M.LinearForm(S.fes)
+    lf.AddDomainIntegrator(M.DomainLFIntegrator(
+        M.QuadratureFunctionCoefficient(S.dev(np.full(nq * ne, 1.5)))))
+    b = lf.Assemble()
+    assert _rel(S.host(b), S.P.rhs(1.5)) < RTOL
+    # a quadrature function of the wrong size is refused
+    lf = M.LinearForm(S.fes)
+    lf.AddDomainIntegrator(M.DomainLFIntegrator(M.QuadratureFunctionCoefficient(S.dev(fq[:-1]))))
+    with pytest.raises(ValueError, match="QuadratureFunctionCoefficient"):
+        lf.Assemble()
+
+
 def test_pcg_cfg1(orc, h1pa, torch_cuda):
     """Config 1: 16^3, p = 3, k = 1, f = 1, homogeneous Dirichlet, Jacobi-PCG
     with CGSolver::SetRelTol(1e-12) semantics (preconditioned norm)."""

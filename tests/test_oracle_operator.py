@@ -207,6 +207,40 @@ def test_rhs_vs_dense(orc):
     assert abs(b.sum() - 1.0) < 1e-13  # sum of basis = 1 -> integral of f = 1
 
 
+def _qpoint_coords(orc, P):
+    """Physical coordinates [ne][nq][3] of the (p+1)^3 Gauss-Legendre points
+    (x fastest) of every element of a trilinear mesh."""
+    S, _ = qpoints(orc, P.p + 1)
+    return np.array([trilinear_map(lex_corners(orc, P.mesh, P.xyz, e), S) for e in range(P.ne)])
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_rhs_per_point_coefficient(orc, p):
+    """DomainLFIntegrator with f given at the quadrature points: a constant
+    array is the constant rhs, and on the unit cube b.u for u in the space
+    is the exact integral of f u (Gauss-Legendre with p+1 points is exact for
+    degree 2p+1 per coordinate)."""
+    P = orc.Problem(3, p)
+    nq = (p + 1) ** 3
+    assert np.array_equal(P.rhs(fq=np.full(nq * P.ne, 2.0)), P.rhs(2.0))
+    Xq = _qpoint_coords(orc, P)  # [ne][nq][3]
+    x, y, z = (Xq[..., c].ravel() for c in range(3))
+    one = np.ones(P.ndofs)
+    # linear u = x in the space
+    Xn = P.mesh.nodes_curved(P.xyz, p, 0.0).reshape(P.ne, 3, -1)
+    ux = np.zeros(P.ndofs)
+    ux[P.gather_map] = Xn[:, 0, :].ravel()
+    cases = [(y, one, 0.5), (x * y * z, one, 0.125), (y, ux, 0.25), (z * z, ux, 1.0 / 6.0)]
+    if p >= 2:
+        cases.append((x * y ** 2, ux, 1.0 / 9.0))
+    for fq, u, want in cases:
+        b = P.rhs(fq=fq)
+        assert abs(b @ u - want) < 1e-13, (p, want, b @ u)
+    # linearity in f
+    b1, b2 = P.rhs(fq=y), P.rhs(fq=z * z)
+    assert np.allclose(P.rhs(fq=2 * y - 3 * z * z), 2 * b1 - 3 * b2, rtol=0, atol=1e-14)
+
+
 def textbook_pcg(A, dinv, b, rel_tol, max_iter):
     """Jacobi-PCG with MFEM's (Br, r) stopping rule; returns (x, iterations)."""
     x = np.zeros_like(b)
