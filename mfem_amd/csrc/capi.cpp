@@ -258,7 +258,8 @@ int h1pa_space_create(h1pa_mesh *mesh, int order, h1pa_space **space) {
         r.nperm = s.nvdofs;
         r.bubble_begin = s.nvdofs + s.nedofs + s.nfdofs;
         // non-bubble rows in first-contribution order (bubble rows too were
-        // measured no faster: 0.375 vs 0.365 ms at p = 4, 32M DOFs)
+        // measured slower at p = 4, 6, 8 -- 0.394 vs 0.360 ms at p = 4, 32M
+        // DOFs -- and faster only at p = 2)
         r.nperm_lex = r.bubble_begin;
         kern::first_contribution_order(s.ne * s.dof, r.gather_map.p, r.offsets.p, r.indices.p,
                                        r.nperm_lex, r.perm_lex, st);

@@ -58,5 +58,66 @@ __device__ __forceinline__ T row_sum(const int *__restrict__ indices, int b, int
     return s;
 }
 
+// R rows at once (rows_sum): every index load of the R rows, then every value
+// load, then the sums -- R times the loads in flight of one row_sum, each
+// row's entries still added in table order.  Needs n[r] >= 1; the loads are
+// unconditional (a predicated load per entry costs more than it saves).
+template <int R, int NB, typename T>
+__device__ __forceinline__ void rows_sum_batch(const int *__restrict__ indices, const int (&b)[R],
+                                               const int (&n)[R], const T *__restrict__ xe,
+                                               bool unsigned_, T (&s)[R]) {
+    int ix[R][NB];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) ix[r][k] = indices[b[r] + (k < n[r] ? k : n[r] - 1)];
+    T v[R][NB];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int k = 0; k < NB; ++k) v[r][k] = xe[ix[r][k] >= 0 ? ix[r][k] : -1 - ix[r][k]];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        T acc = T(0);
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const T t = (ix[r][k] >= 0 || unsigned_) ? v[r][k] : -v[r][k];
+            acc = k < n[r] ? acc + t : acc;
+        }
+        s[r] = acc;
+    }
+}
+
+// s[r] = the sum of row r's entries indices[b[r], b[r] + n[r]) in table order
+// (as row_sum), for R rows of one thread.  A wave holding an empty row takes
+// row_sum row by row.
+template <int R, typename T>
+__device__ __forceinline__ void rows_sum(const int *__restrict__ indices, const int (&b)[R],
+                                         const int (&n)[R], const T *__restrict__ xe, T (&s)[R],
+                                         bool unsigned_ = false) {
+    int nmin = n[0], nmax = n[0];
+#pragma unroll
+    for (int r = 1; r < R; ++r) {
+        nmin = n[r] < nmin ? n[r] : nmin;
+        nmax = n[r] > nmax ? n[r] : nmax;
+    }
+    if (__ballot(nmin <= 0)) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) s[r] = row_sum(indices, b[r], b[r] + n[r], xe, unsigned_);
+        return;
+    }
+    if (!__ballot(nmax > 2)) rows_sum_batch<R, 2>(indices, b, n, xe, unsigned_, s);
+    else if (!__ballot(nmax > 4)) rows_sum_batch<R, 4>(indices, b, n, xe, unsigned_, s);
+    else rows_sum_batch<R, 8>(indices, b, n, xe, unsigned_, s);
+    if (__ballot(nmax > 8)) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            for (int j = b[r] + 8; j < b[r] + n[r]; ++j) {
+                const int k = indices[j];
+                s[r] += k >= 0 ? xe[k] : (unsigned_ ? xe[-1 - k] : -xe[-1 - k]);
+            }
+    }
+}
+
 }  // namespace kern
 }  // namespace h1pa
