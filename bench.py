@@ -16,9 +16,8 @@ Legs (all on by default; --legs selects):
          updates) at rel_tol 1e-10 (CGSolver's measure), p = 4 and 6.
   batch  configuration 4: 512 problems, n ~ U[4, 24], p ~ U{2..6}, vertex
          perturbation 0.15h, k = exp(8 seeded Fourier modes), RHS U[-1, 1]:
-         sequential FP64 CGSolver, FP64 batch (graph replays, and with the
-         solves of at most 60000 DOFs in persistent work-groups) and BF16
-         mixed batch; the same for 512 small problems (n ~ U[4, 8]).
+         sequential FP64 CGSolver, FP64 batch and BF16 mixed batch; the same
+         for 512 small problems (n ~ U[4, 8]).
   serve  configuration 5 (shortened arrival window): Poisson arrivals, n in
          [6, 32] with P(n) ~ 1/n, p ~ U{1..8}, 200 GB HBM budget; the native
          request loop mfem_amd/h1pa_serve: serial FP64 Jacobi-PCG (the
@@ -160,10 +159,6 @@ def _event_ms(torch, fn, reps):
     e1.synchronize()
     return e0.elapsed_time(e1) / reps
 
-
-# FP64 batch solves of at most this many DOFs run in persistent work-groups in
-# the batch leg's persistent variant
-PERSISTENT_MAX_DOFS = 60000
 
 # vector streams of one PCG iteration beyond the fused apply: the scatter's
 # (d, q) dot reads d; the update reads x, d, r, q, dinv and writes x, r, z;
@@ -420,11 +415,10 @@ def run_solve(M, torch, p, n=48, seed=7):
 # ---------------------------------------------------------------------------
 # batch (configuration 4)
 # ---------------------------------------------------------------------------
-def run_batch(M, torch, nprob=512, seed=2024, nmin=4, nmax=24, persistent_max_dofs=PERSISTENT_MAX_DOFS):
+def run_batch(M, torch, nprob=512, seed=2024, nmin=4, nmax=24):
     """Configuration 4: nprob solves, n uniform in [nmin, nmax], p in 2..6,
     Fourier coefficient, rel_tol 1e-10: sequential FP64 CGSolver, the FP64
-    batch (graph replays; and with the solves of at most persistent_max_dofs
-    DOFs in persistent work-groups), the BF16-PA mixed batch."""
+    batch and the BF16-PA mixed batch (graph replays)."""
     import numpy as np
     rng = np.random.default_rng(seed)
     ns, ps = rng.integers(nmin, nmax + 1, nprob), rng.integers(2, 7, nprob)
@@ -467,11 +461,11 @@ def run_batch(M, torch, nprob=512, seed=2024, nmin=4, nmax=24, persistent_max_do
         torch.cuda.synchronize()
         return time.perf_counter() - t0, its, xs
 
-    def batched(prec, persistent=0):
+    def batched(prec):
         # timed from the first Add: each Add does the per-solve set-up
         # (Jacobi diagonal, constraint mask, shadow PA data) that the
         # sequential CGSolver::Mult does inside its own call
-        bs = M.BatchSolver(prec, iters_per_replay=8, persistent_max_dofs=persistent)
+        bs = M.BatchSolver(prec, iters_per_replay=8)
         xs = [pr[4].clone() for pr in probs]
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -486,12 +480,7 @@ def run_batch(M, torch, nprob=512, seed=2024, nmin=4, nmax=24, persistent_max_do
 
     t_seq, its, x_seq = sequential()
     t_b64, r64, x_b64, _ = batched(M.PREC_FP64)
-    t_bp, rp, x_bp, _ = batched(M.PREC_FP64, persistent_max_dofs)
     t_bm, rm, x_bm, info = batched(M.PREC_BF16_PA)
-    npers = sum(1 for pr in probs if pr[6] <= persistent_max_dofs)
-    dp = max((torch.linalg.norm(a_ - b_) / torch.linalg.norm(b_)).item()
-             for a_, b_ in zip(x_bp, x_b64))
-    dit = max(abs(a_["iterations"] - b_["iterations"]) for a_, b_ in zip(rp, r64))
     diff = max((torch.linalg.norm(a_ - b_) / torch.linalg.norm(b_)).item()
                for a_, b_ in zip(x_bm, x_seq))
     return dict(
@@ -500,12 +489,6 @@ def run_batch(M, torch, nprob=512, seed=2024, nmin=4, nmax=24, persistent_max_do
                              iterations_mean=float(np.mean(its)), iterations_max=int(max(its))),
         batch_fp64=dict(seconds=round(t_b64, 3), solves_per_s=round(nprob / t_b64, 2),
                         converged=sum(r["converged"] for r in r64)),
-        batch_fp64_persistent=dict(seconds=round(t_bp, 3), solves_per_s=round(nprob / t_bp, 2),
-                                   converged=sum(r["converged"] for r in rp),
-                                   persistent_max_dofs=persistent_max_dofs,
-                                   persistent_solves=npers, max_rel_diff_vs_graph=dp,
-                                   max_iteration_diff_vs_graph=dit,
-                                   speedup_vs_graph=round(t_b64 / t_bp, 3)),
         batch_bf16=dict(seconds=round(t_bm, 3), solves_per_s=round(nprob / t_bm, 2),
                         converged=sum(r["converged"] for r in rm),
                         iterations_mean=float(np.mean([r["iterations"] for r in rm])),

@@ -486,12 +486,6 @@ typedef struct {
     double delta;          /* mixed: refinement trigger (as h1pa_ir_opts.delta)  */
     int max_outer;         /* mixed: refinement steps                            */
     int max_inner;         /* mixed: inner iterations in total                   */
-    int64_t persistent_max_dofs;  /* FP64 only; 0: off.  Solves of at most this
-                              many DOFs run start to finish in one persistent
-                              work-group each (one launch per (p, q1d) group
-                              on the batch's second stream) instead of the
-                              graph replays; same CGSolver steps and stopping
-                              rules, dots summed in another fixed order     */
 } h1pa_batch_opts;
 typedef struct {
     int converged;

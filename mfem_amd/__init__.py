@@ -85,8 +85,7 @@ PREC_FP64, PREC_FP32, PREC_BF16_PA = 0, 1, 2
 
 class BatchOpts(C.Structure):
     _fields_ = [("precision", _i), ("iters_per_replay", _i), ("use_graph", _i), ("autotune", _i),
-                ("delta", _d), ("max_outer", _i), ("max_inner", _i),
-                ("persistent_max_dofs", C.c_int64)]
+                ("delta", _d), ("max_outer", _i), ("max_inner", _i)]
 
 
 class BatchResult(C.Structure):
@@ -1187,10 +1186,9 @@ class BatchSolver:
     tensors are kept alive here until the solve is polled."""
 
     def __init__(self, precision=PREC_FP64, iters_per_replay=8, use_graph=True, autotune=True,
-                 delta=0.1, max_outer=200, max_inner=100000, persistent_max_dofs=0):
+                 delta=0.1, max_outer=200, max_inner=100000):
         o = BatchOpts(int(precision), int(iters_per_replay), int(bool(use_graph)),
-                      int(bool(autotune)), float(delta), int(max_outer), int(max_inner),
-                      int(persistent_max_dofs))
+                      int(bool(autotune)), float(delta), int(max_outer), int(max_inner))
         h = C.c_void_p()
         _check(lib().h1pa_batch_create(C.byref(o), C.byref(h)))
         self._h = h

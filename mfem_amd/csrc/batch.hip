@@ -43,8 +43,7 @@ enum : int {
     PH_RU = 1,      // mixed: FP64 refinement step due this iteration
     PH_DONE = 2,    // finished (converged or not)
     PH_INIT = 4,    // freshly added
-    PH_EMPTY = 5,   // slot retired
-    PH_PERSIST = 6  // FP64 solve run start to finish by one persistent work-group
+    PH_EMPTY = 5    // slot retired
 };
 
 struct SlotState {
@@ -658,173 +657,6 @@ kbm_fin_ru(const int *__restrict__ runlist, const SlotDesc *__restrict__ desc,
     }
 }
 
-// ---- persistent FP64 solves: one work-group per solve ------------------------
-// Sum of v over the NT threads of the block (any NT >= 64), the same value in
-// every thread: the values are written to red[0, NT), wave 0 adds them in a
-// fixed strided order and a butterfly, thread 0 leaves the total in red[NT].
-template <int NT>
-__device__ inline double block_sum(double v, double *red) {
-    red[threadIdx.x] = v;
-    __syncthreads();
-    if (threadIdx.x < 64) {
-        double w = 0.0;
-        for (int i = threadIdx.x; i < NT; i += 64) w += red[i];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
-        if (threadIdx.x == 0) red[NT] = w;
-    }
-    __syncthreads();
-    const double r = red[NT];
-    __syncthreads();
-    return r;
-}
-
-// CGSolver with the Jacobi smoother (linalg/solvers.cpp:869-1050, the same
-// steps and stopping rules as kb_init ... kb_dir) on one small solve per
-// work-group, from r = b to the end, in one launch: no per-iteration
-// launches, no partials in HBM, and the solve's vectors stay in the L2 of
-// the work-group's XCD between the steps.  The element kernel runs over the
-// solve's elements NB at a time with the same block shape (NB Q^2 threads);
-// the vector steps stride over the DOFs.  Every __syncthreads orders the
-// global writes of one step before the reads of the next (all waves of a
-// work-group share one CU and its L1).  The loop ends after at most max_iter
-// iterations.  Dots are block sums in a fixed order, so a solve gives bitwise
-// the same result alone and in any mix.
-template <int D, int Q, int NB, int U, int K>
-__global__ void __launch_bounds__(NB * Q * Q)
-kb_persistent(const int *__restrict__ slots, const SlotDesc *__restrict__ desc,
-              SlotState *__restrict__ states, const double *__restrict__ basis) {
-    constexpr int NT = NB * Q * Q;
-    __shared__ double smem[NB * diffusion_buf<D, Q>()];
-    __shared__ double red[NT + 1];
-    const int slot = slots[blockIdx.x];
-    const SlotDesc &sd = desc[slot];
-    SlotState *st = states + slot;
-    if (st->phase != PH_PERSIST) return;  // block-uniform
-    const int64_t n = sd.n, ne = sd.ne, bb = sd.bb, nperm = sd.nperm;
-    const int max_iter = st->max_iter;
-    double *x = global_ptr(sd.x), *r = gv<double>(sd.r), *z = gv<double>(sd.z),
-           *d = gv<double>(sd.d), *ye = gv<double>(sd.ye);
-    const double *dinv = gv<double>(sd.dinv), *b = global_ptr(sd.b);
-    const double *pa = gv<double>(sd.pa);
-    const int *gmap = global_ptr(sd.gmap), *off = global_ptr(sd.offsets),
-              *ind = global_ptr(sd.indices);
-    const int *perm = sd.perm ? global_ptr(sd.perm) : nullptr;
-    const uint8_t *mask = sd.mask ? global_ptr(sd.mask) : nullptr;
-    EOut out;
-    out.ye = ye;
-    out.l2n = global_ptr(sd.l2n);
-    if (bb < n) {  // interior DOFs straight to z
-        out.yL = z;
-        out.bb = bb;
-    }
-    // r = b, x = 0, z = dinv r, d = z, nom = (z, r)
-    double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += NT) {
-        const double ri = b[i], zi = dinv[i] * ri;
-        r[i] = ri;
-        x[i] = 0.0;
-        z[i] = zi;
-        d[i] = zi;
-        acc += zi * ri;
-    }
-    double nom = block_sum<NT>(acc, red);
-    const double nom0 = nom;
-    double r0 = nom * sd.rel_tol * sd.rel_tol;
-    if (sd.abs_tol * sd.abs_tol > r0) r0 = sd.abs_tol * sd.abs_tol;
-    double betanom = nom, beta = 0.0, den = 0.0;
-    int iter = 1, in_loop = 0, final_iter = 0, converged = 0, nonfinite = 0;
-    bool done = false;
-    if (!isfinite(nom)) {
-        nonfinite = 1;
-        done = true;
-    } else if (nom < 0.0) {
-        done = true;
-    } else if (nom <= r0) {
-        converged = 1;
-        done = true;
-    }
-    while (!done) {
-        // z = A d: elements (interior rows straight to z), then the other rows
-        for (int64_t e0 = 0; e0 < ne; e0 += NB) {
-            diffusion_elements<double, double, D, Q, NB, U, K>(e0, ne, basis, pa, gmap, mask, d,
-                                                                out, 0, smem);
-            __syncthreads();  // the next elements reuse the work-group's LDS
-        }
-        acc = 0.0;
-        for (int64_t t = threadIdx.x; t < n; t += NT) {
-            const int64_t i = t < nperm ? perm[t] : t;
-            double v;
-            if (t >= bb) {
-                v = z[i];
-            } else {
-                v = (mask && mask[i]) ? d[i] : row_sum(ind, off[i], off[i + 1], ye);
-                z[i] = v;
-            }
-            acc += d[i] * v;
-        }
-        den = block_sum<NT>(acc, red);
-        if (!isfinite(den)) {  // MFEM_VERIFY(IsFinite(den))
-            nonfinite = 1;
-            final_iter = in_loop ? iter : 0;
-            break;
-        }
-        if (den == 0.0) {  // CGSolver: den == 0 ends the loop
-            final_iter = in_loop ? iter : 0;
-            break;
-        }
-        const double alpha = nom / den;
-        acc = 0.0;
-        for (int64_t i = threadIdx.x; i < n; i += NT) {
-            x[i] = x[i] + alpha * d[i];
-            const double ri = r[i] - alpha * z[i];
-            r[i] = ri;
-            const double zi = dinv[i] * ri;
-            z[i] = zi;
-            acc += ri * zi;
-        }
-        betanom = block_sum<NT>(acc, red);
-        in_loop = 1;
-        if (!isfinite(betanom)) {
-            nonfinite = 1;
-            final_iter = iter;
-            break;
-        }
-        if (betanom < 0.0) {
-            final_iter = iter;
-            break;
-        }
-        if (betanom <= r0) {
-            converged = 1;
-            final_iter = iter;
-            break;
-        }
-        if (iter + 1 > max_iter) {
-            final_iter = max_iter;
-            break;
-        }
-        beta = betanom / nom;
-        nom = betanom;
-        iter = iter + 1;
-        for (int64_t i = threadIdx.x; i < n; i += NT) d[i] = z[i] + beta * d[i];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        st->nom = nom;
-        st->nom0 = nom0;
-        st->den = den;
-        st->betanom = betanom;
-        st->beta = beta;
-        st->r0 = r0;
-        st->iter = iter;
-        st->in_loop = in_loop;
-        st->final_iter = final_iter;
-        st->converged = converged;
-        st->nonfinite = nonfinite;
-        st->phase = PH_DONE;
-    }
-}
-
 // ---- element-kernel dispatch over (D, Q, variant) --------------------------
 template <typename T, typename TP, int D, int Q, int NB, int U, int K, bool RU, int W = 1>
 void launch_kb(const ElemTask *tasks, int ntasks, const SlotDesc *desc, const SlotState *st,
@@ -837,14 +669,6 @@ template <typename T, int D, int Q>
 constexpr int nb_max() { return nb_for(Q, 3 * D * Q * Q, sizeof(T)); }
 template <typename T, int D, int Q>
 constexpr int nb_half() { return nb_max<T, D, Q>() / 2 > 0 ? nb_max<T, D, Q>() / 2 : 1; }
-
-template <int D, int Q>
-void launch_persistent(const int *slots, int nslots, const SlotDesc *desc, SlotState *st,
-                       const double *basis, hipStream_t s) {
-    constexpr int NB = nb_max<double, D, Q>();
-    hipLaunchKernelGGL((kb_persistent<D, Q, NB, 1, 0>), dim3((unsigned)nslots), dim3(NB * Q * Q), 0,
-                       s, slots, desc, st, basis);
-}
 
 // Elements per block of variant v for value type T (same table as the
 // single-problem kernels).
@@ -934,20 +758,6 @@ void launch_group(int prec, bool ru, int D, int Q, int v, const ElemTask *tasks,
     throw Error{H1PA_EUNSUP, "batch: unsupported (p, q1d)"};
 }
 
-void launch_persistent_dq(int D, int Q, const int *slots, int nslots, const SlotDesc *desc,
-                          SlotState *st, const double *basis, hipStream_t s) {
-    if (nslots <= 0) return;
-#define X(DD, QQ)                                                                     \
-    if (D == DD && Q == QQ) {                                                         \
-        kern::launch_persistent<DD, QQ>(slots, nslots, desc, st, basis, s);           \
-        H1PA_HIP(hipGetLastError());                                                  \
-        return;                                                                       \
-    }
-    H1PA_BATCH_DQ(X)
-#undef X
-    throw Error{H1PA_EUNSUP, "batch: unsupported (p, q1d)"};
-}
-
 }  // namespace
 
 struct BatchSlot {
@@ -1011,13 +821,6 @@ struct h1pa_batch {
     // solves added since the last launch; merged into `slots` by the next one
     std::vector<std::unique_ptr<h1pa::BatchSlot>> pending;
     bool in_flight = false;
-    // persistent FP64 solves: launched on their own stream, joined before the
-    // state copy of the launch that started them
-    hipStream_t pstream = nullptr;
-    hipEvent_t pjoin = nullptr, pdone = nullptr;
-    h1pa::DevBuf<int> plist;
-    std::vector<int> h_plist;
-    int64_t persistent_launches = 0;
     h1pa::SlotState *h_pinned = nullptr;  // replay's state copy (pinned host memory)
     size_t h_pinned_n = 0;
 
@@ -1033,10 +836,6 @@ struct h1pa_batch {
         pending.clear();
         slots.clear();
         if (h_pinned) (void)hipHostFree(h_pinned);
-        if (pstream) (void)hipStreamSynchronize(pstream);
-        if (pjoin) (void)hipEventDestroy(pjoin);
-        if (pdone) (void)hipEventDestroy(pdone);
-        if (pstream) (void)hipStreamDestroy(pstream);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -1065,10 +864,7 @@ int group_for(Batch &B, Form &f) {
     return (int)B.groups.size() - 1;
 }
 
-// Phases the graph replays serve; a running solve is in one of them or runs
-// in a persistent work-group.
-bool graph_phase(int ph) { return ph == PH_INIT || ph == PH_INNER || ph == PH_RU; }
-bool running_phase(int ph) { return graph_phase(ph) || ph == PH_PERSIST; }
+bool running_phase(int ph) { return ph == PH_INIT || ph == PH_INNER || ph == PH_RU; }
 
 void drop_graph(Batch &B) {
     if (B.exec) {
@@ -1087,7 +883,7 @@ void element_tasks(const Batch &B, int g, int nb, std::vector<ElemTask> &out) {
     out.clear();
     for (size_t k = 0; k < B.slots.size(); ++k) {
         const BatchSlot &sl = *B.slots[k];
-        if (sl.group != g || !sl.live || !graph_phase(B.h_state[k].phase)) continue;
+        if (sl.group != g || !sl.live || !running_phase(B.h_state[k].phase)) continue;
         for (int64_t e = 0; e < sl.ne; e += nb) out.push_back({(int)k, (int)e});
     }
 }
@@ -1106,7 +902,7 @@ void rebuild(Batch &B) {
         desc[k] = sl.desc;
         desc[k].part0 = 0;
         desc[k].nchunks = 0;
-        if (!sl.live || !graph_phase(B.h_state[k].phase)) continue;
+        if (!sl.live || !running_phase(B.h_state[k].phase)) continue;
         const int nch = (int)((sl.n + kern::kChunk - 1) / kern::kChunk);
         desc[k].part0 = part;
         desc[k].nchunks = nch;
@@ -1287,17 +1083,9 @@ Batch *batch_create(const h1pa_batch_opts &o) {
         H1PA_CHECK(o.max_outer >= 1 && o.max_inner >= 1 && o.delta > 0.0 && o.delta < 1.0,
                    H1PA_EINVAL,
                    "batch: mixed precision needs max_outer, max_inner >= 1 and 0 < delta < 1");
-    H1PA_CHECK(o.persistent_max_dofs >= 0, H1PA_EINVAL, "batch: persistent_max_dofs must be >= 0");
-    H1PA_CHECK(o.persistent_max_dofs == 0 || o.precision == H1PA_PREC_FP64, H1PA_EUNSUP,
-               "batch: persistent solves are FP64 only");
     std::unique_ptr<Batch> B(new Batch);
     B->o = o;
     H1PA_HIP(hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking));
-    if (o.persistent_max_dofs > 0) {
-        H1PA_HIP(hipStreamCreateWithFlags(&B->pstream, hipStreamNonBlocking));
-        H1PA_HIP(hipEventCreateWithFlags(&B->pjoin, hipEventDisableTiming));
-        H1PA_HIP(hipEventCreateWithFlags(&B->pdone, hipEventDisableTiming));
-    }
     return B.release();
 }
 
@@ -1399,9 +1187,7 @@ int batch_add(Batch &B, h1pa_form *form, const uint8_t *mask_src, const double *
     sl->bytes = bytes;
     sl->live = true;
     ++f.batch_refs;  // the descriptor holds f's PA pointers until poll retires it
-    sl->init.phase = (!mixed && B.o.persistent_max_dofs > 0 && n <= B.o.persistent_max_dofs)
-                         ? PH_PERSIST
-                         : PH_INIT;
+    sl->init.phase = PH_INIT;
     sl->init.max_iter = mixed ? B.o.max_inner : po.max_iter;
     H1PA_HIP(hipEventCreateWithFlags(&sl->ready, hipEventDisableTiming));
     H1PA_HIP(hipEventRecord(sl->ready, cs));
@@ -1473,7 +1259,7 @@ bool batch_launch(Batch &B) {
         for (size_t k = 0; k < B.slots.size(); ++k) {
             const int nch = B.slots[k]->desc.nchunks;
             all += nch;
-            if (!graph_phase(B.h_state[k].phase)) dead += nch;
+            if (!running_phase(B.h_state[k].phase)) dead += nch;
         }
         if (all > 0 && 3 * dead > all) B.dirty = true;
     }
@@ -1482,39 +1268,7 @@ bool batch_launch(Batch &B) {
         autotune(B);
         if (B.dirty) rebuild(B);  // a variant with another block size was chosen
     }
-    // solves run start to finish by persistent work-groups: one launch per
-    // group on the persistent stream (after the set-up above), joined before
-    // this launch's state copy
-    bool persistent = false;
-    if (B.pstream) {
-        B.h_plist.clear();
-        std::vector<std::pair<int, int>> ranges;  // (first, count) per group
-        for (size_t g = 0; g < B.groups.size(); ++g) {
-            const int first = (int)B.h_plist.size();
-            for (size_t k = 0; k < B.slots.size(); ++k)
-                if (B.slots[k]->live && B.slots[k]->group == (int)g &&
-                    B.h_state[k].phase == PH_PERSIST)
-                    B.h_plist.push_back((int)k);
-            ranges.push_back({first, (int)B.h_plist.size() - first});
-        }
-        if (!B.h_plist.empty()) {
-            B.plist.upload(B.h_plist.data(), B.h_plist.size(), s);
-            H1PA_HIP(hipEventRecord(B.pjoin, s));
-            H1PA_HIP(hipStreamWaitEvent(B.pstream, B.pjoin, 0));
-            for (size_t g = 0; g < B.groups.size(); ++g) {
-                if (ranges[g].second == 0) continue;
-                const BatchGroup &G = B.groups[g];
-                launch_persistent_dq(G.p + 1, G.q1d, B.plist.p + ranges[g].first, ranges[g].second,
-                                     B.d_desc.p, B.d_state.p, G.basis64, B.pstream);
-            }
-            H1PA_HIP(hipEventRecord(B.pdone, B.pstream));
-            persistent = true;
-            ++B.persistent_launches;
-        }
-    }
-    if (B.nvtasks == 0) {
-        // persistent solves only: no replay
-    } else if (B.o.use_graph) {
+    if (B.o.use_graph) {
         if (!B.exec) {
             H1PA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
@@ -1548,7 +1302,6 @@ bool batch_launch(Batch &B) {
         H1PA_HIP(hipHostMalloc((void **)&B.h_pinned, cap * sizeof(SlotState), hipHostMallocDefault));
         B.h_pinned_n = cap;
     }
-    if (persistent) H1PA_HIP(hipStreamWaitEvent(s, B.pdone, 0));
     H1PA_HIP(hipMemcpyAsync(B.h_pinned, B.d_state.p, sizeof(SlotState) * B.slots.size(),
                             hipMemcpyDeviceToHost, s));
     B.in_flight = true;

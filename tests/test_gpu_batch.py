@@ -9,9 +9,6 @@ Bars, written per test:
     reduction order do not depend on the other solves).
   - Mixed batch: CGSolver's measure on the FP64 true residual <= 1e-10 and the
     solution within 1e-6 relative of the oracle's FP64 PCG solution.
-  - Persistent FP64 solves (one work-group per solve, one launch) vs the
-    graph replays: iteration counts within +-1, solutions within 1e-9; beside
-    graph-replayed solves in one batch, bitwise as alone.
 """
 import numpy as np
 import pytest
@@ -54,13 +51,10 @@ class Prob:
 SPECS = [(4, 2, 1), (6, 3, 2), (3, 5, 3), (5, 4, 4), (3, 6, 5), (7, 2, 6), (4, 3, 7)]
 
 
-def _run_batch(M, probs, precision, use_graph=True, k=8, rel_tol=1e-12, add_later=(),
-               persistent=0):
+def _run_batch(M, probs, precision, use_graph=True, k=8, rel_tol=1e-12, add_later=()):
     """Solve probs together (and add_later after two replays); returns
-    {seed: (x, result)}.  persistent: FP64 solves of at most that many DOFs
-    run in persistent work-groups."""
-    bs = M.BatchSolver(precision, iters_per_replay=k, use_graph=use_graph,
-                       persistent_max_dofs=persistent)
+    {seed: (x, result)}."""
+    bs = M.BatchSolver(precision, iters_per_replay=k, use_graph=use_graph)
     xs = {}
     for P in probs:
         xs[P.seed] = P.fresh_x()
@@ -118,35 +112,6 @@ def test_alone_equals_mix_bitwise(h1pa, torch_cuda, probs, prec):
               flush=True)
         assert ra == rm
         assert np.array_equal(xa, xm)
-
-
-def test_persistent_matches_graph(h1pa, torch_cuda, probs):
-    """FP64 solves run start to finish by persistent work-groups against the
-    graph replays: the same CGSolver steps with the dots summed in another
-    fixed order, so iteration counts within +-1 and solutions within 1e-9."""
-    g = _run_batch(h1pa, probs, h1pa.PREC_FP64)
-    q = _run_batch(h1pa, probs, h1pa.PREC_FP64, persistent=10 ** 9)
-    for P in probs:
-        (xg, rg), (xq, rq) = g[P.seed], q[P.seed]
-        print(f"[persistent] n={P.n_} p={P.p} iters graph={rg['iterations']} "
-              f"persistent={rq['iterations']} diff={_rel(xq, xg):.2e}", flush=True)
-        assert rq["converged"] and rq["status"] == h1pa.H1PA_OK
-        assert abs(rq["iterations"] - rg["iterations"]) <= 1
-        assert _rel(xq, xg) < 1e-9
-
-
-def test_persistent_hybrid_alone_equals_mix(h1pa, torch_cuda, probs):
-    """A size threshold inside the mix: the small solves persistent, the
-    others in graph replays beside them, some added after the first replays;
-    every solve gives bitwise what it gives alone."""
-    thr = sorted(P.N for P in probs)[len(probs) // 2]
-    mix = _run_batch(h1pa, probs[:4], h1pa.PREC_FP64, persistent=thr, add_later=probs[4:])
-    for Q in probs:
-        alone = _run_batch(h1pa, [Q], h1pa.PREC_FP64, persistent=thr)
-        print(f"[hybrid] n={Q.n_} p={Q.p} N={Q.N} persistent={Q.N <= thr} "
-              f"iters={mix[Q.seed][1]['iterations']}", flush=True)
-        assert alone[Q.seed][1] == mix[Q.seed][1]
-        assert np.array_equal(alone[Q.seed][0], mix[Q.seed][0])
 
 
 def test_graph_equals_plain_launches(h1pa, torch_cuda, probs):

@@ -59,22 +59,21 @@ def test_nan_rhs_single_and_mixed(h1pa, torch_cuda):
     assert ei.value.code == M.H1PA_ENONFINITE
 
 
-@pytest.mark.parametrize("prec", ["fp64", "bf16", "fp64-persistent"])
+@pytest.mark.parametrize("prec", ["fp64", "bf16"])
 def test_nan_slot_retires_alone(h1pa, torch_cuda, prec):
     M, torch = h1pa, torch_cuda
-    pr = M.PREC_BF16_PA if prec == "bf16" else M.PREC_FP64
-    pmax = 10 ** 9 if prec == "fp64-persistent" else 0
+    pr = M.PREC_FP64 if prec == "fp64" else M.PREC_BF16_PA
     probs = [_problem(M, torch, n, p, s) for (n, p, s) in [(4, 2, 11), (3, 4, 12), (5, 3, 13)]]
     bad = probs[1]
     bad["B"][_free_dof(bad)] = float("nan")
     alone = {}
     for P in (probs[0], probs[2]):
-        bs = M.BatchSolver(pr, persistent_max_dofs=pmax)
+        bs = M.BatchSolver(pr)
         x = P["X"].clone()
         bs.Add(P["A"], P["B"], x, rel_tol=1e-10, max_iter=5000, tag=P["seed"])
         assert bs.Run() == 0
         alone[P["seed"]] = (x.cpu().numpy(), dict(bs.Poll())[P["seed"]])
-    bs = M.BatchSolver(pr, persistent_max_dofs=pmax)
+    bs = M.BatchSolver(pr)
     xs = {}
     for P in probs:
         xs[P["seed"]] = P["X"].clone()
