@@ -65,7 +65,7 @@ template <int D, int Q, int NB>
 __global__ void __launch_bounds__(NB * Q * Q)
 k_diffusion_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
                  const EOut out, int accumulate) {
-    constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
+    constexpr int QQ = Q * Q, QQQ = Q * Q * Q;
     constexpr int DQQ = D * Q * Q, DDQ = D * D * Q;
     __shared__ double smem[NB * 6 * DQQ];
     const double *__restrict__ Bq = basis;
@@ -160,7 +160,7 @@ template <int D, int Q, int NB>
 __global__ void __launch_bounds__(NB * Q * Q)
 k_mass_diag(int64_t ne, const double *__restrict__ basis, const double *__restrict__ pa,
             const EOut out, int accumulate) {
-    constexpr int QQ = Q * Q, QQQ = Q * Q * Q, DDD = D * D * D;
+    constexpr int QQ = Q * Q, QQQ = Q * Q * Q;
     constexpr int DQQ = D * Q * Q;
     __shared__ double smem[NB * DQQ];
     const double *__restrict__ Bq = basis;

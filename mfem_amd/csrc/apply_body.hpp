@@ -604,6 +604,14 @@ mass_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *
     const int64_t e = e0 + le;
     const bool valid = e < ne;
     T *s = smem + le * BUF;
+    // the column's Q PA values are loaded first, so that their round trip
+    // overlaps the gather and the x and y stages instead of the z stage
+    T pz[Q];
+    if (valid) {
+        const TP *pcol = pa + (int64_t)QQQ * e + a + Q * b;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) pz[q] = load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * q);
+    }
     // forward 1D contraction of a line held in even-odd form, B only
     auto fwd = [&](const T (&le_)[HD > 0 ? HD : 1], const T (&lo_)[HD > 0 ? HD : 1], T lm, int q,
                    T &lo, T &hi) {
@@ -678,18 +686,17 @@ mass_elements_eo(int64_t e0, int64_t ne, const T *__restrict__ basis, const TP *
         }
         __syncthreads();
         if (valid) {
-            const TP *pcol = pa + (int64_t)QQQ * e + a + Q * b;
             T Ez[HD > 0 ? HD : 1], Fz[HD > 0 ? HD : 1], Mz = T(0);
 #pragma unroll
             for (int d = 0; d < HD; ++d) Ez[d] = Fz[d] = T(0);
-#pragma unroll 1
+#pragma unroll
             for (int j = 0; j < R; ++j) {
                 T lo, hi;
                 fwd(ze, zo, zm, j, lo, hi);
-                const T v0 = lo * load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * j);
+                const T v0 = lo * pz[j];
                 T P, N;
                 if (j < HQ) {
-                    const T v1 = hi * load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * (Q - 1 - j));
+                    const T v1 = hi * pz[Q - 1 - j];
                     P = v0 + v1;
                     N = v0 - v1;
                 } else {
@@ -1058,6 +1065,14 @@ mass_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const 
     const int64_t e = e0 + le;
     const bool valid = e < ne;
     T *s = smem + le * BUF;
+    // the column's Q PA values first (their round trip overlaps the gather
+    // and the x and y stages)
+    T pz[Q];
+    if (valid) {
+        const TP *pcol = pa + (int64_t)QQQ * e + a + Q * b;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) pz[q] = load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * q);
+    }
     if (valid && a < D && b < D) {
         T u[D];
         const int64_t base = (int64_t)DDD * e + D * (a + D * b);
@@ -1097,17 +1112,16 @@ mass_elements_direct(int64_t e0, int64_t ne, const T *__restrict__ basis, const 
             for (int dz = 0; dz < D; ++dz) z[dz] = s[a + Q * (b + Q * dz)];
         __syncthreads();
         if (valid) {
-            const TP *pcol = pa + (int64_t)QQQ * e + a + Q * b;
             T tz[D];
 #pragma unroll
             for (int dz = 0; dz < D; ++dz) tz[dz] = 0.0;
-#pragma unroll 1
+#pragma unroll
             for (int qz = 0; qz < Q; ++qz) {
                 const T *br = Bd + D * qz;
                 T v = 0.0;
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) v = fma(br[dz], z[dz], v);
-                v *= load_pa<T, stream_pa<Q, TP>()>(pcol + QQ * qz);
+                v *= pz[qz];
 #pragma unroll
                 for (int dz = 0; dz < D; ++dz) tz[dz] = fma(br[dz], v, tz[dz]);
             }
