@@ -403,10 +403,13 @@ typedef struct {
                                 1: OperatorJacobiSmoother on the coarse level    */
 } h1pa_mg_opts;
 /* orders[0 .. nlevels-1]: increasing, orders[nlevels-1] = the fine space's
- * order (2 <= nlevels <= 8).  The fine form must be assembled and stay alive;
- * a QuadratureFunction coefficient (tied to the fine rule) gives
- * H1PA_EUNSUP.  The coarser spaces, forms, smoothers and transfers are made
- * and assembled here, on `stream`. */
+ * order (2 <= nlevels <= 8).  The fine form must be assembled; the hierarchy
+ * holds it until h1pa_multigrid_destroy (meanwhile h1pa_form_assemble,
+ * h1pa_form_destroy and a reduced-precision shadow of another kind on it
+ * give H1PA_EINVAL, as for a form in a live batch solve).  A
+ * QuadratureFunction coefficient (tied to the fine rule) gives H1PA_EUNSUP.
+ * The coarser spaces, forms, smoothers and transfers are made and assembled
+ * here, on `stream`. */
 int h1pa_multigrid_create(h1pa_form *fine, int nlevels, const int *orders,
                           const h1pa_mg_opts *opts, h1pa_multigrid **mg, void *stream);
 int h1pa_multigrid_destroy(h1pa_multigrid *mg);

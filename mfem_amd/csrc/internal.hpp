@@ -180,17 +180,18 @@ struct Form {
     DevBuf<uint32_t> pa_diff_lp, pa_mass_lp;  // raw storage, 4-byte aligned
     int lp_kind = -1;
     DevBuf<float> ework32;
-    // Batch solves (added, not yet retired by poll) whose descriptors hold
-    // raw pointers to this form's PA data: while nonzero, calls that would
-    // free or move those buffers (Assemble, a reduced-precision shadow of the
-    // other kind, destroy) are refused.
+    // Batch solves (added, not yet retired by poll) and multigrid
+    // hierarchies that hold raw pointers to this form's PA data: while
+    // nonzero, calls that would free or move those buffers (Assemble, a
+    // reduced-precision shadow of the other kind, destroy) are refused.
     std::atomic<int> batch_refs{0};
     // fused column apply (Cartesian meshes, diffusion only, q1d = p + 2)
     ColumnGrid cols;
     DevBuf<double> iface;
     int use_columns = -1;  // -1 not decided yet, 0 no, 1 yes
 };
-// Refuses (H1PA_EINVAL) when live batch solves reference f's device data.
+// Refuses (H1PA_EINVAL) when live batch solves or multigrid hierarchies
+// reference f's device data.
 void require_no_batch_refs(const Form &f, const char *what);
 
 // ---- kernel launchers (HIP translation units) ------------------------------
@@ -413,6 +414,7 @@ struct Multigrid {
     DevBuf<double> coarse_dinv;                // ones: CG without a preconditioner
     PcgWork coarse_work, outer_work;           // kept between cycles and solves
     int64_t cycles = 0, coarse_solves = 0, coarse_iterations = 0;
+    Form *fine_ref = nullptr;  // the caller's fine form, held (batch_refs) while this lives
     ~Multigrid();
 };
 Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1pa_mg_opts &o,

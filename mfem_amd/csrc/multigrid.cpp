@@ -81,6 +81,7 @@ void Transfer::mult_transpose(const double *xh, double *yl, hipStream_t s) {
 
 // ---- GeometricMultigrid with order-refined levels ---------------------------
 Multigrid::~Multigrid() {
+    if (fine_ref) --fine_ref->batch_refs;
     P.clear();
     for (auto it = lv.rbegin(); it != lv.rend(); ++it) {
         it->smoother.reset();
@@ -169,6 +170,10 @@ Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1
         fill(C.sp->ndofs, 1.0, mg->coarse_dinv.p, s);
     }
     H1PA_HIP(hipStreamSynchronize(s));
+    // the finest level reads the caller's form: hold it (Assemble, destroy
+    // or a reduced-precision shadow of the other kind are refused meanwhile)
+    mg->fine_ref = &fine;
+    ++fine.batch_refs;
     return mg.release();
 }
 

@@ -1145,7 +1145,8 @@ void require_no_batch_refs(const Form &f, const char *what) {
     const int n = f.batch_refs.load();
     H1PA_CHECK(n == 0, H1PA_EINVAL,
                std::string(what) + ": the form's device data are in use by " + std::to_string(n) +
-                   " batch solve(s) not yet retired (h1pa_batch_poll)");
+                   " batch solve(s) not yet retired (h1pa_batch_poll) or multigrid "
+                   "hierarchies not yet destroyed (h1pa_multigrid_destroy)");
 }
 
 void ensure_low_precision(Form &f, int precision, hipStream_t s) {

@@ -162,6 +162,15 @@ def test_multigrid_argument_errors(h1pa, torch_cuda):
     with pytest.raises(M.H1paError) as e:
         M.PMultigrid(aq, (1, 3))
     assert e.value.code == M.H1PA_EUNSUP
+    # the hierarchy holds the fine form: re-assembly and destroy are refused
+    # until the hierarchy is destroyed
+    mg = M.PMultigrid(a, (1, 3))
+    with pytest.raises(M.H1paError) as e:
+        a.Assemble()
+    assert e.value.code == M.H1PA_EINVAL and "multigrid" in str(e.value)
+    assert M.lib().h1pa_form_destroy(a._single()._h) == M.H1PA_EINVAL
+    del mg
+    a.Assemble()
 
 
 def test_bf16_cycle_matches_oracle(orc, h1pa, torch_cuda):
