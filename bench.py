@@ -400,7 +400,7 @@ def run_solve(M, torch, p, n=48, seed=7):
                                            / torch.linalg.norm(X64)).item(),
                          speedup_vs_fp64=round(t64 / tm, 3),
                          speedup_vs_fp64_with_setup=round(t64 / (tm + t_setup), 3))
-        del mg
+        del mg, cg  # the hierarchy holds the form until it is destroyed
     for prec, name in ((M.PREC_FP32, "fp32"), (M.PREC_BF16_PA, "bf16")):
         mixed(prec, 20)
         ir, Xi, ti = mixed(prec, 100000)

@@ -353,7 +353,7 @@ int h1pa_restriction_create(int ne, int dof, int ndofs, const int *h_elem_dof, c
         R.dof = dof;
         R.ndofs = ndofs;
         hipStream_t s = nullptr;
-        DevBuf<int> ed, dm;
+        StreamBuf<int> ed, dm;
         ed.upload(h_elem_dof, (size_t)ne * dof, s);
         if (h_dof_map) dm.upload(h_dof_map, dof, s);
         R.gather_map.alloc((size_t)ne * dof);
@@ -507,7 +507,7 @@ void autotune_diffusion(Form &f, hipStream_t s) {
     if ((at && at[0] == '0') || (sp.ne < 20000 && sp.ndofs < 500000)) return;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     if (g_tuned[p][Q]) return;
-    DevBuf<double> x(sp.ndofs);
+    StreamBuf<double> x(sp.ndofs, s);
     H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * sp.ndofs, s));
     hipEvent_t e0, e1;
     H1PA_HIP(hipEventCreate(&e0));
@@ -665,6 +665,27 @@ const uint8_t *constraint_mask(Form &f, const int *d_ess, int64_t n_ess, hipStre
 }
 
 }  // namespace
+
+// The default pool of each device keeps the blocks StreamBuf frees (release
+// threshold at its maximum), so a temporary's hipMallocAsync after the first
+// reuses memory instead of going back to the driver.
+void h1pa::keep_stream_pool() {
+    static std::atomic<uint64_t> done{0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return;
+    }
+    const uint64_t bit = uint64_t(1) << dev;
+    if (done.load(std::memory_order_acquire) & bit) return;
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess && pool) {
+        uint64_t threshold = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+    }
+    (void)hipGetLastError();
+    done.fetch_or(bit, std::memory_order_acq_rel);
+}
 
 // Jacobi diagonal of the unconstrained operator into diag (L-vector): the
 // element diagonals in the fused layout (bubble DOFs straight into diag, the
@@ -924,16 +945,16 @@ int h1pa_form_linear_system(h1pa_form *form, const int *d_ess, int64_t n_ess, co
     require_assembled(f);
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
-    DevBuf<uint8_t> zero_mask;
+    StreamBuf<uint8_t> zero_mask;
     const uint8_t *mask;
     if (n_ess > 0) {
         mask = constraint_mask(f, d_ess, n_ess, s);
     } else {
-        zero_mask.alloc(sp.ndofs);
+        zero_mask.alloc(sp.ndofs, s);
         H1PA_HIP(hipMemsetAsync(zero_mask.p, 0, sp.ndofs, s));
         mask = zero_mask.p;
     }
-    DevBuf<double> z(sp.ndofs);
+    StreamBuf<double> z(sp.ndofs, s);
     kern::form_x(sp.ndofs, mask, x, X, s);  // X = x on ess, 0 elsewhere (= w)
     element_apply(f, sp.restr.gather_map.p, nullptr, X, f.ework0.p, z.p, false, s);
     kern::restrict_mult_transpose(sp.restr.bubble_begin, sp.restr.offsets.p,
@@ -970,7 +991,7 @@ int h1pa_space_domain_lf_qf(h1pa_space *space, double fval, const double *d_fq, 
     Space &sp = space->s;
     hipStream_t s = as_stream(stream);
     sp.mesh->ensure_nodes(s);
-    DevBuf<double> bE(sp.ne * sp.dof);
+    StreamBuf<double> bE(sp.ne * sp.dof, s);
     kern::domain_lf(*sp.mesh, sp.p, fval, d_fq, bE.p, s);
     kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, bE.p, b, 0,
                                   sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
@@ -988,7 +1009,7 @@ int h1pa_pcg_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const doubl
     Space &sp = *f.space;
     hipStream_t s = as_stream(stream);
     const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
-    DevBuf<double> diag(sp.ndofs);
+    StreamBuf<double> diag(sp.ndofs, s);
     operator_diagonal(f, diag.p, s);
     pcg_solve(f, mask, diag.p, b, x, *opts, *stats, s);
     if (!stats->converged) {
@@ -1094,7 +1115,7 @@ int h1pa_ir_solve(h1pa_form *form, const int *d_ess, int64_t n_ess, const double
                "ir_solve: reduced-precision applies support q1d = p + 2 only");
     hipStream_t s = as_stream(stream);
     const uint8_t *mask = n_ess > 0 ? constraint_mask(f, d_ess, n_ess, s) : nullptr;
-    DevBuf<double> diag(sp.ndofs);
+    StreamBuf<double> diag(sp.ndofs, s);
     operator_diagonal(f, diag.p, s);
     ir_solve(f, mask, diag.p, b, x, *opts, *stats, s);
     if (!stats->converged) {

@@ -260,7 +260,7 @@ __global__ void k_domain_lf(int64_t ne, int D, GeomBasis gb, const double *__res
 }  // namespace
 
 void nodes_trilinear(const Mesh &m, double *d_nodes, hipStream_t s) {
-    DevBuf<double> xyz;
+    StreamBuf<double> xyz;
     xyz.upload(m.xyz.data(), m.xyz.size(), s);
     hipLaunchKernelGGL(k_nodes_trilinear, dim3(nblocks(m.ne * 8)), dim3(kBlock), 0, s, m.ne,
                        m.d_topo.p, xyz.p, d_nodes);
@@ -272,7 +272,7 @@ void nodes_curved(const Mesh &m, int pg, double amp, double *d_nodes, hipStream_
     const int Dg = pg + 1;
     double z[16], w[16];
     gauss_lobatto(Dg, z, w);
-    DevBuf<double> xyz, gll;
+    StreamBuf<double> xyz, gll;
     xyz.upload(m.xyz.data(), m.xyz.size(), s);
     gll.upload(z, Dg, s);
     hipLaunchKernelGGL(k_nodes_curved, dim3(nblocks(m.ne * Dg * Dg * Dg)), dim3(kBlock), 0, s, m.ne,
@@ -316,7 +316,7 @@ void domain_lf(const Mesh &m, int p, double f, const double *fq, double *bE, hip
     double x[16], w[16], B[256], G[256];
     gauss_legendre(Q, x, w);
     lagrange_basis(p, Q, x, B, G);
-    DevBuf<double> Bs;
+    StreamBuf<double> Bs;
     Bs.upload(B, Q * D, s);
     hipLaunchKernelGGL(k_domain_lf, dim3((unsigned)m.ne), dim3(256), 0, s, m.ne, D, gb, Bs.p,
                        m.d_nodes.p, f, fq, bE);

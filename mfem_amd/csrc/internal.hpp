@@ -74,6 +74,46 @@ struct DevBuf {
     }
 };
 
+// Device memory ordered on one stream (hipMallocAsync / hipFreeAsync from the
+// device's default pool, which keeps freed blocks for reuse): for
+// temporaries that only that stream uses.  Releasing one does not wait: a
+// hipFree synchronizes the whole device, which stalls a set-up or a solve
+// running beside others on their own streams.
+void keep_stream_pool();  // the default pool keeps freed memory (once per device)
+template <typename T>
+struct StreamBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipStream_t s = nullptr;
+    StreamBuf() = default;
+    StreamBuf(size_t count, hipStream_t st) { alloc(count, st); }
+    StreamBuf(const StreamBuf &) = delete;
+    StreamBuf &operator=(const StreamBuf &) = delete;
+    ~StreamBuf() { release(); }
+    void alloc(size_t count, hipStream_t st) {
+        release();
+        s = st;
+        if (count == 0) return;
+        keep_stream_pool();
+        if (hipMallocAsync(reinterpret_cast<void **>(&p), count * sizeof(T), st) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            throw Error{H1PA_ENOMEM, "hipMallocAsync of " + std::to_string(count * sizeof(T)) +
+                                         " bytes failed"};
+        }
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFreeAsync(p, s);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T *h, size_t count, hipStream_t st) {
+        if (count > n || st != s) alloc(count, st);
+        if (count) H1PA_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+    }
+};
+
 // ---- 1D rules and basis (host, long double) --------------------------------
 void gauss_legendre(int n, double *x, double *w);
 void gauss_lobatto(int n, double *x, double *w);
@@ -340,6 +380,10 @@ struct Chebyshev {
     double max_eig = 0.0;
     double coeffs[5] = {0, 0, 0, 0, 0};
     DevBuf<double> dinv, t, u;
+    // power-method work kept with the smoother: freeing device memory waits
+    // for the whole device, which stalls a set-up that runs beside solves
+    DevBuf<double> w, partials;
+    DevBuf<uint32_t> starts;  // the start vector's generator states
 };
 // dinv from d_diag (null: the form's assembled diagonal), the eigenvalue
 // estimate (max_eig > 0: given; else the power method), the coefficients.

@@ -161,11 +161,9 @@ Multigrid *multigrid_create(Form &fine, int nlevels, const int *orders, const h1
     // dinv = 1 gives z = r, the same iteration), or with OperatorJacobiSmoother
     MgLevel &C = mg->lv[0];
     mg->coarse_dinv.alloc(C.sp->ndofs);
-    if (o.coarse_jacobi) {
-        DevBuf<double> diag(C.sp->ndofs);
-        operator_diagonal(*C.f, diag.p, s);
-        kern::jacobi_dinv(C.sp->ndofs, diag.p, C.mask, mg->coarse_dinv.p, s);
-        H1PA_HIP(hipStreamSynchronize(s));
+    if (o.coarse_jacobi) {  // the diagonal through the coarse level's X (free until a cycle)
+        operator_diagonal(*C.f, C.X.p, s);
+        kern::jacobi_dinv(C.sp->ndofs, C.X.p, C.mask, mg->coarse_dinv.p, s);
     } else {
         fill(C.sp->ndofs, 1.0, mg->coarse_dinv.p, s);
     }

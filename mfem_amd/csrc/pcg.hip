@@ -19,6 +19,7 @@
 // and the result does not depend on timing.  The vector type T is double or
 // float; scalars and dot accumulation are double.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -1003,9 +1004,9 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
     const unsigned nb = ch.blocks;
     const int per = ch.per;
     // work vectors: the caller's (FP64, kept between solves) or this solve's own
-    DevBuf<T> r_own, z_own, d_own;
-    DevBuf<double> part_own;
-    DevBuf<PcgState> st_own;
+    StreamBuf<T> r_own, z_own, d_own;
+    StreamBuf<double> part_own;
+    StreamBuf<PcgState> st_own;
     T *r = nullptr, *z = nullptr, *d = nullptr;
     double *partials = nullptr;
     PcgState *st = nullptr;
@@ -1026,11 +1027,11 @@ void pcg_run(Form &f, int prec, const uint8_t *mask, const T *dinv, const T *b, 
         }
     }
     if (!r) {
-        r_own.alloc(n);
-        z_own.alloc(n);
-        d_own.alloc(n);
-        part_own.alloc(nb);
-        st_own.alloc(1);
+        r_own.alloc(n, s);
+        z_own.alloc(n, s);
+        d_own.alloc(n, s);
+        part_own.alloc(nb, s);
+        st_own.alloc(1, s);
         r = r_own.p;
         z = z_own.p;
         d = d_own.p;
@@ -1108,7 +1109,7 @@ void autotune_lp(Form &f, int kind, hipStream_t s) {
     std::lock_guard<std::mutex> lock(g_lp_tune_mutex);
     if (g_lp_tuned[kind][p][Q]) return;
     g_lp_tuned[kind][p][Q] = true;
-    DevBuf<float> x(sp.ndofs);
+    StreamBuf<float> x(sp.ndofs, s);
     H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * sp.ndofs, s));
     hipEvent_t e0, e1;
     H1PA_HIP(hipEventCreate(&e0));
@@ -1201,7 +1202,7 @@ void form_mult_lp(Form &f, int precision, const float *x, float *y, hipStream_t 
 void pcg_solve(Form &f, const uint8_t *mask, const double *diag, const double *b, double *x,
                const h1pa_pcg_opts &o, h1pa_pcg_stats &stats, hipStream_t s) {
     Space &sp = *f.space;
-    DevBuf<double> dinv(sp.ndofs);
+    StreamBuf<double> dinv(sp.ndofs, s);
     kern::jacobi_dinv(sp.ndofs, diag, mask, dinv.p, s);
     pcg_run<double>(f, H1PA_PREC_FP64, mask, dinv.p, b, x, f.ework0.p, o, stats, s);
 }
@@ -1228,21 +1229,89 @@ void constrained_apply(Form &f, const uint8_t *mask, const double *v, double *u,
 namespace {
 
 // Vector::Randomize(seed) (linalg/vector.cpp:955-966): srand(seed), then
-// rand() / (RAND_MAX + 1) per entry.  Drawn from the C library's own
-// generator through its reentrant interface with a private 128-byte state
-// (glibc's rand() is that generator on a 128-byte state), so the caller's
-// rand() sequence is left alone.
-void randomize_like_mfem(int seed, std::vector<double> &v) {
+// rand() / (RAND_MAX + 1) per entry, drawn on the device.  glibc's rand() on
+// its 128-byte state is the additive generator h[k] = h[k-31] + h[k-3]
+// (mod 2^32) returning h[k] >> 1; the library's reentrant interface
+// (initstate_r, with a private state, so the caller's rand() sequence is
+// left alone) gives the 31 words h[k-31 .. k-1] before the first draw.  The
+// recurrence is linear over the integers mod 2^32, so the state c L draws
+// later is A^L applied c times (A the 31 x 31 step matrix, A^L by repeated
+// squaring): the host computes one start state per chunk of L draws and each
+// device thread runs the recurrence over its chunk -- the same draws as n
+// calls of rand(), without generating and uploading n values on the host.
+using Mat31 = std::array<uint32_t, 31 * 31>;
+
+Mat31 mat31_mul(const Mat31 &a, const Mat31 &b) {
+    Mat31 c{};
+    for (int i = 0; i < 31; ++i)
+        for (int k = 0; k < 31; ++k) {
+            const uint32_t aik = a[31 * i + k];
+            if (!aik) continue;
+            for (int j = 0; j < 31; ++j) c[31 * i + j] += aik * b[31 * k + j];
+        }
+    return c;
+}
+
+__global__ void k_glibc_rand(int64_t n, int64_t L, const uint32_t *__restrict__ start,
+                             double *__restrict__ v) {
+    const int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = c * L;
+    if (i >= n) return;
+    const int64_t e = i + L < n ? i + L : n;
+    uint32_t h[31];  // h[j] = h[k - 31 + j] at the start of each block of 31 draws
+#pragma unroll
+    for (int j = 0; j < 31; ++j) h[j] = start[31 * c + j];
+    for (; i < e; i += 31) {
+#pragma unroll
+        for (int j = 0; j < 31; ++j) {
+            h[j] += h[(j + 28) % 31];  // h[k] = h[k - 31] + h[k - 3]
+            if (i + j < e) v[i + j] = (double)(h[j] >> 1) * (1.0 / 2147483648.0);
+        }
+    }
+}
+
+void randomize_like_mfem(int seed, int64_t n, double *v, DevBuf<uint32_t> &d_starts,
+                         hipStream_t s) {
+    if (n <= 0) return;
     struct random_data rd;
     std::memset(&rd, 0, sizeof(rd));
     char state[128];
     H1PA_CHECK(initstate_r((unsigned)seed, state, sizeof(state), &rd) == 0, H1PA_EINVAL,
                "chebyshev: initstate_r failed");
-    for (double &x : v) {
-        int32_t r = 0;
-        random_r(&rd, &r);
-        x = (double)r / ((double)RAND_MAX + 1.0);
+    const int f = (int)(rd.fptr - rd.state), r = (int)(rd.rptr - rd.state);
+    H1PA_CHECK(rd.rand_type == 3 && rd.rand_deg == 31 && rd.rand_sep == 3 &&
+                   ((f - r) % 31 + 31) % 31 == 3,
+               H1PA_EUNSUP, "chebyshev: unexpected C library generator");
+    std::array<uint32_t, 31> x;  // h[k - 31 .. k - 1]
+    for (int j = 0; j < 31; ++j) x[j] = (uint32_t)rd.state[(f + j) % 31];
+    // chunks of L draws, at most 4096 of them
+    const int64_t L = std::max<int64_t>(31 * 64, (n + 4095) / 4096);
+    const int64_t T = (n + L - 1) / L;
+    Mat31 step{}, AL{};
+    for (int i = 0; i < 30; ++i) step[31 * i + i + 1] = 1;
+    step[31 * 30 + 0] = 1;
+    step[31 * 30 + 28] = 1;
+    for (int i = 0; i < 31; ++i) AL[31 * i + i] = 1;
+    for (int64_t e = L; e; e >>= 1) {  // AL = step^L
+        if (e & 1) AL = mat31_mul(AL, step);
+        if (e > 1) step = mat31_mul(step, step);
     }
+    std::vector<uint32_t> starts((size_t)T * 31);
+    for (int64_t c = 0; c < T; ++c) {
+        std::copy(x.begin(), x.end(), starts.begin() + 31 * c);
+        std::array<uint32_t, 31> y{};
+        for (int i = 0; i < 31; ++i) {
+            uint32_t acc = 0;
+            for (int k = 0; k < 31; ++k) acc += AL[31 * i + k] * x[k];
+            y[i] = acc;
+        }
+        x = y;
+    }
+    d_starts.upload(starts.data(), starts.size(), s);
+    hipLaunchKernelGGL(k_glibc_rand, dim3((unsigned)((T + 63) / 64)), dim3(64), 0, s, n, L,
+                       d_starts.p, v);
+    H1PA_HIP(hipGetLastError());
+    H1PA_HIP(hipStreamSynchronize(s));  // the host copy `starts` is freed on return
 }
 
 // Coefficients of OperatorChebyshevSmoother::Setup (solvers.cpp:552-628):
@@ -1321,16 +1390,14 @@ void chebyshev_setup(Chebyshev &c, const double *d_diag, double max_eig, int pow
     Form &f = *c.f;
     Space &sp = *f.space;
     const int64_t n = sp.ndofs;
-    DevBuf<double> diag;
-    if (!d_diag) {
-        diag.alloc(n);
-        operator_diagonal(f, diag.p, s);
-        d_diag = diag.p;
-    }
     c.dinv.alloc(n);
-    kern::jacobi_dinv(n, d_diag, c.mask, c.dinv.p, s);
     c.t.alloc(n);
     c.u.alloc(n);
+    if (!d_diag) {  // the assembled diagonal, through u (free until the first Mult)
+        operator_diagonal(f, c.u.p, s);
+        d_diag = c.u.p;
+    }
+    kern::jacobi_dinv(n, d_diag, c.mask, c.dinv.p, s);
     if (max_eig > 0.0) {
         c.max_eig = max_eig;
     } else {
@@ -1341,33 +1408,34 @@ void chebyshev_setup(Chebyshev &c, const double *d_diag, double max_eig, int pow
         // each step: v /= |v|, u = A v, w = D^-1 u, eig = (v, w), v <- w; the
         // state (norm, estimate, stop) stays on the device and every launch
         // after the stop does nothing, so the host waits once, at the end
-        std::vector<double> h(n);
-        randomize_like_mfem(power_seed, h);
         const kern::Chunks ch = kern::chunks(n);
-        DevBuf<double> v0(n), v1(n), partials(ch.blocks);
-        DevBuf<kern::PowerState> st(1);
-        v0.upload(h.data(), n, s);
+        // v0 in t, v1 in w, the state in the partials' tail
+        c.w.alloc(n);
+        c.partials.alloc(ch.blocks + (sizeof(kern::PowerState) + 7) / 8);
+        double *const partials = c.partials.p;
+        kern::PowerState *const st = reinterpret_cast<kern::PowerState *>(c.partials.p + ch.blocks);
+        double *const v0 = c.t.p, *const v1 = c.w.p;
+        randomize_like_mfem(power_seed, n, v0, c.starts, s);
         const kern::PowerState h0{0.0, 1.0, 0};
-        H1PA_HIP(hipMemcpyAsync(st.p, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
-        const int *stop = &st.p->done;
+        H1PA_HIP(hipMemcpyAsync(st, &h0, sizeof(h0), hipMemcpyHostToDevice, s));
+        const int *stop = &st->done;
         const dim3 grid(ch.blocks), blk(kern::kThreads), one(1);
-        double *a = v0.p, *b = v1.p;
+        double *a = v0, *b = v1;
         for (int it = 0; it < power_iterations; ++it) {
-            hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, ch.per, a, a, partials.p, stop);
-            hipLaunchKernelGGL(kern::k_power_nrm, one, blk, 0, s, (int)ch.blocks, partials.p,
-                               st.p);
-            hipLaunchKernelGGL(kern::k_power_normalize, grid, blk, 0, s, n, ch.per, a, st.p);
+            hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, ch.per, a, a, partials, stop);
+            hipLaunchKernelGGL(kern::k_power_nrm, one, blk, 0, s, (int)ch.blocks, partials, st);
+            hipLaunchKernelGGL(kern::k_power_normalize, grid, blk, 0, s, n, ch.per, a, st);
             constrained_apply(*c.f, c.mask, a, c.u.p, stop, s, c.prec);
             hipLaunchKernelGGL(kern::k_jacobi_mult, grid, blk, 0, s, n, ch.per, c.dinv.p, c.u.p,
                                b);
-            hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, ch.per, a, b, partials.p, stop);
-            hipLaunchKernelGGL(kern::k_power_eig, one, blk, 0, s, (int)ch.blocks, partials.p,
-                               st.p, power_tolerance);
+            hipLaunchKernelGGL(kern::k_dot, grid, blk, 0, s, n, ch.per, a, b, partials, stop);
+            hipLaunchKernelGGL(kern::k_power_eig, one, blk, 0, s, (int)ch.blocks, partials, st,
+                               power_tolerance);
             H1PA_HIP(hipGetLastError());
             std::swap(a, b);
         }
         kern::PowerState hs{};
-        H1PA_HIP(hipMemcpyAsync(&hs, st.p, sizeof(hs), hipMemcpyDeviceToHost, s));
+        H1PA_HIP(hipMemcpyAsync(&hs, st, sizeof(hs), hipMemcpyDeviceToHost, s));
         H1PA_HIP(hipStreamSynchronize(s));
         c.max_eig = hs.eig;
     }
@@ -1473,10 +1541,10 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
     const int per = ch.per;
     const unsigned nbx = std::min<unsigned>(nb, 1024);
     ensure_low_precision(f, o.inner_precision, s);
-    DevBuf<float> dinv(n), r(n), xs(n), z(n), d(n), q(n);
-    DevBuf<double> dinv64(n), r64(n);
-    DevBuf<double> partials(3 * (size_t)nb);
-    DevBuf<kern::IrState> st(1);
+    StreamBuf<float> dinv(n, s), r(n, s), xs(n, s), z(n, s), d(n, s), q(n, s);
+    StreamBuf<double> dinv64(n, s), r64(n, s);
+    StreamBuf<double> partials(3 * (size_t)nb, s);
+    StreamBuf<kern::IrState> st(1, s);
     hipLaunchKernelGGL(kern::k_jacobi_dinv<float>, dim3(nb256), dim3(256), 0, s, n, diag, mask,
                        dinv.p);
     hipLaunchKernelGGL(kern::k_jacobi_dinv<double>, dim3(nb256), dim3(256), 0, s, n, diag, mask,

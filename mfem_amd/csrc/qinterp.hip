@@ -171,22 +171,22 @@ void qinterp(Space &sp, int q1d, int vdim, unsigned flags, int by_vdim, const do
     std::vector<double> xq(Q), wq(Q), h(2 * Q * D);
     gauss_legendre(Q, xq.data(), wq.data());
     lagrange_basis(p, Q, xq.data(), h.data(), h.data() + Q * D);
-    DevBuf<double> basis(2 * Q * D);
+    StreamBuf<double> basis;
     basis.upload(h.data(), h.size(), s);
-    DevBuf<double> J;
+    StreamBuf<double> J;
     if (phys) {
         sp.mesh->ensure_nodes(s);
-        J.alloc(nq * 9 * ne);
+        J.alloc(nq * 9 * ne, s);
         geometric_factors(*sp.mesh, q1d, nullptr, J.p, nullptr, s);
     }
     // determinants need byNODES reference derivatives
-    DevBuf<double> dtmp;
+    StreamBuf<double> dtmp;
     const double *dref = nullptr;
     if (want_det) {
         if ((flags & H1PA_QI_DERIVATIVES) && !by_vdim) {
             dref = der;
         } else {
-            dtmp.alloc(nq * 9 * ne);
+            dtmp.alloc(nq * 9 * ne, s);
             dref = dtmp.p;
         }
     }

@@ -181,14 +181,13 @@ __global__ void k_scan_add(int *__restrict__ a, int64_t n, const int *__restrict
 void inclusive_scan(int *a, int64_t n, hipStream_t s) {
     if (n <= 0) return;
     const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-    DevBuf<int> sums(tiles);
+    StreamBuf<int> sums(tiles, s);
     hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, a, n, sums.p);
     if (tiles > 1) {
         hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kScanThreads), 0, s, sums.p, tiles);
         hipLaunchKernelGGL(k_scan_add, dim3((unsigned)tiles), dim3(kScanThreads), 0, s, a, n, sums.p);
     }
     H1PA_HIP(hipGetLastError());
-    H1PA_HIP(hipStreamSynchronize(s));  // `sums` is freed on return
 }
 
 // DOFs on the element's boundary faces (topo[26]: bit j set when local face j
@@ -398,7 +397,7 @@ void build_gather_map(const Mesh &m, int p, int64_t nvdofs, int64_t nedofs, int6
     const int D = p + 1, dof = D * D * D;
     std::vector<int> dm(dof);
     h1_dof_map(p, dm.data());
-    DevBuf<int> d_dm;
+    StreamBuf<int> d_dm;
     d_dm.upload(dm.data(), dof, s);
     const int64_t n = m.ne * dof;
     hipLaunchKernelGGL(k_build_gather, dim3(nblocks(n)), dim3(kBlock), 0, s, m.ne, p, dof,
@@ -421,7 +420,7 @@ void build_transpose(int64_t ne, int dof, int64_t ndofs, const int *gmap, int *o
     hipLaunchKernelGGL(k_count, dim3(nblocks(n)), dim3(kBlock), 0, s, n, gmap, offsets);
     H1PA_HIP(hipGetLastError());
     inclusive_scan(offsets + 1, ndofs, s);  // offsets[i] = start of row i
-    DevBuf<int> cursor(ndofs + 1);
+    StreamBuf<int> cursor(ndofs + 1, s);
     H1PA_HIP(hipMemcpyAsync(cursor.p, offsets, sizeof(int) * (ndofs + 1), hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_fill, dim3(nblocks(n)), dim3(kBlock), 0, s, n, gmap, cursor.p, indices);
     hipLaunchKernelGGL(k_sort_rows, dim3(nblocks(ndofs)), dim3(kBlock), 0, s, ndofs, offsets,
@@ -441,7 +440,7 @@ void boundary_mask(const Mesh &m, int p, const int *gmap, uint8_t *mask, int64_t
 }
 
 int64_t mask_to_list(const uint8_t *mask, int64_t n, DevBuf<int> &list, hipStream_t s) {
-    DevBuf<int> incl(n);
+    StreamBuf<int> incl(n, s);
     hipLaunchKernelGGL(k_mask_to_int, dim3(nblocks(n)), dim3(kBlock), 0, s, n, mask, incl.p);
     H1PA_HIP(hipGetLastError());
     inclusive_scan(incl.p, n, s);
@@ -482,7 +481,7 @@ void native_indices(int64_t nE, int dof, const int *indices, const int *lex2nat,
 
 void first_contribution_order(int64_t nE, const int *gmap, const int *offsets,
                               const int *indices, int64_t nv, DevBuf<int> &perm, hipStream_t s) {
-    DevBuf<int> incl(nE);
+    StreamBuf<int> incl(nE, s);
     hipLaunchKernelGGL(k_first_flag, dim3(nblocks(nE)), dim3(kBlock), 0, s, nE, gmap, offsets,
                        indices, nv, incl.p);
     H1PA_HIP(hipGetLastError());
