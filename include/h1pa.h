@@ -537,6 +537,17 @@ int h1pa_batch_info(const h1pa_batch *batch, int64_t *info);
 /* Device bytes held by a form, its space and its mesh (admission control; no
  * MFEM counterpart). */
 int h1pa_form_device_bytes(const h1pa_form *form, int64_t *bytes);
+/* Block cache for the library's device memory (MFEM's pooled device memory
+ * types, general/mem_manager.hpp:52-55, DEVICE_UMPIRE, play this part there).
+ * max_bytes > 0: freed blocks, up to max_bytes per device, are kept and
+ * reused by later allocations of a similar size instead of going back through
+ * hipFree, which synchronizes the whole device -- for a caller that releases
+ * objects while other work runs on other streams.  The caller must then not
+ * destroy an object while device work on its buffers is still pending (with
+ * the cache off, hipFree waits for that work).  max_bytes = 0 (the default)
+ * turns it off and frees what it kept.  *bytes: the bytes kept now. */
+int h1pa_set_device_cache(int64_t max_bytes);
+int h1pa_device_cache_bytes(int64_t *bytes);
 
 #ifdef __cplusplus
 }

@@ -178,6 +178,8 @@ SIGNATURES = {
     "h1pa_batch_poll": (_i, [_vp, _i, _pi64, C.POINTER(BatchResult), C.POINTER(_i)]),
     "h1pa_batch_info": (_i, [_vp, _pi64]),
     "h1pa_form_device_bytes": (_i, [_vp, _pi64]),
+    "h1pa_set_device_cache": (_i, [C.c_int64]),
+    "h1pa_device_cache_bytes": (_i, [_vp]),
     "h1pa_ir_solve": (_i, [_vp, _vp, _i64, _vp, _vp, C.POINTER(IrOpts), C.POINTER(IrStats),
                            _vp]),
 }
@@ -1175,6 +1177,20 @@ def Randomize(x, seed, lo=0.0, hi=1.0):
     _check_vec(x, x.numel())
     _check(lib().h1pa_vector_randomize(_ptr(x), x.numel(), int(seed) & ((1 << 64) - 1),
                                        float(lo), float(hi), _stream()))
+
+
+def SetDeviceCache(max_bytes):
+    """Keep up to max_bytes of freed device blocks per device for reuse
+    instead of hipFree (which synchronizes the device); 0 turns it off and
+    frees them.  With it on, destroy no object while device work on its
+    buffers is pending."""
+    _check(lib().h1pa_set_device_cache(int(max_bytes)))
+
+
+def DeviceCacheBytes():
+    v = C.c_int64()
+    _check(lib().h1pa_device_cache_bytes(C.byref(v)))
+    return v.value
 
 
 class BatchSolver:

@@ -26,7 +26,7 @@ CEILINGS = os.path.join(HERE, "h1pa_ceilings")
 
 SOURCES = ["basis.cpp", "mesh.cpp", "capi.cpp", "restriction.hip", "geometry.hip",
            "apply.hip", "columns.hip", "pcg.hip", "batch.hip", "qinterp.hip", "transfer.hip",
-           "multigrid.cpp"]
+           "multigrid.cpp", "devmem.cpp"]
 
 ARCH = "gfx950"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Wall",

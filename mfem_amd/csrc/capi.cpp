@@ -473,6 +473,20 @@ int h1pa_form_create(h1pa_space *space, const h1pa_form_desc *desc, h1pa_form **
     API_END
 }
 
+int h1pa_set_device_cache(int64_t max_bytes) {
+    API_BEGIN
+    REQUIRE(max_bytes >= 0, "set_device_cache: max_bytes must be >= 0");
+    device_cache_set(max_bytes);
+    API_END
+}
+
+int h1pa_device_cache_bytes(int64_t *bytes) {
+    API_BEGIN
+    REQUIRE(bytes, "device_cache_bytes: null argument");
+    *bytes = device_cache_bytes();
+    API_END
+}
+
 int h1pa_form_destroy(h1pa_form *form) {
     API_BEGIN
     if (form) require_no_batch_refs(form->f, "form_destroy");

@@ -39,34 +39,57 @@ struct bf16_t {
 };
 
 // ---- device buffer ---------------------------------------------------------
+// Device blocks for DevBuf (devmem.cpp).  With the block cache off (the
+// default) these are hipMalloc and hipFree.  With it on
+// (h1pa_set_device_cache), a freed block is kept on a per-device list instead
+// of hipFree, which synchronizes the whole device, and an allocation reuses a
+// kept block of at least its size (at most 1.25x); the caller then must not
+// release an object while device work on its buffers is pending.
+void *device_alloc(size_t bytes, size_t &cap, int &dev);
+void device_free(void *p, size_t cap, int dev);
+void device_cache_set(int64_t max_bytes);  // 0: off (kept blocks are freed)
+int64_t device_cache_bytes();               // bytes kept, all devices
+
 template <typename T>
 struct DevBuf {
     T *p = nullptr;
     size_t n = 0;
+    size_t cap = 0;  // bytes of the block behind p
+    int dev = 0;     // device it was allocated on
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n), cap(o.cap), dev(o.dev) {
+        o.p = nullptr;
+        o.n = 0;
+        o.cap = 0;
+    }
     DevBuf &operator=(DevBuf &&o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            cap = o.cap;
+            dev = o.dev;
+            o.p = nullptr;
+            o.n = 0;
+            o.cap = 0;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
         if (count == 0) return;
-        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
-            (void)hipGetLastError();
-            p = nullptr;
-            throw Error{H1PA_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed"};
-        }
+        p = static_cast<T *>(device_alloc(count * sizeof(T), cap, dev));
         n = count;
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) device_free(p, cap, dev);
         p = nullptr;
         n = 0;
+        cap = 0;
     }
     void upload(const T *h, size_t count, hipStream_t s) {
         if (count > n) alloc(count);

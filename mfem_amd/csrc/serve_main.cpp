@@ -197,7 +197,10 @@ Live setup(const Request &q, hipStream_t s, bool mg = false,
     int64_t info[4];
     check(h1pa_space_info(L.space, info), "space_info");
     const int64_t N = info[3];
-    hcheck(hipMalloc(&L.b, 4 * N * sizeof(double)), "hipMalloc vectors");
+    // stream-ordered: the release (on another stream, after every use has
+    // been waited for) does not synchronize the device
+    hcheck(hipMallocAsync(reinterpret_cast<void **>(&L.b), 4 * N * sizeof(double), s),
+           "hipMallocAsync vectors");
     L.x = L.b + N;
     L.B = L.x + N;
     L.X = L.B + N;
@@ -238,12 +241,15 @@ void dump_solution(const std::string &dir, int id, const Live &L) {
     std::fclose(f);
 }
 
-void release(Live &L) {
+// Frees a request's objects; every device use of them has been waited for.
+// With the library's block cache on and the vectors stream-ordered, nothing
+// here synchronizes the device (the solves of other requests run on).
+void release(Live &L, hipStream_t s) {
     if (L.mg) h1pa_multigrid_destroy(L.mg);
     if (L.form) h1pa_form_destroy(L.form);
     if (L.space) h1pa_space_destroy(L.space);
     if (L.mesh) h1pa_mesh_destroy(L.mesh);
-    if (L.b) (void)hipFree(L.b);
+    if (L.b) (void)hipFreeAsync(L.b, s);
     L = Live{};
 }
 
@@ -390,7 +396,8 @@ double jacobi_measure(const Live &L, hipStream_t s) {
     check(h1pa_space_info(L.space, info), "space_info");
     const int64_t N = info[3];
     double *r = nullptr;
-    hcheck(hipMalloc(&r, 2 * N * sizeof(double)), "hipMalloc");
+    hcheck(hipMallocAsync(reinterpret_cast<void **>(&r), 2 * N * sizeof(double), s),
+           "hipMallocAsync");
     double *dg = r + N;
     check(h1pa_form_constrained_mult(L.form, ess, nEss, L.X, r, s), "residual");
     check(h1pa_vector_add(r, -1.0, L.B, N, s), "residual");
@@ -402,7 +409,7 @@ double jacobi_measure(const Live &L, hipStream_t s) {
     hcheck(hipMemcpy(hd.data(), dg, N * 8, hipMemcpyDeviceToHost), "copy");
     hcheck(hipMemcpy(hb.data(), L.B, N * 8, hipMemcpyDeviceToHost), "copy");
     if (nEss) hcheck(hipMemcpy(he.data(), ess, nEss * 4, hipMemcpyDeviceToHost), "copy");
-    hcheck(hipFree(r), "hipFree");
+    hcheck(hipFreeAsync(r, s), "hipFreeAsync");
     for (int i : he) hd[i] = 1.0;
     double rr = 0.0, bb = 0.0;
     for (int64_t i = 0; i < N; ++i) {
@@ -429,7 +436,7 @@ void Reaper::run() {
         }
         const double meas = jacobi_measure(sv.L, rs);
         if (!dump_.empty()) dump_solution(dump_, sv.id, sv.L);
-        release(sv.L);
+        release(sv.L, rs);
         std::lock_guard<std::mutex> g(m_);
         done_.emplace_back(sv.id, meas);
     }
@@ -588,6 +595,19 @@ int main(int argc, char **argv) {
     hipStream_t s;
     hcheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
     const int64_t budget = (int64_t)(o.budget_gb * 1e9);
+    // Releases must not synchronize the device while other requests solve:
+    // the library keeps freed blocks (up to 64 GB) instead of hipFree, and the
+    // default pool behind hipMallocAsync keeps what it is given back.
+    check(h1pa_set_device_cache(int64_t(64) << 30), "device cache");
+    {
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        hcheck(hipGetDevice(&dev), "device");
+        hcheck(hipDeviceGetDefaultMemPool(&pool, dev), "default pool");
+        uint64_t threshold = UINT64_MAX;
+        hcheck(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold),
+               "pool threshold");
+    }
     const bool batch = o.mode == "batch";
     const bool mgmode = o.mode == "mg";
     // batch: the batch's precision; mg: the cycle's (FP64, or BF16-PA applies)
@@ -610,7 +630,7 @@ int main(int argc, char **argv) {
         h1pa_pcg_stats st{};
         check(h1pa_pcg_solve(L.form, ess, nEss, L.B, L.X, &po, &st, s), "warmup");
         hcheck(hipStreamSynchronize(s), "sync");
-        release(L);
+        release(L, s);
     }
     // The apply-variant autotuner runs on the first form of each (p, Q) with
     // >= 20000 elements.  Run it here, for every order of the mix, before the
@@ -631,7 +651,7 @@ int main(int argc, char **argv) {
             hcheck(hipStreamSynchronize(s), "sync");
             hcheck(hipFree(v), "hipFree tune");
         }
-        release(L);
+        release(L, s);
     }
 
     h1pa_batch *B = nullptr;
@@ -687,7 +707,7 @@ int main(int argc, char **argv) {
                     q.iterations = res[i].iterations;
                     q.converged = res[i].converged;
                     if (!o.dump.empty()) dump_solution(o.dump, (int)tags[i], live[tags[i]]);
-                    release(live[tags[i]]);
+                    release(live[tags[i]], s);
                     live.erase(tags[i]);
                     used -= live_est[tags[i]];
                     live_est.erase(tags[i]);
@@ -841,7 +861,7 @@ int main(int argc, char **argv) {
             q.iterations = st.iterations;
             q.converged = st.converged;
             if (!o.dump.empty()) dump_solution(o.dump, id, L);
-            release(L);
+            release(L, s);
             ++done;
         }
     }

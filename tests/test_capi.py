@@ -42,6 +42,13 @@ def test_errors_are_reported(h1pa):
     assert h1pa.lib().h1pa_batch_query(None, C.byref(v)) == h1pa.H1PA_EINVAL
     assert h1pa.lib().h1pa_batch_wait(None, C.byref(v)) == h1pa.H1PA_EINVAL
     assert h1pa.lib().h1pa_mesh_load(None, 0, None) == h1pa.H1PA_EINVAL
+    # the device block cache's switch (no device work while nothing is kept)
+    with pytest.raises(h1pa.H1paError) as e:
+        h1pa.SetDeviceCache(-1)
+    assert e.value.code == h1pa.H1PA_EINVAL
+    h1pa.SetDeviceCache(1 << 20)
+    h1pa.SetDeviceCache(0)
+    assert h1pa.DeviceCacheBytes() == 0
 
 
 @pytest.mark.parametrize("dims", [(1, 1, 1), (2, 2, 2), (3, 3, 3), (4, 3, 5), (7, 5, 6),
