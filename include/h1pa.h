@@ -544,8 +544,9 @@ int h1pa_form_device_bytes(const h1pa_form *form, int64_t *bytes);
  * hipFree, which synchronizes the whole device -- for a caller that releases
  * objects while other work runs on other streams.  The caller must then not
  * destroy an object while device work on its buffers is still pending (with
- * the cache off, hipFree waits for that work).  max_bytes = 0 (the default)
- * turns it off and frees what it kept.  *bytes: the bytes kept now. */
+ * the cache off, hipFree waits for that work).  A lower max_bytes frees kept
+ * blocks down to it; max_bytes = 0 (the default) turns the cache off and frees
+ * what it kept.  *bytes: the bytes kept now. */
 int h1pa_set_device_cache(int64_t max_bytes);
 int h1pa_device_cache_bytes(int64_t *bytes);
 

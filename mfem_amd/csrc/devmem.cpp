@@ -11,6 +11,7 @@
 // device memory types (general/mem_manager.hpp:52-55, DEVICE_UMPIRE).
 #include "internal.hpp"
 
+#include <iterator>
 #include <map>
 #include <mutex>
 
@@ -113,8 +114,14 @@ void device_free(void *p, size_t cap, int dev) {
 void device_cache_set(int64_t max_bytes) {
     std::lock_guard<std::mutex> lock(g_mu);
     g_limit = max_bytes;
-    if (max_bytes == 0)
-        for (auto &c : g_cache) flush_locked(c);
+    for (auto &c : g_cache) {  // keep at most the new limit (largest blocks go first)
+        while (!c.blocks.empty() && (int64_t)c.bytes > max_bytes) {
+            auto it = std::prev(c.blocks.end());
+            (void)hipFree(it->second);
+            c.bytes -= it->first;
+            c.blocks.erase(it);
+        }
+    }
 }
 
 int64_t device_cache_bytes() {

@@ -42,10 +42,11 @@ def test_cache_reuses_blocks_and_changes_nothing(h1pa, torch_cuda):
         assert M.DeviceCacheBytes() <= kept
         for a_, b_, c_ in zip(ref, first, second):
             assert np.array_equal(a_, b_) and np.array_equal(a_, c_)
-        # a smaller cap keeps less; off frees everything
+        # a lower limit trims what is kept; off frees everything
         M.SetDeviceCache(1 << 16)
+        assert M.DeviceCacheBytes() <= (1 << 16)
         _apply(M, torch, 4, 2, 6)
-        assert M.DeviceCacheBytes() <= (1 << 30)
+        assert M.DeviceCacheBytes() <= (1 << 16)
     finally:
         M.SetDeviceCache(0)
     assert M.DeviceCacheBytes() == 0
