@@ -407,9 +407,7 @@ int h1pa_restriction_mult_transpose(const h1pa_restriction *r, const double *x, 
                                     void *stream) {
     API_BEGIN
     REQUIRE(r && x && y, "restriction_mult_transpose: null argument");
-    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 0,
-                                  r->R().perm_lex.p, r->R().nperm_lex,
-                                  as_stream(stream));
+    restriction_scatter(r->R(), r->view != nullptr, x, y, 0, as_stream(stream));
     API_END
 }
 
@@ -418,9 +416,7 @@ int h1pa_restriction_add_mult_transpose(const h1pa_restriction *r, const double 
     API_BEGIN
     REQUIRE(r && x && y, "restriction_add_mult_transpose: null argument");
     REQUIRE(a == 1.0, "General coefficient case is not yet supported!");
-    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 1,
-                                  r->R().perm_lex.p, r->R().nperm_lex,
-                                  as_stream(stream));
+    restriction_scatter(r->R(), r->view != nullptr, x, y, 1, as_stream(stream));
     API_END
 }
 
@@ -428,9 +424,7 @@ int h1pa_restriction_abs_mult_transpose(const h1pa_restriction *r, const double 
                                         void *stream) {
     API_BEGIN
     REQUIRE(r && x && y, "restriction_abs_mult_transpose: null argument");
-    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 2,
-                                  r->R().perm_lex.p, r->R().nperm_lex,
-                                  as_stream(stream));
+    restriction_scatter(r->R(), r->view != nullptr, x, y, 2, as_stream(stream));
     API_END
 }
 
@@ -438,9 +432,7 @@ int h1pa_restriction_mult_left_inverse(const h1pa_restriction *r, const double *
                                        void *stream) {
     API_BEGIN
     REQUIRE(r && x && y, "restriction_mult_left_inverse: null argument");
-    kern::restrict_mult_transpose(r->R().ndofs, r->R().offsets.p, r->R().indices.p, x, y, 3,
-                                  r->R().perm_lex.p, r->R().nperm_lex,
-                                  as_stream(stream));
+    restriction_scatter(r->R(), r->view != nullptr, x, y, 3, as_stream(stream));
     API_END
 }
 
@@ -521,7 +513,7 @@ void autotune_diffusion(Form &f, hipStream_t s) {
     if ((at && at[0] == '0') || (sp.ne < 20000 && sp.ndofs < 500000)) return;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     if (g_tuned[p][Q]) return;
-    StreamBuf<double> x(sp.ndofs, s);
+    DevBuf<double> x(sp.ndofs);  // once per (p, Q) and process: its free may wait
     H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * sp.ndofs, s));
     hipEvent_t e0, e1;
     H1PA_HIP(hipEventCreate(&e0));
@@ -679,6 +671,28 @@ const uint8_t *constraint_mask(Form &f, const int *d_ess, int64_t n_ess, hipStre
 }
 
 }  // namespace
+
+// MultTranspose family on a lexicographic E-vector.  A space's restriction
+// (bubble_block: MFEM's interior-DOF block from bubble_begin on) sums its
+// non-bubble rows through the transpose table in element-major order and
+// writes the bubble block by position; other restrictions use the table for
+// every row.
+void h1pa::restriction_scatter(const Restriction &R, bool bubble_block, const double *x,
+                               double *y, int mode, hipStream_t s) {
+    if (bubble_block && R.bubble_begin < R.ndofs) {
+        int D = 1;
+        while ((int64_t)D * D * D < R.dof) ++D;
+        H1PA_CHECK((int64_t)D * D * D == R.dof && R.nperm_lex == R.bubble_begin &&
+                       R.bubble_begin + R.ne * (int64_t)(D - 2) * (D - 2) * (D - 2) == R.ndofs,
+                   H1PA_EINVAL, "restriction: unexpected bubble block");
+        kern::restrict_mult_transpose(R.bubble_begin, R.offsets.p, R.indices.p, x, y, mode,
+                                      R.perm_lex.p, R.nperm_lex, s);
+        kern::bubble_mult_transpose(R.ne, D, R.bubble_begin, x, y, mode, s);
+        return;
+    }
+    kern::restrict_mult_transpose(R.ndofs, R.offsets.p, R.indices.p, x, y, mode, R.perm_lex.p,
+                                  R.nperm_lex, s);
+}
 
 // The default pool of each device keeps the blocks StreamBuf frees (release
 // threshold at its maximum), so a temporary's hipMallocAsync after the first
@@ -1007,8 +1021,7 @@ int h1pa_space_domain_lf_qf(h1pa_space *space, double fval, const double *d_fq, 
     sp.mesh->ensure_nodes(s);
     StreamBuf<double> bE(sp.ne * sp.dof, s);
     kern::domain_lf(*sp.mesh, sp.p, fval, d_fq, bE.p, s);
-    kern::restrict_mult_transpose(sp.ndofs, sp.restr.offsets.p, sp.restr.indices.p, bE.p, b, 0,
-                                  sp.restr.perm_lex.p, sp.restr.nperm_lex, s);
+    restriction_scatter(sp.restr, true, bE.p, b, 0, s);
     H1PA_HIP(hipStreamSynchronize(s));
     API_END
 }

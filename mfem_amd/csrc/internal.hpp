@@ -283,6 +283,10 @@ void first_contribution_order(int64_t nE, const int *gmap, const int *offsets,
 void restrict_mult_transpose(int64_t nrows, const int *offsets, const int *indices,
                              const double *xe, double *y, int mode, const int *perm,
                              int64_t nperm, hipStream_t s);
+// The bubble block of an H1 space's restriction (rows bb .. bb + ne (D-2)^3)
+// from a lexicographic E-vector, without the transpose table; same modes.
+void bubble_mult_transpose(int64_t ne, int D, int64_t bb, const double *xe, double *y, int mode,
+                           hipStream_t s);
 // p-refinement transfer (transfer.hip): emask[E] = 1 at the first E-entry
 // (indices order) of each L-DOF; prolongation low L -> high L (every high
 // L-DOF written); restriction high L -> low E-vector (lexicographic), to be
@@ -452,6 +456,11 @@ void ir_solve(Form &f, const uint8_t *mask, const double *diag, const double *b,
 void form_mult_lp(Form &f, int precision, const float *x, float *y, hipStream_t s);
 // Jacobi diagonal of the unconstrained operator (L-vector, set).
 void operator_diagonal(Form &f, double *diag, hipStream_t s);
+// ElementRestriction::MultTranspose family (mode 0 set, 1 add, 2 abs, 3 left
+// inverse) from a lexicographic E-vector; bubble_block: R is an H1 space's
+// restriction, whose interior-DOF block is written by position.
+void restriction_scatter(const Restriction &R, bool bubble_block, const double *x, double *y,
+                         int mode, hipStream_t s);
 
 // ---- p-multigrid (transfer.hip, multigrid.cpp) -----------------------------
 // TensorProductPRefinementTransferOperator between two spaces of one mesh.

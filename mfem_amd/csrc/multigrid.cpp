@@ -74,9 +74,7 @@ void Transfer::mult(const double *xl, double *yh, hipStream_t s) {
 void Transfer::mult_transpose(const double *xh, double *yl, hipStream_t s) {
     kern::transfer_restrict_e(lo->ne, lo->p, hi->p, B.p, hi->restr.gather_map.p, emask_h.p, xh,
                               eL.p, s);
-    const Restriction &r = lo->restr;
-    kern::restrict_mult_transpose(r.ndofs, r.offsets.p, r.indices.p, eL.p, yl, 0, r.perm_lex.p,
-                                  r.nperm_lex, s);
+    restriction_scatter(lo->restr, true, eL.p, yl, 0, s);
 }
 
 // ---- GeometricMultigrid with order-refined levels ---------------------------

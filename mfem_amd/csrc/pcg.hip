@@ -1109,7 +1109,7 @@ void autotune_lp(Form &f, int kind, hipStream_t s) {
     std::lock_guard<std::mutex> lock(g_lp_tune_mutex);
     if (g_lp_tuned[kind][p][Q]) return;
     g_lp_tuned[kind][p][Q] = true;
-    StreamBuf<float> x(sp.ndofs, s);
+    DevBuf<float> x(sp.ndofs);  // once per (p, Q) and process: its free may wait
     H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(float) * sp.ndofs, s));
     hipEvent_t e0, e1;
     H1PA_HIP(hipEventCreate(&e0));

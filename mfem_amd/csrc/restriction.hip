@@ -320,6 +320,28 @@ k_restrict_mult_t(int64_t nrows, const int *__restrict__ offsets, const int *__r
     }
 }
 
+// E -> L for MFEM's bubble block of an H1 space (interior DOFs, one E-entry
+// each, fem/fespace.cpp:3518-3527): row bb + e NI^3 + j holds the interior
+// DOF j (x fastest) of element e, at lexicographic position (jx+1, jy+1,
+// jz+1) of the element's E-entries -- no table is read.  The sum of the row
+// is MFEM's (0 + x; the left inverse takes x itself).
+__global__ void __launch_bounds__(kBlock)
+k_bubble_mult_t(int64_t nrows, int NI, int D, int64_t bb, const double *__restrict__ xe,
+                double *__restrict__ y, int mode) {
+    const int64_t r = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (r >= nrows) return;
+    const int ni3 = NI * NI * NI;
+    const int64_t e = r / ni3;
+    const int j = (int)(r - e * ni3);
+    const int jx = j % NI, jy = (j / NI) % NI, jz = j / (NI * NI);
+    const double v = xe[e * (int64_t)(D * D * D) + (jx + 1) + D * ((jy + 1) + D * (jz + 1))];
+    double t = 0.0;
+    t = t + v;
+    if (mode == 3) y[bb + r] = v;
+    else if (mode == 1) y[bb + r] += t;
+    else y[bb + r] = t;
+}
+
 // ElementRestriction::BooleanMask (fem/restriction.cpp:255-290): the first
 // E-entry of each L-DOF's row (in indices order) is 1, the others 0.  A signed
 // entry -1-k stands for E-index k (MFEM's loop indexes with the raw value,
@@ -505,6 +527,16 @@ void restrict_mult_transpose(int64_t nrows, const int *offsets, const int *indic
     // p = 2, 4, 6, 8 (32M, 32M, 24M, 33M DOFs), fastest at every order
     hipLaunchKernelGGL(k_restrict_mult_t<2>, dim3(nblocks((nrows + 1) / 2)), dim3(kBlock), 0, s,
                        nrows, offsets, indices, xe, y, mode, perm, perm ? nperm : 0);
+    H1PA_HIP(hipGetLastError());
+}
+
+void bubble_mult_transpose(int64_t ne, int D, int64_t bb, const double *xe, double *y, int mode,
+                           hipStream_t s) {
+    const int NI = D - 2;
+    if (NI <= 0 || ne <= 0) return;
+    const int64_t nrows = ne * (int64_t)(NI * NI * NI);
+    hipLaunchKernelGGL(k_bubble_mult_t, dim3(nblocks(nrows)), dim3(kBlock), 0, s, nrows, NI, D, bb,
+                       xe, y, mode);
     H1PA_HIP(hipGetLastError());
 }
 
