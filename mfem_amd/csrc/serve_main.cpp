@@ -9,8 +9,9 @@
 // relative measure at rel_tol.
 //
 //   --mode batch   continuous batching: arrived requests are admitted in
-//                  arrival order while the HBM budget allows and set up on a
-//                  host thread of their own (own stream) beside the replays;
+//                  arrival order while the HBM budget allows and set up on
+//                  --setup-threads host threads (own streams) beside the
+//                  replays;
 //                  all admitted solves advance together in graph replays that
 //                  run back to back (the loop checks h1pa_batch_query,
 //                  retires the finished solves, releases their memory, adds
@@ -79,7 +80,7 @@ struct Opts {
     bool solo = false;
     std::string dump;
     int workers = 4;          // mg mode: solver threads (one stream each)
-    int setup_threads = 2;    // mg mode: set-up threads
+    int setup_threads = 2;    // mg and batch modes: set-up threads
 };
 
 struct Request {
@@ -676,7 +677,7 @@ int main(int argc, char **argv) {
     bool inflight = false;  // batch mode: a replay is running on the GPU
     std::vector<int64_t> tags(4096);
     std::vector<h1pa_batch_result> res(4096);
-    SetupWorker W(reqs, mgmode, mgmode ? o.setup_threads : 1, prec);
+    SetupWorker W(reqs, mgmode, (mgmode || batch) ? o.setup_threads : 1, prec);
     const auto t0 = std::chrono::steady_clock::now();
     std::unique_ptr<SolverPool> pool;
     std::unique_ptr<Reaper> reaper;
