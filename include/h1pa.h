@@ -546,7 +546,10 @@ int h1pa_form_device_bytes(const h1pa_form *form, int64_t *bytes);
  * destroy an object while device work on its buffers is still pending (with
  * the cache off, hipFree waits for that work).  A lower max_bytes frees kept
  * blocks down to it; max_bytes = 0 (the default) turns the cache off and frees
- * what it kept.  *bytes: the bytes kept now. */
+ * what it kept.  *bytes: the bytes kept now.  (Temporaries inside a call are
+ * stream-ordered, hipMallocAsync / hipFreeAsync on the call's stream, from
+ * the device's default memory pool, whose release threshold the library
+ * raises to its maximum so that freed blocks are reused.) */
 int h1pa_set_device_cache(int64_t max_bytes);
 int h1pa_device_cache_bytes(int64_t *bytes);
 
