@@ -2,6 +2,7 @@
 //
 // Every entry point catches h1pa::Error (and std::exception) and turns it into
 // a status code plus a thread-local message; nothing throws across the ABI.
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -513,29 +514,40 @@ void autotune_diffusion(Form &f, hipStream_t s) {
     if ((at && at[0] == '0') || (sp.ne < 20000 && sp.ndofs < 500000)) return;
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     if (g_tuned[p][Q]) return;
-    DevBuf<double> x(sp.ndofs);  // once per (p, Q) and process: its free may wait
-    H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * sp.ndofs, s));
+    // x and an L-vector for the interior DOFs; once per (p, Q) and process
+    DevBuf<double> x(2 * sp.ndofs);
+    H1PA_HIP(hipMemsetAsync(x.p, 0, sizeof(double) * 2 * sp.ndofs, s));
     hipEvent_t e0, e1;
     H1PA_HIP(hipEventCreate(&e0));
     H1PA_HIP(hipEventCreate(&e1));
-    int best = 0;
-    float best_ms = 1e30f;
-    kern::EOut plain;
-    plain.ye = f.ework0.p;
-    for (int v = 0; v < kern::diffusion_variant_count(); ++v) {
-        for (int w = 0; w < 2; ++w)
-            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
-                                          sp.restr.gather_map.p, nullptr, x.p, plain, false, s);
-        H1PA_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < 5; ++r)
-            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
-                                          sp.restr.gather_map.p, nullptr, x.p, plain, false, s);
-        H1PA_HIP(hipEventRecord(e1, s));
-        H1PA_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        H1PA_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) { best_ms = ms; best = v; }
+    // the Mult's own output: native layout, interior DOFs straight to an L-vector
+    kern::EOut out;
+    out.ye = f.ework0.p;
+    out.l2n = sp.restr.lex2nat.p;
+    if (sp.restr.bubble_begin < sp.ndofs) {
+        out.yL = x.p + sp.ndofs;
+        out.bb = sp.restr.bubble_begin;
     }
+    // three interleaved rounds (1 warm-up + 4 timed launches per variant each),
+    // so that a drift of the clock or of the other work on the device does
+    // not pick the variant that happened to run in a quiet moment
+    const int nv = kern::diffusion_variant_count();
+    std::vector<float> total(nv, 0.f);
+    for (int round = 0; round < 3; ++round)
+        for (int v = 0; v < nv; ++v) {
+            kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
+                                          sp.restr.gather_map.p, nullptr, x.p, out, false, s);
+            H1PA_HIP(hipEventRecord(e0, s));
+            for (int r = 0; r < 4; ++r)
+                kern::diffusion_apply_variant(v, p, Q, sp.ne, f.basis.p, f.pa_diff.p,
+                                              sp.restr.gather_map.p, nullptr, x.p, out, false, s);
+            H1PA_HIP(hipEventRecord(e1, s));
+            H1PA_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            H1PA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            total[v] += ms;
+        }
+    const int best = (int)(std::min_element(total.begin(), total.end()) - total.begin());
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     kern::set_diffusion_variant(p, Q, best);
