@@ -14,6 +14,8 @@
 //      wave per element on 16x16x4 FP64 matrix-core tiles.  Both read their
 //      input from LDS and write their output to LDS on every repetition; the
 //      program checks that they agree before timing them.
+//   4. The same z-stage at p = 8 in FP32 (D = 9, Q = 10, 100 columns), in the
+//      same two forms, the matrix-core one on 16x16x4 FP32 tiles.
 //
 // Output: one line per measurement (ms per launch, GB/s or TFLOP/s).
 #include <hip/hip_runtime.h>
@@ -322,6 +324,201 @@ __global__ void __launch_bounds__(64 * ZE) k_zstage_mfma(const double *__restric
     }
 }
 
+// ---- 4. the z-stage at D = 9, Q = 10 in FP32 ------------------------------------
+//
+// The same z-stage as section 3 at p = 8 in single precision: Z_c[dz][col] with
+// col = qx + 10 qy < 100, stored as [c][dz (12)][col (112)] so that the
+// matrix-core form covers the columns with 7 tiles of 16 and the DOF rows with 3
+// K-steps of 4 (rows 9..11 and columns 100..111 hold zeros).  PA as
+// pa[k][qz * 112 + col], read at pa + r * pstep on repetition r (pstep = 0: the
+// runtime stride keeps the compiler from holding PA in registers across
+// repetitions, which the real kernel, streaming PA per element, cannot do);
+// basis bm[0] = B, bm[1] = G as [qz][dz].
+constexpr int YD = 9, YQ = 10, YN = 100;  // D, Q, columns (Q^2)
+constexpr int YR = 12, YC = 112;          // stored DOF rows, stored columns
+constexpr int YS = 3 * YR * YC;           // LDS floats per element
+constexpr int YV = 2;                     // elements per block, vector form (128 threads each)
+constexpr int YM = 2;                     // elements (waves) per block, matrix-core form
+
+__global__ void __launch_bounds__(128 * YV) k_zstage8_valu(const float *__restrict__ zin,
+                                                           const float *__restrict__ bm,
+                                                           const float *__restrict__ pa,
+                                                           float *__restrict__ tout, int64_t ne,
+                                                           int reps, int pstep) {
+    __shared__ float zs[YV * YS];
+    const int w = threadIdx.x >> 7, col = threadIdx.x & 127;
+    const int64_t e = (int64_t)blockIdx.x * YV + w;
+    float *z = zs + w * YS;
+    const float *src = zin + (e % 4096) * YS;
+    for (int i = col; i < YS; i += 128) z[i] = src[i];
+    __syncthreads();
+    const bool on = col < YN;
+    float t0[YD], t1[YD], t2[YD];
+    for (int r = 0; r < reps; ++r) {
+        const float *pr = pa + (int64_t)r * pstep;
+        if (on) {
+            float z0[YD], z1[YD], z2[YD];
+#pragma unroll
+            for (int d = 0; d < YD; ++d) {
+                z0[d] = z[(0 * YR + d) * YC + col];
+                z1[d] = z[(1 * YR + d) * YC + col];
+                z2[d] = z[(2 * YR + d) * YC + col];
+                t0[d] = t1[d] = t2[d] = 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < YQ; ++q) {
+                float g0 = 0.f, g1 = 0.f, g2 = 0.f;
+#pragma unroll
+                for (int d = 0; d < YD; ++d) {
+                    g0 = fmaf(bm[q * YD + d], z0[d], g0);
+                    g1 = fmaf(bm[q * YD + d], z1[d], g1);
+                    g2 = fmaf(bm[YQ * YD + q * YD + d], z2[d], g2);
+                }
+                const int pq = q * YC + col;
+                const float d11 = pr[0 * YQ * YC + pq], d21 = pr[1 * YQ * YC + pq];
+                const float d31 = pr[2 * YQ * YC + pq], d22 = pr[3 * YQ * YC + pq];
+                const float d32 = pr[4 * YQ * YC + pq], d33 = pr[5 * YQ * YC + pq];
+                const float o0 = fmaf(d11, g0, fmaf(d21, g1, d31 * g2));
+                const float o1 = fmaf(d21, g0, fmaf(d22, g1, d32 * g2));
+                const float o2 = fmaf(d31, g0, fmaf(d32, g1, d33 * g2));
+#pragma unroll
+                for (int d = 0; d < YD; ++d) {
+                    t0[d] = fmaf(bm[q * YD + d], o0, t0[d]);
+                    t1[d] = fmaf(bm[q * YD + d], o1, t1[d]);
+                    t2[d] = fmaf(bm[YQ * YD + q * YD + d], o2, t2[d]);
+                }
+            }
+        }
+        __syncthreads();
+        if (on) {
+#pragma unroll
+            for (int d = 0; d < YD; ++d) {
+                z[(0 * YR + d) * YC + col] += 1e-6f * t0[d];
+                z[(1 * YR + d) * YC + col] += 1e-6f * t1[d];
+                z[(2 * YR + d) * YC + col] += 1e-6f * t2[d];
+            }
+        }
+        __syncthreads();
+    }
+    if (on && e < ne) {
+        float *o = tout + e * (3 * YD * YN);
+#pragma unroll
+        for (int d = 0; d < YD; ++d) {
+            o[(0 * YD + d) * YN + col] = t0[d];
+            o[(1 * YD + d) * YN + col] = t1[d];
+            o[(2 * YD + d) * YN + col] = t2[d];
+        }
+    }
+}
+
+// Tile row held by lane group g (= lane / 16) in output register r of
+// v_mfma_f32_16x16x4f32: 4 g + r when G4, g + 4 r otherwise.  The program
+// builds both and keeps the one that agrees with the vector form.
+template <bool G4>
+__device__ __forceinline__ int yrow(int g, int r) {
+    return G4 ? 4 * g + r : g + 4 * r;
+}
+
+// One wave per element.  Forward, step s (3 steps): k-slot g is dz = g + 4 s,
+// read from LDS.  The transposed contraction's k-slot g at step s is the
+// quadrature row this lane group already holds in register s of the forward
+// result, qz = yrow(g, s), so its B operand needs no exchange between lanes;
+// that takes 4 steps when G4 (rows s, 4+s, 8+s, 12+s) and 3 otherwise.
+template <bool G4>
+__global__ void __launch_bounds__(64 * YM) k_zstage8_mfma(const float *__restrict__ zin,
+                                                          const float *__restrict__ bm,
+                                                          const float *__restrict__ pa,
+                                                          float *__restrict__ tout, int64_t ne,
+                                                          int reps, int pstep) {
+    constexpr int NT = YC / 16, NF = 3, NS = G4 ? 4 : 3;
+    __shared__ float zs[YM * YS];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane >> 4, c16 = lane & 15;
+    const int64_t e = (int64_t)blockIdx.x * YM + w;
+    float *z = zs + w * YS;
+    const float *src = zin + (e % 4096) * YS;
+    for (int i = lane; i < YS; i += 64) z[i] = src[i];
+    float af[2][NF], at[2][NS];  // [B or G][step]
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+        for (int s = 0; s < NF; ++s) {
+            const int dz = g + 4 * s;
+            af[b][s] = (c16 < YQ && dz < YD) ? bm[b * YQ * YD + c16 * YD + dz] : 0.f;
+        }
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int qz = yrow<G4>(g, s);
+            at[b][s] = (c16 < YD && qz < YQ) ? bm[b * YQ * YD + qz * YD + c16] : 0.f;
+        }
+    }
+    __syncthreads();
+    // A tile's update writes only its own 16 columns of this wave's slice, and
+    // no later tile of the repetition reads them, so each tile is updated (and,
+    // on the last repetition, stored) as soon as it is done.
+    for (int r = 0; r < reps; ++r) {
+        const float *pr = pa + (int64_t)r * pstep;
+#pragma unroll 1
+        for (int nt = 0; nt < NT; ++nt) {
+            const int col = 16 * nt + c16;
+            v4f gc[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                gc[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < NF; ++s) {
+                    const float zb = z[(c * YR + g + 4 * s) * YC + col];  // rows 9..11 are zero
+                    gc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[c == 2][s], zb, gc[c], 0, 0, 0);
+                }
+            }
+            float o[3][NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int qz = yrow<G4>(g, s);
+                if (qz < YQ) {
+                    const int pq = qz * YC + col;
+                    const float d11 = pr[0 * YQ * YC + pq], d21 = pr[1 * YQ * YC + pq];
+                    const float d31 = pr[2 * YQ * YC + pq], d22 = pr[3 * YQ * YC + pq];
+                    const float d32 = pr[4 * YQ * YC + pq], d33 = pr[5 * YQ * YC + pq];
+                    const float g0 = gc[0][s], g1 = gc[1][s], g2 = gc[2][s];
+                    o[0][s] = fmaf(d11, g0, fmaf(d21, g1, d31 * g2));
+                    o[1][s] = fmaf(d21, g0, fmaf(d22, g1, d32 * g2));
+                    o[2][s] = fmaf(d31, g0, fmaf(d32, g1, d33 * g2));
+                } else {
+                    o[0][s] = o[1][s] = o[2][s] = 0.f;
+                }
+            }
+            v4f tacc[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                tacc[c] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+                    tacc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(at[c == 2][s], o[c][s], tacc[c],
+                                                                   0, 0, 0);
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    const int dz = yrow<G4>(g, s);
+                    if (dz < YD) z[(c * YR + dz) * YC + col] += 1e-6f * tacc[c][s];
+                }
+            if (r == reps - 1 && e < ne) {
+                float *out = tout + e * (3 * YD * YN);
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        const int dz = yrow<G4>(g, s);
+                        if (dz < YD && col < YN) out[(c * YD + dz) * YN + col] = tacc[c][s];
+                    }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---- timing ------------------------------------------------------------------
 
 template <typename F>
@@ -458,6 +655,82 @@ int main() {
         CK(hipFree(pz));
         CK(hipFree(t1));
         CK(hipFree(t2));
+    }
+    // 4. z-stage, D = 9, Q = 10, FP32
+    {
+        const int64_t nz = 32768;  // elements (a multiple of YV and YM)
+        std::vector<float> hz(4096 * (int64_t)YS, 0.f), hb(2 * YQ * YD), hp(6 * YQ * YC, 0.f);
+        unsigned s = 54321u;
+        auto rnd = [&] { s = s * 1664525u + 1013904223u; return (float)((s >> 8) * (1.0 / 16777216.0) - 0.5); };
+        for (int64_t el = 0; el < 4096; ++el)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < YD; ++d)  // rows 9..11 and columns 100..111 stay zero
+                    for (int k = 0; k < YN; ++k) hz[el * YS + (c * YR + d) * YC + k] = rnd();
+        for (auto &v : hb) v = rnd();
+        for (int q = 0; q < YQ; ++q)
+            for (int k = 0; k < YC; ++k) {
+                const int i = q * YC + k;
+                hp[0 * YQ * YC + i] = 1.f + 0.2f * rnd();
+                hp[1 * YQ * YC + i] = 0.1f * rnd();
+                hp[2 * YQ * YC + i] = 0.1f * rnd();
+                hp[3 * YQ * YC + i] = 1.f + 0.2f * rnd();
+                hp[4 * YQ * YC + i] = 0.1f * rnd();
+                hp[5 * YQ * YC + i] = 1.f + 0.2f * rnd();
+            }
+        float *zin, *bm, *pz, *tv, *tm;
+        CK(hipMalloc(&zin, hz.size() * sizeof(float)));
+        CK(hipMalloc(&bm, hb.size() * sizeof(float)));
+        CK(hipMalloc(&pz, hp.size() * sizeof(float)));
+        const int64_t nt = nz * 3 * YD * YN;
+        CK(hipMalloc(&tv, nt * sizeof(float)));
+        CK(hipMalloc(&tm, nt * sizeof(float)));
+        CK(hipMemcpy(zin, hz.data(), hz.size() * sizeof(float), hipMemcpyHostToDevice));
+        CK(hipMemcpy(bm, hb.data(), hb.size() * sizeof(float), hipMemcpyHostToDevice));
+        CK(hipMemcpy(pz, hp.data(), hp.size() * sizeof(float), hipMemcpyHostToDevice));
+        const unsigned gv = (unsigned)(nz / YV), gm = (unsigned)(nz / YM);
+        std::vector<float> a(nt), b(nt);
+        auto launch_m = [&](bool g4, int reps) {
+            if (g4)
+                hipLaunchKernelGGL(k_zstage8_mfma<true>, dim3(gm), dim3(64 * YM), 0, 0, zin, bm, pz, tm, nz, reps, 0);
+            else
+                hipLaunchKernelGGL(k_zstage8_mfma<false>, dim3(gm), dim3(64 * YM), 0, 0, zin, bm, pz, tm, nz, reps, 0);
+        };
+        int layout = -1;  // 1: row 4 g + r, 0: row g + 4 r
+        for (int g4 = 1; g4 >= 0 && layout < 0; --g4) {
+            bool ok = true;
+            for (int reps : {1, 3}) {
+                hipLaunchKernelGGL(k_zstage8_valu, dim3(gv), dim3(128 * YV), 0, 0, zin, bm, pz, tv, nz, reps, 0);
+                launch_m(g4 != 0, reps);
+                CK(hipGetLastError());
+                CK(hipDeviceSynchronize());
+                CK(hipMemcpy(a.data(), tv, nt * sizeof(float), hipMemcpyDeviceToHost));
+                CK(hipMemcpy(b.data(), tm, nt * sizeof(float), hipMemcpyDeviceToHost));
+                double num = 0.0, den = 0.0;
+                for (int64_t i = 0; i < nt; ++i) {
+                    num += ((double)a[i] - b[i]) * ((double)a[i] - b[i]);
+                    den += (double)a[i] * a[i];
+                }
+                const double rel = std::sqrt(num / den);
+                std::printf("z-stage p=8 FP32 check, output rows %s, %d repetition(s): "
+                            "matrix-core vs vector rel. diff %.3e\n",
+                            g4 ? "4g+r" : "g+4r", reps, rel);
+                ok = ok && rel < 1e-5;
+            }
+            if (ok) layout = g4;
+        }
+        if (layout < 0) {
+            std::printf("z-stage p=8 FP32 check FAILED for both output layouts\n");
+            return 2;
+        }
+        const int reps = 32;
+        const double flops = 2.0 * (2.0 * 3 * YD * YQ + 9.0 * YQ) * YN * nz * reps;
+        line("z-stage p=8 FP32, vector (thread per column)", time_ms([&] { hipLaunchKernelGGL(k_zstage8_valu, dim3(gv), dim3(128 * YV), 0, 0, zin, bm, pz, tv, nz, reps, 0); }, 5), flops * 1e-3, "TFLOP/s");
+        line("z-stage p=8 FP32, matrix cores (wave per element)", time_ms([&] { launch_m(layout == 1, reps); }, 5), flops * 1e-3, "TFLOP/s");
+        CK(hipFree(zin));
+        CK(hipFree(bm));
+        CK(hipFree(pz));
+        CK(hipFree(tv));
+        CK(hipFree(tm));
     }
     CK(hipFree(scratch));
     std::printf("done\n");
