@@ -244,8 +244,11 @@ __global__ void k_restrict_mult_1(int64_t nE, const int *__restrict__ gmap,
 // P consecutive E-entries per thread (P a multiple of 4; 4 is used: 8 was
 // measured 20-30% slower): 16-byte index loads, P gathers issued together,
 // 16-byte stores (the tail thread goes entry by entry).  Needs 16-byte
-// aligned gmap and y.
-template <int P>
+// aligned gmap and y.  NT: the index loads and E-vector stores, each touched
+// once, go non-temporal.
+typedef int gv4i __attribute__((ext_vector_type(4)));
+typedef double gv2d __attribute__((ext_vector_type(2)));
+template <int P, bool NT>
 __global__ void __launch_bounds__(kBlock)
 k_restrict_mult(int64_t nE, const int *__restrict__ gmap, const double *__restrict__ x,
                 double *__restrict__ y, bool abs_) {
@@ -255,7 +258,8 @@ k_restrict_mult(int64_t nE, const int *__restrict__ gmap, const double *__restri
         int gg[P];
 #pragma unroll
         for (int k = 0; k < P; k += 4) {
-            const int4 g = *reinterpret_cast<const int4 *>(gmap + i0 + k);
+            const gv4i *src = reinterpret_cast<const gv4i *>(gmap + i0 + k);
+            const gv4i g = NT ? __builtin_nontemporal_load(src) : *src;
             gg[k] = g.x;
             gg[k + 1] = g.y;
             gg[k + 2] = g.z;
@@ -267,8 +271,14 @@ k_restrict_mult(int64_t nE, const int *__restrict__ gmap, const double *__restri
 #pragma unroll
         for (int k = 0; k < P; ++k) v[k] = (gg[k] >= 0 || abs_) ? v[k] : -v[k];
 #pragma unroll
-        for (int k = 0; k < P; k += 2)
-            reinterpret_cast<double2 *>(y + i0 + k)[0] = make_double2(v[k], v[k + 1]);
+        for (int k = 0; k < P; k += 2) {
+            gv2d *dst = reinterpret_cast<gv2d *>(y + i0 + k);
+            const gv2d w = {v[k], v[k + 1]};
+            if (NT)
+                __builtin_nontemporal_store(w, dst);
+            else
+                *dst = w;
+        }
         return;
     }
     for (int64_t i = i0; i < nE; ++i) {
@@ -485,9 +495,11 @@ void list_to_mask(const int *list, int64_t nl, uint8_t *mask, int64_t n, hipStre
 
 void restrict_mult(int64_t nE, const int *gmap, const double *x, double *y, bool abs_,
                    hipStream_t s) {
+    // non-temporal index loads and E-vector stores: 0.242 -> 0.210 ms at p = 4
+    // (79^3), 0.198 -> 0.172 ms at p = 6, the L-vector reads keeping the caches
     if (((reinterpret_cast<uintptr_t>(gmap) | reinterpret_cast<uintptr_t>(y)) & 15) == 0)
-        hipLaunchKernelGGL(k_restrict_mult<4>, dim3(nblocks((nE + 3) / 4)), dim3(kBlock), 0, s, nE,
-                           gmap, x, y, abs_);
+        hipLaunchKernelGGL((k_restrict_mult<4, true>), dim3(nblocks((nE + 3) / 4)), dim3(kBlock), 0,
+                           s, nE, gmap, x, y, abs_);
     else
         hipLaunchKernelGGL(k_restrict_mult_1, dim3(nblocks(nE)), dim3(kBlock), 0, s, nE, gmap, x,
                            y, abs_);
